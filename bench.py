@@ -1,0 +1,189 @@
+#!/usr/bin/env python3
+"""Benchmark of the earthquake-cycle hot path on one MI355X.
+
+Workload: transform fault 80 x 20 km on 128 x 32 = 4096 Okada patches
+(down-dip cells graded finer toward the surface) coupled to a hex8 mantle
+block of 16 x 16 x 8 = 2048 cells (12288 strain dofs), linear Maxwell
+viscosity 1e19 Pa s; K = 16384^2 FP64 (2.15 GB), assembled on the device
+from the mesh.  One "step" = one evaluation of the ODE right-hand side
+(prologue + the fused K-streaming/friction kernel) on a state resident in
+HBM.  Also reported: the FP32- and BF16-streamed operator (throughput and
+error against FP64), simulated years per hour of the on-device Tsit5, the
+HBM roofline of the dominant kernel, and the reference algorithm (Toeplitz
+DFT K_ff + dense mantle products) timed on the host cores with the oracle.
+
+Prints ONE JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "oetqf.jl_amd"))
+
+HBM_PEAK_GBPS = 8000.0
+METRIC = "RHS evals/s & simulated yr/hr vs plain FP64 GPU at stated err; HBM GB/s"
+
+
+def algorithmic_bytes(nf, ne, kbytes_per_entry):
+    """Bytes one RHS must move: K once, the state it reads (v, theta, sigma),
+    the derivative it writes (dv, dtheta, deps, dsigma, ddelta) and the
+    per-patch/per-cell parameters."""
+    nm = 6 * ne
+    n = nf + nm
+    return (kbytes_per_entry * n * n + 8 * (2 * nf + nm) + 8 * (3 * nf + 2 * nm)
+            + 8 * (4 * nf + 2 * ne))
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(w, u0, sample_seconds):
+    """The reference algorithm (equation.jl:185-205: Toeplitz DFT K_ff and
+    dense FP64 mantle products, OpenMP) timed on the host with the oracle."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    import oetqf_amd as oq
+
+    mf, ma, lam, mu = w["mf"], w["ma"], w["lam"], w["mu"]
+    st = oq.stress_greens_function(mf, lam, mu, fourier=False)
+    g12 = oq.stress_greens_function(mf, ma, lam, mu)
+    g21 = oq.stress_greens_function(ma, mf, lam, mu)
+    g22 = oq.stress_greens_function(ma, lam, mu, checkeigvals=False)
+    v, th, sig = u0.x[0], u0.x[1], u0.x[3]
+    out = oracle.rhs_coupled(w["pf"], w["pa"], st, g12, g21, g22, v, th, sig)  # warm
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        out = oracle.rhs_coupled(w["pf"], w["pa"], st, g12, g21, g22, v, th, sig)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= sample_seconds:
+            break
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(value=n / el, unit="RHS evals/s", cores=cores, kind="port",
+                sample="%d RHS evaluations of the same workload (%.1f s)" % (n, el)), out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cycle-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-variants", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(requested --gpus %d)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import oetqf_amd as oq
+    from oetqf_amd.workloads import YEAR, transform_fault_composite
+
+    w = transform_fault_composite()
+    mf, ma = w["mf"], w["ma"]
+    nf, ne = mf.nx * mf.nxi, ma.ne
+    u0 = w["u0"]
+    rng = np.random.default_rng(11)
+    # a generic interseismic state: perturbed slip rates, nonzero mantle stress
+    u0.x[0][:] = w["pf"].vpl * np.exp(0.5 * rng.standard_normal(u0.x[0].shape))
+    u0.x[1][:] = w["pf"].L / u0.x[0]
+    u0.x[3][:] = 1e6 * rng.standard_normal(u0.x[3].shape)
+
+    log("assembling K on the device (%s)" % w["name"])
+    t0 = time.perf_counter()
+    prob = oq.assemble_from_mesh(mf, ma, w["lam"], w["mu"], w["pf"], w["pa"], u0, w["tspan"],
+                                 nrept=w["nrept"], buffer_ratio=w["buffer_ratio"])
+    t_asm = time.perf_counter() - t0
+    f = prob.f
+    du64 = oq.similar(u0)
+    f(du64, u0, None, 0.0)  # uploads the state; the timed loop reuses it in HBM
+    log("assembly %.2f s; timing %d RHS evaluations" % (t_asm, args.steps))
+    f.time(args.warmup)
+    t0 = time.perf_counter()
+    ms_eval, ms_mv = f.time(args.steps)
+    wall = time.perf_counter() - t0
+    value = 1000.0 / ms_eval
+    nbytes = algorithmic_bytes(nf, ne, 8)
+    achieved = nbytes / (ms_mv * 1e-3) / 1e9
+    log("FP64: %.4f ms/eval, matvec %.4f ms, %.0f GB/s" % (ms_eval, ms_mv, achieved))
+
+    variants = {}
+    if not args.no_variants:
+        for kd, esz in (("fp32", 4), ("bf16", 2)):
+            p = oq.assemble_from_mesh(mf, ma, w["lam"], w["mu"], w["pf"], w["pa"], u0, w["tspan"],
+                                      nrept=w["nrept"], buffer_ratio=w["buffer_ratio"], kdtype=kd)
+            du = oq.similar(u0)
+            p.f(du, u0, None, 0.0)
+            p.f.time(args.warmup)
+            me, mm = p.f.time(args.steps)
+            err_tau = float(np.linalg.norm(du.x[0] - du64.x[0]) / np.linalg.norm(du64.x[0]))
+            err_sig = float(np.linalg.norm(du.x[3] - du64.x[3]) / np.linalg.norm(du64.x[3]))
+            variants[kd] = dict(rhs_per_s=1000.0 / me, ms_matvec=mm,
+                                hbm_GBps=algorithmic_bytes(nf, ne, esz) / (mm * 1e-3) / 1e9,
+                                rel_err_dv=err_tau, rel_err_dsigma=err_sig,
+                                speedup_vs_fp64=ms_eval / me)
+            log("%s: %.4f ms/eval (%.2fx), err dv %.2e dsigma %.2e" % (kd, me, ms_eval / me, err_tau, err_sig))
+            p.f.close()
+
+    cycle = None
+    if args.cycle_seconds > 0:
+        it = oq.Integrator(prob, reltol=w["reltol"], abstol=w["abstol"])
+        t0 = time.perf_counter()
+        steps = 0
+        while time.perf_counter() - t0 < args.cycle_seconds:
+            ret = it.step(64)
+            steps += 64
+            if ret != "Running":
+                break
+        el = time.perf_counter() - t0
+        t, dt, _, na, nr = it.state()
+        it.close()
+        cycle = dict(sim_years=t / YEAR, wall_s=el, sim_yr_per_hr=(t / YEAR) / (el / 3600.0),
+                     accepted=na, rejected=nr, reltol=w["reltol"], abstol=w["abstol"],
+                     note="interseismic start of the cycle; Tsit5 on device, FP64 K")
+        log("Tsit5: %.3f yr in %.1f s (%d+%d steps)" % (t / YEAR, el, na, nr))
+
+    cpu = None
+    if not args.no_cpu:
+        log("CPU baseline (oracle, reference algorithm)")
+        cpu, ref = cpu_baseline(w, u0, args.cpu_seconds)
+        cpu["rhs_rel_err_gpu_vs_cpu"] = float(np.linalg.norm(du64.x[0] - ref[0]) / np.linalg.norm(ref[0]))
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "RHS evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_eval,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded state; Green's functions assembled from the mesh)",
+        "config": {"workload": w["name"], "n_fault": nf, "n_cells": ne, "N": nf + 6 * ne,
+                   "k_bytes": 8 * (nf + 6 * ne) ** 2, "assembly_s": t_asm,
+                   "wall_s_timed_loop": wall},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "kernel": "k_rhs_matvec<FP64>", "ms_per_launch": ms_mv,
+                     "algorithmic_bytes": nbytes},
+        "variants": variants,
+        "cycle": cycle,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

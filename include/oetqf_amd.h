@@ -1,0 +1,213 @@
+/*
+ * oetqf_amd.h -- C ABI of the MI355X (gfx950) earthquake-cycle hot path.
+ *
+ * Drop-in for the CPU hot path of Oetqf.jl v0.3.6: the stress Green's-function
+ * builders (src/BEM/GF.jl), the ODE right-hand side that `assemble` hands to
+ * OrdinaryDiffEq (src/BEM/equation.jl) and the Tsit5 time loop the solver runs
+ * over it.  Plain C: pointers, sizes and status codes; no exceptions cross it.
+ *
+ * Conventions (those of the reference, Julia column-major):
+ *  - fault fields: nx*nxi doubles, strike index fastest (vec(v), equation.jl:35-42);
+ *  - mantle fields: ne*6 doubles, component-major [xx xy xz yy yz zz]
+ *    (equation.jl:207-222, GF.jl:163-169);
+ *  - tuple vectors (ax, axi: Vector{NTuple{2,Float64}}) are 2n interleaved doubles;
+ *  - ftype: OQ_STRIKE_SLIP / OQ_DIP_SLIP (GF.jl:3-5);
+ *  - every function returns OQ_OK (0) or an error code; oq_last_error() gives
+ *    a message (thread-local).  Argument checks mirror the reference's
+ *    @assert's (GF.jl:36,131,326; property.jl:20-24; mesh.jl:19-20,70-71).
+ *  - host-pointer entry points are synchronous; *_device entry points enqueue
+ *    on the handle's HIP stream and take device pointers.
+ */
+#ifndef OETQF_AMD_H
+#define OETQF_AMD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OQ_OK 0
+#define OQ_ERR_ARG 1     /* invalid argument (reference: AssertionError / ArgumentError) */
+#define OQ_ERR_HIP 2     /* HIP runtime failure */
+#define OQ_ERR_STATE 3   /* handle in an invalid state */
+#define OQ_ERR_NOMEM 4   /* device allocation failed */
+
+#define OQ_STRIKE_SLIP 0 /* StrikeSlip (GF.jl:4) */
+#define OQ_DIP_SLIP 1    /* DipSlip (GF.jl:5) */
+
+/* Storage type of the dense operator K streamed by the RHS. */
+#define OQ_K_FP64 0
+#define OQ_K_FP32 1
+#define OQ_K_BF16 2
+
+/* RectOkadaMesh (src/BEM/mesh.jl:5-21).  Only x, ax, y, z, axi, nx, nxi, dx,
+ * dep and dip enter the Green's functions (GF.jl:38,49-53,150-160). */
+typedef struct oq_fault_mesh {
+    const double *x;   /* nx strike centroids */
+    const double *ax;  /* 2*nx: (x - dx/2, x + dx/2) */
+    const double *y;   /* nxi */
+    const double *z;   /* nxi */
+    const double *axi; /* 2*nxi: down-dip extents */
+    int64_t nx, nxi;
+    double dx, dep, dip; /* dip in degrees */
+} oq_fault_mesh;
+
+/* BEMHex8Mesh (src/BEM/mesh.jl:58-72). */
+typedef struct oq_hex8_mesh {
+    const double *cx, *cy, *cz; /* barycentres */
+    const double *qx, *qy, *qz; /* reference points (cx, cy - dy/2, cz + dz/2) */
+    const double *dx, *dy, *dz; /* cell extents (T, L, W) */
+    int64_t ne;
+} oq_hex8_mesh;
+
+/* Quadrature over a hex8 cell in [-1,1]^3, weights normalised to sum 1
+ * (get_quadrature, GF.jl:318-328).  coords: 3*nq. */
+typedef struct oq_quadrature {
+    const double *coords;
+    const double *weights;
+    int64_t nq;
+} oq_quadrature;
+
+/* RateStateQuasiDynamicProperty (src/BEM/property.jl:10-25), aging law
+ * (DieterichStateLaw, property.jl:4-5). a, b, L, sigma: nx*nxi each. */
+typedef struct oq_rs_props {
+    const double *a, *b, *L, *sigma;
+    double eta, vpl, f0, v0;
+} oq_rs_props;
+
+/* PowerLawViscosityProperty (property.jl:34-41), or with nlaws > 1 the
+ * CompositePowerLawViscosityProperty (property.jl:43-48) whose laws are
+ * stacked: gamma[law*ne + i], n[law*ne + i] (n = power - 1). */
+typedef struct oq_visc_props {
+    const double *gamma;
+    const double *n;
+    int64_t nlaws;
+    double deps0[6];
+} oq_visc_props;
+
+/* Solver options (the reference passes these to OrdinaryDiffEq.solve:
+ * examples/otf-with-mantle.jl:160-161). dt0 <= 0 selects the automatic
+ * initial step; dtmax <= 0 means unbounded. */
+typedef struct oq_solver_opts {
+    double reltol, abstol, dt0, dtmax, dtmin;
+    int64_t maxiters;
+} oq_solver_opts;
+
+/* Stepper return codes (OrdinaryDiffEq ReturnCode analogues). */
+#define OQ_RET_RUNNING 0
+#define OQ_RET_SUCCESS 1
+#define OQ_RET_MAXITERS 2
+#define OQ_RET_DTMIN 3
+#define OQ_RET_NONFINITE 4
+
+typedef struct oq_rhs oq_rhs;
+typedef struct oq_stepper oq_stepper;
+
+/* ---- library ---- */
+const char *oq_last_error(void);
+int oq_version(void);
+int oq_device_count(int *count);
+
+/* ---- Green's functions (host output, caller-owned, Julia layout) ---- */
+
+/* Replaces stress_greens_function(mesh::RectOkadaMesh, λ, μ; ftype,
+ * fourier=false, nrept, buffer_ratio) (src/BEM/GF.jl:31-58).
+ * st: nx*nxi*nxi, st[i + nx*j + nx*nxi*l]. */
+int oq_gf_okada_ff(const oq_fault_mesh *mesh, double lambda, double mu,
+                   int ftype, int64_t nrept, double buffer_ratio, double *st);
+
+/* The fourier=true branch (src/BEM/GF.jl:60-68): the length-(2nx-1) real DFT
+ * of [st[:,j,l]; reverse(st[2:end,j,l])]; st_dft: 2*nx*nxi*nxi doubles,
+ * interleaved (re, im), Complex{Float64} layout. */
+int oq_gf_okada_ff_dft(const oq_fault_mesh *mesh, double lambda, double mu,
+                       int ftype, int64_t nrept, double buffer_ratio,
+                       double *st_dft);
+
+/* Replaces stress_greens_function(mf::RectOkadaMesh, ma::BEMHex8Mesh, λ, μ;
+ * ftype, qtype, nrept, buffer_ratio) (src/BEM/GF.jl:123-174), gf₁₂.
+ * st: (6ne) x (nx*nxi), column-major. */
+int oq_gf_okada_mf(const oq_fault_mesh *mf, const oq_hex8_mesh *ma,
+                   double lambda, double mu, int ftype,
+                   const oq_quadrature *quad, int64_t nrept,
+                   double buffer_ratio, double *st);
+
+/* Replaces stress_greens_function(ma::BEMHex8Mesh, mf::RectOkadaMesh, λ, μ;
+ * ftype) (src/BEM/GF.jl:194-227), gf₂₁.  st: (nx*nxi) x (6ne), column-major. */
+int oq_gf_hex8_fm(const oq_hex8_mesh *ma, const oq_fault_mesh *mf,
+                  double lambda, double mu, int ftype, double *st);
+
+/* Replaces stress_greens_function(ma::BEMHex8Mesh, λ, μ; qtype,
+ * checkeigvals=false) (src/BEM/GF.jl:250-290), gf₂₂.
+ * st: (6ne) x (6ne), column-major.  The eigenvalue diagnostic of
+ * GF.jl:291-294 stays with the caller. */
+int oq_gf_hex8_mm(const oq_hex8_mesh *ma, double lambda, double mu,
+                  const oq_quadrature *quad, double *st);
+
+/* ---- right-hand side ---- */
+
+/* Replaces assemble(gf, p, u0, tspan) (equation.jl:81-89; fault only: ne = 0,
+ * visc, gf12, gf21, gf22 NULL) and assemble(gf₁₁, gf₁₂, gf₂₁, gf₂₂, pf, pa,
+ * u0, tspan) (equation.jl:141-154): binds the Green's functions and
+ * properties to a device-resident handle.  gf11 is the real Toeplitz kernel
+ * (nx,nxi,nxi) when gf11_is_dft = 0, or its complex DFT (2*nx*nxi*nxi
+ * doubles, the default output of the reference builder) when 1.  The
+ * library expands K into one dense receiver-major operator stored as
+ * kdtype and owns it until oq_rhs_destroy. */
+int oq_rhs_create(int device, int64_t nx, int64_t nxi, int64_t ne,
+                  const oq_rs_props *fault, const oq_visc_props *visc,
+                  const double *gf11, int gf11_is_dft, const double *gf12,
+                  const double *gf21, const double *gf22, int kdtype,
+                  oq_rhs **out);
+
+/* As oq_rhs_create, but the four blocks are assembled directly on the device
+ * (GF.jl:31-58,123-174,194-227,250-290) -- no host copy of K. */
+int oq_rhs_create_from_mesh(int device, const oq_fault_mesh *mf,
+                            const oq_hex8_mesh *ma, double lambda, double mu,
+                            int ftype, int64_t nrept, double buffer_ratio,
+                            const oq_quadrature *quad,
+                            const oq_rs_props *fault,
+                            const oq_visc_props *visc, int kdtype,
+                            oq_rhs **out);
+
+/* The RHS `ode(du,u,p,t)` (equation.jl:156-166 fault only, :185-205 coupled)
+ * on host arrays: the ArrayPartition components are separate pointers.
+ * Fault only: eps, sigma, deps, dsigma are ignored (may be NULL). */
+int oq_rhs_eval(oq_rhs *h, double t, const double *v, const double *theta,
+                const double *eps, const double *sigma, const double *delta,
+                double *dv, double *dtheta, double *deps, double *dsigma,
+                double *ddelta);
+
+/* Same on device arrays in packed ArrayPartition order
+ * ([v θ δ] or [v θ ϵ σ δ]), enqueued on the handle's stream. */
+int oq_rhs_eval_device(oq_rhs *h, double t, const double *u, double *du);
+
+/* Sizes: number of state entries, matrix order N = nf + 6ne, bytes of K. */
+int oq_rhs_info(oq_rhs *h, int64_t *nstate, int64_t *n, int64_t *kbytes);
+
+/* Copies the dense operator back (N*N doubles, row r = receiver), for tests. */
+int oq_rhs_get_k(oq_rhs *h, double *k_rowmajor);
+
+/* Times `iters` device RHS evaluations on the handle's stream with HIP
+ * events; reports the mean per evaluation and the mean of the dominant
+ * (matrix-streaming) kernel. */
+int oq_rhs_time(oq_rhs *h, int iters, double *ms_eval, double *ms_matvec);
+
+int oq_rhs_destroy(oq_rhs *h);
+
+/* ---- on-device Tsit5 stepper (replaces solve(prob, Tsit5(); reltol,
+ * abstol, dt, dtmax, maxiters), io.jl:125-130, test/tests.jl:11) ---- */
+int oq_stepper_create(oq_rhs *h, const double *u0, double t0, double t1,
+                      const oq_solver_opts *opts, oq_stepper **out);
+/* Advances up to max_steps attempted steps (or to t1).  *retcode gets one of
+ * OQ_RET_*. */
+int oq_stepper_run(oq_stepper *s, int64_t max_steps, int *retcode);
+/* Current state: t, dt, u (packed, host), step counters. */
+int oq_stepper_state(oq_stepper *s, double *t, double *dt, double *u,
+                     int64_t *naccept, int64_t *nreject);
+int oq_stepper_destroy(oq_stepper *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

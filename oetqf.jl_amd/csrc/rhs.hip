@@ -1,0 +1,643 @@
+// rhs.hip -- the quasi-dynamic earthquake-cycle RHS on gfx950.
+//
+// Replaces `ode(du,u,p,t)` of src/BEM/equation.jl (:156-166 fault only,
+// :185-205 coupled) for the reference's `assemble` (equation.jl:81-89,
+// :141-154).  The four Green's blocks are held as ONE dense receiver-major
+// operator K (N = nf + 6ne rows; row r = receiver, column c = source):
+//
+//     [tau_dot  ]   [K_ff  K_fm] [v - vpl       ]
+//     [sigma_dot] = [K_mf  K_mm] [eps_dot - e0  ]
+//
+// and each RHS is two launches:
+//   1. prologue  (O(N)):   s = [v - vpl ; eps_dot(sigma) - deps0], also
+//                          writes du.eps = eps_dot (power-law viscosity,
+//                          equation.jl:207-230);
+//   2. matvec    (O(N^2)): every 64-wide wavefront streams R rows of K from
+//                          HBM with 16-byte non-temporal loads, FP64
+//                          accumulation, a cross-lane reduction, then the
+//                          fused epilogue: rate-and-state update for fault
+//                          rows (update_fault!, equation.jl:233-246) or
+//                          du.sigma for mantle rows.
+// K may be stored as FP64, FP32 or BF16; products are always accumulated in
+// FP64 (the FP64 FMA rate is ~40x what an 8 TB/s stream needs).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <vector>
+
+#include "rhs_impl.h"
+
+namespace oq {
+
+void toeplitz_idft(const double *dft, int64_t nx, int64_t nxi, double *st);
+
+static constexpr int kWaves = 4;      // wavefronts per workgroup
+static constexpr int kRows = 4;       // rows per wavefront
+static constexpr int64_t kPad = 512;  // row pitch multiple (elements)
+
+// ---------------- prologue ----------------
+__global__ void __launch_bounds__(256) k_rhs_prologue(RhsDev d, const double *__restrict__ u,
+                                                      double *__restrict__ du,
+                                                      double *__restrict__ s)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nf = d.lay.nf, ne = d.lay.ne;
+    if (t < nf) {
+        s[t] = u[d.lay.v + t] - d.vpl;  // relative_velocity! (equation.jl:35-42)
+        return;
+    }
+    const int64_t i = t - nf;
+    if (i >= ne) return;
+    // update_strain_rate! + relative_strain_rate! (equation.jl:207-230)
+    const double *sg = u + d.lay.sig;
+    const double sxx0 = sg[i], sxy = sg[i + ne], sxz = sg[i + 2 * ne];
+    const double syy0 = sg[i + 3 * ne], syz = sg[i + 4 * ne], szz0 = sg[i + 5 * ne];
+    const double skk = (sxx0 + syy0 + szz0) / 3.0;
+    const double sxx = sxx0 - skk, syy = syy0 - skk, szz = szz0 - skk;
+    const double tn = sqrt(sxx * sxx + syy * syy + szz * szz +
+                           2.0 * (sxy * sxy + sxz * sxz + syz * syz));
+    double f = 0.0;
+    for (int l = 0; l < d.nlaws; ++l) f += d.gamma[l * ne + i] * pow(tn, d.nexp[l * ne + i]);
+    const double e[6] = {f * sxx, f * sxy, f * sxz, f * syy, f * syz, f * szz};
+    double *de = du + d.lay.eps;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        de[i + k * ne] = e[k];
+        s[nf + i + k * ne] = e[k] - d.deps0[k];
+    }
+}
+
+// ---------------- K streaming + fused epilogue ----------------
+template <int KT> struct KTraits;
+template <> struct KTraits<OQ_K_FP64> {
+    using T = double;
+    static constexpr int kPerLane = 2;  // 16 B per lane per load
+};
+template <> struct KTraits<OQ_K_FP32> {
+    using T = float;
+    static constexpr int kPerLane = 4;
+};
+template <> struct KTraits<OQ_K_BF16> {
+    using T = unsigned short;
+    static constexpr int kPerLane = 8;
+};
+
+__device__ __forceinline__ double bf16_lo(unsigned int w) { return (double)__uint_as_float(w << 16); }
+__device__ __forceinline__ double bf16_hi(unsigned int w) { return (double)__uint_as_float(w & 0xffff0000u); }
+
+// acc += dot(16 bytes of K row at element c, s[c..])
+template <int KT>
+__device__ __forceinline__ double dot16(const uint4 kv, const double *__restrict__ s, int64_t c);
+
+template <>
+__device__ __forceinline__ double dot16<OQ_K_FP64>(const uint4 kv, const double *__restrict__ s,
+                                                   int64_t c)
+{
+    const double2 sv = *reinterpret_cast<const double2 *>(s + c);
+    const double k0 = __hiloint2double(kv.y, kv.x), k1 = __hiloint2double(kv.w, kv.z);
+    return fma(k0, sv.x, k1 * sv.y);
+}
+
+template <>
+__device__ __forceinline__ double dot16<OQ_K_FP32>(const uint4 kv, const double *__restrict__ s,
+                                                   int64_t c)
+{
+    const double4 sv = *reinterpret_cast<const double4 *>(s + c);
+    double r = (double)__uint_as_float(kv.x) * sv.x;
+    r = fma((double)__uint_as_float(kv.y), sv.y, r);
+    r = fma((double)__uint_as_float(kv.z), sv.z, r);
+    return fma((double)__uint_as_float(kv.w), sv.w, r);
+}
+
+template <>
+__device__ __forceinline__ double dot16<OQ_K_BF16>(const uint4 kv, const double *__restrict__ s,
+                                                   int64_t c)
+{
+    const double4 s0 = *reinterpret_cast<const double4 *>(s + c);
+    const double4 s1 = *reinterpret_cast<const double4 *>(s + c + 4);
+    double r = bf16_lo(kv.x) * s0.x;
+    r = fma(bf16_hi(kv.x), s0.y, r);
+    r = fma(bf16_lo(kv.y), s0.z, r);
+    r = fma(bf16_hi(kv.y), s0.w, r);
+    r = fma(bf16_lo(kv.z), s1.x, r);
+    r = fma(bf16_hi(kv.z), s1.y, r);
+    r = fma(bf16_lo(kv.w), s1.z, r);
+    return fma(bf16_hi(kv.w), s1.w, r);
+}
+
+__device__ __forceinline__ uint4 ld_nt(const uint4 *p)
+{
+    uint4 v;
+    v.x = __builtin_nontemporal_load(&p->x);
+    v.y = __builtin_nontemporal_load(&p->y);
+    v.z = __builtin_nontemporal_load(&p->z);
+    v.w = __builtin_nontemporal_load(&p->w);
+    return v;
+}
+
+__device__ __forceinline__ double wave_sum(double x)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+
+// Rate-and-state update of one fault patch (update_fault! with the aging
+// law, equation.jl:237-245 and :279).
+__device__ __forceinline__ void fault_update(const RhsDev &d, int64_t i, double dtau,
+                                            const double *__restrict__ u, double *__restrict__ du)
+{
+    const double v = u[d.lay.v + i], th = u[d.lay.th + i];
+    const double a = d.a[i], b = d.b[i], L = d.L[i];
+    const double thp = th > 0.0 ? th : 0.0;
+    const double psi1 = exp((d.f0 + b * log(d.v0 * thp / L)) / a) / (2.0 * d.v0);
+    const double psi2 = d.sigma[i] * psi1 / hypot(1.0, v * psi1);
+    const double dmu_dv = a * psi2;
+    const double dmu_dth = b / th * v * psi2;
+    const double dth = 1.0 - v * th / L;
+    du[d.lay.th + i] = dth;
+    du[d.lay.v + i] = (dtau - dmu_dth * dth) / (dmu_dv + d.eta);
+    du[d.lay.del + i] = v;
+}
+
+template <int KT>
+__global__ void __launch_bounds__(kWaves * 64) k_rhs_matvec(RhsDev d, const void *__restrict__ Kp,
+                                                            const double *__restrict__ s,
+                                                            const double *__restrict__ u,
+                                                            double *__restrict__ du)
+{
+    using T = typename KTraits<KT>::T;
+    constexpr int E = KTraits<KT>::kPerLane;  // elements per 16-byte load
+    constexpr int64_t STEP = 64 * E;          // columns per wavefront pass
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int64_t row0 = ((int64_t)blockIdx.x * kWaves + wave) * kRows;
+    if (row0 >= d.N) return;
+    const T *K = static_cast<const T *>(Kp);
+    const uint4 *rp[kRows];
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+        const int64_t row = row0 + r < d.N ? row0 + r : d.N - 1;
+        rp[r] = reinterpret_cast<const uint4 *>(K + row * d.ld) + lane;
+    }
+    double acc[kRows];
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) acc[r] = 0.0;
+    const int64_t npass = d.ld / STEP;
+    int64_t p = 0;
+    // two passes per iteration: 2*kRows 16-byte loads in flight per lane
+    for (; p + 2 <= npass; p += 2) {
+        uint4 k0[kRows], k1[kRows];
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) {
+            k0[r] = ld_nt(rp[r] + p * 64);
+            k1[r] = ld_nt(rp[r] + (p + 1) * 64);
+        }
+        const int64_t c0 = p * STEP + lane * E, c1 = c0 + STEP;
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) {
+            acc[r] += dot16<KT>(k0[r], s, c0);
+            acc[r] += dot16<KT>(k1[r], s, c1);
+        }
+    }
+    for (; p < npass; ++p) {
+        const int64_t c0 = p * STEP + lane * E;
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) acc[r] += dot16<KT>(ld_nt(rp[r] + p * 64), s, c0);
+    }
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) acc[r] = wave_sum(acc[r]);
+    if (lane < kRows) {
+        double y = acc[0];
+#pragma unroll
+        for (int r = 1; r < kRows; ++r)
+            if (lane == r) y = acc[r];
+        const int64_t row = row0 + lane;
+        if (row < d.N) {
+            if (row < d.lay.nf) fault_update(d, row, y, u, du);
+            else du[d.lay.sig + (row - d.lay.nf)] = y;
+        }
+    }
+}
+
+static inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+int rhs_enqueue_matvec(oq_rhs *h, const double *u, double *du)
+{
+    const unsigned grid = nblk(h->N, kWaves * kRows);
+    switch (h->kdtype) {
+    case OQ_K_FP64:
+        hipLaunchKernelGGL(k_rhs_matvec<OQ_K_FP64>, dim3(grid), dim3(kWaves * 64), 0, h->stream,
+                           h->dev, h->K.p, h->s64.as<double>(), u, du);
+        break;
+    case OQ_K_FP32:
+        hipLaunchKernelGGL(k_rhs_matvec<OQ_K_FP32>, dim3(grid), dim3(kWaves * 64), 0, h->stream,
+                           h->dev, h->K.p, h->s64.as<double>(), u, du);
+        break;
+    default:
+        hipLaunchKernelGGL(k_rhs_matvec<OQ_K_BF16>, dim3(grid), dim3(kWaves * 64), 0, h->stream,
+                           h->dev, h->K.p, h->s64.as<double>(), u, du);
+        break;
+    }
+    OQ_HIP(hipGetLastError());
+    return OQ_OK;
+}
+
+int rhs_enqueue(oq_rhs *h, const double *u, double *du)
+{
+    const int64_t n = h->nf + h->ne;
+    hipLaunchKernelGGL(k_rhs_prologue, dim3(nblk(n, 256)), dim3(256), 0, h->stream, h->dev, u, du,
+                       h->s64.as<double>());
+    OQ_HIP(hipGetLastError());
+    return rhs_enqueue_matvec(h, u, du);
+}
+
+// ---------------- operator assembly helpers ----------------
+
+// dst[r*ld + c] = src[r*srs + c*scs] for an m x n block.
+__global__ void __launch_bounds__(256) k_copy_block(const double *__restrict__ src, int64_t m,
+                                                    int64_t n, int64_t srs, int64_t scs,
+                                                    double *__restrict__ dst, int64_t ld)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m * n) return;
+    const int64_t c = t % n, r = t / n;
+    dst[r * ld + c] = src[r * srs + c * scs];
+}
+
+__global__ void __launch_bounds__(256) k_to_fp32(const double *__restrict__ src, float *dst, int64_t n)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) dst[t] = (float)src[t];
+}
+
+// Round-to-nearest-even FP64 -> BF16 (via the exactly rounded FP32 value).
+__global__ void __launch_bounds__(256) k_to_bf16(const double *__restrict__ src,
+                                                 unsigned short *dst, int64_t n)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const unsigned int w = __float_as_uint((float)src[t]);
+    const unsigned int r = w + 0x7fffu + ((w >> 16) & 1u);
+    dst[t] = (unsigned short)(r >> 16);
+}
+
+__global__ void __launch_bounds__(256) k_from_lowp(const void *src, int kt, double *dst, int64_t n)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    if (kt == OQ_K_FP32) dst[t] = (double)static_cast<const float *>(src)[t];
+    else dst[t] = (double)__uint_as_float(((unsigned int)static_cast<const unsigned short *>(src)[t]) << 16);
+}
+
+static int alloc_common(oq_rhs *h, int device, int64_t nx, int64_t nxi, int64_t ne, int kdtype)
+{
+    OQ_REQUIRE(nx > 0 && nxi > 0 && ne >= 0, "sizes must be positive");
+    OQ_REQUIRE(kdtype == OQ_K_FP64 || kdtype == OQ_K_FP32 || kdtype == OQ_K_BF16,
+               "kdtype must be OQ_K_FP64, OQ_K_FP32 or OQ_K_BF16");
+    int ndev = 0;
+    OQ_HIP(hipGetDeviceCount(&ndev));
+    OQ_REQUIRE(device >= 0 && device < ndev, "invalid device index");
+    OQ_HIP(hipSetDevice(device));
+    h->device = device;
+    OQ_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    OQ_HIP(hipEventCreate(&h->ev0));
+    OQ_HIP(hipEventCreate(&h->ev1));
+    h->nx = nx;
+    h->nxi = nxi;
+    h->ne = ne;
+    h->nf = nx * nxi;
+    h->N = h->nf + 6 * ne;
+    h->ld = (h->N + kPad - 1) / kPad * kPad;
+    h->kdtype = kdtype;
+    h->lay.set(h->nf, ne);
+    int rc;
+    if ((rc = h->s64.alloc((size_t)h->ld * 8))) return rc;
+    OQ_HIP(hipMemsetAsync(h->s64.p, 0, (size_t)h->ld * 8, h->stream));
+    if ((rc = h->u.alloc((size_t)h->lay.n * 8))) return rc;
+    if ((rc = h->du.alloc((size_t)h->lay.n * 8))) return rc;
+    OQ_HIP(hipMemsetAsync(h->u.p, 0, (size_t)h->lay.n * 8, h->stream));
+    OQ_HIP(hipMemsetAsync(h->du.p, 0, (size_t)h->lay.n * 8, h->stream));
+    return OQ_OK;
+}
+
+static int upload_props(oq_rhs *h, const oq_rs_props *f, const oq_visc_props *v)
+{
+    OQ_REQUIRE(f && f->a && f->b && f->L && f->sigma, "fault property array is NULL");
+    // RateStateQuasiDynamicProperty invariants (property.jl:20-24)
+    OQ_REQUIRE(f->f0 > 0.0, "f₀ > 0 required");
+    OQ_REQUIRE(f->v0 > 0.0, "v₀ > 0 required");
+    OQ_REQUIRE(f->eta > 0.0, "η > 0 required");
+    OQ_REQUIRE(f->vpl > 0.0, "vpl > 0 required");
+    const size_t nf8 = (size_t)h->nf * 8;
+    int rc;
+    if ((rc = h->a.upload(f->a, nf8, h->stream))) return rc;
+    if ((rc = h->b.upload(f->b, nf8, h->stream))) return rc;
+    if ((rc = h->L.upload(f->L, nf8, h->stream))) return rc;
+    if ((rc = h->sigma.upload(f->sigma, nf8, h->stream))) return rc;
+    RhsDev &d = h->dev;
+    d.a = h->a.as<double>();
+    d.b = h->b.as<double>();
+    d.L = h->L.as<double>();
+    d.sigma = h->sigma.as<double>();
+    d.eta = f->eta;
+    d.vpl = f->vpl;
+    d.f0 = f->f0;
+    d.v0 = f->v0;
+    d.nlaws = 0;
+    d.gamma = d.nexp = nullptr;
+    if (h->ne > 0) {
+        OQ_REQUIRE(v && v->gamma && v->n && v->nlaws >= 1, "viscosity property is NULL");
+        const size_t n8 = (size_t)(v->nlaws * h->ne) * 8;
+        if ((rc = h->gamma.upload(v->gamma, n8, h->stream))) return rc;
+        if ((rc = h->nexp.upload(v->n, n8, h->stream))) return rc;
+        d.gamma = h->gamma.as<double>();
+        d.nexp = h->nexp.as<double>();
+        d.nlaws = (int)v->nlaws;
+        for (int k = 0; k < 6; ++k) d.deps0[k] = v->deps0[k];
+    }
+    d.lay = h->lay;
+    d.N = h->N;
+    d.ld = h->ld;
+    return OQ_OK;
+}
+
+// Converts the FP64 operator in k64 (N x ld) to h->K in h->kdtype.
+static int finalize_k(oq_rhs *h, DevBuf &k64)
+{
+    const int64_t n = h->N * h->ld;
+    if (h->kdtype == OQ_K_FP64) {
+        h->K.p = k64.p;
+        h->K.bytes = k64.bytes;
+        k64.p = nullptr;
+        k64.bytes = 0;
+        return OQ_OK;
+    }
+    int rc;
+    if (h->kdtype == OQ_K_FP32) {
+        if ((rc = h->K.alloc((size_t)n * 4))) return rc;
+        hipLaunchKernelGGL(k_to_fp32, dim3(nblk(n, 256)), dim3(256), 0, h->stream, k64.as<double>(),
+                           h->K.as<float>(), n);
+    } else {
+        if ((rc = h->K.alloc((size_t)n * 2))) return rc;
+        hipLaunchKernelGGL(k_to_bf16, dim3(nblk(n, 256)), dim3(256), 0, h->stream, k64.as<double>(),
+                           h->K.as<unsigned short>(), n);
+    }
+    OQ_HIP(hipGetLastError());
+    OQ_HIP(hipStreamSynchronize(h->stream));
+    k64.release();
+    return OQ_OK;
+}
+
+static void destroy(oq_rhs *h)
+{
+    if (!h) return;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+}  // namespace oq
+
+using namespace oq;
+
+extern "C" {
+
+int oq_rhs_create(int device, int64_t nx, int64_t nxi, int64_t ne, const oq_rs_props *fault,
+                  const oq_visc_props *visc, const double *gf11, int gf11_is_dft,
+                  const double *gf12, const double *gf21, const double *gf22, int kdtype,
+                  oq_rhs **out)
+{
+    OQ_REQUIRE(out != nullptr, "out is NULL");
+    *out = nullptr;
+    OQ_REQUIRE(gf11 != nullptr, "gf11 is NULL");
+    OQ_REQUIRE(ne == 0 || (gf12 && gf21 && gf22), "coupled model needs gf12, gf21 and gf22");
+    OQ_REQUIRE(nx > 0 && nxi > 0 && ne >= 0, "sizes must be positive");
+    OQ_REQUIRE(fault && fault->a && fault->b && fault->L && fault->sigma,
+               "fault property array is NULL");
+    OQ_REQUIRE(fault->f0 > 0.0 && fault->v0 > 0.0 && fault->eta > 0.0 && fault->vpl > 0.0,
+               "RateStateQuasiDynamicProperty requires f₀ > 0, v₀ > 0, η > 0, vpl > 0");
+    OQ_REQUIRE(ne == 0 || (visc && visc->gamma && visc->n && visc->nlaws >= 1),
+               "viscosity property is NULL");
+    oq_rhs *h = new oq_rhs();
+    int rc = alloc_common(h, device, nx, nxi, ne, kdtype);
+    if (!rc) rc = upload_props(h, fault, visc);
+    if (rc) {
+        destroy(h);
+        return rc;
+    }
+    const int64_t nf = h->nf, m6 = 6 * ne, ld = h->ld;
+    DevBuf k64, tmp;
+    rc = k64.alloc((size_t)(h->N * ld) * 8);
+    if (!rc && hipMemsetAsync(k64.p, 0, (size_t)(h->N * ld) * 8, h->stream) != hipSuccess) {
+        set_error("hipMemsetAsync failed");
+        rc = OQ_ERR_HIP;
+    }
+    if (!rc) {
+        std::vector<double> st;
+        const double *stp = gf11;
+        if (gf11_is_dft) {
+            st.resize((size_t)(nf * nxi));
+            toeplitz_idft(gf11, nx, nxi, st.data());
+            stp = st.data();
+        }
+        rc = tmp.upload(stp, (size_t)(nf * nxi) * 8, h->stream);
+        if (!rc) rc = launch_expand_toeplitz(tmp.as<double>(), nx, nxi, k64.as<double>(), ld, h->stream);
+    }
+    struct Blk { const double *src; int64_t m, n, srs, scs, r0, c0; };
+    const Blk blks[3] = {{gf21, nf, m6, 1, nf, 0, nf},     // K_fm: nf x 6ne col-major
+                         {gf12, m6, nf, 1, m6, nf, 0},     // K_mf: 6ne x nf
+                         {gf22, m6, m6, 1, m6, nf, nf}};   // K_mm: 6ne x 6ne
+    for (int bi = 0; !rc && ne > 0 && bi < 3; ++bi) {
+        const Blk &b = blks[bi];
+        if (hipStreamSynchronize(h->stream) != hipSuccess) { rc = OQ_ERR_HIP; break; }
+        rc = tmp.upload(b.src, (size_t)(b.m * b.n) * 8, h->stream);
+        if (rc) break;
+        hipLaunchKernelGGL(k_copy_block, dim3(nblk(b.m * b.n, 256)), dim3(256), 0, h->stream,
+                           tmp.as<double>(), b.m, b.n, b.srs, b.scs,
+                           k64.as<double>() + b.r0 * ld + b.c0, ld);
+        if (hipGetLastError() != hipSuccess) {
+            set_error("k_copy_block launch failed");
+            rc = OQ_ERR_HIP;
+        }
+    }
+    if (!rc && hipStreamSynchronize(h->stream) != hipSuccess) {
+        set_error("assembly failed");
+        rc = OQ_ERR_HIP;
+    }
+    tmp.release();
+    if (!rc) rc = finalize_k(h, k64);
+    if (rc) {
+        destroy(h);
+        return rc;
+    }
+    *out = h;
+    return OQ_OK;
+}
+
+int oq_rhs_create_from_mesh(int device, const oq_fault_mesh *mf, const oq_hex8_mesh *ma,
+                            double lambda, double mu, int ftype, int64_t nrept,
+                            double buffer_ratio, const oq_quadrature *quad,
+                            const oq_rs_props *fault, const oq_visc_props *visc, int kdtype,
+                            oq_rhs **out)
+{
+    OQ_REQUIRE(out != nullptr, "out is NULL");
+    *out = nullptr;
+    int rc;
+    if ((rc = check_fault(mf))) return rc;
+    OQ_REQUIRE(ftype == 0 || ftype == 1, "ftype must be StrikeSlip (0) or DipSlip (1)");
+    OQ_REQUIRE(buffer_ratio >= 0.0, "Argument `buffer_ratio` must be ≥ 0.");
+    OQ_REQUIRE(nrept >= 0, "nrept must be >= 0");
+    const int64_t ne = ma ? ma->ne : 0;
+    if (ma && (rc = check_hex8(ma))) return rc;
+    if (ma && (rc = check_quad(quad))) return rc;
+    oq_rhs *h = new oq_rhs();
+    rc = alloc_common(h, device, mf->nx, mf->nxi, ne, kdtype);
+    if (!rc) rc = upload_props(h, fault, visc);
+    FaultUpload fu;
+    Hex8Upload hu;
+    QuadUpload qu;
+    DevBuf k64, st;
+    if (!rc) rc = fu.load(mf);
+    if (!rc) fu.dev.lrept = (buffer_ratio + 1.0) * (mf->dx * (double)mf->nx);
+    if (!rc && ma) rc = hu.load(ma);
+    if (!rc && ma) rc = qu.load(quad);
+    const int64_t nf = h->nf, ld = h->ld;
+    if (!rc) rc = k64.alloc((size_t)(h->N * ld) * 8);
+    if (!rc && hipMemsetAsync(k64.p, 0, (size_t)(h->N * ld) * 8, h->stream) != hipSuccess) {
+        set_error("hipMemsetAsync failed");
+        rc = OQ_ERR_HIP;
+    }
+    if (!rc) rc = st.alloc((size_t)(nf * mf->nxi) * 8);
+    if (!rc) rc = launch_gf_ff(fu.dev, lambda, mu, ftype, (int)nrept, st.as<double>(), h->stream);
+    if (!rc) rc = launch_expand_toeplitz(st.as<double>(), mf->nx, mf->nxi, k64.as<double>(), ld, h->stream);
+    if (!rc && ne > 0) {
+        double *K = k64.as<double>();
+        rc = launch_gf_fm(hu.dev, fu.dev, lambda, mu, ftype, K + nf, ld, 1, h->stream);
+        if (!rc)
+            rc = launch_gf_mf(fu.dev, hu.dev, qu.c.as<double>(), qu.w.as<double>(), qu.nq, lambda,
+                              mu, ftype, (int)nrept, K + nf * ld, ld, 1, h->stream);
+        if (!rc)
+            rc = launch_gf_mm(hu.dev, qu.c.as<double>(), qu.w.as<double>(), qu.nq, lambda, mu,
+                              K + nf * ld + nf, ld, 1, h->stream);
+    }
+    if (!rc && hipStreamSynchronize(h->stream) != hipSuccess) {
+        set_error("device assembly failed");
+        rc = OQ_ERR_HIP;
+    }
+    if (!rc) rc = finalize_k(h, k64);
+    if (rc) {
+        destroy(h);
+        return rc;
+    }
+    *out = h;
+    return OQ_OK;
+}
+
+int oq_rhs_eval(oq_rhs *h, double t, const double *v, const double *theta, const double *eps,
+                const double *sigma, const double *delta, double *dv, double *dtheta,
+                double *deps, double *dsigma, double *ddelta)
+{
+    (void)t;
+    OQ_REQUIRE(h != nullptr, "handle is NULL");
+    OQ_REQUIRE(v && theta && delta && dv && dtheta && ddelta, "state pointer is NULL");
+    OQ_REQUIRE(h->ne == 0 || (eps && sigma && deps && dsigma), "mantle state pointer is NULL");
+    OQ_HIP(hipSetDevice(h->device));
+    const StateLayout &L = h->lay;
+    double *u = h->u.as<double>(), *du = h->du.as<double>();
+    const size_t f8 = (size_t)L.nf * 8, m8 = (size_t)(6 * L.ne) * 8;
+    OQ_HIP(hipMemcpyAsync(u + L.v, v, f8, hipMemcpyHostToDevice, h->stream));
+    OQ_HIP(hipMemcpyAsync(u + L.th, theta, f8, hipMemcpyHostToDevice, h->stream));
+    OQ_HIP(hipMemcpyAsync(u + L.del, delta, f8, hipMemcpyHostToDevice, h->stream));
+    if (L.ne > 0) {
+        OQ_HIP(hipMemcpyAsync(u + L.eps, eps, m8, hipMemcpyHostToDevice, h->stream));
+        OQ_HIP(hipMemcpyAsync(u + L.sig, sigma, m8, hipMemcpyHostToDevice, h->stream));
+    }
+    int rc = rhs_enqueue(h, u, du);
+    if (rc) return rc;
+    OQ_HIP(hipMemcpyAsync(dv, du + L.v, f8, hipMemcpyDeviceToHost, h->stream));
+    OQ_HIP(hipMemcpyAsync(dtheta, du + L.th, f8, hipMemcpyDeviceToHost, h->stream));
+    OQ_HIP(hipMemcpyAsync(ddelta, du + L.del, f8, hipMemcpyDeviceToHost, h->stream));
+    if (L.ne > 0) {
+        OQ_HIP(hipMemcpyAsync(deps, du + L.eps, m8, hipMemcpyDeviceToHost, h->stream));
+        OQ_HIP(hipMemcpyAsync(dsigma, du + L.sig, m8, hipMemcpyDeviceToHost, h->stream));
+    }
+    OQ_HIP(hipStreamSynchronize(h->stream));
+    return OQ_OK;
+}
+
+int oq_rhs_eval_device(oq_rhs *h, double t, const double *u, double *du)
+{
+    (void)t;
+    OQ_REQUIRE(h != nullptr && u != nullptr && du != nullptr, "NULL argument");
+    OQ_HIP(hipSetDevice(h->device));
+    return rhs_enqueue(h, u, du);
+}
+
+int oq_rhs_info(oq_rhs *h, int64_t *nstate, int64_t *n, int64_t *kbytes)
+{
+    OQ_REQUIRE(h != nullptr, "handle is NULL");
+    if (nstate) *nstate = h->lay.n;
+    if (n) *n = h->N;
+    if (kbytes) *kbytes = (int64_t)h->K.bytes;
+    return OQ_OK;
+}
+
+int oq_rhs_get_k(oq_rhs *h, double *k)
+{
+    OQ_REQUIRE(h != nullptr && k != nullptr, "NULL argument");
+    OQ_HIP(hipSetDevice(h->device));
+    const int64_t n = h->N * h->ld;
+    DevBuf tmp;
+    const double *src = h->K.as<double>();
+    if (h->kdtype != OQ_K_FP64) {
+        int rc = tmp.alloc((size_t)n * 8);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_from_lowp, dim3(nblk(n, 256)), dim3(256), 0, h->stream, h->K.p,
+                           h->kdtype, tmp.as<double>(), n);
+        OQ_HIP(hipGetLastError());
+        src = tmp.as<double>();
+    }
+    OQ_HIP(hipMemcpy2DAsync(k, (size_t)h->N * 8, src, (size_t)h->ld * 8, (size_t)h->N * 8,
+                            (size_t)h->N, hipMemcpyDeviceToHost, h->stream));
+    OQ_HIP(hipStreamSynchronize(h->stream));
+    return OQ_OK;
+}
+
+int oq_rhs_time(oq_rhs *h, int iters, double *ms_eval, double *ms_matvec)
+{
+    OQ_REQUIRE(h != nullptr && iters > 0, "invalid argument");
+    OQ_HIP(hipSetDevice(h->device));
+    double *u = h->u.as<double>(), *du = h->du.as<double>();
+    int rc;
+    OQ_HIP(hipEventRecord(h->ev0, h->stream));
+    for (int i = 0; i < iters; ++i)
+        if ((rc = rhs_enqueue(h, u, du))) return rc;
+    OQ_HIP(hipEventRecord(h->ev1, h->stream));
+    OQ_HIP(hipEventSynchronize(h->ev1));
+    float ms = 0.f;
+    OQ_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    if (ms_eval) *ms_eval = (double)ms / iters;
+    OQ_HIP(hipEventRecord(h->ev0, h->stream));
+    for (int i = 0; i < iters; ++i)
+        if ((rc = rhs_enqueue_matvec(h, u, du))) return rc;
+    OQ_HIP(hipEventRecord(h->ev1, h->stream));
+    OQ_HIP(hipEventSynchronize(h->ev1));
+    OQ_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    if (ms_matvec) *ms_matvec = (double)ms / iters;
+    return OQ_OK;
+}
+
+int oq_rhs_destroy(oq_rhs *h)
+{
+    if (h) {
+        (void)hipSetDevice(h->device);
+        destroy(h);
+    }
+    return OQ_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,64 @@
+// rhs_impl.h -- the device RHS handle shared by rhs.hip and stepper.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "oq_internal.h"
+
+namespace oq {
+
+// Packed ArrayPartition offsets (equation.jl:159-165, :193-195).
+struct StateLayout {
+    int64_t nf = 0, ne = 0;
+    int64_t v = 0, th = 0, eps = 0, sig = 0, del = 0, n = 0;
+    void set(int64_t nf_, int64_t ne_)
+    {
+        nf = nf_;
+        ne = ne_;
+        v = 0;
+        th = nf;
+        if (ne > 0) {
+            eps = 2 * nf;
+            sig = 2 * nf + 6 * ne;
+            del = 2 * nf + 12 * ne;
+            n = 3 * nf + 12 * ne;
+        } else {
+            eps = sig = 0;
+            del = 2 * nf;
+            n = 3 * nf;
+        }
+    }
+};
+
+// Arguments of the fused RHS kernels (all device pointers).
+struct RhsDev {
+    const double *a, *b, *L, *sigma;  // nf each
+    double eta, vpl, f0, v0;
+    const double *gamma, *nexp;       // nlaws*ne
+    int nlaws;
+    double deps0[6];
+    StateLayout lay;
+    int64_t N, ld;
+};
+
+}  // namespace oq
+
+struct oq_rhs {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t nx = 0, nxi = 0, ne = 0, nf = 0, N = 0, ld = 0;
+    int kdtype = OQ_K_FP64;
+    oq::StateLayout lay;
+    oq::DevBuf K;                       // N x ld, kdtype
+    oq::DevBuf a, b, L, sigma, gamma, nexp;
+    oq::DevBuf s64;                     // source vector, ld doubles (zero padded)
+    oq::DevBuf u, du;                   // packed state scratch for the host API
+    oq::RhsDev dev{};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace oq {
+// Enqueues one RHS evaluation du = f(u) on h->stream (device pointers).
+int rhs_enqueue(oq_rhs *h, const double *u, double *du);
+// Enqueues only the K-streaming kernel (requires a prior prologue).
+int rhs_enqueue_matvec(oq_rhs *h, const double *u, double *du);
+}  // namespace oq

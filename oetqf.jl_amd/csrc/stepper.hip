@@ -1,0 +1,387 @@
+// stepper.hip -- on-device adaptive Tsit5 for the earthquake-cycle ODE.
+//
+// Replaces the OrdinaryDiffEq time loop the reference runs over `ode`
+// (solve(prob, Tsit5(); ...), test/tests.jl:11, io.jl:125-130): Tsitouras'
+// 5(4) pair with FSAL, the RMS norm of the embedded error scaled by
+// abstol + reltol*max(|u_old|,|u_new|), and OrdinaryDiffEq's PI controller
+// (beta1 = 7/50, beta2 = 2/25, qmin = 1/5, qmax = 10, gamma = 9/10,
+// qoldinit = 1e-4).  All state -- u, the seven stages, the error sums, t,
+// dt and the controller memory -- lives in device memory, so one step is a
+// fixed launch sequence that is captured once into a HIP graph and replayed;
+// the host only polls a small control block every few steps.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <vector>
+
+#include "rhs_impl.h"
+
+namespace oq {
+
+// Tsitouras (2011) tableau, b = a[6][:] (FSAL), btilde = b - bhat.
+__constant__ double kTsC[7] = {0.0, 0.161, 0.327, 0.9, 0.9800255409045097, 1.0, 1.0};
+__constant__ double kTsA[7][6] = {
+    {0, 0, 0, 0, 0, 0},
+    {0.161, 0, 0, 0, 0, 0},
+    {-0.008480655492356989, 0.335480655492357, 0, 0, 0, 0},
+    {2.897153057105493, -6.359448489975075, 4.3622954328695815, 0, 0, 0},
+    {5.325864828439257, -11.748883564062828, 7.4955393428898365, -0.09249506636175525, 0, 0},
+    {5.86145544294642, -12.92096931784711, 8.159367898576159, -0.071584973281401,
+     -0.028269050394068383, 0},
+    {0.09646076681806523, 0.01, 0.4798896504144996, 1.379008574103742, -3.290069515436081,
+     2.324710524099774}};
+__constant__ double kTsBt[7] = {-0.00178001105222577714, -0.0008164344596567469,
+                                0.007880878010261995, -0.1447110071732629,
+                                0.5823571654525552, -0.45808210592918697,
+                                0.015151515151515152};
+
+struct Ctrl {
+    double t, t1, dt, dtcur, qold, eest;
+    double reltol, abstol, dtmax, dtmin;
+    long long iter, maxiters, naccept, nreject;
+    int done, retcode, accept, pad;
+};
+
+static constexpr int kErrBlocks = 256;
+
+// Start of a step: termination checks, dt clamp to dtmax and t1.
+__global__ void k_step_begin(Ctrl *c)
+{
+    c->accept = 0;
+    if (c->done) return;
+    if (!(c->t < c->t1)) {
+        c->done = 1;
+        c->retcode = OQ_RET_SUCCESS;
+        return;
+    }
+    if (c->iter >= c->maxiters) {
+        c->done = 1;
+        c->retcode = OQ_RET_MAXITERS;
+        return;
+    }
+    double dt = c->dt;
+    if (dt > c->dtmax) dt = c->dtmax;
+    if (c->t + dt > c->t1) dt = c->t1 - c->t;
+    if (!(dt >= c->dtmin) || !isfinite(dt)) {
+        c->done = 1;
+        c->retcode = isfinite(dt) ? OQ_RET_DTMIN : OQ_RET_NONFINITE;
+        return;
+    }
+    c->dtcur = dt;
+    c->iter += 1;
+}
+
+// y = u + dt * sum_{j<s} a[s][j] k_j   (stage s = 1..6)
+__global__ void __launch_bounds__(256) k_stage(const Ctrl *__restrict__ c, int s, int64_t n,
+                                               const double *__restrict__ u,
+                                               const double *__restrict__ k,
+                                               double *__restrict__ y)
+{
+    if (c->done) return;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc = 0.0;
+    for (int j = 0; j < s; ++j) acc += kTsA[s][j] * k[j * n + i];
+    y[i] = u[i] + c->dtcur * acc;
+}
+
+// Per-block sums of the squared scaled error (deterministic order).
+__global__ void __launch_bounds__(256) k_err(const Ctrl *__restrict__ c, int64_t n,
+                                             const double *__restrict__ u,
+                                             const double *__restrict__ y,
+                                             const double *__restrict__ k,
+                                             double *__restrict__ part)
+{
+    __shared__ double red[256];
+    if (c->done) return;
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        double e = 0.0;
+#pragma unroll
+        for (int j = 0; j < 7; ++j) e += kTsBt[j] * k[j * n + i];
+        const double w = c->dtcur * e / (c->abstol + fmax(fabs(u[i]), fabs(y[i])) * c->reltol);
+        acc += w * w;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// PI step-size control (OrdinaryDiffEq's PIController semantics).
+__global__ void __launch_bounds__(256) k_controller(Ctrl *c, int64_t n, const double *part)
+{
+    __shared__ double red[256];
+    if (c->done) return;
+    red[threadIdx.x] = part[threadIdx.x];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const double beta1 = 7.0 / 50.0, beta2 = 2.0 / 25.0, qmin = 0.2, qmax = 10.0, gamma = 0.9,
+                 qoldinit = 1e-4;
+    const double eest = sqrt(red[0] / (double)n);
+    double q11, q;
+    c->eest = eest;
+    if (eest == 0.0) {
+        q11 = 0.0;
+        q = 1.0 / qmax;
+    } else {
+        q11 = pow(eest, beta1);
+        q = q11 / pow(c->qold, beta2);
+        q = fmax(1.0 / qmax, fmin(1.0 / qmin, q / gamma));
+    }
+    if (eest <= 1.0) {
+        c->t = (c->t1 - (c->t + c->dtcur) <= 0.0) ? c->t1 : c->t + c->dtcur;
+        c->qold = fmax(eest, qoldinit);
+        c->dt = c->dtcur / q;
+        c->naccept += 1;
+        c->accept = 1;
+    } else {
+        c->nreject += 1;
+        if (!isfinite(eest)) {
+            c->done = 1;
+            c->retcode = OQ_RET_NONFINITE;
+            return;
+        }
+        c->dt = c->dtcur / fmin(1.0 / qmin, q11 / gamma);
+    }
+}
+
+// Accepted step: u <- y, k1 <- k7 (FSAL).
+__global__ void __launch_bounds__(256) k_commit(const Ctrl *__restrict__ c, int64_t n,
+                                                double *__restrict__ u,
+                                                const double *__restrict__ y,
+                                                double *__restrict__ k)
+{
+    if (!c->accept) return;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    u[i] = y[i];
+    k[i] = k[6 * n + i];
+}
+
+}  // namespace oq
+
+struct oq_stepper {
+    oq_rhs *h = nullptr;
+    int64_t n = 0;
+    oq::DevBuf u, y, k, part, ctrl;
+    oq::Ctrl *ctrl_host = nullptr;  // pinned
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace oq {
+
+static inline unsigned nb(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+// The launch sequence of one attempted step (captured into a graph).
+static int enqueue_step(oq_stepper *st)
+{
+    oq_rhs *h = st->h;
+    const int64_t n = st->n;
+    Ctrl *c = st->ctrl.as<Ctrl>();
+    double *u = st->u.as<double>(), *y = st->y.as<double>(), *k = st->k.as<double>();
+    int rc;
+    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, h->stream, c);
+    for (int s = 1; s < 7; ++s) {
+        hipLaunchKernelGGL(k_stage, dim3(nb(n)), dim3(256), 0, h->stream, c, s, n, u, k, y);
+        // the RHS kernels have no done-check of their own: evaluating one
+        // more RHS on a finished state is harmless and keeps them graph-safe
+        if ((rc = rhs_enqueue(h, y, k + s * n))) return rc;
+    }
+    hipLaunchKernelGGL(k_err, dim3(kErrBlocks), dim3(256), 0, h->stream, c, n, u, y, k,
+                       st->part.as<double>());
+    hipLaunchKernelGGL(k_controller, dim3(1), dim3(256), 0, h->stream, c, n, st->part.as<double>());
+    hipLaunchKernelGGL(k_commit, dim3(nb(n)), dim3(256), 0, h->stream, c, n, u, y, k);
+    OQ_HIP(hipGetLastError());
+    return OQ_OK;
+}
+
+static double rms_scaled(const std::vector<double> &x, const std::vector<double> &u, double at,
+                         double rt)
+{
+    double acc = 0.0;
+    for (size_t i = 0; i < x.size(); ++i) {
+        const double w = x[i] / (at + fabs(u[i]) * rt);
+        acc += w * w;
+    }
+    return sqrt(acc / (double)x.size());
+}
+
+static void destroy_stepper(oq_stepper *s)
+{
+    if (!s) return;
+    if (s->h && s->h->stream) (void)hipStreamSynchronize(s->h->stream);
+    if (s->exec) (void)hipGraphExecDestroy(s->exec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    if (s->ctrl_host) (void)hipHostFree(s->ctrl_host);
+    delete s;
+}
+
+}  // namespace oq
+
+using namespace oq;
+
+extern "C" {
+
+int oq_stepper_create(oq_rhs *h, const double *u0, double t0, double t1,
+                      const oq_solver_opts *o, oq_stepper **out)
+{
+    OQ_REQUIRE(h && u0 && o && out, "NULL argument");
+    OQ_REQUIRE(t1 > t0, "tspan must satisfy t1 > t0");
+    OQ_REQUIRE(o->reltol > 0.0 && o->abstol >= 0.0, "tolerances must be positive");
+    *out = nullptr;
+    OQ_HIP(hipSetDevice(h->device));
+    oq_stepper *st = new oq_stepper();
+    st->h = h;
+    st->n = h->lay.n;
+    const int64_t n = st->n;
+    int rc = st->u.alloc((size_t)n * 8);
+    if (!rc) rc = st->y.alloc((size_t)n * 8);
+    if (!rc) rc = st->k.alloc((size_t)(7 * n) * 8);
+    if (!rc) rc = st->part.alloc(kErrBlocks * 8);
+    if (!rc) rc = st->ctrl.alloc(sizeof(Ctrl));
+    if (!rc && hipHostMalloc((void **)&st->ctrl_host, sizeof(Ctrl), hipHostMallocDefault) != hipSuccess) {
+        set_error("hipHostMalloc failed");
+        rc = OQ_ERR_NOMEM;
+    }
+    if (rc) {
+        destroy_stepper(st);
+        return rc;
+    }
+    // f0 and the initial step (Hairer-Norsett-Wanner II.4, order 5), as
+    // OrdinaryDiffEq's ode_determine_initdt, evaluated with device RHS calls.
+    const double dtmax = o->dtmax > 0.0 ? o->dtmax : INFINITY;
+    std::vector<double> hu(u0, u0 + n), f0(n), u1(n), f1(n);
+    double *du = st->k.as<double>();
+    if (hipMemcpyAsync(st->u.p, u0, (size_t)n * 8, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        rhs_enqueue(h, st->u.as<double>(), du) != OQ_OK ||
+        hipMemcpyAsync(f0.data(), du, (size_t)n * 8, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess) {
+        set_error("initial RHS evaluation failed");
+        destroy_stepper(st);
+        return OQ_ERR_HIP;
+    }
+    double dt;
+    if (o->dt0 > 0.0) {
+        dt = o->dt0;
+    } else {
+        const double d0 = rms_scaled(hu, hu, o->abstol, o->reltol);
+        const double d1 = rms_scaled(f0, hu, o->abstol, o->reltol);
+        double dt0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * (d0 / d1);
+        dt0 = fmin(dt0, dtmax);
+        for (int64_t i = 0; i < n; ++i) u1[i] = hu[i] + dt0 * f0[i];
+        double *yd = st->y.as<double>(), *fd = du + n;
+        if (hipMemcpyAsync(yd, u1.data(), (size_t)n * 8, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+            rhs_enqueue(h, yd, fd) != OQ_OK ||
+            hipMemcpyAsync(f1.data(), fd, (size_t)n * 8, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+            hipStreamSynchronize(h->stream) != hipSuccess) {
+            set_error("initial step-size RHS evaluation failed");
+            destroy_stepper(st);
+            return OQ_ERR_HIP;
+        }
+        double acc = 0.0;
+        for (int64_t i = 0; i < n; ++i) {
+            const double w = (f1[i] - f0[i]) / (o->abstol + fabs(hu[i]) * o->reltol);
+            acc += w * w;
+        }
+        const double d2 = sqrt(acc / (double)n) / dt0;
+        const double m = fmax(d1, d2);
+        const double dt1 = (m <= 1e-15) ? fmax(1e-6, dt0 * 1e-3) : pow(0.01 / m, 1.0 / 5.0);
+        dt = fmin(fmin(100.0 * dt0, dt1), dtmax);
+    }
+    Ctrl c;
+    memset(&c, 0, sizeof(c));
+    c.t = t0;
+    c.t1 = t1;
+    c.dt = dt;
+    c.qold = 1e-4;
+    c.reltol = o->reltol;
+    c.abstol = o->abstol;
+    c.dtmax = dtmax;
+    c.dtmin = o->dtmin;
+    c.maxiters = o->maxiters > 0 ? o->maxiters : (long long)1e18;
+    *st->ctrl_host = c;
+    if (hipMemcpyAsync(st->ctrl.p, st->ctrl_host, sizeof(Ctrl), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess) {
+        set_error("control block upload failed");
+        destroy_stepper(st);
+        return OQ_ERR_HIP;
+    }
+    // Capture one attempted step into a graph.
+    if (hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        set_error("hipStreamBeginCapture failed");
+        destroy_stepper(st);
+        return OQ_ERR_HIP;
+    }
+    rc = enqueue_step(st);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(h->stream, &g);
+    if (rc || ce != hipSuccess || hipGraphInstantiate(&st->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (!rc) set_error("graph capture/instantiate failed");
+        destroy_stepper(st);
+        return rc ? rc : OQ_ERR_HIP;
+    }
+    st->graph = g;
+    *out = st;
+    return OQ_OK;
+}
+
+int oq_stepper_run(oq_stepper *st, int64_t max_steps, int *retcode)
+{
+    OQ_REQUIRE(st != nullptr && max_steps > 0, "invalid argument");
+    oq_rhs *h = st->h;
+    OQ_HIP(hipSetDevice(h->device));
+    const int64_t chunk = 16;
+    int64_t done_steps = 0;
+    while (done_steps < max_steps) {
+        const int64_t m = (max_steps - done_steps) < chunk ? (max_steps - done_steps) : chunk;
+        for (int64_t i = 0; i < m; ++i) OQ_HIP(hipGraphLaunch(st->exec, h->stream));
+        done_steps += m;
+        OQ_HIP(hipMemcpyAsync(st->ctrl_host, st->ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost,
+                              h->stream));
+        OQ_HIP(hipStreamSynchronize(h->stream));
+        if (st->ctrl_host->done) break;
+    }
+    if (retcode) *retcode = st->ctrl_host->done ? st->ctrl_host->retcode : OQ_RET_RUNNING;
+    return OQ_OK;
+}
+
+int oq_stepper_state(oq_stepper *st, double *t, double *dt, double *u, int64_t *naccept,
+                     int64_t *nreject)
+{
+    OQ_REQUIRE(st != nullptr, "stepper is NULL");
+    oq_rhs *h = st->h;
+    OQ_HIP(hipSetDevice(h->device));
+    OQ_HIP(hipMemcpyAsync(st->ctrl_host, st->ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost,
+                          h->stream));
+    if (u)
+        OQ_HIP(hipMemcpyAsync(u, st->u.p, (size_t)st->n * 8, hipMemcpyDeviceToHost, h->stream));
+    OQ_HIP(hipStreamSynchronize(h->stream));
+    if (t) *t = st->ctrl_host->t;
+    if (dt) *dt = st->ctrl_host->dt;
+    if (naccept) *naccept = st->ctrl_host->naccept;
+    if (nreject) *nreject = st->ctrl_host->nreject;
+    return OQ_OK;
+}
+
+int oq_stepper_destroy(oq_stepper *st)
+{
+    if (st) {
+        (void)hipSetDevice(st->h->device);
+        destroy_stepper(st);
+    }
+    return OQ_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,123 @@
+"""ctypes binding of lib/liboetqf_amd.so (C ABI in include/oetqf_amd.h).
+
+The shared library is the product: every Green's function, RHS evaluation
+and time step below runs in its HIP kernels.  There is no CPU fallback --
+if the library is missing or no GPU is visible, calls raise.
+"""
+import ctypes
+import os
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIBPATH = os.path.join(_PKG, "lib", "liboetqf_amd.so")
+
+_d = ctypes.c_double
+_i64 = ctypes.c_int64
+_int = ctypes.c_int
+_pd = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+
+OQ_OK = 0
+OQ_ERR_ARG = 1
+OQ_K_FP64, OQ_K_FP32, OQ_K_BF16 = 0, 1, 2
+RETCODES = {0: "Running", 1: "Success", 2: "MaxIters", 3: "DtLessThanMin", 4: "NonFinite"}
+
+
+class FaultMesh(ctypes.Structure):
+    _fields_ = [("x", _pd), ("ax", _pd), ("y", _pd), ("z", _pd), ("axi", _pd),
+                ("nx", _i64), ("nxi", _i64), ("dx", _d), ("dep", _d), ("dip", _d)]
+
+
+class Hex8Mesh(ctypes.Structure):
+    _fields_ = [("cx", _pd), ("cy", _pd), ("cz", _pd), ("qx", _pd), ("qy", _pd),
+                ("qz", _pd), ("dx", _pd), ("dy", _pd), ("dz", _pd), ("ne", _i64)]
+
+
+class Quadrature(ctypes.Structure):
+    _fields_ = [("coords", _pd), ("weights", _pd), ("nq", _i64)]
+
+
+class RSProps(ctypes.Structure):
+    _fields_ = [("a", _pd), ("b", _pd), ("L", _pd), ("sigma", _pd),
+                ("eta", _d), ("vpl", _d), ("f0", _d), ("v0", _d)]
+
+
+class ViscProps(ctypes.Structure):
+    _fields_ = [("gamma", _pd), ("n", _pd), ("nlaws", _i64), ("deps0", _d * 6)]
+
+
+class SolverOpts(ctypes.Structure):
+    _fields_ = [("reltol", _d), ("abstol", _d), ("dt0", _d), ("dtmax", _d), ("dtmin", _d),
+                ("maxiters", _i64)]
+
+
+class OetqfError(RuntimeError):
+    """A non-zero status from the C ABI."""
+
+
+_lib = None
+
+# name -> (restype, argtypes); the exported surface of include/oetqf_amd.h
+PROTOTYPES = {
+    "oq_last_error": (ctypes.c_char_p, []),
+    "oq_version": (_int, []),
+    "oq_device_count": (_int, [ctypes.POINTER(_int)]),
+    "oq_gf_okada_ff": (_int, [ctypes.POINTER(FaultMesh), _d, _d, _int, _i64, _d, _pd]),
+    "oq_gf_okada_ff_dft": (_int, [ctypes.POINTER(FaultMesh), _d, _d, _int, _i64, _d, _pd]),
+    "oq_gf_okada_mf": (_int, [ctypes.POINTER(FaultMesh), ctypes.POINTER(Hex8Mesh), _d, _d, _int,
+                              ctypes.POINTER(Quadrature), _i64, _d, _pd]),
+    "oq_gf_hex8_fm": (_int, [ctypes.POINTER(Hex8Mesh), ctypes.POINTER(FaultMesh), _d, _d, _int,
+                             _pd]),
+    "oq_gf_hex8_mm": (_int, [ctypes.POINTER(Hex8Mesh), _d, _d, ctypes.POINTER(Quadrature), _pd]),
+    "oq_rhs_create": (_int, [_int, _i64, _i64, _i64, ctypes.POINTER(RSProps),
+                             ctypes.POINTER(ViscProps), _pd, _int, _pd, _pd, _pd, _int,
+                             ctypes.POINTER(_vp)]),
+    "oq_rhs_create_from_mesh": (_int, [_int, ctypes.POINTER(FaultMesh), ctypes.POINTER(Hex8Mesh),
+                                       _d, _d, _int, _i64, _d, ctypes.POINTER(Quadrature),
+                                       ctypes.POINTER(RSProps), ctypes.POINTER(ViscProps), _int,
+                                       ctypes.POINTER(_vp)]),
+    "oq_rhs_eval": (_int, [_vp, _d] + [_pd] * 10),
+    "oq_rhs_eval_device": (_int, [_vp, _d, _vp, _vp]),
+    "oq_rhs_info": (_int, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
+    "oq_rhs_get_k": (_int, [_vp, _pd]),
+    "oq_rhs_time": (_int, [_vp, _int, ctypes.POINTER(_d), ctypes.POINTER(_d)]),
+    "oq_rhs_destroy": (_int, [_vp]),
+    "oq_stepper_create": (_int, [_vp, _pd, _d, _d, ctypes.POINTER(SolverOpts), ctypes.POINTER(_vp)]),
+    "oq_stepper_run": (_int, [_vp, _i64, ctypes.POINTER(_int)]),
+    "oq_stepper_state": (_int, [_vp, ctypes.POINTER(_d), ctypes.POINTER(_d), _pd,
+                                ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
+    "oq_stepper_destroy": (_int, [_vp]),
+}
+
+
+def load():
+    """Loads the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise OetqfError("liboetqf_amd.so is not built (run __graft_entry__.build() or "
+                             "make -C oetqf.jl_amd); there is no CPU fallback")
+        lib = ctypes.CDLL(LIBPATH)
+        for name, (res, args) in PROTOTYPES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(rc):
+    if rc != OQ_OK:
+        msg = load().oq_last_error().decode("utf-8", "replace")
+        if rc == OQ_ERR_ARG:
+            raise ValueError(msg)
+        raise OetqfError("oetqf_amd error %d: %s" % (rc, msg))
+
+
+def ptr(a):
+    return a.ctypes.data_as(_pd)
+
+
+def device_count():
+    n = _int(0)
+    check(load().oq_device_count(ctypes.byref(n)))
+    return n.value

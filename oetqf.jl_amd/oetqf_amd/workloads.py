@@ -1,0 +1,78 @@
+"""Synthetic earthquake-cycle workloads (meshes, properties, initial state).
+
+Each builder returns a dict with the fault mesh `mf`, optional hex8 mantle
+`ma`, elastic constants, friction/viscosity properties and a seeded initial
+ArrayPartition.  Randomness uses numpy's PCG64 with the given seed; the
+arrays themselves are what tests and benchmarks consume.
+"""
+import numpy as np
+
+from .equation import ArrayPartition
+from .mesh import gen_mesh, gen_mesh_graded, hex8_box
+from .property import PowerLawViscosityProperty, RateStateQuasiDynamicProperty
+
+YEAR = 365 * 86400.0
+
+
+def _b_profile(depth_m, top=2e3, bottom=12e3, taper=2e3, b_vw=0.019, b_vs=0.011):
+    """b = b_vw between `top` and `bottom`, tapering linearly to b_vs over
+    `taper` above `top` and below `bottom`, b_vs elsewhere."""
+    d = np.asarray(depth_m, dtype=np.float64)
+    b = np.full(d.shape, b_vs)
+    up = d < top
+    b[up] = b_vs + (b_vw - b_vs) * d[up] / top
+    mid = (d >= top) & (d <= bottom)
+    b[mid] = b_vw
+    lo = (d > bottom) & (d < bottom + taper)
+    b[lo] = b_vw - (b_vw - b_vs) * (d[lo] - bottom) / taper
+    return b
+
+
+def strike_slip_512(seed=20261202):
+    """Vertical strike-slip fault, 40 km x 20 km, 32 x 16 = 512 Okada patches
+    in an elastic half-space; mu = 30 GPa, nu = 0.25, cs = 3.04 km/s,
+    eta = mu/(2 cs), vpl = 1e-9 m/s, sigma = 50 MPa, L = 12 mm, a = 0.015,
+    b = 0.019 at 2-12 km tapering to 0.011; V0 = vpl exp(0.1 N(0,1)),
+    theta0 = L/V0.  Tsit5 reltol 1e-6 (abstol 1e-8), 300 years."""
+    rng = np.random.default_rng(seed)
+    mf = gen_mesh("RectOkada", 40e3, 20e3, 1250.0, 1250.0, 90.0)
+    mu = 30e9
+    lam = mu  # nu = 0.25
+    shp = (mf.nx, mf.nxi)
+    depth = np.broadcast_to(-mf.z, shp)
+    a = np.full(shp, 0.015)
+    b = _b_profile(depth)
+    pf = RateStateQuasiDynamicProperty(a, b, np.full(shp, 0.012), np.full(shp, 50e6),
+                                       mu / (2 * 3040.0), 1e-9, 0.6, 1e-6)
+    v0 = pf.vpl * np.exp(0.1 * rng.standard_normal(shp))
+    u0 = ArrayPartition(v0, pf.L / v0, np.zeros(shp))
+    return dict(name="strike-slip fault, 512 patches (32x16), elastic", mf=mf, ma=None, lam=lam,
+                mu=mu, pf=pf, pa=None, u0=u0, tspan=(0.0, 300 * YEAR), nrept=2, buffer_ratio=0.0,
+                reltol=1e-6, abstol=1e-8)
+
+
+def transform_fault_composite(nx=128, nxi=32, mx=16, my=16, mz=8, seed=20261202,
+                              grading=1.05, length=80e3, depth=20e3):
+    """Transform fault `length` x `depth` on nx x nxi patches (down-dip cells
+    graded finer toward the surface by `grading`) coupled to a viscoelastic
+    mantle block below 10 km (80 x 40 x 30 km, mx x my x mz hex8 cells, six
+    strain components each), linear Maxwell viscosity 1e19 Pa s.  Friction
+    as strike_slip_512 with velocity strengthening below 10 km.  Defaults:
+    4096 patches + 2048 cells (N = 16384)."""
+    rng = np.random.default_rng(seed)
+    mf = gen_mesh_graded(length, depth, length / nx, nxi, 90.0, ratio=grading)
+    ma = hex8_box(-40e3, -20e3, -10e3, 80e3, 40e3, -30e3, mx, my, mz)
+    mu = 30e9
+    lam = mu
+    shp = (mf.nx, mf.nxi)
+    d = np.broadcast_to(-mf.z, shp)
+    pf = RateStateQuasiDynamicProperty(np.full(shp, 0.015), _b_profile(d, bottom=8e3),
+                                       np.full(shp, 0.012), np.full(shp, 50e6),
+                                       mu / (2 * 3040.0), 1e-9, 0.6, 1e-6)
+    pa = PowerLawViscosityProperty(np.full(ma.ne, 1.0 / (2 * 1e19)), np.zeros(ma.ne), np.zeros(6))
+    v0 = pf.vpl * np.exp(0.1 * rng.standard_normal(shp))
+    u0 = ArrayPartition(v0, pf.L / v0, np.zeros((ma.ne, 6)), np.zeros((ma.ne, 6)), np.zeros(shp))
+    return dict(name="transform fault %dx%d=%d patches + hex8 mantle %dx%dx%d=%d cells"
+                % (nx, nxi, nx * nxi, mx, my, mz, ma.ne),
+                mf=mf, ma=ma, lam=lam, mu=mu, pf=pf, pa=pa, u0=u0, tspan=(0.0, 500 * YEAR),
+                nrept=2, buffer_ratio=0.0, reltol=1e-6, abstol=1e-8)
