@@ -18,8 +18,9 @@
 //                          fused epilogue: rate-and-state update for fault
 //                          rows (update_fault!, equation.jl:233-246) or
 //                          du.sigma for mantle rows.
-// K may be stored as FP64, FP32 or BF16; products are always accumulated in
-// FP64 (the FP64 FMA rate is ~40x what an 8 TB/s stream needs).
+// K may be stored as FP64, FP32 or BF16; row sums are accumulated in FP64
+// (the FP64 FMA rate is far above what an 8 TB/s stream needs); BF16 forms
+// its products in FP32 per 16-byte chunk.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
@@ -32,19 +33,20 @@ namespace oq {
 
 void toeplitz_idft(const double *dft, int64_t nx, int64_t nxi, double *st);
 
-static constexpr int kWaves = 4;      // wavefronts per workgroup
-static constexpr int kRows = 4;       // rows per wavefront
 static constexpr int64_t kPad = 512;  // row pitch multiple (elements)
 
 // ---------------- prologue ----------------
 __global__ void __launch_bounds__(256) k_rhs_prologue(RhsDev d, const double *__restrict__ u,
                                                       double *__restrict__ du,
-                                                      double *__restrict__ s)
+                                                      double *__restrict__ s,
+                                                      float *__restrict__ s32)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t nf = d.lay.nf, ne = d.lay.ne;
     if (t < nf) {
-        s[t] = u[d.lay.v + t] - d.vpl;  // relative_velocity! (equation.jl:35-42)
+        const double x = u[d.lay.v + t] - d.vpl;  // relative_velocity! (equation.jl:35-42)
+        s[t] = x;
+        if (s32) s32[t] = (float)x;
         return;
     }
     const int64_t i = t - nf;
@@ -63,77 +65,103 @@ __global__ void __launch_bounds__(256) k_rhs_prologue(RhsDev d, const double *__
     double *de = du + d.lay.eps;
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
+        const double x = e[k] - d.deps0[k];
         de[i + k * ne] = e[k];
-        s[nf + i + k * ne] = e[k] - d.deps0[k];
+        s[nf + i + k * ne] = x;
+        if (s32) s32[nf + i + k * ne] = (float)x;
     }
 }
 
 // ---------------- K streaming + fused epilogue ----------------
+// Per storage type: elements per 16-byte load, rows per wavefront and
+// wavefronts per workgroup (the shapes that streamed fastest on MI355X at
+// N = 16384: FP64 6.15 TB/s, FP32 6.87 TB/s, BF16 6.46 TB/s).
 template <int KT> struct KTraits;
 template <> struct KTraits<OQ_K_FP64> {
     using T = double;
-    static constexpr int kPerLane = 2;  // 16 B per lane per load
+    static constexpr int kPerLane = 2;
+    static constexpr int kRows = 8;
+    static constexpr int kWaves = 4;
 };
 template <> struct KTraits<OQ_K_FP32> {
     using T = float;
     static constexpr int kPerLane = 4;
+    static constexpr int kRows = 8;
+    static constexpr int kWaves = 2;
 };
 template <> struct KTraits<OQ_K_BF16> {
     using T = unsigned short;
     static constexpr int kPerLane = 8;
+    static constexpr int kRows = 8;
+    static constexpr int kWaves = 4;
 };
 
-__device__ __forceinline__ double bf16_lo(unsigned int w) { return (double)__uint_as_float(w << 16); }
-__device__ __forceinline__ double bf16_hi(unsigned int w) { return (double)__uint_as_float(w & 0xffff0000u); }
+// 16-byte vector types of one lane's load, per storage type.
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+template <int KT> struct KVec;
+template <> struct KVec<OQ_K_FP64> { using V = f64x2; };
+template <> struct KVec<OQ_K_FP32> { using V = f32x4; };
+template <> struct KVec<OQ_K_BF16> { using V = u32x4; };
 
-// acc += dot(16 bytes of K row at element c, s[c..])
-template <int KT>
-__device__ __forceinline__ double dot16(const uint4 kv, const double *__restrict__ s, int64_t c);
+// Streaming (non-temporal) 128-bit load: K is read once per RHS, so it
+// should not evict the source vector from the caches.
+template <class V> __device__ __forceinline__ V ld_nt(const V *p) { return __builtin_nontemporal_load(p); }
 
+// The slice of the source vector matching one lane's 16 bytes of K, and the
+// dot product of the two.
+template <int KT> struct SVec;
+template <> struct SVec<OQ_K_FP64> { double2 a; };
+template <> struct SVec<OQ_K_FP32> { double4 a; };
+template <> struct SVec<OQ_K_BF16> { float4 a, b; };
+
+template <int KT> __device__ __forceinline__ SVec<KT> s_load(const double *__restrict__ s, int64_t c);
 template <>
-__device__ __forceinline__ double dot16<OQ_K_FP64>(const uint4 kv, const double *__restrict__ s,
-                                                   int64_t c)
+__device__ __forceinline__ SVec<OQ_K_FP64> s_load<OQ_K_FP64>(const double *__restrict__ s, int64_t c)
 {
-    const double2 sv = *reinterpret_cast<const double2 *>(s + c);
-    const double k0 = __hiloint2double(kv.y, kv.x), k1 = __hiloint2double(kv.w, kv.z);
-    return fma(k0, sv.x, k1 * sv.y);
+    return {*reinterpret_cast<const double2 *>(s + c)};
+}
+template <>
+__device__ __forceinline__ SVec<OQ_K_FP32> s_load<OQ_K_FP32>(const double *__restrict__ s, int64_t c)
+{
+    return {*reinterpret_cast<const double4 *>(s + c)};
+}
+template <>
+__device__ __forceinline__ SVec<OQ_K_BF16> s_load<OQ_K_BF16>(const double *__restrict__ sd, int64_t c)
+{
+    const float *s = reinterpret_cast<const float *>(sd);
+    return {*reinterpret_cast<const float4 *>(s + c), *reinterpret_cast<const float4 *>(s + c + 4)};
 }
 
-template <>
-__device__ __forceinline__ double dot16<OQ_K_FP32>(const uint4 kv, const double *__restrict__ s,
-                                                   int64_t c)
+__device__ __forceinline__ double dot16(const f64x2 kv, const SVec<OQ_K_FP64> &sv)
 {
-    const double4 sv = *reinterpret_cast<const double4 *>(s + c);
-    double r = (double)__uint_as_float(kv.x) * sv.x;
-    r = fma((double)__uint_as_float(kv.y), sv.y, r);
-    r = fma((double)__uint_as_float(kv.z), sv.z, r);
-    return fma((double)__uint_as_float(kv.w), sv.w, r);
+    return fma(kv.x, sv.a.x, kv.y * sv.a.y);
 }
 
-template <>
-__device__ __forceinline__ double dot16<OQ_K_BF16>(const uint4 kv, const double *__restrict__ s,
-                                                   int64_t c)
+__device__ __forceinline__ double dot16(const f32x4 kv, const SVec<OQ_K_FP32> &sv)
 {
-    const double4 s0 = *reinterpret_cast<const double4 *>(s + c);
-    const double4 s1 = *reinterpret_cast<const double4 *>(s + c + 4);
-    double r = bf16_lo(kv.x) * s0.x;
-    r = fma(bf16_hi(kv.x), s0.y, r);
-    r = fma(bf16_lo(kv.y), s0.z, r);
-    r = fma(bf16_hi(kv.y), s0.w, r);
-    r = fma(bf16_lo(kv.z), s1.x, r);
-    r = fma(bf16_hi(kv.z), s1.y, r);
-    r = fma(bf16_lo(kv.w), s1.z, r);
-    return fma(bf16_hi(kv.w), s1.w, r);
+    double r = (double)kv.x * sv.a.x;
+    r = fma((double)kv.y, sv.a.y, r);
+    r = fma((double)kv.z, sv.a.z, r);
+    return fma((double)kv.w, sv.a.w, r);
 }
 
-__device__ __forceinline__ uint4 ld_nt(const uint4 *p)
+// BF16: the eight products of a 16-byte chunk are formed in FP32 against
+// the FP32 copy of s (product rounding 2^-24, far below BF16's 2^-9), then
+// added to the FP64 accumulator.
+__device__ __forceinline__ double dot16(const u32x4 kv, const SVec<OQ_K_BF16> &sv)
 {
-    uint4 v;
-    v.x = __builtin_nontemporal_load(&p->x);
-    v.y = __builtin_nontemporal_load(&p->y);
-    v.z = __builtin_nontemporal_load(&p->z);
-    v.w = __builtin_nontemporal_load(&p->w);
-    return v;
+    const float4 a = sv.a, b = sv.b;
+    float r = __uint_as_float(kv.x << 16) * a.x;
+    r = fmaf(__uint_as_float(kv.x & 0xffff0000u), a.y, r);
+    r = fmaf(__uint_as_float(kv.y << 16), a.z, r);
+    r = fmaf(__uint_as_float(kv.y & 0xffff0000u), a.w, r);
+    float q = __uint_as_float(kv.z << 16) * b.x;
+    q = fmaf(__uint_as_float(kv.z & 0xffff0000u), b.y, q);
+    q = fmaf(__uint_as_float(kv.w << 16), b.z, q);
+    q = fmaf(__uint_as_float(kv.w & 0xffff0000u), b.w, q);
+    return (double)r + (double)q;
 }
 
 __device__ __forceinline__ double wave_sum(double x)
@@ -161,57 +189,70 @@ __device__ __forceinline__ void fault_update(const RhsDev &d, int64_t i, double 
     du[d.lay.del + i] = v;
 }
 
+// One wavefront streams KTraits::kRows consecutive rows.  Each wavefront
+// starts its sweep over the columns at a different pass (rotation by a
+// multiple of its index): with all wavefronts reading the same column
+// offset of rows whose pitch is a large power of two, the loads would
+// concentrate on a few HBM channels (measured 4.9 vs 6.1 TB/s, FP64).
 template <int KT>
-__global__ void __launch_bounds__(kWaves * 64) k_rhs_matvec(RhsDev d, const void *__restrict__ Kp,
-                                                            const double *__restrict__ s,
-                                                            const double *__restrict__ u,
-                                                            double *__restrict__ du)
+__global__ void __launch_bounds__(KTraits<KT>::kWaves * 64)
+    k_rhs_matvec(RhsDev d, const void *__restrict__ Kp, const double *__restrict__ s,
+                 const double *__restrict__ u, double *__restrict__ du)
 {
     using T = typename KTraits<KT>::T;
     constexpr int E = KTraits<KT>::kPerLane;  // elements per 16-byte load
+    constexpr int R = KTraits<KT>::kRows;
+    constexpr int W = KTraits<KT>::kWaves;
     constexpr int64_t STEP = 64 * E;          // columns per wavefront pass
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int64_t row0 = ((int64_t)blockIdx.x * kWaves + wave) * kRows;
+    const int64_t gw = (int64_t)blockIdx.x * W + (threadIdx.x >> 6);
+    const int64_t row0 = gw * R;
     if (row0 >= d.N) return;
+    using V = typename KVec<KT>::V;
     const T *K = static_cast<const T *>(Kp);
-    const uint4 *rp[kRows];
+    const V *rp[R];
 #pragma unroll
-    for (int r = 0; r < kRows; ++r) {
+    for (int r = 0; r < R; ++r) {
         const int64_t row = row0 + r < d.N ? row0 + r : d.N - 1;
-        rp[r] = reinterpret_cast<const uint4 *>(K + row * d.ld) + lane;
+        rp[r] = reinterpret_cast<const V *>(K + row * d.ld) + lane;
     }
-    double acc[kRows];
+    double acc[R];
 #pragma unroll
-    for (int r = 0; r < kRows; ++r) acc[r] = 0.0;
+    for (int r = 0; r < R; ++r) acc[r] = 0.0;
     const int64_t npass = d.ld / STEP;
-    int64_t p = 0;
-    // two passes per iteration: 2*kRows 16-byte loads in flight per lane
-    for (; p + 2 <= npass; p += 2) {
-        uint4 k0[kRows], k1[kRows];
+    // Software pipeline: the R loads of the next pass are issued before the
+    // FMAs of the current one, so 2R 16-byte loads per lane are in flight;
+    // the scheduling barriers keep the compiler from interleaving each load
+    // with its use (which it does by default to save registers).
+    int64_t p = (gw * 13) % npass;
+    V cur[R];
 #pragma unroll
-        for (int r = 0; r < kRows; ++r) {
-            k0[r] = ld_nt(rp[r] + p * 64);
-            k1[r] = ld_nt(rp[r] + (p + 1) * 64);
-        }
-        const int64_t c0 = p * STEP + lane * E, c1 = c0 + STEP;
+    for (int r = 0; r < R; ++r) cur[r] = ld_nt(rp[r] + p * 64);
+    for (int64_t it = 1; it < npass; ++it) {
+        const int64_t pn = (p + 1 == npass) ? 0 : p + 1;
+        const SVec<KT> sv = s_load<KT>(s, p * STEP + lane * E);
+        V nxt[R];
 #pragma unroll
-        for (int r = 0; r < kRows; ++r) {
-            acc[r] += dot16<KT>(k0[r], s, c0);
-            acc[r] += dot16<KT>(k1[r], s, c1);
-        }
+        for (int r = 0; r < R; ++r) nxt[r] = ld_nt(rp[r] + pn * 64);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] += dot16(cur[r], sv);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int r = 0; r < R; ++r) cur[r] = nxt[r];
+        p = pn;
     }
-    for (; p < npass; ++p) {
-        const int64_t c0 = p * STEP + lane * E;
+    {
+        const SVec<KT> sv = s_load<KT>(s, p * STEP + lane * E);
 #pragma unroll
-        for (int r = 0; r < kRows; ++r) acc[r] += dot16<KT>(ld_nt(rp[r] + p * 64), s, c0);
+        for (int r = 0; r < R; ++r) acc[r] += dot16(cur[r], sv);
     }
 #pragma unroll
-    for (int r = 0; r < kRows; ++r) acc[r] = wave_sum(acc[r]);
-    if (lane < kRows) {
+    for (int r = 0; r < R; ++r) acc[r] = wave_sum(acc[r]);
+    if (lane < R) {
         double y = acc[0];
 #pragma unroll
-        for (int r = 1; r < kRows; ++r)
+        for (int r = 1; r < R; ++r)
             if (lane == r) y = acc[r];
         const int64_t row = row0 + lane;
         if (row < d.N) {
@@ -223,22 +264,19 @@ __global__ void __launch_bounds__(kWaves * 64) k_rhs_matvec(RhsDev d, const void
 
 static inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+template <int KT> static void launch_matvec(oq_rhs *h, const double *s, const double *u, double *du)
+{
+    constexpr int rows = KTraits<KT>::kRows * KTraits<KT>::kWaves;
+    hipLaunchKernelGGL(k_rhs_matvec<KT>, dim3(nblk(h->N, rows)), dim3(KTraits<KT>::kWaves * 64), 0,
+                       h->stream, h->dev, h->K.p, s, u, du);
+}
+
 int rhs_enqueue_matvec(oq_rhs *h, const double *u, double *du)
 {
-    const unsigned grid = nblk(h->N, kWaves * kRows);
     switch (h->kdtype) {
-    case OQ_K_FP64:
-        hipLaunchKernelGGL(k_rhs_matvec<OQ_K_FP64>, dim3(grid), dim3(kWaves * 64), 0, h->stream,
-                           h->dev, h->K.p, h->s64.as<double>(), u, du);
-        break;
-    case OQ_K_FP32:
-        hipLaunchKernelGGL(k_rhs_matvec<OQ_K_FP32>, dim3(grid), dim3(kWaves * 64), 0, h->stream,
-                           h->dev, h->K.p, h->s64.as<double>(), u, du);
-        break;
-    default:
-        hipLaunchKernelGGL(k_rhs_matvec<OQ_K_BF16>, dim3(grid), dim3(kWaves * 64), 0, h->stream,
-                           h->dev, h->K.p, h->s64.as<double>(), u, du);
-        break;
+    case OQ_K_FP64: launch_matvec<OQ_K_FP64>(h, h->s64.as<double>(), u, du); break;
+    case OQ_K_FP32: launch_matvec<OQ_K_FP32>(h, h->s64.as<double>(), u, du); break;
+    default: launch_matvec<OQ_K_BF16>(h, reinterpret_cast<const double *>(h->s32.p), u, du); break;
     }
     OQ_HIP(hipGetLastError());
     return OQ_OK;
@@ -248,7 +286,7 @@ int rhs_enqueue(oq_rhs *h, const double *u, double *du)
 {
     const int64_t n = h->nf + h->ne;
     hipLaunchKernelGGL(k_rhs_prologue, dim3(nblk(n, 256)), dim3(256), 0, h->stream, h->dev, u, du,
-                       h->s64.as<double>());
+                       h->s64.as<double>(), h->kdtype == OQ_K_BF16 ? h->s32.as<float>() : nullptr);
     OQ_HIP(hipGetLastError());
     return rhs_enqueue_matvec(h, u, du);
 }
@@ -315,6 +353,8 @@ static int alloc_common(oq_rhs *h, int device, int64_t nx, int64_t nxi, int64_t 
     int rc;
     if ((rc = h->s64.alloc((size_t)h->ld * 8))) return rc;
     OQ_HIP(hipMemsetAsync(h->s64.p, 0, (size_t)h->ld * 8, h->stream));
+    if ((rc = h->s32.alloc((size_t)h->ld * 4))) return rc;
+    OQ_HIP(hipMemsetAsync(h->s32.p, 0, (size_t)h->ld * 4, h->stream));
     if ((rc = h->u.alloc((size_t)h->lay.n * 8))) return rc;
     if ((rc = h->du.alloc((size_t)h->lay.n * 8))) return rc;
     OQ_HIP(hipMemsetAsync(h->u.p, 0, (size_t)h->lay.n * 8, h->stream));

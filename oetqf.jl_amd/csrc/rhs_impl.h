@@ -51,6 +51,7 @@ struct oq_rhs {
     oq::DevBuf K;                       // N x ld, kdtype
     oq::DevBuf a, b, L, sigma, gamma, nexp;
     oq::DevBuf s64;                     // source vector, ld doubles (zero padded)
+    oq::DevBuf s32;                     // FP32 copy of s (BF16 operator)
     oq::DevBuf u, du;                   // packed state scratch for the host API
     oq::RhsDev dev{};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;

@@ -137,6 +137,8 @@ def assemble(*args, se=DieterichStateLaw(), kdtype="fp64", device=0, **kwargs):
     builder's default, fourier=true)."""
     if not isinstance(se, DieterichStateLaw):
         raise NotImplementedError("only the aging law (DieterichStateLaw) is provided")
+    if kdtype not in _KDTYPES:
+        raise ValueError("kdtype must be one of %s" % sorted(_KDTYPES))
     kdt = _KDTYPES[kdtype]
     keep = _Keep()
     L = _lib.load()
@@ -177,6 +179,8 @@ def assemble_from_mesh(mf, ma, lam, mu, pf, pa, u0, tspan, ftype=StrikeSlip(), n
     """The four stress_greens_function calls followed by assemble, with the
     Green's functions built straight into the device operator (no host copy
     of K).  ma = None gives the fault-only problem."""
+    if kdtype not in _KDTYPES:
+        raise ValueError("kdtype must be one of %s" % sorted(_KDTYPES))
     keep = _Keep()
     L = _lib.load()
     h = ctypes.c_void_p()

@@ -214,8 +214,14 @@ def gf_mm(ma, lam, mu, qc=(0.0, 0.0, 0.0), qw=(1.0,), nquad=6):
     return st.reshape(6 * ne, 6 * ne).T
 
 
+def _fortran_flat(a):
+    return np.asarray(a, dtype=np.float64).reshape(-1, order="F")
+
+
 def _rs(p, keep):
-    return RSC(_p(keep.arr(p.a)), _p(keep.arr(p.b)), _p(keep.arr(p.L)), _p(keep.arr(p.sigma)),
+    """Friction parameters in Julia (column-major) order, like the state."""
+    return RSC(_p(keep.arr(_fortran_flat(p.a))), _p(keep.arr(_fortran_flat(p.b))),
+               _p(keep.arr(_fortran_flat(p.L))), _p(keep.arr(_fortran_flat(p.sigma))),
                p.eta, p.vpl, p.f0, p.v0)
 
 

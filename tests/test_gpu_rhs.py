@@ -105,10 +105,13 @@ def test_dense_operator_layout(oq, oracle, rng):
     assert np.array_equal(K[nf:, nf:], g22)
 
 
-@pytest.mark.parametrize("kdtype,tol", [("fp32", 1e-5), ("bf16", 5e-2)])
-def test_rhs_reduced_precision_operator(oq, oracle, rng, kdtype, tol):
-    """K streamed as FP32/BF16 with FP64 accumulation: the traction-rate error
-    is bounded by the storage rounding of K (stated per dtype)."""
+@pytest.mark.parametrize("kdtype,tol,exact_tol", [("fp32", 1e-5, 1e-12), ("bf16", 5e-2, 1e-6)])
+def test_rhs_reduced_precision_operator(oq, oracle, rng, kdtype, tol, exact_tol):
+    """K streamed as FP32/BF16 with FP64 row accumulation: the RHS error
+    against FP64 is bounded by the storage rounding of K (tol, per dtype).
+    Against the FP64 oracle applied to the rounded operator the FP32 path
+    agrees to FP64 accuracy; BF16 forms its products in FP32 against an
+    FP32 copy of the source vector (rounding 2^-24), hence exact_tol."""
     mf, ma, st, g12, g21, g22, pf, pa, u = _coupled_inputs(oracle, rng)
     u0 = oq.ArrayPartition(*u)
     p64 = oq.assemble(st, g12, g21, g22, pf, pa, u0, (0.0, 1.0))
@@ -123,8 +126,8 @@ def test_rhs_reduced_precision_operator(oq, oracle, rng, kdtype, tol):
     v, th, eps, sig, dl = u
     ref = oracle.rhs_dense(pf, pa, K, nf, ma.ne, v.reshape(-1, order="F"), th.reshape(-1, order="F"),
                            sig.reshape(-1, order="F"))
-    assert relerr(dlo.x[0].reshape(-1, order="F"), ref[0]) <= 1e-12
-    assert relerr(dlo.x[3].reshape(-1, order="F"), ref[3]) <= 1e-12
+    assert relerr(dlo.x[0].reshape(-1, order="F"), ref[0]) <= exact_tol
+    assert relerr(dlo.x[3].reshape(-1, order="F"), ref[3]) <= exact_tol
     err = relerr(dlo.x[3], d64.x[3])
     print("%s mantle stress-rate rel err vs FP64: %.3e" % (kdtype, err))
     assert err <= tol
