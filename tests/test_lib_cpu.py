@@ -1,0 +1,130 @@
+"""The C-ABI library without a GPU: it is built, loads, exports every symbol
+include/oetqf_amd.h declares, and rejects bad arguments before touching a
+device (the reference's @assert / ArgumentError behaviour)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    src = open(os.path.join(ROOT, "include", "oetqf_amd.h")).read()
+    return sorted(set(re.findall(r"\b(oq_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol(oq):
+    lib = oq.load_library()
+    syms = declared_symbols()
+    assert len(syms) >= 20
+    for s in syms:
+        assert hasattr(lib, s), s
+    from oetqf_amd import _lib
+    assert set(_lib.PROTOTYPES) == set(syms)
+
+
+def test_version_and_error_string(oq):
+    lib = oq.load_library()
+    assert lib.oq_version() >= 1
+    assert isinstance(lib.oq_last_error(), bytes)
+
+
+def test_buffer_ratio_assert(oq):
+    """@assert buffer_ratio ≥ 0 (GF.jl:36) -> ValueError from the C ABI."""
+    m = oq.gen_mesh("RectOkada", 20e3, 10e3, 2.5e3, 2.5e3, 90.0)
+    from oetqf_amd import _lib
+    from oetqf_amd.greens import _Keep, fault_struct
+    keep = _Keep()
+    fs = fault_struct(m, keep)
+    out = np.zeros(m.nx * m.nxi * m.nxi)
+    rc = oq.load_library().oq_gf_okada_ff(ctypes.byref(fs), 3e10, 3e10, 0, 2, -1.0, _lib.ptr(out))
+    assert rc == _lib.OQ_ERR_ARG
+    assert "buffer_ratio" in oq.load_library().oq_last_error().decode()
+    with pytest.raises(ValueError):
+        _lib.check(rc)
+
+
+def test_bad_ftype_and_null_mesh(oq):
+    from oetqf_amd import _lib
+    lib = oq.load_library()
+    out = np.zeros(8)
+    assert lib.oq_gf_okada_ff(None, 1.0, 1.0, 0, 2, 0.0, _lib.ptr(out)) == _lib.OQ_ERR_ARG
+    m = oq.gen_mesh("RectOkada", 20e3, 10e3, 2.5e3, 2.5e3, 90.0)
+    from oetqf_amd.greens import _Keep, fault_struct
+    keep = _Keep()
+    fs = fault_struct(m, keep)
+    assert lib.oq_gf_okada_ff(ctypes.byref(fs), 1.0, 1.0, 7, 2, 0.0, _lib.ptr(out)) == _lib.OQ_ERR_ARG
+
+
+def test_rhs_create_rejects_bad_properties(oq):
+    """RateStateQuasiDynamicProperty invariants (property.jl:20-24) are checked
+    by the C ABI before any device work."""
+    from oetqf_amd import _lib
+    lib = oq.load_library()
+    a = np.ones(4)
+    rs = _lib.RSProps(_lib.ptr(a), _lib.ptr(a), _lib.ptr(a), _lib.ptr(a), -1.0, 1e-9, 0.6, 1e-6)
+    h = ctypes.c_void_p()
+    st = np.zeros(8)
+    rc = lib.oq_rhs_create(0, 2, 2, 0, ctypes.byref(rs), None, _lib.ptr(st), 0, None, None, None, 0,
+                           ctypes.byref(h))
+    assert rc == _lib.OQ_ERR_ARG and not h.value
+
+
+def test_property_asserts(oq):
+    with pytest.raises(AssertionError):
+        oq.RateStateQuasiDynamicProperty(np.ones(3), np.ones(4), np.ones(3), np.ones(3), 1.0, 1.0)
+    with pytest.raises(AssertionError):
+        oq.RateStateQuasiDynamicProperty(np.ones(3), np.ones(3), np.ones(3), np.ones(3), 1.0, 1.0, 0.0)
+    with pytest.raises(AssertionError):
+        oq.PowerLawViscosityProperty(np.ones(3), np.ones(3), np.ones(5))
+    with pytest.raises(AssertionError):
+        oq.PowerLawViscosityProperty(np.ones(3), np.ones(2), np.ones(6))
+
+
+def test_quadrature_format(oq):
+    c, w = oq.get_quadrature("Gauss1")
+    assert np.allclose(c, 0) and np.allclose(w, 1)
+    c, w = oq.get_quadrature("Gauss2")
+    assert len(w) == 8 and np.isclose(w.sum(), 1) and np.allclose(np.abs(c), 1 / np.sqrt(3))
+    with pytest.raises(AssertionError):
+        oq.get_quadrature((np.zeros(5), np.ones(2)))
+
+
+def test_viscosity_law_host(oq, rng):
+    # test/BEM/tests.jl:142-154
+    A, n, s, t = rng.random(4)
+    p = oq.PowerLawViscosityProperty([A], [n], rng.random(6))
+    assert oq.dϵ_dt(p, 0, s, t) == A * s * t**n
+    A2, n2 = rng.random(2)
+    p2 = oq.PowerLawViscosityProperty([A2], [n2], rng.random(6))
+    pc = oq.CompositePowerLawViscosityProperty([p, p2], rng.random(6))
+    assert oq.dϵ_dt(pc, 0, s, t) == A * s * t**n + A2 * s * t**n2
+
+
+def test_matvec_backend_preference(oq, caplog):
+    # test/tests.jl:60-65
+    assert oq.get_matvecmul() == "LinearAlgebra"
+    assert oq.set_matvecmul("Octavian") == \
+        "New backend Octavian set; restart your Julia session for this change to take effect!"
+    assert oq.set_matvecmul("LinearAlgebra").startswith("New backend LinearAlgebra set")
+    with pytest.raises(ValueError):
+        oq.set_matvecmul("DummyMatVec")
+
+
+def test_product_fails_loudly_without_library(oq, monkeypatch):
+    from oetqf_amd import _lib
+    monkeypatch.setattr(_lib, "LIBPATH", "/nonexistent/liboetqf_amd.so")
+    monkeypatch.setattr(_lib, "_lib", None)
+    with pytest.raises(_lib.OetqfError):
+        _lib.load()
+
+
+def test_bench_refuses_multi_gpu():
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True)
+    assert r.returncode == 2 and "one GPU" in r.stderr
