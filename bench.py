@@ -79,6 +79,8 @@ def main():
     ap.add_argument("--cycle-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 1 GiB device copy (known traffic) for counter scaling")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -87,6 +89,17 @@ def main():
 
     import oetqf_amd as oq
     from oetqf_amd.workloads import YEAR, transform_fault_composite
+
+    if args.copy_calibration:
+        # a copy of known size (reads 1 GiB, writes 1 GiB): the reference for
+        # scaling the FETCH_SIZE / WRITE_SIZE counters of the other kernels
+        import torch
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     w = transform_fault_composite()
     mf, ma = w["mf"], w["ma"]
@@ -158,6 +171,13 @@ def main():
         cpu, ref = cpu_baseline(w, u0, args.cpu_seconds)
         cpu["rhs_rel_err_gpu_vs_cpu"] = float(np.linalg.norm(du64.x[0] - ref[0]) / np.linalg.norm(ref[0]))
 
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "k_rhs_matvec_fp64_transform_fault_4096.traffic.txt")
+    if os.path.exists(tfile):
+        for ln in open(tfile):
+            if ln.startswith("traffic_bytes_per_launch:"):
+                traffic = float(ln.split(":")[1])
+
     line = {
         "metric": METRIC,
         "value": value,
@@ -175,7 +195,7 @@ def main():
                    "k_bytes": 8 * (nf + 6 * ne) ** 2, "assembly_s": t_asm,
                    "wall_s_timed_loop": wall},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
                      "kernel": "k_rhs_matvec<FP64>", "ms_per_launch": ms_mv,
                      "algorithmic_bytes": nbytes},
         "variants": variants,

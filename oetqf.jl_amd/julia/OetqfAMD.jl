@@ -1,0 +1,230 @@
+# OetqfAMD.jl -- Julia binding of liboetqf_amd.so (C ABI: include/oetqf_amd.h).
+#
+# Drop-in methods for Oetqf.jl's hot path on one MI355X:
+#   stress_greens_function(...)  -> GPU builders     (src/BEM/GF.jl:31,123,194,250)
+#   assemble(...)                -> device RHS handle (src/BEM/equation.jl:81,141)
+#   prob.f(du, u, p, t)          -> fused GPU RHS     (src/BEM/equation.jl:156,185)
+#   solve(prob, Tsit5(); ...)    -> on-device Tsit5   (OrdinaryDiffEq call sites io.jl:125-130)
+# Written against Oetqf v0.3.6 types (RectOkadaMesh, BEMHex8Mesh,
+# RateStateQuasiDynamicProperty, PowerLawViscosityProperty).  It needs no
+# change to Oetqf itself: `using Oetqf, OetqfAMD` and call OetqfAMD.* .
+module OetqfAMD
+
+using Oetqf
+using RecursiveArrayTools: ArrayPartition
+
+const libpath = joinpath(@__DIR__, "..", "lib", "liboetqf_amd.so")
+
+struct CFaultMesh
+    x::Ptr{Float64}; ax::Ptr{Float64}; y::Ptr{Float64}; z::Ptr{Float64}; axi::Ptr{Float64}
+    nx::Int64; nxi::Int64; dx::Float64; dep::Float64; dip::Float64
+end
+struct CHex8Mesh
+    cx::Ptr{Float64}; cy::Ptr{Float64}; cz::Ptr{Float64}
+    qx::Ptr{Float64}; qy::Ptr{Float64}; qz::Ptr{Float64}
+    dx::Ptr{Float64}; dy::Ptr{Float64}; dz::Ptr{Float64}; ne::Int64
+end
+struct CQuadrature
+    coords::Ptr{Float64}; weights::Ptr{Float64}; nq::Int64
+end
+struct CRSProps
+    a::Ptr{Float64}; b::Ptr{Float64}; L::Ptr{Float64}; sigma::Ptr{Float64}
+    eta::Float64; vpl::Float64; f0::Float64; v0::Float64
+end
+struct CViscProps
+    gamma::Ptr{Float64}; n::Ptr{Float64}; nlaws::Int64; deps0::NTuple{6,Float64}
+end
+struct CSolverOpts
+    reltol::Float64; abstol::Float64; dt0::Float64; dtmax::Float64; dtmin::Float64; maxiters::Int64
+end
+
+lasterror() = unsafe_string(ccall((:oq_last_error, libpath), Cstring, ()))
+function check(rc::Cint)
+    rc == 0 && return nothing
+    rc == 1 ? throw(ArgumentError(lasterror())) : error("oetqf_amd: " * lasterror())
+end
+
+# Vector{NTuple{2,T}} is laid out as 2n contiguous T: pass it as is.
+tupleptr(v::Vector{NTuple{2,Float64}}) = Ptr{Float64}(pointer(v))
+
+cmesh(m::Oetqf.RectOkadaMesh) = CFaultMesh(pointer(m.x), tupleptr(m.ax), pointer(m.y), pointer(m.z),
+    tupleptr(m.aξ), m.nx, m.nξ, m.Δx, m.dep, m.dip)
+cmesh(m::Oetqf.BEMHex8Mesh) = CHex8Mesh(pointer(m.cx), pointer(m.cy), pointer(m.cz),
+    pointer(m.qx), pointer(m.qy), pointer(m.qz), pointer(m.Δx), pointer(m.Δy), pointer(m.Δz), length(m.cx))
+ftcode(::StrikeSlip) = Cint(0)
+ftcode(::DipSlip) = Cint(1)
+
+function quadrature(qtype)
+    c, w = Oetqf.get_quadrature(qtype)   # GF.jl:318-328 (gmsh rule or user tuple)
+    c = collect(Float64, vec(c)); w = collect(Float64, w)
+    return c, w
+end
+
+# ---- Green's functions (GF.jl) ----
+function stress_greens_function(mesh::Oetqf.RectOkadaMesh, λ::Float64, μ::Float64;
+        ftype=StrikeSlip(), fourier::Bool=true, nrept::Integer=2, buffer_ratio::Real=0, kwargs...)
+    @assert buffer_ratio ≥ 0 "Argument `buffer_ratio` must be ≥ 0."
+    GC.@preserve mesh begin
+        cm = Ref(cmesh(mesh))
+        if fourier
+            out = Array{ComplexF64,3}(undef, mesh.nx, mesh.nξ, mesh.nξ)
+            check(ccall((:oq_gf_okada_ff_dft, libpath), Cint,
+                (Ref{CFaultMesh}, Float64, Float64, Cint, Int64, Float64, Ptr{ComplexF64}),
+                cm, λ, μ, ftcode(ftype), nrept, buffer_ratio, out))
+        else
+            out = Array{Float64,3}(undef, mesh.nx, mesh.nξ, mesh.nξ)
+            check(ccall((:oq_gf_okada_ff, libpath), Cint,
+                (Ref{CFaultMesh}, Float64, Float64, Cint, Int64, Float64, Ptr{Float64}),
+                cm, λ, μ, ftcode(ftype), nrept, buffer_ratio, out))
+        end
+    end
+    return out
+end
+
+function stress_greens_function(mf::Oetqf.RectOkadaMesh, ma::Oetqf.BEMHex8Mesh, λ::Float64, μ::Float64;
+        ftype=StrikeSlip(), qtype="Gauss1", nrept::Integer=2, buffer_ratio::Real=0)
+    @assert buffer_ratio ≥ 0 "Argument `buffer_ratio` must be ≥ 0."
+    c, w = quadrature(qtype)
+    out = zeros(Float64, 6length(ma.cx), mf.nx * mf.nξ)
+    GC.@preserve mf ma c w begin
+        q = Ref(CQuadrature(pointer(c), pointer(w), length(w)))
+        check(ccall((:oq_gf_okada_mf, libpath), Cint,
+            (Ref{CFaultMesh}, Ref{CHex8Mesh}, Float64, Float64, Cint, Ref{CQuadrature}, Int64, Float64, Ptr{Float64}),
+            Ref(cmesh(mf)), Ref(cmesh(ma)), λ, μ, ftcode(ftype), q, nrept, buffer_ratio, out))
+    end
+    return out
+end
+
+function stress_greens_function(ma::Oetqf.BEMHex8Mesh, mf::Oetqf.RectOkadaMesh, λ::Float64, μ::Float64;
+        ftype=StrikeSlip())
+    out = zeros(Float64, mf.nx * mf.nξ, 6length(ma.cx))
+    GC.@preserve mf ma begin
+        check(ccall((:oq_gf_hex8_fm, libpath), Cint,
+            (Ref{CHex8Mesh}, Ref{CFaultMesh}, Float64, Float64, Cint, Ptr{Float64}),
+            Ref(cmesh(ma)), Ref(cmesh(mf)), λ, μ, ftcode(ftype), out))
+    end
+    return out
+end
+
+function stress_greens_function(ma::Oetqf.BEMHex8Mesh, λ::Float64, μ::Float64;
+        qtype="Gauss1", checkeigvals::Bool=true)
+    c, w = quadrature(qtype)
+    out = zeros(Float64, 6length(ma.cx), 6length(ma.cx))
+    GC.@preserve ma c w begin
+        q = Ref(CQuadrature(pointer(c), pointer(w), length(w)))
+        check(ccall((:oq_gf_hex8_mm, libpath), Cint,
+            (Ref{CHex8Mesh}, Float64, Float64, Ref{CQuadrature}, Ptr{Float64}),
+            Ref(cmesh(ma)), λ, μ, q, out))
+    end
+    if checkeigvals   # GF.jl:291-294
+        maxrλ = maximum(real, Oetqf.eigvals(out))
+        println("Maximum real part of eigval is: ", round(maxrλ; digits=4))
+    end
+    return out
+end
+
+# ---- RHS handle (equation.jl) ----
+mutable struct GPURHS
+    h::Ptr{Cvoid}
+    ne::Int
+    keep::Any
+    function GPURHS(h, ne, keep)
+        r = new(h, ne, keep)
+        finalizer(x -> (x.h != C_NULL && ccall((:oq_rhs_destroy, libpath), Cint, (Ptr{Cvoid},), x.h); x.h = C_NULL), r)
+        return r
+    end
+end
+
+crs(p::RateStateQuasiDynamicProperty) = CRSProps(pointer(p.a), pointer(p.b), pointer(p.L), pointer(p.σ),
+    p.η, p.vpl, p.f₀, p.v₀)
+function cvisc(p::PowerLawViscosityProperty, ne)
+    γ = collect(Float64, p.γ); n = collect(Float64, p.n)
+    return CViscProps(pointer(γ), pointer(n), 1, Tuple(Float64.(p.dϵ₀))), (γ, n)
+end
+function cvisc(p::CompositePowerLawViscosityProperty, ne)
+    γ = reduce(vcat, [collect(Float64, q.γ) for q in p.piter])
+    n = reduce(vcat, [collect(Float64, q.n) for q in p.piter])
+    return CViscProps(pointer(γ), pointer(n), length(p.piter), Tuple(Float64.(p.dϵ₀))), (γ, n)
+end
+
+kcode(T) = T === Float64 ? Cint(0) : T === Float32 ? Cint(1) : Cint(2)  # :bf16 -> 2
+
+"""assemble(gf, p, u0, tspan; Kstorage=Float64): fault only (equation.jl:81-89)."""
+function assemble(gf::AbstractArray, p::RateStateQuasiDynamicProperty, u0::ArrayPartition, tspan::NTuple{2};
+        Kstorage=Float64, device::Integer=0, kwargs...)
+    nx, nξ = size(u0.x[1])
+    isdft = eltype(gf) <: Complex
+    g = isdft ? collect(ComplexF64, gf) : collect(Float64, gf)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve p g begin
+        check(ccall((:oq_rhs_create, libpath), Cint,
+            (Cint, Int64, Int64, Int64, Ref{CRSProps}, Ptr{CViscProps}, Ptr{Cvoid}, Cint,
+             Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cint, Ref{Ptr{Cvoid}}),
+            device, nx, nξ, 0, Ref(crs(p)), C_NULL, pointer(g), Cint(isdft), C_NULL, C_NULL, C_NULL,
+            kcode(Kstorage), h))
+    end
+    f = GPURHS(h[], 0, (p,))
+    return Oetqf.ODEProblem{true}((du, u, pp, t) -> rhs!(f, du, u, t), u0, tspan, (p, f))
+end
+
+"""assemble(gf₁₁, gf₁₂, gf₂₁, gf₂₂, pf, pa, u0, tspan) (equation.jl:141-154)."""
+function assemble(gf₁₁::AbstractArray, gf₁₂::AbstractMatrix, gf₂₁::AbstractMatrix, gf₂₂::AbstractMatrix,
+        pf::RateStateQuasiDynamicProperty, pa, u0::ArrayPartition, tspan::NTuple{2};
+        Kstorage=Float64, device::Integer=0, kwargs...)
+    nx, nξ = size(u0.x[1]); ne = size(u0.x[3], 1)
+    isdft = eltype(gf₁₁) <: Complex
+    g = isdft ? collect(ComplexF64, gf₁₁) : collect(Float64, gf₁₁)
+    vs, keepv = cvisc(pa, ne)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve pf keepv g gf₁₂ gf₂₁ gf₂₂ begin
+        check(ccall((:oq_rhs_create, libpath), Cint,
+            (Cint, Int64, Int64, Int64, Ref{CRSProps}, Ref{CViscProps}, Ptr{Cvoid}, Cint,
+             Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cint, Ref{Ptr{Cvoid}}),
+            device, nx, nξ, ne, Ref(crs(pf)), Ref(vs), pointer(g), Cint(isdft),
+            pointer(gf₁₂), pointer(gf₂₁), pointer(gf₂₂), kcode(Kstorage), h))
+    end
+    f = GPURHS(h[], ne, (pf, pa, keepv))
+    return Oetqf.ODEProblem{true}((du, u, pp, t) -> rhs!(f, du, u, t), u0, tspan, (pf, pa, f))
+end
+
+"""The `ode(du,u,p,t)` contract on host arrays (equation.jl:156-166, :185-205)."""
+function rhs!(f::GPURHS, du::ArrayPartition, u::ArrayPartition, t)
+    if f.ne == 0
+        v, θ, δ = u.x; dv, dθ, dδ = du.x
+        check(ccall((:oq_rhs_eval, libpath), Cint,
+            (Ptr{Cvoid}, Float64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+             Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+            f.h, t, v, θ, C_NULL, C_NULL, δ, dv, dθ, C_NULL, C_NULL, dδ))
+    else
+        v, θ, ϵ, σ, δ = u.x; dv, dθ, dϵ, dσ, dδ = du.x
+        check(ccall((:oq_rhs_eval, libpath), Cint,
+            (Ptr{Cvoid}, Float64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+             Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+            f.h, t, v, θ, ϵ, σ, δ, dv, dθ, dϵ, dσ, dδ))
+    end
+    return nothing
+end
+
+"""solve on the device (Tsit5 with OrdinaryDiffEq's controller); returns
+(t, u::ArrayPartition, retcode::Symbol)."""
+function solve_gpu(prob; reltol=1e-6, abstol=1e-8, dt=0.0, dtmax=0.0, dtmin=0.0, maxiters=10^9)
+    f = prob.p[end]::GPURHS
+    u0 = collect(Float64, prob.u0)                     # packed ArrayPartition order
+    s = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:oq_stepper_create, libpath), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Float64, Float64, Ref{CSolverOpts}, Ref{Ptr{Cvoid}}),
+        f.h, u0, prob.tspan[1], prob.tspan[2], Ref(CSolverOpts(reltol, abstol, dt, dtmax, dtmin, maxiters)), s))
+    rc = Ref{Cint}(0)
+    while rc[] == 0
+        check(ccall((:oq_stepper_run, libpath), Cint, (Ptr{Cvoid}, Int64, Ref{Cint}), s[], 4096, rc))
+    end
+    t = Ref{Float64}(0); dtr = Ref{Float64}(0); na = Ref{Int64}(0); nr = Ref{Int64}(0)
+    u = similar(u0)
+    check(ccall((:oq_stepper_state, libpath), Cint,
+        (Ptr{Cvoid}, Ref{Float64}, Ref{Float64}, Ptr{Float64}, Ref{Int64}, Ref{Int64}), s[], t, dtr, u, na, nr))
+    ccall((:oq_stepper_destroy, libpath), Cint, (Ptr{Cvoid},), s[])
+    out = similar(prob.u0); copyto!(out, u)
+    return t[], out, (:Running, :Success, :MaxIters, :DtLessThanMin, :NonFinite)[rc[] + 1]
+end
+
+end # module
