@@ -79,6 +79,8 @@ def main():
     ap.add_argument("--cycle-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--ensemble", type=int, default=256,
+                    help="members of the FP32 MFMA ensemble leg (0 = skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (known traffic) for counter scaling")
     args = ap.parse_args()
@@ -147,6 +149,39 @@ def main():
             log("%s: %.4f ms/eval (%.2fx), err dv %.2e dsigma %.2e" % (kd, me, ms_eval / me, err_tau, err_sig))
             p.f.close()
 
+    ensemble = None
+    if args.ensemble > 0:
+        from oetqf_amd.workloads import ensemble_members
+        M = args.ensemble
+        members, U0 = ensemble_members(w, M)
+        log("ensemble: %d members, FP32 operator, MFMA GEMM" % M)
+        ens = oq.Ensemble(mf, ma, w["lam"], w["mu"], members, w["pa"], nrept=w["nrept"],
+                          buffer_ratio=w["buffer_ratio"], kdtype="fp32")
+        dU = ens.eval(U0)
+        ens.time(2)
+        ms_ens = ens.time(max(5, args.steps // 10))
+        member_rps = M * 1000.0 / ms_ens
+        nn = nf + 6 * ne
+        tflops = 2.0 * nn * nn * M / (ms_ens * 1e-3) / 1e12
+        # error of two members against their own FP64 RHS
+        errs = []
+        for m in (0, M - 1):
+            pm = oq.assemble_from_mesh(mf, ma, w["lam"], w["mu"], members[m], w["pa"], u0, w["tspan"],
+                                       nrept=w["nrept"], buffer_ratio=w["buffer_ratio"])
+            um = u0.similar().unpack(U0[m])
+            dm = oq.similar(um)
+            pm.f(dm, um, None, 0.0)
+            ref = dm.pack()
+            errs.append(float(np.linalg.norm(dU[m] - ref) / np.linalg.norm(ref)))
+            pm.f.close()
+        ensemble = dict(members=M, ms_per_ensemble_rhs=ms_ens, member_rhs_per_s=member_rps,
+                        speedup_vs_fp64_plain=member_rps / value, gemm_tflops=tflops,
+                        mfma_fp32_peak_tflops=157.3, rel_err_vs_fp64_members=errs)
+        log("ensemble: %.3f ms per RHS of %d members -> %.0f member-RHS/s (%.1fx FP64 plain), "
+            "%.1f TFLOP/s, err %s" % (ms_ens, M, member_rps, member_rps / value, tflops,
+                                      ", ".join("%.1e" % e for e in errs)))
+        ens.close()
+
     cycle = None
     if args.cycle_seconds > 0:
         it = oq.Integrator(prob, reltol=w["reltol"], abstol=w["abstol"])
@@ -199,6 +234,7 @@ def main():
                      "kernel": "k_rhs_matvec<FP64>", "ms_per_launch": ms_mv,
                      "algorithmic_bytes": nbytes},
         "variants": variants,
+        "ensemble": ensemble,
         "cycle": cycle,
         "cpu_baseline": cpu,
     }

@@ -207,6 +207,37 @@ int oq_stepper_state(oq_stepper *s, double *t, double *dt, double *u,
                      int64_t *naccept, int64_t *nreject);
 int oq_stepper_destroy(oq_stepper *s);
 
+/* ---- ensembles of frictional-parameter realizations sharing one
+ * geometry.  No entry point of the reference corresponds: a parameter study
+ * there is one `assemble` + `solve` per realization (equation.jl:141-154,
+ * examples/otf-with-mantle.jl:154-161).  Here the members' RHS is one MFMA
+ * GEMM over a shared FP32/BF16 operator and a lockstep Tsit5 advances all
+ * members with per-member dt and acceptance.  Member states are packed
+ * ArrayPartitions laid one after another (member-major, n entries each). */
+typedef struct oq_ensemble oq_ensemble;
+
+/* members: nmembers oq_rs_props (per-member a, b, L, sigma arrays and
+ * scalars); visc is shared.  kdtype: OQ_K_FP32 or OQ_K_BF16. */
+int oq_ensemble_create(int device, const oq_fault_mesh *mf, const oq_hex8_mesh *ma,
+                       double lambda, double mu, int ftype, int64_t nrept,
+                       double buffer_ratio, const oq_quadrature *quad,
+                       int64_t nmembers, const oq_rs_props *members,
+                       const oq_visc_props *visc, int kdtype, oq_ensemble **out);
+/* du = f(u) for all members (host arrays, nmembers * nstate each). */
+int oq_ensemble_eval(oq_ensemble *e, const double *u, double *du);
+/* Sets the members' initial states and options (opts->dt0 > 0 required)
+ * and captures one lockstep step into a HIP graph. */
+int oq_ensemble_start(oq_ensemble *e, const double *u0, double t0, double t1,
+                      const oq_solver_opts *opts);
+/* Advances up to max_steps lockstep attempts; *nrunning = members not done. */
+int oq_ensemble_run(oq_ensemble *e, int64_t max_steps, int64_t *nrunning);
+/* Per-member t[M], u[M*nstate], step counters and OQ_RET_* codes (0 = running). */
+int oq_ensemble_state(oq_ensemble *e, double *t, double *u, int64_t *naccept,
+                      int64_t *nreject, int *retcode);
+/* Mean time of one ensemble RHS (all members) over `iters` evaluations. */
+int oq_ensemble_time(oq_ensemble *e, int iters, double *ms_eval);
+int oq_ensemble_destroy(oq_ensemble *e);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,614 @@
+// ensemble.hip -- many frictional-parameter realizations sharing one geometry.
+//
+// The reference integrates one `assemble`d problem at a time; a parameter
+// study runs it once per realization (typically as an OrdinaryDiffEq
+// EnsembleProblem over remade problems, examples/otf-with-mantle.jl:154-161).
+// Here M members that share the Green's operator K advance together:
+//
+//   - every RHS is ONE GEMM  Y = K S  (N x N by N x M), K streamed once in
+//     FP32 (or BF16) through v_mfma_f32_32x32x2f32 tiles, FP32 products
+//     accumulated in the MFMA registers for 256-column chunks and folded into
+//     an FP32 (hi, lo) two-sum pair between chunks, then the fused epilogue
+//     (rate-and-state update / viscous stress rate) per (row, member);
+//   - a lockstep Tsit5: all members evaluate stage s together, each with its
+//     own dt, error norm, PI controller state and accept/reject decision.
+//
+// Layout: every per-member field is entry-major, x[i * M + m], so that the
+// members of one row are contiguous (coalesced prologue/epilogue/stage
+// loads, and the GEMM's B operand S is stored [source][member]).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <vector>
+
+#include "rhs_impl.h"
+
+typedef float f16v __attribute__((ext_vector_type(16)));
+typedef float f4x __attribute__((ext_vector_type(4)));
+typedef unsigned int u4x __attribute__((ext_vector_type(4)));
+
+namespace oq {
+
+struct EnsDev {
+    int64_t nf, ne, N, ld, M, Mp, n;  // Mp: padded member count (multiple of 64)
+    StateLayout lay;                   // offsets in entries (rows)
+    const double *a, *b, *L, *sigma;   // nf x M
+    const double *eta, *vpl, *f0, *v0; // M
+    const double *gamma, *nexp;        // nlaws*ne (shared)
+    int nlaws;
+    double deps0[6];
+};
+
+// ---------------- prologue: S = [v - vpl ; eps_dot - deps0] ----------------
+__global__ void __launch_bounds__(256) k_ens_prologue(EnsDev d, const double *__restrict__ u,
+                                                      double *__restrict__ du,
+                                                      float *__restrict__ S)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t M = d.M;
+    if (t < d.nf * M) {
+        const int64_t i = t / M, m = t % M;
+        S[i * d.Mp + m] = (float)(u[(d.lay.v + i) * M + m] - d.vpl[m]);
+        return;
+    }
+    const int64_t t2 = t - d.nf * M;
+    if (t2 >= d.ne * M) return;
+    const int64_t i = t2 / M, m = t2 % M, ne = d.ne;
+    const double *sg = u + d.lay.sig * M;
+    const double sxx0 = sg[i * M + m], sxy = sg[(i + ne) * M + m], sxz = sg[(i + 2 * ne) * M + m];
+    const double syy0 = sg[(i + 3 * ne) * M + m], syz = sg[(i + 4 * ne) * M + m];
+    const double szz0 = sg[(i + 5 * ne) * M + m];
+    const double skk = (sxx0 + syy0 + szz0) / 3.0;
+    const double sxx = sxx0 - skk, syy = syy0 - skk, szz = szz0 - skk;
+    const double tn = sqrt(sxx * sxx + syy * syy + szz * szz + 2.0 * (sxy * sxy + sxz * sxz + syz * syz));
+    double f = 0.0;
+    for (int l = 0; l < d.nlaws; ++l) f += d.gamma[l * ne + i] * pow(tn, d.nexp[l * ne + i]);
+    const double e[6] = {f * sxx, f * sxy, f * sxz, f * syy, f * syz, f * szz};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        du[(d.lay.eps + i + k * ne) * M + m] = e[k];
+        S[(d.nf + i + k * ne) * d.Mp + m] = (float)(e[k] - d.deps0[k]);
+    }
+}
+
+__device__ __forceinline__ void two_sum(float &hi, float &lo, float x)
+{
+    const float s = hi + x;
+    const float bp = s - hi;
+    const float err = (hi - (s - bp)) + (x - bp);
+    hi = s;
+    lo += err;
+}
+
+// ---------------- GEMM + fused epilogue ----------------
+// Block: 256 threads = 4 wavefronts; tile 128 rows x 64 members; each
+// wavefront owns 32 rows x 64 members = two 32x32 MFMA tiles.  K-slices of
+// BK = 16 columns are staged through LDS.
+static constexpr int EBM = 128, EBN = 64, EBK = 16, ECHUNK = 256;
+
+template <int KT>
+__global__ void __launch_bounds__(256) k_ens_gemm(EnsDev d, const void *__restrict__ Kp,
+                                                  const float *__restrict__ S,
+                                                  const double *__restrict__ u,
+                                                  double *__restrict__ du)
+{
+    __shared__ float As[EBM][EBK + 1];
+    __shared__ float Bs[EBK][EBN + 4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t rb = (int64_t)blockIdx.x * EBM, cb = (int64_t)blockIdx.y * EBN;
+    // global -> LDS assignments
+    const int ar = tid >> 1, ak = (tid & 1) * 8;      // A: row ar, 8 columns from ak
+    const int bk = tid >> 4, bc = (tid & 15) * 4;     // B: k row bk, 4 members from bc
+    const int64_t arow = rb + ar < d.N ? rb + ar : d.N - 1;
+    f16v acc0 = {0}, acc1 = {0};
+    float hi0[16], lo0[16], hi1[16], lo1[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) hi0[i] = lo0[i] = hi1[i] = lo1[i] = 0.f;
+    const int64_t nk = d.ld;  // zero-padded columns
+    for (int64_t k0 = 0; k0 < nk; k0 += EBK) {
+        // stage A (K rows) and B (S slice)
+        if (KT == OQ_K_FP32) {
+            const float *Krow = static_cast<const float *>(Kp) + arow * d.ld + k0 + ak;
+            const f4x x0 = __builtin_nontemporal_load(reinterpret_cast<const f4x *>(Krow));
+            const f4x x1 = __builtin_nontemporal_load(reinterpret_cast<const f4x *>(Krow) + 1);
+            As[ar][ak + 0] = x0.x; As[ar][ak + 1] = x0.y; As[ar][ak + 2] = x0.z; As[ar][ak + 3] = x0.w;
+            As[ar][ak + 4] = x1.x; As[ar][ak + 5] = x1.y; As[ar][ak + 6] = x1.z; As[ar][ak + 7] = x1.w;
+        } else {
+            const unsigned short *Krow = static_cast<const unsigned short *>(Kp) + arow * d.ld + k0 + ak;
+            const u4x w = __builtin_nontemporal_load(reinterpret_cast<const u4x *>(Krow));
+            As[ar][ak + 0] = __uint_as_float(w.x << 16); As[ar][ak + 1] = __uint_as_float(w.x & 0xffff0000u);
+            As[ar][ak + 2] = __uint_as_float(w.y << 16); As[ar][ak + 3] = __uint_as_float(w.y & 0xffff0000u);
+            As[ar][ak + 4] = __uint_as_float(w.z << 16); As[ar][ak + 5] = __uint_as_float(w.z & 0xffff0000u);
+            As[ar][ak + 6] = __uint_as_float(w.w << 16); As[ar][ak + 7] = __uint_as_float(w.w & 0xffff0000u);
+        }
+        {
+            const float4 s4 = *reinterpret_cast<const float4 *>(S + (k0 + bk) * d.Mp + cb + bc);
+            Bs[bk][bc + 0] = s4.x; Bs[bk][bc + 1] = s4.y; Bs[bk][bc + 2] = s4.z; Bs[bk][bc + 3] = s4.w;
+        }
+        __syncthreads();
+        const int r = wave * 32 + (lane & 31), kk = lane >> 5;
+#pragma unroll
+        for (int k = 0; k < EBK; k += 2) {
+            const float a = As[r][k + kk];
+            const float b0 = Bs[k + kk][lane & 31];
+            const float b1 = Bs[k + kk][32 + (lane & 31)];
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b0, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b1, acc1, 0, 0, 0);
+        }
+        __syncthreads();
+        if (((k0 + EBK) % ECHUNK) == 0 || k0 + EBK >= nk) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                two_sum(hi0[i], lo0[i], acc0[i]);
+                two_sum(hi1[i], lo1[i], acc1[i]);
+                acc0[i] = 0.f;
+                acc1[i] = 0.f;
+            }
+        }
+    }
+    // fused epilogue: element i of tile t sits at row (i/4)*8 + (lane/32)*4 + i%4
+    // and member 32 t + lane%32 of this wavefront's 32 x 64 block
+    const int64_t M = d.M;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int64_t m = cb + 32 * t + (lane & 31);
+        if (m >= M) continue;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int64_t row = rb + wave * 32 + (i / 4) * 8 + (lane >> 5) * 4 + (i % 4);
+            if (row >= d.N) continue;
+            const double y = t == 0 ? (double)hi0[i] + (double)lo0[i] : (double)hi1[i] + (double)lo1[i];
+            if (row < d.nf) {
+                // update_fault! (equation.jl:237-245), member m's parameters
+                const int64_t ix = row * M + m;
+                const double v = u[(d.lay.v + row) * M + m], th = u[(d.lay.th + row) * M + m];
+                const double a = d.a[ix], b = d.b[ix], L = d.L[ix];
+                const double thp = th > 0.0 ? th : 0.0;
+                const double psi1 = exp((d.f0[m] + b * log(d.v0[m] * thp / L)) / a) / (2.0 * d.v0[m]);
+                const double psi2 = d.sigma[ix] * psi1 / hypot(1.0, v * psi1);
+                const double dth = 1.0 - v * th / L;
+                du[(d.lay.th + row) * M + m] = dth;
+                du[(d.lay.v + row) * M + m] = (y - b / th * v * psi2 * dth) / (a * psi2 + d.eta[m]);
+                du[(d.lay.del + row) * M + m] = v;
+            } else {
+                du[(d.lay.sig + row - d.nf) * M + m] = y;
+            }
+        }
+    }
+}
+
+// ---------------- batched Tsit5 ----------------
+__constant__ double kEA[7][6] = {
+    {0, 0, 0, 0, 0, 0},
+    {0.161, 0, 0, 0, 0, 0},
+    {-0.008480655492356989, 0.335480655492357, 0, 0, 0, 0},
+    {2.897153057105493, -6.359448489975075, 4.3622954328695815, 0, 0, 0},
+    {5.325864828439257, -11.748883564062828, 7.4955393428898365, -0.09249506636175525, 0, 0},
+    {5.86145544294642, -12.92096931784711, 8.159367898576159, -0.071584973281401,
+     -0.028269050394068383, 0},
+    {0.09646076681806523, 0.01, 0.4798896504144996, 1.379008574103742, -3.290069515436081,
+     2.324710524099774}};
+__constant__ double kEBt[7] = {-0.00178001105222577714, -0.0008164344596567469,
+                               0.007880878010261995, -0.1447110071732629, 0.5823571654525552,
+                               -0.45808210592918697, 0.015151515151515152};
+
+struct EnsCtrl {  // arrays of M
+    double *t, *dt, *dtcur, *qold;
+    long long *naccept, *nreject, *iter;
+    int *done, *accept, *retcode;
+    double t1, reltol, abstol, dtmax, dtmin;
+    long long maxiters;
+    int M;
+};
+
+__global__ void k_ens_begin(EnsCtrl c)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= c.M) return;
+    c.accept[m] = 0;
+    c.dtcur[m] = 0.0;
+    if (c.done[m]) return;
+    if (!(c.t[m] < c.t1)) { c.done[m] = 1; c.retcode[m] = OQ_RET_SUCCESS; return; }
+    if (c.iter[m] >= c.maxiters) { c.done[m] = 1; c.retcode[m] = OQ_RET_MAXITERS; return; }
+    double dt = c.dt[m];
+    if (dt > c.dtmax) dt = c.dtmax;
+    if (c.t[m] + dt > c.t1) dt = c.t1 - c.t[m];
+    if (!(dt >= c.dtmin) || !isfinite(dt)) {
+        c.done[m] = 1;
+        c.retcode[m] = isfinite(dt) ? OQ_RET_DTMIN : OQ_RET_NONFINITE;
+        return;
+    }
+    c.dtcur[m] = dt;
+    c.iter[m] += 1;
+}
+
+__global__ void __launch_bounds__(256) k_ens_stage(EnsCtrl c, int s, int64_t nM, int64_t M,
+                                                   const double *__restrict__ u,
+                                                   const double *__restrict__ k,
+                                                   double *__restrict__ y)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nM) return;
+    const double dt = c.dtcur[i % M];
+    double acc = 0.0;
+    for (int j = 0; j < s; ++j) acc += kEA[s][j] * k[j * nM + i];
+    y[i] = u[i] + dt * acc;
+}
+
+// per-(row chunk, member) partial sums of the squared scaled error
+static constexpr int kEnsChunks = 64;
+__global__ void __launch_bounds__(256) k_ens_err(EnsCtrl c, int64_t n, int64_t M,
+                                                 const double *__restrict__ u,
+                                                 const double *__restrict__ y,
+                                                 const double *__restrict__ k,
+                                                 double *__restrict__ part)
+{
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int chunk = blockIdx.y;
+    if (m >= M) return;
+    const int64_t nM = n * M;
+    const int64_t r0 = n * chunk / kEnsChunks, r1 = n * (chunk + 1) / kEnsChunks;
+    const double dt = c.dtcur[m];
+    double acc = 0.0;
+    for (int64_t r = r0; r < r1; ++r) {
+        const int64_t i = r * M + m;
+        double e = 0.0;
+#pragma unroll
+        for (int j = 0; j < 7; ++j) e += kEBt[j] * k[j * nM + i];
+        const double w = dt * e / (c.abstol + fmax(fabs(u[i]), fabs(y[i])) * c.reltol);
+        acc += w * w;
+    }
+    part[chunk * M + m] = acc;
+}
+
+__global__ void k_ens_controller(EnsCtrl c, int64_t n, const double *__restrict__ part)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= c.M || c.done[m]) return;
+    double acc = 0.0;
+    for (int ch = 0; ch < kEnsChunks; ++ch) acc += part[ch * c.M + m];
+    const double beta1 = 7.0 / 50.0, beta2 = 2.0 / 25.0, qmin = 0.2, qmax = 10.0, gamma = 0.9,
+                 qoldinit = 1e-4;
+    const double eest = sqrt(acc / (double)n);
+    double q11, q;
+    if (eest == 0.0) {
+        q11 = 0.0;
+        q = 1.0 / qmax;
+    } else {
+        q11 = pow(eest, beta1);
+        q = q11 / pow(c.qold[m], beta2);
+        q = fmax(1.0 / qmax, fmin(1.0 / qmin, q / gamma));
+    }
+    const double dtc = c.dtcur[m];
+    if (eest <= 1.0) {
+        c.t[m] = (c.t1 - (c.t[m] + dtc) <= 0.0) ? c.t1 : c.t[m] + dtc;
+        c.qold[m] = fmax(eest, qoldinit);
+        c.dt[m] = dtc / q;
+        c.naccept[m] += 1;
+        c.accept[m] = 1;
+    } else {
+        c.nreject[m] += 1;
+        if (!isfinite(eest)) {
+            c.done[m] = 1;
+            c.retcode[m] = OQ_RET_NONFINITE;
+            return;
+        }
+        c.dt[m] = dtc / fmin(1.0 / qmin, q11 / gamma);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_ens_commit(EnsCtrl c, int64_t nM, int64_t M,
+                                                    double *__restrict__ u,
+                                                    const double *__restrict__ y,
+                                                    double *__restrict__ k)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nM || !c.accept[i % M]) return;
+    u[i] = y[i];
+    k[i] = k[6 * nM + i];
+}
+
+}  // namespace oq
+
+struct oq_ensemble {
+    oq_rhs *h = nullptr;  // owns K (FP32/BF16), stream, sizes, shared viscosity
+    int64_t M = 0, Mp = 0, n = 0;
+    oq::DevBuf a, b, L, sigma, eta, vpl, f0, v0, S;
+    oq::DevBuf u, y, k, part;
+    oq::DevBuf ct, cdt, cdtcur, cqold, cna, cnr, citer, cdone, cacc, cret;
+    oq::EnsDev dev{};
+    oq::EnsCtrl ctrl{};
+    bool stepping = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace oq {
+
+static inline unsigned nbk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+static int ens_rhs(oq_ensemble *e, const double *u, double *du)
+{
+    oq_rhs *h = e->h;
+    const int64_t rows = (h->nf + h->ne) * e->M;
+    hipLaunchKernelGGL(k_ens_prologue, dim3(nbk(rows, 256)), dim3(256), 0, h->stream, e->dev, u, du,
+                       e->S.as<float>());
+    const dim3 grid(nbk(h->N, EBM), (unsigned)(e->Mp / EBN));
+    if (h->kdtype == OQ_K_FP32)
+        hipLaunchKernelGGL(k_ens_gemm<OQ_K_FP32>, grid, dim3(256), 0, h->stream, e->dev, h->K.p,
+                           e->S.as<float>(), u, du);
+    else
+        hipLaunchKernelGGL(k_ens_gemm<OQ_K_BF16>, grid, dim3(256), 0, h->stream, e->dev, h->K.p,
+                           e->S.as<float>(), u, du);
+    OQ_HIP(hipGetLastError());
+    return OQ_OK;
+}
+
+static int ens_step(oq_ensemble *e)
+{
+    oq_rhs *h = e->h;
+    const int64_t nM = e->n * e->M;
+    double *u = e->u.as<double>(), *y = e->y.as<double>(), *k = e->k.as<double>();
+    int rc;
+    hipLaunchKernelGGL(k_ens_begin, dim3(nbk(e->M, 256)), dim3(256), 0, h->stream, e->ctrl);
+    for (int s = 1; s < 7; ++s) {
+        hipLaunchKernelGGL(k_ens_stage, dim3(nbk(nM, 256)), dim3(256), 0, h->stream, e->ctrl, s, nM,
+                           e->M, u, k, y);
+        if ((rc = ens_rhs(e, y, k + s * nM))) return rc;
+    }
+    hipLaunchKernelGGL(k_ens_err, dim3(nbk(e->M, 256), kEnsChunks), dim3(256), 0, h->stream, e->ctrl,
+                       e->n, e->M, u, y, k, e->part.as<double>());
+    hipLaunchKernelGGL(k_ens_controller, dim3(nbk(e->M, 256)), dim3(256), 0, h->stream, e->ctrl, e->n,
+                       e->part.as<double>());
+    hipLaunchKernelGGL(k_ens_commit, dim3(nbk(nM, 256)), dim3(256), 0, h->stream, e->ctrl, nM, e->M,
+                       u, y, k);
+    OQ_HIP(hipGetLastError());
+    return OQ_OK;
+}
+
+static void ens_destroy(oq_ensemble *e)
+{
+    if (!e) return;
+    if (e->h && e->h->stream) (void)hipStreamSynchronize(e->h->stream);
+    if (e->exec) (void)hipGraphExecDestroy(e->exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    if (e->h) oq_rhs_destroy(e->h);
+    delete e;
+}
+
+// entry-major copy of M member arrays (each length n): out[i*M + m]
+static std::vector<double> interleave(const double *const *src, int64_t n, int64_t M)
+{
+    std::vector<double> out((size_t)(n * M));
+    for (int64_t m = 0; m < M; ++m)
+        for (int64_t i = 0; i < n; ++i) out[(size_t)(i * M + m)] = src[m][i];
+    return out;
+}
+
+}  // namespace oq
+
+using namespace oq;
+
+extern "C" {
+
+int oq_ensemble_create(int device, const oq_fault_mesh *mf, const oq_hex8_mesh *ma, double lambda,
+                       double mu, int ftype, int64_t nrept, double buffer_ratio,
+                       const oq_quadrature *quad, int64_t nmembers, const oq_rs_props *members,
+                       const oq_visc_props *visc, int kdtype, oq_ensemble **out)
+{
+    OQ_REQUIRE(out != nullptr && members != nullptr, "NULL argument");
+    *out = nullptr;
+    OQ_REQUIRE(nmembers >= 1, "nmembers must be >= 1");
+    OQ_REQUIRE(kdtype == OQ_K_FP32 || kdtype == OQ_K_BF16, "ensemble operator is FP32 or BF16");
+    for (int64_t m = 0; m < nmembers; ++m) {
+        const oq_rs_props &p = members[m];
+        OQ_REQUIRE(p.a && p.b && p.L && p.sigma, "member property array is NULL");
+        OQ_REQUIRE(p.f0 > 0.0 && p.v0 > 0.0 && p.eta > 0.0 && p.vpl > 0.0,
+                   "RateStateQuasiDynamicProperty requires f₀ > 0, v₀ > 0, η > 0, vpl > 0");
+    }
+    oq_ensemble *e = new oq_ensemble();
+    int rc = oq_rhs_create_from_mesh(device, mf, ma, lambda, mu, ftype, nrept, buffer_ratio, quad,
+                                     &members[0], visc, kdtype, &e->h);
+    if (rc) {
+        ens_destroy(e);
+        return rc;
+    }
+    oq_rhs *h = e->h;
+    const int64_t M = nmembers, nf = h->nf;
+    e->M = M;
+    e->Mp = (M + EBN - 1) / EBN * EBN;
+    e->n = h->lay.n;
+    std::vector<const double *> pa(M), pb(M), pL(M), ps(M);
+    std::vector<double> eta(M), vpl(M), f0(M), v0(M);
+    for (int64_t m = 0; m < M; ++m) {
+        pa[m] = members[m].a; pb[m] = members[m].b; pL[m] = members[m].L; ps[m] = members[m].sigma;
+        eta[m] = members[m].eta; vpl[m] = members[m].vpl; f0[m] = members[m].f0; v0[m] = members[m].v0;
+    }
+    const size_t fM = (size_t)(nf * M) * 8, M8 = (size_t)M * 8;
+    if (!rc) rc = e->a.upload(interleave(pa.data(), nf, M).data(), fM, h->stream);
+    if (!rc) rc = e->b.upload(interleave(pb.data(), nf, M).data(), fM, h->stream);
+    if (!rc) rc = e->L.upload(interleave(pL.data(), nf, M).data(), fM, h->stream);
+    if (!rc) rc = e->sigma.upload(interleave(ps.data(), nf, M).data(), fM, h->stream);
+    if (!rc) rc = e->eta.upload(eta.data(), M8, h->stream);
+    if (!rc) rc = e->vpl.upload(vpl.data(), M8, h->stream);
+    if (!rc) rc = e->f0.upload(f0.data(), M8, h->stream);
+    if (!rc) rc = e->v0.upload(v0.data(), M8, h->stream);
+    if (!rc) rc = e->S.alloc((size_t)(h->ld * e->Mp) * 4);
+    if (!rc && hipMemsetAsync(e->S.p, 0, (size_t)(h->ld * e->Mp) * 4, h->stream) != hipSuccess) rc = OQ_ERR_HIP;
+    const int64_t nM = e->n * M;
+    if (!rc) rc = e->u.alloc((size_t)nM * 8);
+    if (!rc) rc = e->y.alloc((size_t)nM * 8);
+    if (!rc) rc = e->k.alloc((size_t)(7 * nM) * 8);
+    if (!rc) rc = e->part.alloc((size_t)(kEnsChunks * M) * 8);
+    DevBuf *cb8[] = {&e->ct, &e->cdt, &e->cdtcur, &e->cqold, &e->cna, &e->cnr, &e->citer};
+    for (DevBuf *bp : cb8)
+        if (!rc) rc = bp->alloc(M8);
+    DevBuf *cb4[] = {&e->cdone, &e->cacc, &e->cret};
+    for (DevBuf *bp : cb4)
+        if (!rc) rc = bp->alloc((size_t)M * 4);
+    if (!rc && hipStreamSynchronize(h->stream) != hipSuccess) rc = OQ_ERR_HIP;
+    if (rc) {
+        ens_destroy(e);
+        return rc;
+    }
+    EnsDev &d = e->dev;
+    d.nf = nf; d.ne = h->ne; d.N = h->N; d.ld = h->ld; d.M = M; d.Mp = e->Mp; d.n = e->n;
+    d.lay = h->lay;
+    d.a = e->a.as<double>(); d.b = e->b.as<double>(); d.L = e->L.as<double>(); d.sigma = e->sigma.as<double>();
+    d.eta = e->eta.as<double>(); d.vpl = e->vpl.as<double>(); d.f0 = e->f0.as<double>(); d.v0 = e->v0.as<double>();
+    d.gamma = h->dev.gamma; d.nexp = h->dev.nexp; d.nlaws = h->dev.nlaws;
+    for (int k = 0; k < 6; ++k) d.deps0[k] = h->dev.deps0[k];
+    EnsCtrl &c = e->ctrl;
+    c.t = e->ct.as<double>(); c.dt = e->cdt.as<double>(); c.dtcur = e->cdtcur.as<double>();
+    c.qold = e->cqold.as<double>(); c.naccept = e->cna.as<long long>(); c.nreject = e->cnr.as<long long>();
+    c.iter = e->citer.as<long long>(); c.done = e->cdone.as<int>(); c.accept = e->cacc.as<int>();
+    c.retcode = e->cret.as<int>(); c.M = (int)M;
+    *out = e;
+    return OQ_OK;
+}
+
+int oq_ensemble_eval(oq_ensemble *e, const double *u, double *du)
+{
+    OQ_REQUIRE(e && u && du, "NULL argument");
+    oq_rhs *h = e->h;
+    OQ_HIP(hipSetDevice(h->device));
+    const int64_t nM = e->n * e->M;
+    std::vector<const double *> src(e->M);
+    for (int64_t m = 0; m < e->M; ++m) src[m] = u + m * e->n;
+    const std::vector<double> ui = interleave(src.data(), e->n, e->M);
+    OQ_HIP(hipMemcpyAsync(e->y.p, ui.data(), (size_t)nM * 8, hipMemcpyHostToDevice, h->stream));
+    int rc = ens_rhs(e, e->y.as<double>(), e->k.as<double>());
+    if (rc) return rc;
+    std::vector<double> o((size_t)nM);
+    OQ_HIP(hipMemcpyAsync(o.data(), e->k.p, (size_t)nM * 8, hipMemcpyDeviceToHost, h->stream));
+    OQ_HIP(hipStreamSynchronize(h->stream));
+    for (int64_t m = 0; m < e->M; ++m)
+        for (int64_t i = 0; i < e->n; ++i) du[m * e->n + i] = o[(size_t)(i * e->M + m)];
+    return OQ_OK;
+}
+
+int oq_ensemble_start(oq_ensemble *e, const double *u0, double t0, double t1,
+                      const oq_solver_opts *o)
+{
+    OQ_REQUIRE(e && u0 && o, "NULL argument");
+    OQ_REQUIRE(t1 > t0, "tspan must satisfy t1 > t0");
+    OQ_REQUIRE(o->dt0 > 0.0, "the ensemble stepper needs an initial step dt0 > 0");
+    oq_rhs *h = e->h;
+    OQ_HIP(hipSetDevice(h->device));
+    const int64_t M = e->M, nM = e->n * M;
+    std::vector<const double *> src(M);
+    for (int64_t m = 0; m < M; ++m) src[m] = u0 + m * e->n;
+    const std::vector<double> ui = interleave(src.data(), e->n, M);
+    OQ_HIP(hipMemcpyAsync(e->u.p, ui.data(), (size_t)nM * 8, hipMemcpyHostToDevice, h->stream));
+    std::vector<double> tv(M, t0), dtv(M, o->dt0), qv(M, 1e-4);
+    std::vector<long long> zl(M, 0);
+    std::vector<int> zi(M, 0);
+    OQ_HIP(hipMemcpyAsync(e->ct.p, tv.data(), M * 8, hipMemcpyHostToDevice, h->stream));
+    OQ_HIP(hipMemcpyAsync(e->cdt.p, dtv.data(), M * 8, hipMemcpyHostToDevice, h->stream));
+    OQ_HIP(hipMemcpyAsync(e->cqold.p, qv.data(), M * 8, hipMemcpyHostToDevice, h->stream));
+    OQ_HIP(hipMemcpyAsync(e->cna.p, zl.data(), M * 8, hipMemcpyHostToDevice, h->stream));
+    OQ_HIP(hipMemcpyAsync(e->cnr.p, zl.data(), M * 8, hipMemcpyHostToDevice, h->stream));
+    OQ_HIP(hipMemcpyAsync(e->citer.p, zl.data(), M * 8, hipMemcpyHostToDevice, h->stream));
+    OQ_HIP(hipMemcpyAsync(e->cdone.p, zi.data(), M * 4, hipMemcpyHostToDevice, h->stream));
+    OQ_HIP(hipMemcpyAsync(e->cacc.p, zi.data(), M * 4, hipMemcpyHostToDevice, h->stream));
+    OQ_HIP(hipMemcpyAsync(e->cret.p, zi.data(), M * 4, hipMemcpyHostToDevice, h->stream));
+    EnsCtrl &c = e->ctrl;
+    c.t1 = t1;
+    c.reltol = o->reltol;
+    c.abstol = o->abstol;
+    c.dtmax = o->dtmax > 0.0 ? o->dtmax : INFINITY;
+    c.dtmin = o->dtmin;
+    c.maxiters = o->maxiters > 0 ? o->maxiters : (long long)1e18;
+    // FSAL seed: k1 = f(u0)
+    int rc = ens_rhs(e, e->u.as<double>(), e->k.as<double>());
+    if (rc) return rc;
+    if (e->exec) { (void)hipGraphExecDestroy(e->exec); e->exec = nullptr; }
+    if (e->graph) { (void)hipGraphDestroy(e->graph); e->graph = nullptr; }
+    OQ_HIP(hipStreamSynchronize(h->stream));
+    OQ_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    rc = ens_step(e);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(h->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    OQ_HIP(ce);
+    e->graph = g;
+    OQ_HIP(hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0));
+    e->stepping = true;
+    return OQ_OK;
+}
+
+int oq_ensemble_run(oq_ensemble *e, int64_t max_steps, int64_t *nrunning)
+{
+    OQ_REQUIRE(e && max_steps > 0, "invalid argument");
+    OQ_REQUIRE(e->stepping, "call oq_ensemble_start first");
+    oq_rhs *h = e->h;
+    OQ_HIP(hipSetDevice(h->device));
+    std::vector<int> done(e->M);
+    int64_t left = e->M;
+    for (int64_t s = 0; s < max_steps; s += 8) {
+        const int64_t m = (max_steps - s) < 8 ? (max_steps - s) : 8;
+        for (int64_t i = 0; i < m; ++i) OQ_HIP(hipGraphLaunch(e->exec, h->stream));
+        OQ_HIP(hipMemcpyAsync(done.data(), e->cdone.p, e->M * 4, hipMemcpyDeviceToHost, h->stream));
+        OQ_HIP(hipStreamSynchronize(h->stream));
+        left = 0;
+        for (int v : done) left += v ? 0 : 1;
+        if (left == 0) break;
+    }
+    if (nrunning) *nrunning = left;
+    return OQ_OK;
+}
+
+int oq_ensemble_state(oq_ensemble *e, double *t, double *u, int64_t *naccept, int64_t *nreject,
+                      int *retcode)
+{
+    OQ_REQUIRE(e != nullptr, "NULL argument");
+    oq_rhs *h = e->h;
+    OQ_HIP(hipSetDevice(h->device));
+    const int64_t M = e->M, nM = e->n * M;
+    if (t) OQ_HIP(hipMemcpyAsync(t, e->ct.p, M * 8, hipMemcpyDeviceToHost, h->stream));
+    if (naccept) OQ_HIP(hipMemcpyAsync(naccept, e->cna.p, M * 8, hipMemcpyDeviceToHost, h->stream));
+    if (nreject) OQ_HIP(hipMemcpyAsync(nreject, e->cnr.p, M * 8, hipMemcpyDeviceToHost, h->stream));
+    if (retcode) OQ_HIP(hipMemcpyAsync(retcode, e->cret.p, M * 4, hipMemcpyDeviceToHost, h->stream));
+    std::vector<double> o;
+    if (u) {
+        o.resize((size_t)nM);
+        OQ_HIP(hipMemcpyAsync(o.data(), e->u.p, (size_t)nM * 8, hipMemcpyDeviceToHost, h->stream));
+    }
+    OQ_HIP(hipStreamSynchronize(h->stream));
+    if (u)
+        for (int64_t m = 0; m < M; ++m)
+            for (int64_t i = 0; i < e->n; ++i) u[m * e->n + i] = o[(size_t)(i * M + m)];
+    return OQ_OK;
+}
+
+int oq_ensemble_time(oq_ensemble *e, int iters, double *ms_eval)
+{
+    OQ_REQUIRE(e && iters > 0, "invalid argument");
+    oq_rhs *h = e->h;
+    OQ_HIP(hipSetDevice(h->device));
+    int rc;
+    OQ_HIP(hipEventRecord(h->ev0, h->stream));
+    for (int i = 0; i < iters; ++i)
+        if ((rc = ens_rhs(e, e->y.as<double>(), e->k.as<double>()))) return rc;
+    OQ_HIP(hipEventRecord(h->ev1, h->stream));
+    OQ_HIP(hipEventSynchronize(h->ev1));
+    float ms = 0.f;
+    OQ_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    if (ms_eval) *ms_eval = (double)ms / iters;
+    return OQ_OK;
+}
+
+int oq_ensemble_destroy(oq_ensemble *e)
+{
+    if (e) {
+        (void)hipSetDevice(e->h ? e->h->device : 0);
+        ens_destroy(e);
+    }
+    return OQ_OK;
+}
+
+}  // extern "C"

@@ -5,6 +5,7 @@ generation, stress Green's functions, properties, `assemble` and a Tsit5
 `solve`, all backed by the HIP library lib/liboetqf_amd.so through its C ABI.
 """
 from ._lib import OetqfError, device_count, load as load_library  # noqa: F401
+from .ensemble import Ensemble  # noqa: F401
 from .equation import (ArrayPartition, Integrator, ODEProblem, ODESolution, Tsit5,  # noqa: F401
                        assemble, assemble_from_mesh, similar, solve)
 from .greens import (DipSlip, StrikeSlip, get_quadrature, gmsh_quadrature,  # noqa: F401

@@ -86,6 +86,17 @@ PROTOTYPES = {
     "oq_stepper_state": (_int, [_vp, ctypes.POINTER(_d), ctypes.POINTER(_d), _pd,
                                 ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
     "oq_stepper_destroy": (_int, [_vp]),
+    "oq_ensemble_create": (_int, [_int, ctypes.POINTER(FaultMesh), ctypes.POINTER(Hex8Mesh), _d, _d,
+                                  _int, _i64, _d, ctypes.POINTER(Quadrature), _i64,
+                                  ctypes.POINTER(RSProps), ctypes.POINTER(ViscProps), _int,
+                                  ctypes.POINTER(_vp)]),
+    "oq_ensemble_eval": (_int, [_vp, _pd, _pd]),
+    "oq_ensemble_start": (_int, [_vp, _pd, _d, _d, ctypes.POINTER(SolverOpts)]),
+    "oq_ensemble_run": (_int, [_vp, _i64, ctypes.POINTER(_i64)]),
+    "oq_ensemble_state": (_int, [_vp, _pd, _pd, ctypes.POINTER(_i64), ctypes.POINTER(_i64),
+                                 ctypes.POINTER(_int)]),
+    "oq_ensemble_time": (_int, [_vp, _int, ctypes.POINTER(_d)]),
+    "oq_ensemble_destroy": (_int, [_vp]),
 }
 
 

@@ -76,3 +76,33 @@ def transform_fault_composite(nx=128, nxi=32, mx=16, my=16, mz=8, seed=20261202,
                 % (nx, nxi, nx * nxi, mx, my, mz, ma.ne),
                 mf=mf, ma=ma, lam=lam, mu=mu, pf=pf, pa=pa, u0=u0, tspan=(0.0, 500 * YEAR),
                 nrept=2, buffer_ratio=0.0, reltol=1e-6, abstol=1e-8)
+
+
+def ensemble_members(w, M=256, seed=20261203):
+    """M frictional realizations on workload w's geometry: in the
+    seismogenic zone (where the base b exceeds a) a - b ~ U[-0.006, -0.002];
+    L ~ logU[4, 40] mm; sigma ~ U[30, 80] MPa (per-member scalars); initial
+    V = vpl exp(0.1 N(0,1)) per patch, theta = L/V.  Returns (members, U0)
+    with U0 (M, nstate) packed ArrayPartition states."""
+    rng = np.random.default_rng(seed)
+    base = w["pf"]
+    shp = base.a.shape
+    vw = base.b > base.a
+    members, states = [], []
+    for _ in range(M):
+        amb = rng.uniform(-0.006, -0.002)
+        L = np.exp(rng.uniform(np.log(4e-3), np.log(4e-2)))
+        sg = rng.uniform(30e6, 80e6)
+        b = base.b.copy()
+        b[vw] = base.a[vw] - amb
+        p = RateStateQuasiDynamicProperty(base.a.copy(), b, np.full(shp, L), np.full(shp, sg),
+                                          base.η, base.vpl, base.f0, base.v0)
+        v0 = p.vpl * np.exp(0.1 * rng.standard_normal(shp))
+        parts = [v0, p.L / v0]
+        if w["ma"] is not None:
+            ne = w["ma"].ne
+            parts += [np.zeros((ne, 6)), np.zeros((ne, 6))]
+        parts.append(np.zeros(shp))
+        members.append(p)
+        states.append(ArrayPartition(*parts).pack())
+    return members, np.stack(states)

@@ -13,7 +13,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_symbols():
     src = open(os.path.join(ROOT, "include", "oetqf_amd.h")).read()
-    return sorted(set(re.findall(r"\b(oq_[a-z0-9_]+)\s*\(", src)))
+    # function declarations only: "<type> oq_name(" at the start of a line
+    return sorted(set(re.findall(r"^(?:int|void|const char \*)\s*(oq_[a-z0-9_]+)\s*\(", src, re.M)))
 
 
 def test_library_exports_every_declared_symbol(oq):
