@@ -81,6 +81,8 @@ def main():
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--ensemble", type=int, default=256,
                     help="members of the FP32 MFMA ensemble leg (0 = skip)")
+    ap.add_argument("--no-cycle512", dest="cycle512", action="store_false",
+                    help="skip the 300-year run of the 512-patch fault")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (known traffic) for counter scaling")
     args = ap.parse_args()
@@ -200,6 +202,27 @@ def main():
                      note="interseismic start of the cycle; Tsit5 on device, FP64 K")
         log("Tsit5: %.3f yr in %.1f s (%d+%d steps)" % (t / YEAR, el, na, nr))
 
+    cycle512 = None
+    if args.cycle512:
+        # the reference's CPU-runnable case: 300 years of the 512-patch fault
+        from oetqf_amd.cycle import events, run_cycle
+        from oetqf_amd.workloads import strike_slip_512
+        w0 = strike_slip_512()
+        p0 = oq.assemble_from_mesh(w0["mf"], None, w0["lam"], w0["mu"], w0["pf"], None, w0["u0"],
+                                   w0["tspan"])
+        r0 = run_cycle(p0, reltol=w0["reltol"], abstol=w0["abstol"],
+                       progress=lambda t: None)
+        ev = events(r0["t"], r0["vmax"], 1e-3, merge_gap=365 * 86400.0)
+        yrs = (w0["tspan"][1] - w0["tspan"][0]) / YEAR
+        cycle512 = dict(workload=w0["name"], sim_years=yrs, wall_s=r0["wall"],
+                        sim_yr_per_hr=yrs / (r0["wall"] / 3600.0), accepted=int(r0["naccept"]),
+                        rejected=int(r0["nreject"]), events=len(ev), retcode=r0["retcode"],
+                        event_def="max v > 1e-3 m/s, crossings within 1 yr merged",
+                        reltol=w0["reltol"], abstol=w0["abstol"])
+        log("512-patch cycle: %.0f yr in %.1f s (%.0f yr/hr), %d events" %
+            (yrs, r0["wall"], cycle512["sim_yr_per_hr"], len(ev)))
+        p0.f.close()
+
     cpu = None
     if not args.no_cpu:
         log("CPU baseline (oracle, reference algorithm)")
@@ -236,6 +259,7 @@ def main():
         "variants": variants,
         "ensemble": ensemble,
         "cycle": cycle,
+        "cycle_512": cycle512,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -205,6 +205,20 @@ int oq_stepper_run(oq_stepper *s, int64_t max_steps, int *retcode);
 /* Current state: t, dt, u (packed, host), step counters. */
 int oq_stepper_state(oq_stepper *s, double *t, double *dt, double *u,
                      int64_t *naccept, int64_t *nreject);
+/* Device recorder behind wsolve (src/io.jl:51-82): after every accepted
+ * step the time and max|v| over the fault go to a step ring of
+ * step_capacity entries; when the accepted-step count is a multiple of
+ * `stride` the whole packed state goes to a snapshot ring of snap_capacity
+ * states (the caller saves the initial state itself, as wsolve's callback
+ * does at t0).  Rings must be drained (oq_stepper_drain) before they fill:
+ * running at most min(step_capacity, snap_capacity) attempted steps between
+ * drains guarantees it. */
+int oq_stepper_record(oq_stepper *s, int64_t stride, int64_t snap_capacity,
+                      int64_t step_capacity);
+/* Copies out and clears the rings: *nsteps step records (t, max|v|) and
+ * *nsnaps snapshots (t, packed u).  Buffers must hold the capacities. */
+int oq_stepper_drain(oq_stepper *s, int64_t *nsteps, double *step_t, double *step_vmax,
+                     int64_t *nsnaps, double *snap_t, double *snap_u);
 int oq_stepper_destroy(oq_stepper *s);
 
 /* ---- ensembles of frictional-parameter realizations sharing one

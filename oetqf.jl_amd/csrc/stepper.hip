@@ -168,6 +168,51 @@ __global__ void __launch_bounds__(256) k_commit(const Ctrl *__restrict__ c, int6
     k[i] = k[6 * n + i];
 }
 
+// Recorder (the device side of wsolve, io.jl:51-66): after an accepted
+// step, the step counter advances, (t, max v) goes to a per-step ring, and
+// every `stride`-th accepted step the state goes to a snapshot ring.
+struct Recorder {
+    long long *nstep, *nsnap;  // totals since the last drain
+    double *step_t, *step_vmax;
+    double *snap_t, *snap_u;
+    long long step_cap, snap_cap, stride, stride_count;
+    int64_t n, nf;
+};
+
+__global__ void __launch_bounds__(256) k_record(const Ctrl *__restrict__ c, Recorder r,
+                                                const double *__restrict__ u)
+{
+    __shared__ double red[256];
+    __shared__ int take;
+    if (!c->accept) return;
+    double m = 0.0;
+    for (int64_t i = threadIdx.x; i < r.nf; i += blockDim.x) m = fmax(m, fabs(u[i]));
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const long long k = *r.nstep;
+        if (r.step_cap > 0) {
+            r.step_t[k % r.step_cap] = c->t;
+            r.step_vmax[k % r.step_cap] = red[0];
+        }
+        *r.nstep = k + 1;
+        take = (r.snap_cap > 0) && (c->naccept % r.stride == 0);
+    }
+    __syncthreads();
+    if (!take) return;
+    const long long js = *r.nsnap % r.snap_cap;
+    for (int64_t i = threadIdx.x; i < r.n; i += blockDim.x) r.snap_u[js * r.n + i] = u[i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        r.snap_t[js] = c->t;
+        *r.nsnap = *r.nsnap + 1;
+    }
+}
+
 }  // namespace oq
 
 struct oq_stepper {
@@ -177,6 +222,10 @@ struct oq_stepper {
     oq::Ctrl *ctrl_host = nullptr;  // pinned
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // recorder (disabled until oq_stepper_record)
+    oq::DevBuf rcnt, rstep_t, rstep_v, rsnap_t, rsnap_u;
+    oq::Recorder rec{};
+    bool recording = false;
 };
 
 namespace oq {
@@ -202,7 +251,29 @@ static int enqueue_step(oq_stepper *st)
                        st->part.as<double>());
     hipLaunchKernelGGL(k_controller, dim3(1), dim3(256), 0, h->stream, c, n, st->part.as<double>());
     hipLaunchKernelGGL(k_commit, dim3(nb(n)), dim3(256), 0, h->stream, c, n, u, y, k);
+    if (st->recording)
+        hipLaunchKernelGGL(k_record, dim3(1), dim3(256), 0, h->stream, c, st->rec, u);
     OQ_HIP(hipGetLastError());
+    return OQ_OK;
+}
+
+static int capture_step(oq_stepper *st)
+{
+    oq_rhs *h = st->h;
+    if (st->exec) { (void)hipGraphExecDestroy(st->exec); st->exec = nullptr; }
+    if (st->graph) { (void)hipGraphDestroy(st->graph); st->graph = nullptr; }
+    OQ_HIP(hipStreamSynchronize(h->stream));
+    OQ_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_step(st);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(h->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    OQ_HIP(ce);
+    st->graph = g;
+    OQ_HIP(hipGraphInstantiate(&st->exec, g, nullptr, nullptr, 0));
     return OQ_OK;
 }
 
@@ -318,21 +389,10 @@ int oq_stepper_create(oq_rhs *h, const double *u0, double t0, double t1,
         return OQ_ERR_HIP;
     }
     // Capture one attempted step into a graph.
-    if (hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-        set_error("hipStreamBeginCapture failed");
+    if ((rc = capture_step(st))) {
         destroy_stepper(st);
-        return OQ_ERR_HIP;
+        return rc;
     }
-    rc = enqueue_step(st);
-    hipGraph_t g = nullptr;
-    const hipError_t ce = hipStreamEndCapture(h->stream, &g);
-    if (rc || ce != hipSuccess || hipGraphInstantiate(&st->exec, g, nullptr, nullptr, 0) != hipSuccess) {
-        if (g) (void)hipGraphDestroy(g);
-        if (!rc) set_error("graph capture/instantiate failed");
-        destroy_stepper(st);
-        return rc ? rc : OQ_ERR_HIP;
-    }
-    st->graph = g;
     *out = st;
     return OQ_OK;
 }
@@ -372,6 +432,68 @@ int oq_stepper_state(oq_stepper *st, double *t, double *dt, double *u, int64_t *
     if (dt) *dt = st->ctrl_host->dt;
     if (naccept) *naccept = st->ctrl_host->naccept;
     if (nreject) *nreject = st->ctrl_host->nreject;
+    return OQ_OK;
+}
+
+int oq_stepper_record(oq_stepper *st, int64_t stride, int64_t snap_capacity,
+                      int64_t step_capacity)
+{
+    OQ_REQUIRE(st != nullptr, "stepper is NULL");
+    OQ_REQUIRE(stride >= 1 && snap_capacity >= 0 && step_capacity >= 0,
+               "stride >= 1 and non-negative capacities required");
+    oq_rhs *h = st->h;
+    OQ_HIP(hipSetDevice(h->device));
+    int rc;
+    if ((rc = st->rcnt.alloc(2 * sizeof(long long)))) return rc;
+    if ((rc = st->rstep_t.alloc((size_t)step_capacity * 8))) return rc;
+    if ((rc = st->rstep_v.alloc((size_t)step_capacity * 8))) return rc;
+    if ((rc = st->rsnap_t.alloc((size_t)snap_capacity * 8))) return rc;
+    if ((rc = st->rsnap_u.alloc((size_t)(snap_capacity * st->n) * 8))) return rc;
+    const long long zero[2] = {0, 0};
+    OQ_HIP(hipMemcpyAsync(st->rcnt.p, zero, sizeof(zero), hipMemcpyHostToDevice, h->stream));
+    Recorder &r = st->rec;
+    r.nstep = st->rcnt.as<long long>();
+    r.nsnap = st->rcnt.as<long long>() + 1;
+    r.step_t = st->rstep_t.as<double>();
+    r.step_vmax = st->rstep_v.as<double>();
+    r.snap_t = st->rsnap_t.as<double>();
+    r.snap_u = st->rsnap_u.as<double>();
+    r.step_cap = step_capacity;
+    r.snap_cap = snap_capacity;
+    r.stride = stride;
+    r.n = st->n;
+    r.nf = h->nf;
+    st->recording = true;
+    return capture_step(st);
+}
+
+int oq_stepper_drain(oq_stepper *st, int64_t *nsteps, double *step_t, double *step_vmax,
+                     int64_t *nsnaps, double *snap_t, double *snap_u)
+{
+    OQ_REQUIRE(st != nullptr, "stepper is NULL");
+    OQ_REQUIRE(st->recording, "recording is not enabled (oq_stepper_record)");
+    oq_rhs *h = st->h;
+    OQ_HIP(hipSetDevice(h->device));
+    long long cnt[2];
+    OQ_HIP(hipMemcpyAsync(cnt, st->rcnt.p, sizeof(cnt), hipMemcpyDeviceToHost, h->stream));
+    OQ_HIP(hipStreamSynchronize(h->stream));
+    const Recorder &r = st->rec;
+    OQ_REQUIRE(cnt[0] <= r.step_cap || r.step_cap == 0, "step record overflow: drain more often");
+    OQ_REQUIRE(cnt[1] <= r.snap_cap || r.snap_cap == 0, "snapshot overflow: drain more often");
+    if (nsteps) *nsteps = cnt[0];
+    if (nsnaps) *nsnaps = cnt[1];
+    if (step_t && cnt[0] > 0 && r.step_cap > 0)
+        OQ_HIP(hipMemcpyAsync(step_t, r.step_t, (size_t)cnt[0] * 8, hipMemcpyDeviceToHost, h->stream));
+    if (step_vmax && cnt[0] > 0 && r.step_cap > 0)
+        OQ_HIP(hipMemcpyAsync(step_vmax, r.step_vmax, (size_t)cnt[0] * 8, hipMemcpyDeviceToHost, h->stream));
+    if (snap_t && cnt[1] > 0)
+        OQ_HIP(hipMemcpyAsync(snap_t, r.snap_t, (size_t)cnt[1] * 8, hipMemcpyDeviceToHost, h->stream));
+    if (snap_u && cnt[1] > 0)
+        OQ_HIP(hipMemcpyAsync(snap_u, r.snap_u, (size_t)(cnt[1] * st->n) * 8, hipMemcpyDeviceToHost,
+                              h->stream));
+    const long long zero[2] = {0, 0};
+    OQ_HIP(hipMemcpyAsync(st->rcnt.p, zero, sizeof(zero), hipMemcpyHostToDevice, h->stream));
+    OQ_HIP(hipStreamSynchronize(h->stream));
     return OQ_OK;
 }
 

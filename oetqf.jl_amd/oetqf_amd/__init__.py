@@ -8,6 +8,7 @@ from ._lib import OetqfError, device_count, load as load_library  # noqa: F401
 from .ensemble import Ensemble  # noqa: F401
 from .equation import (ArrayPartition, Integrator, ODEProblem, ODESolution, Tsit5,  # noqa: F401
                        assemble, assemble_from_mesh, similar, solve)
+from .io import wread, wsolve  # noqa: F401
 from .greens import (DipSlip, StrikeSlip, get_quadrature, gmsh_quadrature,  # noqa: F401
                      stress_greens_function)
 from .mesh import (BEMHex8Mesh, RectOkadaMesh, cosd, gen_gmsh_mesh, gen_mesh,  # noqa: F401

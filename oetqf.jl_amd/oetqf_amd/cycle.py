@@ -1,0 +1,78 @@
+"""Earthquake-cycle runs on the device and their event catalogue.
+
+`run_cycle` drives the on-device Tsit5 through the recorder (one (t, max v)
+record per accepted step); `events` turns that record into an earthquake
+catalogue: an event starts when the largest slip rate on the fault crosses
+`vthresh` upward and ends when it falls back below it.
+"""
+import time
+
+import numpy as np
+
+from .equation import Integrator
+
+
+def run_cycle(prob, reltol=1e-6, abstol=1e-8, dt=0.0, dtmax=0.0, chunk=4096, wall_limit=None,
+              progress=None):
+    """Integrates prob over its tspan on the device.  Returns a dict with the
+    per-step record (t, vmax), the final state and counters."""
+    it = Integrator(prob, reltol=reltol, abstol=abstol, dt=dt, dtmax=dtmax)
+    it.record(stride=1, snap_capacity=0, step_capacity=chunk)
+    ts, vs = [], []
+    t0 = time.perf_counter()
+    ret = "Running"
+    while ret == "Running":
+        ret = it.step(chunk)
+        st, sv, _, _ = it.drain()
+        ts.append(st)
+        vs.append(sv)
+        if progress is not None:
+            progress(st[-1] if len(st) else None)
+        if wall_limit is not None and time.perf_counter() - t0 > wall_limit:
+            break
+    wall = time.perf_counter() - t0
+    t, dtl, u, na, nr = it.state()
+    it.close()
+    return dict(t=np.concatenate(ts), vmax=np.concatenate(vs), retcode=ret, t_end=t, u=u,
+                naccept=na, nreject=nr, wall=wall)
+
+
+def events(step_t, step_vmax, vthresh=1e-3, merge_gap=0.0):
+    """Catalogue of events from a (t, max v) record: onset time (linear
+    interpolation of log v at the upward crossing), peak slip rate and the
+    time of the peak.  Crossings that start less than `merge_gap` seconds
+    after the previous event ended belong to that event (a rupture that
+    flickers around the threshold is one event)."""
+    v = np.asarray(step_vmax)
+    t = np.asarray(step_t)
+    above = v >= vthresh
+    out = []
+    i = 1
+    n = len(v)
+    while i < n:
+        if above[i] and not above[i - 1]:
+            la, lb = np.log(v[i - 1]), np.log(v[i])
+            f = (np.log(vthresh) - la) / (lb - la) if lb != la else 0.0
+            onset = t[i - 1] + f * (t[i] - t[i - 1])
+            j = i
+            while j < n and above[j]:
+                j += 1
+            k = i + int(np.argmax(v[i:j]))
+            end = t[j - 1]
+            if out and onset - out[-1]["end"] < merge_gap:
+                ev = out[-1]
+                if v[k] > ev["peak_v"]:
+                    ev["peak_v"], ev["t_peak"] = float(v[k]), float(t[k])
+                ev["end"] = float(end)
+            else:
+                out.append(dict(onset=float(onset), peak_v=float(v[k]), t_peak=float(t[k]),
+                                end=float(end)))
+            i = j
+        else:
+            i += 1
+    return out
+
+
+def recurrence(evs):
+    on = np.array([e["onset"] for e in evs])
+    return np.diff(on)

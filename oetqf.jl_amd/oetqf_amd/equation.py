@@ -243,6 +243,26 @@ class Integrator:
                                                 ctypes.byref(nr)))
         return t.value, dt.value, self.u0.similar().unpack(flat), na.value, nr.value
 
+    def record(self, stride=1, snap_capacity=0, step_capacity=4096):
+        """Enables the device recorder: per accepted step (t, max|v|), and a
+        state snapshot every `stride` accepted steps."""
+        _lib.check(_lib.load().oq_stepper_record(self.s, int(stride), int(snap_capacity),
+                                                 int(step_capacity)))
+        self._caps = (int(snap_capacity), int(step_capacity))
+
+    def drain(self):
+        """Returns (step_t, step_vmax, snap_t, snaps (k, nstate)) recorded
+        since the last drain."""
+        sc, kc = self._caps
+        st_t, st_v = np.zeros(max(kc, 1)), np.zeros(max(kc, 1))
+        sn_t, sn_u = np.zeros(max(sc, 1)), np.zeros(max(sc, 1) * self._u0flat.size)
+        ns, nn = ctypes.c_int64(), ctypes.c_int64()
+        _lib.check(_lib.load().oq_stepper_drain(self.s, ctypes.byref(ns), _lib.ptr(st_t), _lib.ptr(st_v),
+                                                ctypes.byref(nn), _lib.ptr(sn_t), _lib.ptr(sn_u)))
+        k, j = ns.value, nn.value
+        return st_t[:k].copy(), st_v[:k].copy(), sn_t[:j].copy(), \
+            sn_u[:j * self._u0flat.size].reshape(j, self._u0flat.size).copy()
+
     def close(self):
         if self.s:
             _lib.load().oq_stepper_destroy(self.s)

@@ -149,7 +149,18 @@ typedef void (*oq_rhs_fn)(void *ctx, double t, const double *u, double *du);
 /* Integrates u (length n) from t0 to t1 in place. */
 void oq_tsit5(oq_rhs_fn f, void *ctx, int64_t n, double *u, double t0,
               double t1, const oq_ode_opts_t *o, oq_ode_stats_t *st);
+void oq_tsit5_record(oq_rhs_fn f, void *ctx, int64_t n, double *u, double t0,
+                     double t1, const oq_ode_opts_t *o, oq_ode_stats_t *st,
+                     int64_t nrec_n, double *step_t, double *step_v, int64_t cap,
+                     int64_t *nrec);
 const double *oq_tsit5_tableau(void); /* c[7], a[7][7], b[7], btilde[7] */
+
+/* A whole earthquake cycle in C: Tsit5 over oq_rhs_dense (packed state
+ * [v θ δ] or [v θ ϵ σ δ]), recording (t, max v) per accepted step. */
+void oq_cycle_dense(const oq_rs_t *pf, const oq_visc_t *pa, const double *K,
+                    int64_t nf, int64_t ne, double *u, double t0, double t1,
+                    const oq_ode_opts_t *o, oq_ode_stats_t *st, double *step_t,
+                    double *step_v, int64_t cap, int64_t *nrec);
 
 #ifdef __cplusplus
 }

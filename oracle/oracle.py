@@ -374,3 +374,26 @@ def tsit5_tableau():
     p = lib().oq_tsit5_tableau()
     a = np.ctypeslib.as_array(p, shape=(70,)).copy()
     return a[:7], a[7:56].reshape(7, 7), a[56:63], a[63:70]
+
+
+def cycle_dense(pf, pa, K, nf, ne, u0, t0, t1, reltol=1e-6, abstol=1e-8, dtmax=np.inf, dtmin=0.0,
+                dt0=0.0, maxiters=10**9, cap=2_000_000):
+    """A whole cycle in C (oq_cycle_dense): Tsit5 over the dense RHS; returns
+    (u, stats, step_t, step_vmax) for every accepted step."""
+    keep = _Keep()
+    cf = _rs(pf, keep)
+    ca = _visc(pa, keep) if ne > 0 else ViscC()
+    u = np.array(u0, dtype=np.float64, copy=True)
+    st_t, st_v = np.zeros(cap), np.zeros(cap)
+    nrec = ctypes.c_int64()
+    o = OdeOpts(reltol, abstol, dtmax, dtmin, dt0, maxiters)
+    st = OdeStats()
+    L = lib()
+    L.oq_cycle_dense.argtypes = [ctypes.POINTER(RSC), ctypes.POINTER(ViscC), _pd, _i64, _i64, _pd,
+                                 _d, _d, ctypes.POINTER(OdeOpts), ctypes.POINTER(OdeStats), _pd, _pd,
+                                 _i64, ctypes.POINTER(_i64)]
+    Kc = keep.arr(np.asarray(K).reshape(-1))
+    L.oq_cycle_dense(ctypes.byref(cf), ctypes.byref(ca), _p(Kc), nf, ne, _p(u), t0, t1,
+                     ctypes.byref(o), ctypes.byref(st), _p(st_t), _p(st_v), cap, ctypes.byref(nrec))
+    k = min(nrec.value, cap)
+    return u, st, st_t[:k].copy(), st_v[:k].copy()

@@ -247,3 +247,34 @@ void oq_rhs_dense(const oq_rs_t *pf, const oq_visc_t *pa, const double *K,
     free(s);
     free(y);
 }
+
+/* Adapter: the dense RHS on a packed ArrayPartition state. */
+typedef struct {
+    const oq_rs_t *pf;
+    const oq_visc_t *pa;
+    const double *K;
+    int64_t nf, ne;
+} dense_ctx_t;
+
+static void dense_packed(void *vctx, double t, const double *u, double *du)
+{
+    const dense_ctx_t *c = (const dense_ctx_t *)vctx;
+    const int64_t nf = c->nf, nm = 6 * c->ne;
+    (void)t;
+    if (c->ne > 0)
+        oq_rhs_dense(c->pf, c->pa, c->K, nf, c->ne, u, u + nf, u + 2 * nf + nm, du, du + nf,
+                     du + 2 * nf, du + 2 * nf + nm, du + 2 * nf + 2 * nm);
+    else
+        oq_rhs_dense(c->pf, c->pa, c->K, nf, 0, u, u + nf, NULL, du, du + nf, NULL, NULL,
+                     du + 2 * nf);
+}
+
+void oq_cycle_dense(const oq_rs_t *pf, const oq_visc_t *pa, const double *K,
+                    int64_t nf, int64_t ne, double *u, double t0, double t1,
+                    const oq_ode_opts_t *o, oq_ode_stats_t *st, double *step_t,
+                    double *step_v, int64_t cap, int64_t *nrec)
+{
+    dense_ctx_t c = {pf, pa, K, nf, ne};
+    const int64_t n = ne > 0 ? 3 * nf + 12 * ne : 3 * nf;
+    oq_tsit5_record(dense_packed, &c, n, u, t0, t1, o, st, nf, step_t, step_v, cap, nrec);
+}

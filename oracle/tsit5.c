@@ -87,6 +87,17 @@ static double initial_dt(oq_rhs_fn f, void *ctx, int64_t n, const double *u0,
 void oq_tsit5(oq_rhs_fn f, void *ctx, int64_t n, double *u, double t0,
               double t1, const oq_ode_opts_t *o, oq_ode_stats_t *st)
 {
+    oq_tsit5_record(f, ctx, n, u, t0, t1, o, st, 0, NULL, NULL, 0, NULL);
+}
+
+/* Tsit5 that also records, after each accepted step, t and max|u[0:nrec_n]|
+ * (the largest slip rate when u starts with v) into step_t / step_v (up to
+ * cap entries; *nrec gets the number of accepted steps). */
+void oq_tsit5_record(oq_rhs_fn f, void *ctx, int64_t n, double *u, double t0,
+                     double t1, const oq_ode_opts_t *o, oq_ode_stats_t *st,
+                     int64_t nrec_n, double *step_t, double *step_v, int64_t cap,
+                     int64_t *nrec)
+{
     const double beta1 = 7.0 / 50.0, beta2 = 2.0 / 25.0, qmin = 0.2,
                  qmax = 10.0, gamma = 0.9, qoldinit = 1e-4;
     double *k = (double *)malloc(sizeof(double) * (size_t)(7 * n));
@@ -150,6 +161,12 @@ void oq_tsit5(oq_rhs_fn f, void *ctx, int64_t n, double *u, double t0,
             if (q >= 1.0 && q <= 1.0) q = 1.0;
             qold = fmax(eest, qoldinit);
             dt = dt / q;
+            if (nrec_n > 0 && st->naccept < cap && step_t && step_v) {
+                double vm = 0.0;
+                for (i = 0; i < nrec_n; ++i) vm = fmax(vm, fabs(u[i]));
+                step_t[st->naccept] = t;
+                step_v[st->naccept] = vm;
+            }
             st->naccept += 1;
         } else {
             if (!isfinite(eest)) { st->retcode = 3; st->nreject += 1; break; }
@@ -159,6 +176,7 @@ void oq_tsit5(oq_rhs_fn f, void *ctx, int64_t n, double *u, double t0,
     }
     st->t = t;
     st->dt = dt;
+    if (nrec) *nrec = st->naccept;
     free(k);
     free(y);
     free(err);
