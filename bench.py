@@ -182,6 +182,26 @@ def main():
         log("ensemble: %.3f ms per RHS of %d members -> %.0f member-RHS/s (%.1fx FP64 plain), "
             "%.1f TFLOP/s, err %s" % (ms_ens, M, member_rps, member_rps / value, tflops,
                                       ", ".join("%.1e" % e for e in errs)))
+        if args.cycle_seconds > 0:
+            # the lockstep stepper from the members' interseismic start, for a
+            # bounded wall-clock window
+            ens.start(U0, w["tspan"], reltol=w["reltol"], abstol=w["abstol"], dt0=1.0)
+            t0 = time.perf_counter()
+            iters = 0
+            while time.perf_counter() - t0 < args.cycle_seconds:
+                left = ens.run(32)
+                iters += 32
+                if left == 0:
+                    break
+            el = time.perf_counter() - t0
+            tm, _, na, nr, _ = ens.state()
+            ensemble["cycle"] = dict(
+                wall_s=el, lockstep_iterations=iters, mean_member_sim_years=float(tm.mean() / YEAR),
+                member_sim_yr_per_hr=float(tm.sum() / YEAR / (el / 3600.0)),
+                accepted_mean=float(na.mean()), rejection_rate=float(nr.sum() / (na.sum() + nr.sum())),
+                note="all members from their interseismic start, dt0 = 1 s, Tsit5 per member")
+            log("ensemble Tsit5: %d lockstep iterations in %.1f s, mean %.3f yr/member, "
+                "%.0f member-yr/hr" % (iters, el, tm.mean() / YEAR, ensemble["cycle"]["member_sim_yr_per_hr"]))
         ens.close()
 
     cycle = None
@@ -228,6 +248,10 @@ def main():
         log("CPU baseline (oracle, reference algorithm)")
         cpu, ref = cpu_baseline(w, u0, args.cpu_seconds)
         cpu["rhs_rel_err_gpu_vs_cpu"] = float(np.linalg.norm(du64.x[0] - ref[0]) / np.linalg.norm(ref[0]))
+
+    if ensemble and ensemble.get("cycle") and cycle:
+        ensemble["cycle"]["vs_fp64_plain_sim_yr_per_hr"] = (
+            ensemble["cycle"]["member_sim_yr_per_hr"] / cycle["sim_yr_per_hr"])
 
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "k_rhs_matvec_fp64_transform_fault_4096.traffic.txt")

@@ -248,6 +248,17 @@ int oq_ensemble_run(oq_ensemble *e, int64_t max_steps, int64_t *nrunning);
 /* Per-member t[M], u[M*nstate], step counters and OQ_RET_* codes (0 = running). */
 int oq_ensemble_state(oq_ensemble *e, double *t, double *u, int64_t *naccept,
                       int64_t *nreject, int *retcode);
+/* Per-member record of accepted steps, the ensemble form of the single-state
+ * recorder (the saving callback of wsolve, io.jl:51-58, reduced to (t,
+ * max |v|)): after each accepted step of member m, its new t and the
+ * largest slip rate of its new state go into a ring of step_capacity
+ * entries.  0 turns it off.  Takes effect at the next oq_ensemble_start;
+ * while on, oq_ensemble_run accepts at most step_capacity steps per call. */
+int oq_ensemble_record(oq_ensemble *e, int64_t step_capacity);
+/* Hands out the records made since the last drain: counts[m] entries of
+ * member m at step_t[m*cap + j], step_vmax[m*cap + j] (arrays of
+ * nmembers*step_capacity).  OQ_ERR_STATE if a ring was overrun. */
+int oq_ensemble_drain(oq_ensemble *e, int64_t *counts, double *step_t, double *step_vmax);
 /* Mean time of one ensemble RHS (all members) over `iters` evaluations. */
 int oq_ensemble_time(oq_ensemble *e, int iters, double *ms_eval);
 int oq_ensemble_destroy(oq_ensemble *e);

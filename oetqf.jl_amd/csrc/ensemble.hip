@@ -200,6 +200,10 @@ struct EnsCtrl {  // arrays of M
     double t1, reltol, abstol, dtmax, dtmin;
     long long maxiters;
     int M;
+    // accepted-step recorder (rec_cap == 0: off): per-chunk max|v| partials
+    // and per-member rings rec_t/rec_v [M x rec_cap], slot (naccept-1) % cap
+    long long nf, rec_cap;
+    double *vpart, *rec_t, *rec_v;
 };
 
 __global__ void k_ens_begin(EnsCtrl c)
@@ -250,16 +254,19 @@ __global__ void __launch_bounds__(256) k_ens_err(EnsCtrl c, int64_t n, int64_t M
     const int64_t nM = n * M;
     const int64_t r0 = n * chunk / kEnsChunks, r1 = n * (chunk + 1) / kEnsChunks;
     const double dt = c.dtcur[m];
-    double acc = 0.0;
+    double acc = 0.0, vm = 0.0;
     for (int64_t r = r0; r < r1; ++r) {
         const int64_t i = r * M + m;
         double e = 0.0;
 #pragma unroll
         for (int j = 0; j < 7; ++j) e += kEBt[j] * k[j * nM + i];
-        const double w = dt * e / (c.abstol + fmax(fabs(u[i]), fabs(y[i])) * c.reltol);
+        const double ay = fabs(y[i]);
+        const double w = dt * e / (c.abstol + fmax(fabs(u[i]), ay) * c.reltol);
         acc += w * w;
+        if (r < c.nf) vm = fmax(vm, ay);  // rows [0, nf) of the state are v
     }
     part[chunk * M + m] = acc;
+    if (c.rec_cap > 0) c.vpart[chunk * M + m] = vm;
 }
 
 __global__ void k_ens_controller(EnsCtrl c, int64_t n, const double *__restrict__ part)
@@ -287,6 +294,13 @@ __global__ void k_ens_controller(EnsCtrl c, int64_t n, const double *__restrict_
         c.dt[m] = dtc / q;
         c.naccept[m] += 1;
         c.accept[m] = 1;
+        if (c.rec_cap > 0) {
+            double vm = 0.0;
+            for (int ch = 0; ch < kEnsChunks; ++ch) vm = fmax(vm, c.vpart[ch * c.M + m]);
+            const long long slot = (long long)m * c.rec_cap + (c.naccept[m] - 1) % c.rec_cap;
+            c.rec_t[slot] = c.t[m];
+            c.rec_v[slot] = vm;
+        }
     } else {
         c.nreject[m] += 1;
         if (!isfinite(eest)) {
@@ -317,6 +331,9 @@ struct oq_ensemble {
     oq::DevBuf a, b, L, sigma, eta, vpl, f0, v0, S;
     oq::DevBuf u, y, k, part;
     oq::DevBuf ct, cdt, cdtcur, cqold, cna, cnr, citer, cdone, cacc, cret;
+    oq::DevBuf vpart, rec_t, rec_v;  // recorder (oq_ensemble_record)
+    int64_t rec_cap = 0;
+    std::vector<long long> rec_drained;  // records already handed out, per member
     oq::EnsDev dev{};
     oq::EnsCtrl ctrl{};
     bool stepping = false;
@@ -464,6 +481,7 @@ int oq_ensemble_create(int device, const oq_fault_mesh *mf, const oq_hex8_mesh *
     c.qold = e->cqold.as<double>(); c.naccept = e->cna.as<long long>(); c.nreject = e->cnr.as<long long>();
     c.iter = e->citer.as<long long>(); c.done = e->cdone.as<int>(); c.accept = e->cacc.as<int>();
     c.retcode = e->cret.as<int>(); c.M = (int)M;
+    c.nf = nf; c.rec_cap = 0; c.vpart = c.rec_t = c.rec_v = nullptr;
     *out = e;
     return OQ_OK;
 }
@@ -520,6 +538,11 @@ int oq_ensemble_start(oq_ensemble *e, const double *u0, double t0, double t1,
     c.dtmax = o->dtmax > 0.0 ? o->dtmax : INFINITY;
     c.dtmin = o->dtmin;
     c.maxiters = o->maxiters > 0 ? o->maxiters : (long long)1e18;
+    c.rec_cap = e->rec_cap;
+    c.vpart = e->rec_cap ? e->vpart.as<double>() : nullptr;
+    c.rec_t = e->rec_cap ? e->rec_t.as<double>() : nullptr;
+    c.rec_v = e->rec_cap ? e->rec_v.as<double>() : nullptr;
+    e->rec_drained.assign((size_t)M, 0);
     // FSAL seed: k1 = f(u0)
     int rc = ens_rhs(e, e->u.as<double>(), e->k.as<double>());
     if (rc) return rc;
@@ -545,6 +568,8 @@ int oq_ensemble_run(oq_ensemble *e, int64_t max_steps, int64_t *nrunning)
 {
     OQ_REQUIRE(e && max_steps > 0, "invalid argument");
     OQ_REQUIRE(e->stepping, "call oq_ensemble_start first");
+    OQ_REQUIRE(e->rec_cap == 0 || max_steps <= e->rec_cap,
+               "with the recorder on, max_steps must be <= step_capacity (one drain per run)");
     oq_rhs *h = e->h;
     OQ_HIP(hipSetDevice(h->device));
     std::vector<int> done(e->M);
@@ -559,6 +584,69 @@ int oq_ensemble_run(oq_ensemble *e, int64_t max_steps, int64_t *nrunning)
         if (left == 0) break;
     }
     if (nrunning) *nrunning = left;
+    return OQ_OK;
+}
+
+int oq_ensemble_record(oq_ensemble *e, int64_t step_capacity)
+{
+    OQ_REQUIRE(e != nullptr, "NULL argument");
+    OQ_REQUIRE(step_capacity >= 0, "step_capacity must be >= 0");
+    OQ_HIP(hipSetDevice(e->h->device));
+    if (e->rec_cap == step_capacity) return OQ_OK;
+    e->vpart.release();
+    e->rec_t.release();
+    e->rec_v.release();
+    e->rec_cap = 0;
+    if (step_capacity > 0) {
+        const size_t rb = (size_t)(e->M * step_capacity) * 8;
+        int rc = e->vpart.alloc((size_t)(kEnsChunks * e->M) * 8);
+        if (!rc) rc = e->rec_t.alloc(rb);
+        if (!rc) rc = e->rec_v.alloc(rb);
+        if (rc) {
+            e->vpart.release();
+            e->rec_t.release();
+            e->rec_v.release();
+            return rc;
+        }
+        e->rec_cap = step_capacity;
+    }
+    e->stepping = false;  // the captured step must be rebuilt with the recorder
+    return OQ_OK;
+}
+
+int oq_ensemble_drain(oq_ensemble *e, int64_t *counts, double *step_t, double *step_vmax)
+{
+    OQ_REQUIRE(e && counts && step_t && step_vmax, "NULL argument");
+    OQ_REQUIRE(e->rec_cap > 0, "the recorder is off (oq_ensemble_record)");
+    OQ_REQUIRE(e->stepping, "call oq_ensemble_start first");
+    oq_rhs *h = e->h;
+    OQ_HIP(hipSetDevice(h->device));
+    const int64_t M = e->M, cap = e->rec_cap;
+    std::vector<long long> na((size_t)M);
+    std::vector<double> rt((size_t)(M * cap)), rv((size_t)(M * cap));
+    OQ_HIP(hipMemcpyAsync(na.data(), e->cna.p, M * 8, hipMemcpyDeviceToHost, h->stream));
+    OQ_HIP(hipMemcpyAsync(rt.data(), e->rec_t.p, rt.size() * 8, hipMemcpyDeviceToHost, h->stream));
+    OQ_HIP(hipMemcpyAsync(rv.data(), e->rec_v.p, rv.size() * 8, hipMemcpyDeviceToHost, h->stream));
+    OQ_HIP(hipStreamSynchronize(h->stream));
+    int64_t lost = 0;
+    for (int64_t m = 0; m < M; ++m) {
+        const long long d0 = e->rec_drained[m], fresh = na[m] - d0;
+        const long long keep = fresh > cap ? cap : fresh;
+        if (fresh > cap) lost += fresh - cap;
+        for (long long j = 0; j < keep; ++j) {
+            const long long slot = m * cap + (na[m] - keep + j) % cap;
+            step_t[m * cap + j] = rt[(size_t)slot];
+            step_vmax[m * cap + j] = rv[(size_t)slot];
+        }
+        counts[m] = keep;
+        e->rec_drained[m] = na[m];
+    }
+    if (lost) {
+        set_error("step record overflow: " + std::to_string(lost) +
+                  " accepted steps were overwritten before this drain (run at most "
+                  "step_capacity steps between drains)");
+        return OQ_ERR_STATE;
+    }
     return OQ_OK;
 }
 

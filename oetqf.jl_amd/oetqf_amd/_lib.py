@@ -97,6 +97,8 @@ PROTOTYPES = {
     "oq_ensemble_run": (_int, [_vp, _i64, ctypes.POINTER(_i64)]),
     "oq_ensemble_state": (_int, [_vp, _pd, _pd, ctypes.POINTER(_i64), ctypes.POINTER(_i64),
                                  ctypes.POINTER(_int)]),
+    "oq_ensemble_record": (_int, [_vp, _i64]),
+    "oq_ensemble_drain": (_int, [_vp, ctypes.POINTER(_i64), _pd, _pd]),
     "oq_ensemble_time": (_int, [_vp, _int, ctypes.POINTER(_d)]),
     "oq_ensemble_destroy": (_int, [_vp]),
 }

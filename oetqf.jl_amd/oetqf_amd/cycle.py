@@ -76,3 +76,30 @@ def events(step_t, step_vmax, vthresh=1e-3, merge_gap=0.0):
 def recurrence(evs):
     on = np.array([e["onset"] for e in evs])
     return np.diff(on)
+
+
+def run_ensemble_cycle(ens, U0, tspan, reltol=1e-6, abstol=1e-8, dt0=1.0, dtmax=0.0, chunk=2048,
+                       wall_limit=None):
+    """Integrates all members of an Ensemble over tspan with the per-member
+    recorder on.  Returns per-member (t, vmax) records, final states and
+    counters."""
+    ens.record(chunk)
+    ens.start(U0, tspan, reltol=reltol, abstol=abstol, dt0=dt0, dtmax=dtmax)
+    ts = [[] for _ in range(ens.M)]
+    vs = [[] for _ in range(ens.M)]
+    t0 = time.perf_counter()
+    iters = 0
+    while True:
+        left = ens.run(chunk)
+        iters += chunk
+        for m, (t, v) in enumerate(ens.drain()):
+            ts[m].append(t)
+            vs[m].append(v)
+        if left == 0:
+            break
+        if wall_limit is not None and time.perf_counter() - t0 > wall_limit:
+            break
+    wall = time.perf_counter() - t0
+    t, U, na, nr, rc = ens.state()
+    return dict(t=[np.concatenate(x) for x in ts], vmax=[np.concatenate(x) for x in vs],
+                t_end=t, U=U, naccept=na, nreject=nr, retcode=rc, wall=wall, iters=iters)

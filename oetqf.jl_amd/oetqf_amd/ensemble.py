@@ -42,6 +42,7 @@ class Ensemble:
                                             _KDTYPES[kdtype], ctypes.byref(self.h)))
             self.ne = 0
         self.M = M
+        self._rec_cap = 0
         self.nf = mf.nx * mf.nxi
         self.n = 3 * self.nf + 12 * self.ne
 
@@ -59,6 +60,22 @@ class Ensemble:
         o = _lib.SolverOpts(reltol, abstol, dt0, dtmax, dtmin, int(maxiters))
         _lib.check(_lib.load().oq_ensemble_start(self.h, _lib.ptr(self._u0), float(tspan[0]),
                                                  float(tspan[1]), ctypes.byref(o)))
+
+    def record(self, step_capacity):
+        """Per-member (t, max|v|) record of accepted steps; takes effect at
+        the next start().  0 turns it off."""
+        _lib.check(_lib.load().oq_ensemble_record(self.h, int(step_capacity)))
+        self._rec_cap = int(step_capacity)
+
+    def drain(self):
+        """[(t, vmax)] per member: the accepted steps since the last drain."""
+        cap = self._rec_cap
+        cnt = np.zeros(self.M, dtype=np.int64)
+        t = np.zeros((self.M, cap))
+        v = np.zeros((self.M, cap))
+        _lib.check(_lib.load().oq_ensemble_drain(
+            self.h, cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), _lib.ptr(t), _lib.ptr(v)))
+        return [(t[m, :cnt[m]].copy(), v[m, :cnt[m]].copy()) for m in range(self.M)]
 
     def run(self, max_steps):
         n = ctypes.c_int64()
