@@ -70,6 +70,60 @@ def cpu_baseline(w, u0, sample_seconds):
                 sample="%d RHS evaluations of the same workload (%.1f s)" % (n, el)), out
 
 
+def scale_leg(oq, args):
+    """The precision variant at scale: 256x64 fault + 32x32x8 mantle
+    (N = 65536, K 34.4 GB in FP64).  Per storage type: device assembly time,
+    RHS rate and HBM rate, RHS error against FP64 on a perturbed state, and
+    a trajectory of `scale_years` from the interseismic start whose end
+    state (slip rate, slip) is compared with the FP64 trajectory."""
+    from oetqf_amd.workloads import YEAR, precision_scale_65536
+
+    w = precision_scale_65536()
+    mf, ma = w["mf"], w["ma"]
+    nf, ne = mf.nx * mf.nxi, ma.ne
+    rng = np.random.default_rng(12)
+    ug = w["u0"].similar()
+    ug.x[0][:] = w["pf"].vpl * np.exp(0.5 * rng.standard_normal(ug.x[0].shape))
+    ug.x[1][:] = w["pf"].L / ug.x[0]
+    ug.x[3][:] = 1e6 * rng.standard_normal(ug.x[3].shape)
+    span = (0.0, args.scale_years * YEAR)
+    out = dict(workload=w["name"], N=nf + 6 * ne, k_bytes_fp64=8 * (nf + 6 * ne) ** 2,
+               trajectory_years=args.scale_years, reltol=w["reltol"], abstol=w["abstol"])
+    ref = None
+    for kd, esz in (("fp64", 8), ("fp32", 4), ("bf16", 2)):
+        t0 = time.perf_counter()
+        p = oq.assemble_from_mesh(mf, ma, w["lam"], w["mu"], w["pf"], w["pa"], w["u0"], span,
+                                  nrept=w["nrept"], buffer_ratio=w["buffer_ratio"], kdtype=kd)
+        ta = time.perf_counter() - t0
+        du = oq.similar(ug)
+        p.f(du, ug, None, 0.0)
+        p.f.time(3)
+        me, mm = p.f.time(20)
+        it = oq.Integrator(p, reltol=w["reltol"], abstol=w["abstol"])
+        t0 = time.perf_counter()
+        while it.step(256) == "Running":
+            pass
+        tw = time.perf_counter() - t0
+        t, _, ue, na, nr = it.state()
+        it.close()
+        p.f.close()
+        r = dict(assembly_s=ta, ms_per_rhs=me, ms_matvec=mm, rhs_per_s=1000.0 / me,
+                 hbm_GBps=algorithmic_bytes(nf, ne, esz) / (mm * 1e-3) / 1e9,
+                 trajectory_wall_s=tw, accepted=na, rejected=nr)
+        if ref is None:
+            ref = (du, ue)
+        else:
+            rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+            r.update(rel_err_dv=rel(du.x[0], ref[0].x[0]), rel_err_dsigma=rel(du.x[3], ref[0].x[3]),
+                     traj_rel_err_v=rel(ue.x[0], ref[1].x[0]),
+                     traj_rel_err_slip=rel(ue.x[4], ref[1].x[4]))
+        out[kd] = r
+        log("N=65536 %s: assembly %.1f s, %.3f ms/RHS (%.0f GB/s), %.1f yr in %.1f s (%d+%d)%s" % (
+            kd, ta, me, r["hbm_GBps"], t / YEAR, tw, na, nr,
+            "" if kd == "fp64" else ", err dv %.1e traj v %.1e" % (r["rel_err_dv"], r["traj_rel_err_v"])))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +137,10 @@ def main():
                     help="members of the FP32 MFMA ensemble leg (0 = skip)")
     ap.add_argument("--no-cycle512", dest="cycle512", action="store_false",
                     help="skip the 300-year run of the 512-patch fault")
+    ap.add_argument("--no-scale", dest="scale", action="store_false",
+                    help="skip the N = 65536 precision-variant leg")
+    ap.add_argument("--scale-years", type=float, default=1.0,
+                    help="trajectory length of the N = 65536 leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (known traffic) for counter scaling")
     args = ap.parse_args()
@@ -243,6 +301,10 @@ def main():
             (yrs, r0["wall"], cycle512["sim_yr_per_hr"], len(ev)))
         p0.f.close()
 
+    scale = None
+    if args.scale:
+        scale = scale_leg(oq, args)
+
     cpu = None
     if not args.no_cpu:
         log("CPU baseline (oracle, reference algorithm)")
@@ -284,6 +346,7 @@ def main():
         "ensemble": ensemble,
         "cycle": cycle,
         "cycle_512": cycle512,
+        "scale_65536": scale,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -303,21 +303,26 @@ __global__ void __launch_bounds__(256) k_expand_toeplitz(const double *st, int64
                                                          int64_t nxi, double *k, int64_t ld)
 {
     const int64_t nf = nx * nxi;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nf * nf) return;
-    const int64_t c = t % nf, r = t / nf;
-    const int64_t i = r % nx, j = r / nx, kk = c % nx, l = c / nx;
-    const int64_t d = i > kk ? i - kk : kk - i;
-    k[r * ld + c] = st[d + nx * j + nx * nxi * l];
+    const int64_t step = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nf * nf; t += step) {
+        const int64_t c = t % nf, r = t / nf;
+        const int64_t i = r % nx, j = r / nx, kk = c % nx, l = c / nx;
+        const int64_t d = i > kk ? i - kk : kk - i;
+        k[r * ld + c] = st[d + nx * j + nx * nxi * l];
+    }
 }
 
 static inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+// one thread per pair: a dispatch counts at most 2^32 - 1 work-items
+static constexpr int64_t kMaxPairs = (int64_t)0xFFFFFFFFu - 256;
+#define OQ_PAIRS_FIT(n) OQ_REQUIRE((n) <= kMaxPairs, "too many source-receiver pairs for one launch")
 
 int launch_gf_ff(const DevFault &f, double lambda, double mu, int ftype, int nrept, double *st,
                  hipStream_t s)
 {
     const OkConst c = ok_const((lambda + mu) / (lambda + 2.0 * mu), f.sd, f.cd);
     const int64_t n = f.nx * f.nxi * f.nxi;
+    OQ_PAIRS_FIT(n);
     if (ftype == 0)
         hipLaunchKernelGGL(k_gf_ff<0>, dim3(nblocks(n, 256)), dim3(256), 0, s, f, c, lambda, mu,
                            nrept, st);
@@ -334,6 +339,7 @@ int launch_gf_mf(const DevFault &f, const DevHex8 &h, const double *qc, const do
 {
     const OkConst c = ok_const((lambda + mu) / (lambda + 2.0 * mu), f.sd, f.cd);
     const int64_t n = f.nx * f.nxi * h.ne;
+    OQ_PAIRS_FIT(n);
     if (ftype == 0)
         hipLaunchKernelGGL(k_gf_mf<0>, dim3(nblocks(n, 256)), dim3(256), 0, s, f, h, qc, qw, nq, c,
                            lambda, mu, nrept, out, rs, cs);
@@ -349,6 +355,7 @@ int launch_gf_fm(const DevHex8 &h, const DevFault &f, double lambda, double mu, 
 {
     const double nu = lambda / 2.0 / (lambda + mu);  // GF.jl:203
     const int64_t n = f.nx * f.nxi * h.ne;
+    OQ_PAIRS_FIT(n);
     hipLaunchKernelGGL(k_gf_fm, dim3(nblocks(n, 128)), dim3(128), 0, s, h, f, mu, nu, ftype, out,
                        rs, cs);
     OQ_HIP(hipGetLastError());
@@ -360,6 +367,7 @@ int launch_gf_mm(const DevHex8 &h, const double *qc, const double *qw, int nq, d
 {
     const double nu = lambda / 2.0 / (lambda + mu);  // GF.jl:259
     const int64_t n = h.ne * h.ne;
+    OQ_PAIRS_FIT(n);
     hipLaunchKernelGGL(k_gf_mm, dim3(nblocks(n, 128)), dim3(128), 0, s, h, qc, qw, nq, mu, nu, out,
                        rs, cs);
     OQ_HIP(hipGetLastError());
@@ -370,7 +378,7 @@ int launch_expand_toeplitz(const double *st, int64_t nx, int64_t nxi, double *k,
                            hipStream_t s)
 {
     const int64_t nf = nx * nxi;
-    hipLaunchKernelGGL(k_expand_toeplitz, dim3(nblocks(nf * nf, 256)), dim3(256), 0, s, st, nx,
+    hipLaunchKernelGGL(k_expand_toeplitz, dim3(nf * nf < (256 << 16) ? nblocks(nf * nf, 256) : 65536u), dim3(256), 0, s, st, nx,
                        nxi, k, ld);
     OQ_HIP(hipGetLastError());
     return OQ_OK;

@@ -293,40 +293,65 @@ int rhs_enqueue(oq_rhs *h, const double *u, double *du)
 
 // ---------------- operator assembly helpers ----------------
 
+// Element-wise kernels over operator-sized arrays (up to N*ld entries, more
+// than the 2^32 work-items one dispatch can count) use grid-stride loops over
+// a capped grid.
+static constexpr int64_t kStrideBlocks = 1 << 16;
+static inline unsigned grid_capped(int64_t n, int bs)
+{
+    const int64_t b = (n + bs - 1) / bs;
+    return (unsigned)(b < kStrideBlocks ? (b > 0 ? b : 1) : kStrideBlocks);
+}
+
 // dst[r*ld + c] = src[r*srs + c*scs] for an m x n block.
 __global__ void __launch_bounds__(256) k_copy_block(const double *__restrict__ src, int64_t m,
                                                     int64_t n, int64_t srs, int64_t scs,
                                                     double *__restrict__ dst, int64_t ld)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= m * n) return;
-    const int64_t c = t % n, r = t / n;
-    dst[r * ld + c] = src[r * srs + c * scs];
+    const int64_t step = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < m * n; t += step) {
+        const int64_t c = t % n, r = t / n;
+        dst[r * ld + c] = src[r * srs + c * scs];
+    }
 }
 
 __global__ void __launch_bounds__(256) k_to_fp32(const double *__restrict__ src, float *dst, int64_t n)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) dst[t] = (float)src[t];
+    const int64_t step = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += step)
+        dst[t] = (float)src[t];
 }
 
-// Round-to-nearest-even FP64 -> BF16 (via the exactly rounded FP32 value).
+// FP64 -> BF16, round to nearest even.  The FP64 value is first rounded to
+// FP32 precision by round-to-odd (truncate, then set the last bit when
+// inexact); rounding that 24-bit value to nearest-even at 8 bits gives the
+// correctly rounded result (round-to-odd to p >= q + 2 bits composes with
+// any rounding to q bits).
+__device__ inline unsigned short bf16_rne(double x)
+{
+    float f = (float)x;
+    if (fabs((double)f) > fabs(x)) f = __uint_as_float(__float_as_uint(f) - 1u);  // toward zero
+    unsigned int w = __float_as_uint(f);
+    if ((double)f != x) w |= 1u;
+    const unsigned int r = w + 0x7fffu + ((w >> 16) & 1u);
+    return (unsigned short)(r >> 16);
+}
+
 __global__ void __launch_bounds__(256) k_to_bf16(const double *__restrict__ src,
                                                  unsigned short *dst, int64_t n)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const unsigned int w = __float_as_uint((float)src[t]);
-    const unsigned int r = w + 0x7fffu + ((w >> 16) & 1u);
-    dst[t] = (unsigned short)(r >> 16);
+    const int64_t step = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += step)
+        dst[t] = bf16_rne(src[t]);
 }
 
 __global__ void __launch_bounds__(256) k_from_lowp(const void *src, int kt, double *dst, int64_t n)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    if (kt == OQ_K_FP32) dst[t] = (double)static_cast<const float *>(src)[t];
-    else dst[t] = (double)__uint_as_float(((unsigned int)static_cast<const unsigned short *>(src)[t]) << 16);
+    const int64_t step = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += step) {
+        if (kt == OQ_K_FP32) dst[t] = (double)static_cast<const float *>(src)[t];
+        else dst[t] = (double)__uint_as_float(((unsigned int)static_cast<const unsigned short *>(src)[t]) << 16);
+    }
 }
 
 static int alloc_common(oq_rhs *h, int device, int64_t nx, int64_t nxi, int64_t ne, int kdtype)
@@ -417,11 +442,11 @@ static int finalize_k(oq_rhs *h, DevBuf &k64)
     int rc;
     if (h->kdtype == OQ_K_FP32) {
         if ((rc = h->K.alloc((size_t)n * 4))) return rc;
-        hipLaunchKernelGGL(k_to_fp32, dim3(nblk(n, 256)), dim3(256), 0, h->stream, k64.as<double>(),
+        hipLaunchKernelGGL(k_to_fp32, dim3(grid_capped(n, 256)), dim3(256), 0, h->stream, k64.as<double>(),
                            h->K.as<float>(), n);
     } else {
         if ((rc = h->K.alloc((size_t)n * 2))) return rc;
-        hipLaunchKernelGGL(k_to_bf16, dim3(nblk(n, 256)), dim3(256), 0, h->stream, k64.as<double>(),
+        hipLaunchKernelGGL(k_to_bf16, dim3(grid_capped(n, 256)), dim3(256), 0, h->stream, k64.as<double>(),
                            h->K.as<unsigned short>(), n);
     }
     OQ_HIP(hipGetLastError());
@@ -496,7 +521,7 @@ int oq_rhs_create(int device, int64_t nx, int64_t nxi, int64_t ne, const oq_rs_p
         if (hipStreamSynchronize(h->stream) != hipSuccess) { rc = OQ_ERR_HIP; break; }
         rc = tmp.upload(b.src, (size_t)(b.m * b.n) * 8, h->stream);
         if (rc) break;
-        hipLaunchKernelGGL(k_copy_block, dim3(nblk(b.m * b.n, 256)), dim3(256), 0, h->stream,
+        hipLaunchKernelGGL(k_copy_block, dim3(grid_capped(b.m * b.n, 256)), dim3(256), 0, h->stream,
                            tmp.as<double>(), b.m, b.n, b.srs, b.scs,
                            k64.as<double>() + b.r0 * ld + b.c0, ld);
         if (hipGetLastError() != hipSuccess) {
@@ -636,7 +661,7 @@ int oq_rhs_get_k(oq_rhs *h, double *k)
     if (h->kdtype != OQ_K_FP64) {
         int rc = tmp.alloc((size_t)n * 8);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_from_lowp, dim3(nblk(n, 256)), dim3(256), 0, h->stream, h->K.p,
+        hipLaunchKernelGGL(k_from_lowp, dim3(grid_capped(n, 256)), dim3(256), 0, h->stream, h->K.p,
                            h->kdtype, tmp.as<double>(), n);
         OQ_HIP(hipGetLastError());
         src = tmp.as<double>();

@@ -106,3 +106,13 @@ def ensemble_members(w, M=256, seed=20261203):
         members.append(p)
         states.append(ArrayPartition(*parts).pack())
     return members, np.stack(states)
+
+
+def precision_scale_65536(seed=20261202):
+    """The composite model at scale: 256 x 64 = 16384 fault patches coupled
+    to a 32 x 32 x 8 = 8192-cell mantle (49152 strain dofs), N = 65536; the
+    dense operator is 34.4 GB in FP64.  Physics as transform_fault_composite;
+    200 years."""
+    w = transform_fault_composite(nx=256, nxi=64, mx=32, my=32, mz=8, seed=seed)
+    w["tspan"] = (0.0, 200 * YEAR)
+    return w

@@ -264,33 +264,43 @@ void oq_gf_fm(const oq_hex8_mesh_t *ma, const oq_fault_mesh_t *mf,
 /* K_mm = gf22 (GF.jl:250-296): 6ne x 6ne column-major; entry
  * [(k-1)ne + j, (p-1)ne + i] += w_q sigma_k at quadrature points of
  * receiver cell j. */
-void oq_gf_mm(const oq_hex8_mesh_t *ma, double lambda, double mu,
-              const double *qc, const double *qw, int64_t nq, int nquad,
-              double *st)
+void oq_gf_mm_rows(const oq_hex8_mesh_t *src, const oq_hex8_mesh_t *rcv,
+                   double lambda, double mu, const double *qc, const double *qw,
+                   int64_t nq, int nquad, double *st)
 {
-    const int64_t ne = ma->ne, n6 = 6 * ne;
+    /* GF.jl:250-296 with the receiver cells (quadrature-averaged rows)
+     * taken from `rcv` and the eigenstrain sources from `src`: entry
+     * [(k-1)nr + j, (p-1)ns + i] column-major, (6 nr) x (6 ns). */
+    const int64_t ns = src->ne, nr = rcv->ne, r6 = 6 * nr;
     const double nu = lambda / 2.0 / (lambda + mu);
     int64_t i;
-    memset(st, 0, sizeof(double) * (size_t)(n6 * n6));
+    memset(st, 0, sizeof(double) * (size_t)(r6 * 6 * ns));
 #pragma omp parallel for schedule(dynamic)
-    for (i = 0; i < ne; ++i) {
+    for (i = 0; i < ns; ++i) {
         int p, k;
         int64_t j, q;
         for (p = 0; p < 6; ++p) {
             double e[6] = {0, 0, 0, 0, 0, 0}, s[6];
-            double *col = st + n6 * (p * ne + i);
+            double *col = st + r6 * (p * ns + i);
             e[p] = 1.0;
-            for (j = 0; j < ne; ++j) {
+            for (j = 0; j < nr; ++j) {
                 for (q = 0; q < nq; ++q) {
-                    double rx = ma->cx[j] + qc[3 * q] * ma->dx[j] / 2.0;
-                    double ry = ma->cy[j] + qc[3 * q + 1] * ma->dy[j] / 2.0;
-                    double rz = ma->cz[j] + qc[3 * q + 2] * ma->dz[j] / 2.0;
-                    oq_stress_vol_hex8(rx, ry, rz, ma->qx[i], ma->qy[i],
-                                       ma->qz[i], ma->dx[i], ma->dy[i],
-                                       ma->dz[i], e, mu, nu, nquad, s);
-                    for (k = 0; k < 6; ++k) col[k * ne + j] += s[k] * qw[q];
+                    double rx = rcv->cx[j] + qc[3 * q] * rcv->dx[j] / 2.0;
+                    double ry = rcv->cy[j] + qc[3 * q + 1] * rcv->dy[j] / 2.0;
+                    double rz = rcv->cz[j] + qc[3 * q + 2] * rcv->dz[j] / 2.0;
+                    oq_stress_vol_hex8(rx, ry, rz, src->qx[i], src->qy[i],
+                                       src->qz[i], src->dx[i], src->dy[i],
+                                       src->dz[i], e, mu, nu, nquad, s);
+                    for (k = 0; k < 6; ++k) col[k * nr + j] += s[k] * qw[q];
                 }
             }
         }
     }
+}
+
+void oq_gf_mm(const oq_hex8_mesh_t *ma, double lambda, double mu,
+              const double *qc, const double *qw, int64_t nq, int nquad,
+              double *st)
+{
+    oq_gf_mm_rows(ma, ma, lambda, mu, qc, qw, nq, nquad, st);
 }

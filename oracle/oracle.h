@@ -85,6 +85,11 @@ void oq_gf_fm(const oq_hex8_mesh_t *ma, const oq_fault_mesh_t *mf,
 void oq_gf_mm(const oq_hex8_mesh_t *ma, double lambda, double mu,
               const double *qc, const double *qw, int64_t nq, int nquad,
               double *st);
+/* Rows of K_mm for receiver cells rcv over source cells src:
+ * (6 nr) x (6 ns) column-major; oq_gf_mm is rcv = src. */
+void oq_gf_mm_rows(const oq_hex8_mesh_t *src, const oq_hex8_mesh_t *rcv,
+                   double lambda, double mu, const double *qc, const double *qw,
+                   int64_t nq, int nquad, double *st);
 
 /* ---------------- RHS (equation.jl) ---------------- */
 typedef struct {

@@ -214,6 +214,24 @@ def gf_mm(ma, lam, mu, qc=(0.0, 0.0, 0.0), qw=(1.0,), nquad=6):
     return st.reshape(6 * ne, 6 * ne).T
 
 
+def gf_mm_rows(ma_src, ma_rcv, lam, mu, qc=(0.0, 0.0, 0.0), qw=(1.0,), nquad=6):
+    """Rows of K_mm (GF.jl:250-296) for the receiver cells ma_rcv over the
+    source cells ma_src: (6 nr) x (6 ns)."""
+    keep = _Keep()
+    a = hex8_struct(ma_src, keep)
+    b = hex8_struct(ma_rcv, keep)
+    qc = keep.arr(np.asarray(qc, dtype=np.float64).reshape(-1))
+    qw = keep.arr(np.asarray(qw, dtype=np.float64).reshape(-1))
+    ns, nr = len(ma_src.cx), len(ma_rcv.cx)
+    st = np.zeros(36 * ns * nr)
+    L = lib()
+    L.oq_gf_mm_rows.argtypes = [ctypes.POINTER(Hex8MeshC), ctypes.POINTER(Hex8MeshC), _d, _d, _pd,
+                                _pd, _i64, ctypes.c_int, _pd]
+    L.oq_gf_mm_rows(ctypes.byref(a), ctypes.byref(b), lam, mu, _p(qc), _p(qw), len(qw), nquad,
+                    _p(st))
+    return st.reshape(6 * ns, 6 * nr).T
+
+
 def _fortran_flat(a):
     return np.asarray(a, dtype=np.float64).reshape(-1, order="F")
 

@@ -105,6 +105,18 @@ def test_dense_operator_layout(oq, oracle, rng):
     assert np.array_equal(K[nf:, nf:], g22)
 
 
+def _round_nearest_even(x, p):
+    """x rounded to p significant bits, ties to even (normal range): with
+    |x| = m 2^e, m in [0.5, 1), the candidates are floor(m 2^p) and the next
+    integer, scaled back; the distances are exact in FP64."""
+    m, e = np.frexp(np.abs(x))
+    q = m * 2.0 ** p
+    lo = np.floor(q)
+    f = q - lo
+    up = (f > 0.5) | ((f == 0.5) & (np.fmod(lo, 2.0) == 1.0))
+    return np.sign(x) * np.ldexp((lo + up) / 2.0 ** p, e)
+
+
 @pytest.mark.parametrize("kdtype,tol,exact_tol", [("fp32", 1e-5, 1e-12), ("bf16", 5e-2, 1e-6)])
 def test_rhs_reduced_precision_operator(oq, oracle, rng, kdtype, tol, exact_tol):
     """K streamed as FP32/BF16 with FP64 row accumulation: the RHS error
@@ -122,6 +134,9 @@ def test_rhs_reduced_precision_operator(oq, oracle, rng, kdtype, tol, exact_tol)
     # exact check: the low-precision RHS equals the FP64 oracle applied to the
     # rounded operator, to FP64 accuracy
     K = plo.f.dense_operator()
+    K64 = p64.f.dense_operator()
+    # the stored operator is the FP64 one rounded to nearest even
+    assert np.array_equal(K, _round_nearest_even(K64, 24 if kdtype == "fp32" else 8))
     nf = mf.nx * mf.nxi
     v, th, eps, sig, dl = u
     ref = oracle.rhs_dense(pf, pa, K, nf, ma.ne, v.reshape(-1, order="F"), th.reshape(-1, order="F"),

@@ -1,0 +1,91 @@
+"""The precision variant at scale: a 256x64 = 16384-patch fault coupled to a
+32x32x8 = 8192-cell mantle (N = 65536; K is 34.4 GB in FP64).
+
+The oracle rebuilds 12 fault rows and 24 mantle rows of K from the meshes
+with its own Okada and hex8 code (Toeplitz K_ff, then K_fm, K_mf and K_mm
+rows) and evaluates those rows of the RHS.  The device FP64 RHS must agree
+to 1e-10, the Green's-function tolerance: the two operators are assembled
+independently.  FP32 and BF16 storage of the same operator are then
+measured against the device FP64 RHS over the whole state."""
+from types import SimpleNamespace
+
+import numpy as np
+import pytest
+
+import oetqf_amd as oq
+from helpers import relerr
+from oetqf_amd.workloads import precision_scale_65536
+
+pytestmark = pytest.mark.gpu
+
+
+def _state(w, rng):
+    pf, ma = w["pf"], w["ma"]
+    shp = pf.a.shape
+    v = pf.vpl * np.exp(2.0 * rng.standard_normal(shp))
+    th = pf.L / pf.vpl * np.exp(rng.standard_normal(shp))
+    sig = 1e6 * rng.standard_normal((ma.ne, 6))
+    return oq.ArrayPartition(v, th, np.zeros((ma.ne, 6)), sig, np.zeros(shp))
+
+
+@pytest.fixture(scope="module")
+def scale():
+    w = precision_scale_65536()
+    u = _state(w, np.random.default_rng(7))
+    prob = oq.assemble_from_mesh(w["mf"], w["ma"], w["lam"], w["mu"], w["pf"], w["pa"], w["u0"],
+                                 w["tspan"])
+    du = oq.similar(u)
+    prob.f(du, u, None, 0.0)
+    prob.f.close()
+    return w, u, du
+
+
+def test_scale_rows_match_oracle(oracle, scale):
+    w, u, du = scale
+    mf, ma, pf, pa, lam, mu = w["mf"], w["ma"], w["pf"], w["pa"], w["lam"], w["mu"]
+    print("N =", mf.nx * mf.nxi + 6 * ma.ne, flush=True)
+    s_f = u.x[0] - pf.vpl
+    de = oracle.update_strain_rate(pa, u.x[3])
+    s_m = (de - np.asarray(pa.deps0)[None, :]).reshape(-1, order="F")
+
+    # fault rows: patches I x J
+    I, J = np.array([0, 77, 128, 255]), np.array([0, 31, 63])
+    st = oracle.gf_ff(mf, lam, mu)
+    sub = SimpleNamespace(x=mf.x[I], ax=mf.ax[I], y=mf.y[J], z=mf.z[J], axi=mf.axi[J], nx=len(I),
+                          nxi=len(J), dx=mf.dx, dep=mf.dep, dip=mf.dip)
+    kfm = oracle.gf_fm(ma, sub, lam, mu)  # rows i + 4 j
+    dtau = np.zeros((len(I), len(J)))
+    for a, i in enumerate(I):
+        D = np.abs(i - np.arange(mf.nx))
+        for b, j in enumerate(J):
+            dtau[a, b] = np.sum(st[D, j, :] * s_f) + kfm[a + len(I) * b] @ s_m
+    ix = np.ix_(I, J)
+    psub = SimpleNamespace(a=pf.a[ix], b=pf.b[ix], L=pf.L[ix], sigma=pf.sigma[ix], eta=pf.eta,
+                           vpl=pf.vpl, f0=pf.f0, v0=pf.v0)
+    dv, _, _ = oracle.update_fault(psub, u.x[0][ix], u.x[1][ix], dtau)
+    e_dv = relerr(du.x[0][ix].reshape(-1, order="F"), dv)
+
+    # mantle rows: all six stress components of cells C
+    C = np.array([0, 1234, 4095, 8191])
+    msub = SimpleNamespace(**{k: getattr(ma, k)[C] for k in
+                              ("cx", "cy", "cz", "qx", "qy", "qz", "dx", "dy", "dz")})
+    kmf = oracle.gf_mf(mf, msub, lam, mu)
+    kmm = oracle.gf_mm_rows(ma, msub, lam, mu)
+    dsig = kmf @ s_f.reshape(-1, order="F") + kmm @ s_m  # rows k * 4 + c
+    e_ds = relerr(du.x[3][C, :].T.reshape(-1), dsig)
+    print("rows vs oracle: dv %.2e  dsigma %.2e" % (e_dv, e_ds), flush=True)
+    assert e_dv <= 1e-10 and e_ds <= 1e-10
+
+
+@pytest.mark.parametrize("kdtype,tol", [("fp32", 1e-5), ("bf16", 5e-2)])
+def test_scale_precision_variants(scale, kdtype, tol):
+    w, u, du = scale
+    prob = oq.assemble_from_mesh(w["mf"], w["ma"], w["lam"], w["mu"], w["pf"], w["pa"], w["u0"],
+                                 w["tspan"], kdtype=kdtype)
+    d2 = oq.similar(u)
+    prob.f(d2, u, None, 0.0)
+    prob.f.close()
+    e_dv, e_ds = relerr(d2.x[0], du.x[0]), relerr(d2.x[3], du.x[3])
+    print("%s vs FP64: dv %.2e  dsigma %.2e" % (kdtype, e_dv, e_ds), flush=True)
+    assert e_dv <= tol and e_ds <= tol
+    assert relerr(d2.x[1], du.x[1]) <= 1e-14
