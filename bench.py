@@ -124,6 +124,75 @@ def scale_leg(oq, args):
     return out
 
 
+def group_leg(oq, args):
+    """Uneven-size grouped batch: 24 faults of 1024-8192 patches (half with
+    a mantle of 512-4096 cells), 16 realizations each, FP32 operators."""
+    from oetqf_amd.workloads import YEAR, grouped_batch
+
+    ws = grouped_batch()
+    t0 = time.perf_counter()
+    grp = oq.Group([(w["mf"], w["ma"], w["pa"]) for w in ws], ws[0]["lam"], ws[0]["mu"],
+                   [w["members"] for w in ws])
+    t_asm = time.perf_counter() - t0
+    U0 = grp.pack([w["U0"] for w in ws])
+    dU = grp.eval(U0)
+    grp.time(3)
+    ms_default = grp.time(20)
+    choice = grp.autotune(5)
+    grp.eval(U0)
+    grp.time(3)
+    ms = grp.time(args.steps)
+    GM = grp.G * grp.M
+    out = dict(geometries=grp.G, members_per_geometry=grp.M, N_min=int(min(grp.N)),
+               N_max=int(max(grp.N)), k_bytes=grp.kbytes, assembly_s=t_asm,
+               ms_per_group_rhs=ms, ms_per_group_rhs_default_shape=ms_default,
+               member_rhs_per_s=GM * 1000.0 / ms, hbm_GBps=grp.kbytes / (ms * 1e-3) / 1e9,
+               hbm_frac=grp.kbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS,
+               tile_shape_by_bucket={"N<%d" % (2048 << b): c[0] for b, c in choice.items()},
+               autotune_ms={"N<%d" % (2048 << b): t for b, t in grp.tune_table.items()})
+    # error of the first member of the smallest and the largest geometry
+    # against its own FP64 RHS
+    Us, dUs = grp.unpack(U0), grp.unpack(dU)
+    errs = {}
+    for g in (int(np.argmin(grp.N)), int(np.argmax(grp.N))):
+        w = ws[g]
+        p = oq.assemble_from_mesh(w["mf"], w["ma"], w["lam"], w["mu"], w["members"][0], w["pa"],
+                                  w["u0"], w["tspan"])
+        u = w["u0"].similar().unpack(Us[g][0])
+        d = oq.similar(u)
+        p.f(d, u, None, 0.0)
+        p.f.close()
+        ref = d.pack()
+        errs[w["name"]] = float(np.linalg.norm(dUs[g][0] - ref) / np.linalg.norm(ref))
+    out["rel_err_vs_fp64"] = errs
+    log("group: %d geometries x %d members, K %.2f GB: %.3f ms per RHS (default shape %.3f ms), "
+        "%.0f member-RHS/s, %.0f GB/s; shapes %s" % (grp.G, grp.M, grp.kbytes / 1e9, ms, ms_default,
+                                                    out["member_rhs_per_s"], out["hbm_GBps"],
+                                                    out["tile_shape_by_bucket"]))
+    if args.cycle_seconds > 0:
+        grp.start(U0, (0.0, 300 * YEAR), dt0=1.0)
+        t0 = time.perf_counter()
+        iters = 0
+        while time.perf_counter() - t0 < args.cycle_seconds:
+            left = grp.run(32)
+            iters += 32
+            if left == 0:
+                break
+        el = time.perf_counter() - t0
+        tm, _, na, nr, _ = grp.state()
+        rej = nr / np.maximum(na + nr, 1)
+        out["stepping"] = dict(wall_s=el, lockstep_iterations=iters,
+                               mean_member_sim_years=float(tm.mean() / YEAR),
+                               member_sim_yr_per_hr=float(tm.sum() / YEAR / (el / 3600.0)),
+                               rejection_rate_mean=float(rej.mean()),
+                               rejection_rate_max=float(rej.max()),
+                               rejection_rate_overall=float(nr.sum() / (na.sum() + nr.sum())))
+        log("group Tsit5: %d iterations in %.1f s, mean %.2f yr/member, rejection rate mean %.3f "
+            "max %.3f" % (iters, el, tm.mean() / YEAR, rej.mean(), rej.max()))
+    grp.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,6 +208,8 @@ def main():
                     help="skip the 300-year run of the 512-patch fault")
     ap.add_argument("--no-scale", dest="scale", action="store_false",
                     help="skip the N = 65536 precision-variant leg")
+    ap.add_argument("--no-group", dest="group", action="store_false",
+                    help="skip the uneven-size grouped batch leg")
     ap.add_argument("--scale-years", type=float, default=1.0,
                     help="trajectory length of the N = 65536 leg")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -305,6 +376,10 @@ def main():
     if args.scale:
         scale = scale_leg(oq, args)
 
+    group = None
+    if args.group:
+        group = group_leg(oq, args)
+
     cpu = None
     if not args.no_cpu:
         log("CPU baseline (oracle, reference algorithm)")
@@ -347,6 +422,7 @@ def main():
         "cycle": cycle,
         "cycle_512": cycle512,
         "scale_65536": scale,
+        "grouped_batch": group,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -263,6 +263,51 @@ int oq_ensemble_drain(oq_ensemble *e, int64_t *counts, double *step_t, double *s
 int oq_ensemble_time(oq_ensemble *e, int iters, double *ms_eval);
 int oq_ensemble_destroy(oq_ensemble *e);
 
+/* ---- grouped batches: geometries of uneven size, up to 16 frictional
+ * realizations each.  No entry point of the reference corresponds (one
+ * `assemble` + `solve` per geometry and realization, equation.jl:141-154).
+ * Each geometry's operator is FP32; one RHS is one grouped MFMA launch per
+ * tile shape in use over all geometries, and a lockstep Tsit5 advances all
+ * ngeo*nmembers members with per-member dt and acceptance.  Member states
+ * are packed ArrayPartitions, geometry after geometry, member after member
+ * (geometry g: nmembers states of nstate[g] entries). */
+typedef struct oq_group oq_group;
+
+/* mf[ngeo]; ma[ngeo] (NULL entries: fault only; ma itself may be NULL);
+ * visc[ngeo] (used where ma[g] is set); members[ngeo*nmembers], member m of
+ * geometry g at g*nmembers + m, its arrays of that geometry's nx*nxi
+ * patches.  nmembers in [1, 16]; kdtype OQ_K_FP32. */
+int oq_group_create(int device, int64_t ngeo, const oq_fault_mesh *mf,
+                    const oq_hex8_mesh *const *ma, const oq_visc_props *const *visc,
+                    double lambda, double mu, int ftype, int64_t nrept, double buffer_ratio,
+                    const oq_quadrature *quad, int64_t nmembers, const oq_rs_props *members,
+                    int kdtype, oq_group **out);
+/* Total state entries, per-geometry state size nstate[ngeo] and the bytes
+ * of all operators (any pointer may be NULL). */
+int oq_group_info(oq_group *e, int64_t *ntot, int64_t *nstate, int64_t *kbytes);
+/* Times every tile shape (6: rows per wavefront 16/32/64, each with and
+ * without a software-pipelined K stream) on the geometries of each size
+ * bucket (bucket b: N < 2048 << b; 8 buckets) and keeps the fastest;
+ * shape_of_bucket[8] and ms_of_bucket[8] receive the choice (-1 for an
+ * empty bucket), ms_table[8*6] (may be NULL) every timing.  Call before
+ * oq_group_start (a running stepper must be started again). */
+int oq_group_autotune(oq_group *e, int iters, int *shape_of_bucket, double *ms_of_bucket,
+                      double *ms_table);
+/* du = f(u) for all members (host arrays of ntot entries). */
+int oq_group_eval(oq_group *e, const double *u, double *du);
+/* Mean time of one grouped RHS (all geometries and members). */
+int oq_group_time(oq_group *e, int iters, double *ms_eval);
+/* Sets the initial states (opts->dt0 > 0 required) and captures one lockstep
+ * step into a HIP graph. */
+int oq_group_start(oq_group *e, const double *u0, double t0, double t1,
+                   const oq_solver_opts *opts);
+/* Advances up to max_steps lockstep attempts; *nrunning = members not done. */
+int oq_group_run(oq_group *e, int64_t max_steps, int64_t *nrunning);
+/* Per-member t, u (ntot), step counters and OQ_RET_* codes (ngeo*nmembers). */
+int oq_group_state(oq_group *e, double *t, double *u, int64_t *naccept, int64_t *nreject,
+                   int *retcode);
+int oq_group_destroy(oq_group *e);
+
 #ifdef __cplusplus
 }
 #endif

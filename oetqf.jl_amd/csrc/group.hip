@@ -1,0 +1,922 @@
+// group.hip -- grouped batches of geometries of uneven size, each with up to
+// 16 frictional realizations.
+//
+// The reference runs one `assemble` + `solve` per (geometry, realization)
+// (equation.jl:141-154).  Here G geometries with M <= 16 members each are
+// one object on the device:
+//
+//   - every RHS is ONE grouped launch over a tile table that spans all
+//     geometries: a 4-wavefront workgroup takes 64*RB consecutive receiver
+//     rows of one geometry and all M members, so each operator K_g (FP32) is
+//     streamed from HBM once per RHS.  Sixteen members fill the N dimension
+//     of v_mfma_f32_16x16x4f32; the reduction index is permuted so that each
+//     lane streams 16 consecutive columns of one row (64 contiguous bytes),
+//     and the source block S_g is stored member-major so that the matching
+//     B operands are contiguous too.  FP32 MFMA partials are folded into an
+//     FP32 (hi, lo) two-sum pair every 256 columns; the fused epilogue is the
+//     rate-and-state update (fault rows) or the stress-rate store (mantle
+//     rows), per (row, member);
+//   - the grid is persistent (at most the resident workgroups) and walks the
+//     tile table largest geometry first; the table is static, so every
+//     workgroup ends after its last tile;
+//   - tile shape (RB = rows per wavefront / 16, and whether the K stream is
+//     software-pipelined, which doubles the load registers and may halve the
+//     occupancy) is chosen per size bucket of N by timing every shape on the
+//     bucket's geometries (oq_group_autotune);
+//   - a lockstep Tsit5 advances all G*M members, each with its own dt, error
+//     norm, PI controller state and acceptance.
+//
+// Layout: per geometry g the member fields are entry-major, x[soff_g + i*M
+// + m]; geometries follow one another.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "rhs_impl.h"
+
+typedef float f4x __attribute__((ext_vector_type(4)));
+
+namespace oq {
+
+static constexpr int kGM = 16;       // member slots per geometry (MFMA N)
+static constexpr int kGStep = 64;    // columns per wavefront pass (4 lane groups x 16)
+static constexpr int kGWaves = 4;    // wavefronts per workgroup
+static constexpr int kGChunks = 64;  // row chunks of the error norm
+static constexpr int kGBuckets = 8;  // size buckets: N < 2048, < 4096, ... (log2)
+
+struct GrpGeo {
+    const float *K;
+    int64_t N, ld, nf, ne, n;
+    int64_t soff;  // state entries before this geometry
+    int64_t foff;  // fault-parameter entries before this geometry
+    int64_t Soff;  // S floats before this geometry (S[Soff + m*ld + c])
+    StateLayout lay;
+    const double *gamma, *nexp;
+    int nlaws;
+    double deps0[6];
+};
+
+struct GrpDev {
+    const GrpGeo *geo;
+    const int64_t *soffs;  // G+1 state offsets
+    const int64_t *poffs;  // G+1 prologue offsets ((nf+ne)*M per geometry)
+    int G, M;
+    const double *a, *b, *L, *sigma;    // per geometry nf x M, entry-major
+    const double *eta, *vpl, *f0, *v0;  // G*M
+    float *S;
+    int64_t nprolog, ntot;
+};
+
+// segment g with off[g] <= x < off[g+1] (off ascending, off[0] = 0)
+__device__ __forceinline__ int seg_find(const int64_t *__restrict__ off, int G, int64_t x)
+{
+    int lo = 0, hi = G - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// ---------------- prologue: S_g = [v - vpl ; eps_dot - deps0], member-major ----------------
+__global__ void __launch_bounds__(256) k_grp_prologue(GrpDev d, const double *__restrict__ u,
+                                                      double *__restrict__ du)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= d.nprolog) return;
+    const int g = seg_find(d.poffs, d.G, t);
+    const GrpGeo &G = d.geo[g];
+    const int64_t M = d.M, loc = t - d.poffs[g];
+    const int64_t i = loc / M, m = loc % M, gm = (int64_t)g * M + m;
+    const double *ug = u + G.soff;
+    float *S = d.S + G.Soff + m * G.ld;
+    if (i < G.nf) {
+        S[i] = (float)(ug[(G.lay.v + i) * M + m] - d.vpl[gm]);
+        return;
+    }
+    const int64_t c = i - G.nf, ne = G.ne;
+    const double *sg = ug + G.lay.sig * M;
+    const double sxx0 = sg[c * M + m], sxy = sg[(c + ne) * M + m], sxz = sg[(c + 2 * ne) * M + m];
+    const double syy0 = sg[(c + 3 * ne) * M + m], syz = sg[(c + 4 * ne) * M + m];
+    const double szz0 = sg[(c + 5 * ne) * M + m];
+    const double skk = (sxx0 + syy0 + szz0) / 3.0;
+    const double sxx = sxx0 - skk, syy = syy0 - skk, szz = szz0 - skk;
+    const double tn = sqrt(sxx * sxx + syy * syy + szz * szz + 2.0 * (sxy * sxy + sxz * sxz + syz * syz));
+    double f = 0.0;
+    for (int l = 0; l < G.nlaws; ++l) f += G.gamma[l * ne + c] * pow(tn, G.nexp[l * ne + c]);
+    const double e[6] = {f * sxx, f * sxy, f * sxz, f * syy, f * syz, f * szz};
+    double *dug = du + G.soff;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        dug[(G.lay.eps + c + k * ne) * M + m] = e[k];
+        S[G.nf + c + k * ne] = (float)(e[k] - G.deps0[k]);
+    }
+}
+
+__device__ __forceinline__ void grp_two_sum(float &hi, float &lo, float x)
+{
+    const float s = hi + x;
+    const float bp = s - hi;
+    const float err = (hi - (s - bp)) + (x - bp);
+    hi = s;
+    lo += err;
+}
+
+// ---------------- grouped GEMM + fused epilogue ----------------
+// tiles[t] = (geometry, first row); a workgroup tile is kGWaves * RB * 16 rows.
+template <int RB, bool PIPE>
+__global__ void __launch_bounds__(256) k_grp_gemm(GrpDev d, const int2 *__restrict__ tiles,
+                                                  int ntiles, const double *__restrict__ u,
+                                                  double *__restrict__ du)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lr = lane & 15, lq = lane >> 4;
+    for (int tix = blockIdx.x; tix < ntiles; tix += gridDim.x) {
+        const int2 tl = tiles[tix];
+        const GrpGeo &G = d.geo[tl.x];
+        const int64_t N = G.N, ld = G.ld;
+        const int64_t row0 = (int64_t)tl.y + (int64_t)wave * RB * 16;
+        if (row0 >= N) continue;
+        const float *kp[RB];
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) {
+            const int64_t r = row0 + rb * 16 + lr < N ? row0 + rb * 16 + lr : N - 1;
+            kp[rb] = G.K + r * ld + 4 * lq;
+        }
+        const float *sp = d.S + G.Soff + (int64_t)lr * ld + 4 * lq;
+        // NC independent accumulator chains per row block (RB * NC = 4), so
+        // that consecutive MFMAs do not wait on each other's result
+        constexpr int NC = 4 / RB;
+        f4x acc[RB][NC];
+        float hi[RB][4], lo[RB][4];
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) acc[rb][c] = (f4x){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int v = 0; v < 4; ++v) hi[rb][v] = lo[rb][v] = 0.f;
+        }
+        const int64_t nstep = ld / kGStep;
+        // start each wavefront's sweep at a different pass (row pitches are
+        // multiples of 2 KB, so aligned sweeps would load the same channels)
+        int64_t p = (((int64_t)tix * kGWaves + wave) * 13) % nstep;
+        // PIPE: software pipeline, the loads of the next pass are issued
+        // before the MFMAs of the current one (twice the load registers)
+        f4x sv[4], kv[RB][4];
+        if (PIPE) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) sv[t] = *reinterpret_cast<const f4x *>(sp + p * kGStep + 16 * t);
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+                for (int t = 0; t < 4; ++t) kv[rb][t] = *reinterpret_cast<const f4x *>(kp[rb] + p * kGStep + 16 * t);
+        }
+        for (int64_t it = 0; it < nstep; ++it) {
+            const int64_t pn = (p + 1 == nstep) ? 0 : p + 1;
+            const bool more = it + 1 < nstep;
+            if (!PIPE) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) sv[t] = *reinterpret_cast<const f4x *>(sp + p * kGStep + 16 * t);
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) kv[rb][t] = *reinterpret_cast<const f4x *>(kp[rb] + p * kGStep + 16 * t);
+            }
+            f4x sn[4], kn[RB][4];
+            if (PIPE && more) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) sn[t] = *reinterpret_cast<const f4x *>(sp + pn * kGStep + 16 * t);
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) kn[rb][t] = *reinterpret_cast<const f4x *>(kp[rb] + pn * kGStep + 16 * t);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            // lane (row lr, group lq) holds columns c0 + 16t + 4lq + e; MFMA
+            // (t, e) reduces over lq, the 16 MFMAs over all 64 columns
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+#pragma unroll
+                    for (int rb = 0; rb < RB; ++rb)
+                        acc[rb][t % NC] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                            kv[rb][t][e], sv[t][e], acc[rb][t % NC], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
+            if ((it & 3) == 3 || !more) {
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb) {
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) grp_two_sum(hi[rb][v], lo[rb][v], acc[rb][c][v]);
+                        acc[rb][c] = (f4x){0.f, 0.f, 0.f, 0.f};
+                    }
+                }
+            }
+            if (PIPE && more) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) sv[t] = sn[t];
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) kv[rb][t] = kn[rb][t];
+            }
+            p = pn;
+        }
+        // element v of row block rb: row row0 + 16 rb + 4 lq + v, member lr
+        const int64_t M = d.M;
+        if (lr >= M) continue;
+        const int64_t m = lr, gm = (int64_t)tl.x * M + m;
+        const double *ug = u + G.soff;
+        double *dug = du + G.soff;
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int64_t row = row0 + rb * 16 + 4 * lq + v;
+                if (row >= N) continue;
+                const double y = (double)hi[rb][v] + (double)lo[rb][v];
+                if (row < G.nf) {
+                    // update_fault! (equation.jl:237-245), member m's parameters
+                    const int64_t ix = G.foff + row * M + m;
+                    const double vv = ug[(G.lay.v + row) * M + m], th = ug[(G.lay.th + row) * M + m];
+                    const double a = d.a[ix], b = d.b[ix], L = d.L[ix];
+                    const double thp = th > 0.0 ? th : 0.0;
+                    const double psi1 = exp((d.f0[gm] + b * log(d.v0[gm] * thp / L)) / a) / (2.0 * d.v0[gm]);
+                    const double psi2 = d.sigma[ix] * psi1 / hypot(1.0, vv * psi1);
+                    const double dth = 1.0 - vv * th / L;
+                    dug[(G.lay.th + row) * M + m] = dth;
+                    dug[(G.lay.v + row) * M + m] = (y - b / th * vv * psi2 * dth) / (a * psi2 + d.eta[gm]);
+                    dug[(G.lay.del + row) * M + m] = vv;
+                } else {
+                    dug[(G.lay.sig + row - G.nf) * M + m] = y;
+                }
+            }
+        }
+    }
+}
+
+// ---------------- lockstep Tsit5 over all members ----------------
+__constant__ double kGA[7][6] = {
+    {0, 0, 0, 0, 0, 0},
+    {0.161, 0, 0, 0, 0, 0},
+    {-0.008480655492356989, 0.335480655492357, 0, 0, 0, 0},
+    {2.897153057105493, -6.359448489975075, 4.3622954328695815, 0, 0, 0},
+    {5.325864828439257, -11.748883564062828, 7.4955393428898365, -0.09249506636175525, 0, 0},
+    {5.86145544294642, -12.92096931784711, 8.159367898576159, -0.071584973281401,
+     -0.028269050394068383, 0},
+    {0.09646076681806523, 0.01, 0.4798896504144996, 1.379008574103742, -3.290069515436081,
+     2.324710524099774}};
+__constant__ double kGBt[7] = {-0.00178001105222577714, -0.0008164344596567469,
+                               0.007880878010261995, -0.1447110071732629, 0.5823571654525552,
+                               -0.45808210592918697, 0.015151515151515152};
+
+struct GrpCtrl {  // arrays of G*M
+    double *t, *dt, *dtcur, *qold;
+    long long *naccept, *nreject, *iter;
+    int *done, *accept, *retcode;
+    const int64_t *soffs;  // G+1
+    const int64_t *nst;    // G: state entries per member
+    double t1, reltol, abstol, dtmax, dtmin;
+    long long maxiters;
+    int G, M;
+};
+
+__global__ void k_grp_begin(GrpCtrl c)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= c.G * c.M) return;
+    c.accept[q] = 0;
+    c.dtcur[q] = 0.0;
+    if (c.done[q]) return;
+    if (!(c.t[q] < c.t1)) { c.done[q] = 1; c.retcode[q] = OQ_RET_SUCCESS; return; }
+    if (c.iter[q] >= c.maxiters) { c.done[q] = 1; c.retcode[q] = OQ_RET_MAXITERS; return; }
+    double dt = c.dt[q];
+    if (dt > c.dtmax) dt = c.dtmax;
+    if (c.t[q] + dt > c.t1) dt = c.t1 - c.t[q];
+    if (!(dt >= c.dtmin) || !isfinite(dt)) {
+        c.done[q] = 1;
+        c.retcode[q] = isfinite(dt) ? OQ_RET_DTMIN : OQ_RET_NONFINITE;
+        return;
+    }
+    c.dtcur[q] = dt;
+    c.iter[q] += 1;
+}
+
+__device__ __forceinline__ int64_t grp_member(const GrpCtrl &c, int64_t i)
+{
+    const int g = seg_find(c.soffs, c.G, i);
+    return (int64_t)g * c.M + (i - c.soffs[g]) % c.M;
+}
+
+__global__ void __launch_bounds__(256) k_grp_stage(GrpCtrl c, int s, int64_t ntot,
+                                                   const double *__restrict__ u,
+                                                   const double *__restrict__ k,
+                                                   double *__restrict__ y)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntot) return;
+    const double dt = c.dtcur[grp_member(c, i)];
+    double acc = 0.0;
+    for (int j = 0; j < s; ++j) acc += kGA[s][j] * k[j * ntot + i];
+    y[i] = u[i] + dt * acc;
+}
+
+__global__ void __launch_bounds__(256) k_grp_err(GrpCtrl c, int64_t ntot,
+                                                 const double *__restrict__ u,
+                                                 const double *__restrict__ y,
+                                                 const double *__restrict__ k,
+                                                 double *__restrict__ part)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int chunk = blockIdx.y;
+    const int64_t GM = (int64_t)c.G * c.M;
+    if (q >= GM) return;
+    const int64_t g = q / c.M, m = q % c.M, M = c.M, n = c.nst[g], base = c.soffs[g];
+    const int64_t r0 = n * chunk / kGChunks, r1 = n * (chunk + 1) / kGChunks;
+    const double dt = c.dtcur[q];
+    double acc = 0.0;
+    for (int64_t r = r0; r < r1; ++r) {
+        const int64_t i = base + r * M + m;
+        double e = 0.0;
+#pragma unroll
+        for (int j = 0; j < 7; ++j) e += kGBt[j] * k[j * ntot + i];
+        const double w = dt * e / (c.abstol + fmax(fabs(u[i]), fabs(y[i])) * c.reltol);
+        acc += w * w;
+    }
+    part[chunk * GM + q] = acc;
+}
+
+__global__ void k_grp_controller(GrpCtrl c, const double *__restrict__ part)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int GM = c.G * c.M;
+    if (q >= GM || c.done[q]) return;
+    double acc = 0.0;
+    for (int ch = 0; ch < kGChunks; ++ch) acc += part[ch * GM + q];
+    const double beta1 = 7.0 / 50.0, beta2 = 2.0 / 25.0, qmin = 0.2, qmax = 10.0, gamma = 0.9,
+                 qoldinit = 1e-4;
+    const double eest = sqrt(acc / (double)c.nst[q / c.M]);
+    double q11, qq;
+    if (eest == 0.0) {
+        q11 = 0.0;
+        qq = 1.0 / qmax;
+    } else {
+        q11 = pow(eest, beta1);
+        qq = q11 / pow(c.qold[q], beta2);
+        qq = fmax(1.0 / qmax, fmin(1.0 / qmin, qq / gamma));
+    }
+    const double dtc = c.dtcur[q];
+    if (eest <= 1.0) {
+        c.t[q] = (c.t1 - (c.t[q] + dtc) <= 0.0) ? c.t1 : c.t[q] + dtc;
+        c.qold[q] = fmax(eest, qoldinit);
+        c.dt[q] = dtc / qq;
+        c.naccept[q] += 1;
+        c.accept[q] = 1;
+    } else {
+        c.nreject[q] += 1;
+        if (!isfinite(eest)) {
+            c.done[q] = 1;
+            c.retcode[q] = OQ_RET_NONFINITE;
+            return;
+        }
+        c.dt[q] = dtc / fmin(1.0 / qmin, q11 / gamma);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_grp_commit(GrpCtrl c, int64_t ntot,
+                                                    double *__restrict__ u,
+                                                    const double *__restrict__ y,
+                                                    double *__restrict__ k)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntot || !c.accept[grp_member(c, i)]) return;
+    u[i] = y[i];
+    k[i] = k[6 * ntot + i];
+}
+
+// tile shapes: RB in {1, 2, 4} x {software-pipelined, plain}
+static constexpr int kGShapes = 6;
+static inline int shape_rb(int s) { return s / 2 == 0 ? 1 : (s / 2 == 1 ? 2 : 4); }
+
+}  // namespace oq
+
+struct oq_group {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int64_t G = 0, M = 0, ntot = 0, nprolog = 0;
+    std::vector<oq_rhs *> rhs;  // per geometry: K (FP32), viscosity arrays
+    std::vector<oq::GrpGeo> geo;
+    std::vector<int64_t> soffs, nst;
+    int bucket_shape[oq::kGBuckets];
+    std::vector<int> geo_bucket;
+    std::vector<std::vector<int2>> tiles_of_shape;  // host tables per shape
+    oq::DevBuf dgeo, dsoffs, dpoffs, dnst, a, b, L, sigma, eta, vpl, f0, v0, S;
+    oq::DevBuf dtiles[oq::kGShapes];
+    int ntiles[oq::kGShapes] = {0};
+    int grid[oq::kGShapes] = {0};
+    oq::DevBuf u, y, k, part;
+    oq::DevBuf ct, cdt, cdtcur, cqold, cna, cnr, citer, cdone, cacc, cret;
+    oq::GrpDev dev{};
+    oq::GrpCtrl ctrl{};
+    bool stepping = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace oq {
+
+static inline unsigned gbk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+static int bucket_of(int64_t N)
+{
+    int b = 0;
+    while (b + 1 < kGBuckets && N >= ((int64_t)2048 << b)) ++b;
+    return b;
+}
+
+template <int RB, bool PIPE>
+static int occupancy_grid(int *grid)
+{
+    int nb = 0, dev = 0;
+    OQ_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    OQ_HIP(hipGetDeviceProperties(&prop, dev));
+    OQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_grp_gemm<RB, PIPE>, 256, 0));
+    *grid = (nb > 0 ? nb : 1) * prop.multiProcessorCount;
+    return OQ_OK;
+}
+
+// host tile table of shape s over the geometries in `sel`, largest first
+static std::vector<int2> build_tiles(const oq_group *e, int s, const std::vector<int> &sel)
+{
+    const int rows = kGWaves * shape_rb(s) * 16;
+    std::vector<int> order(sel);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return e->geo[x].ld > e->geo[y].ld; });
+    std::vector<int2> t;
+    for (int g : order)
+        for (int64_t r = 0; r < e->geo[g].N; r += rows) t.push_back(make_int2(g, (int)r));
+    return t;
+}
+
+static int upload_tiles(oq_group *e)
+{
+    for (int s = 0; s < kGShapes; ++s) {
+        std::vector<int> sel;
+        for (int g = 0; g < (int)e->G; ++g)
+            if (e->bucket_shape[e->geo_bucket[g]] == s) sel.push_back(g);
+        const std::vector<int2> t = build_tiles(e, s, sel);
+        e->ntiles[s] = (int)t.size();
+        e->dtiles[s].release();
+        if (!t.empty()) {
+            int rc = e->dtiles[s].upload(t.data(), t.size() * sizeof(int2), e->stream);
+            if (rc) return rc;
+        }
+    }
+    OQ_HIP(hipStreamSynchronize(e->stream));
+    return OQ_OK;
+}
+
+static int launch_gemm(oq_group *e, int s, const int2 *tiles, int ntiles, const double *u,
+                       double *du)
+{
+    if (ntiles == 0) return OQ_OK;
+    const unsigned grid = (unsigned)std::min(ntiles, e->grid[s]);
+    switch (s) {
+    case 0: hipLaunchKernelGGL((k_grp_gemm<1, true>), dim3(grid), dim3(256), 0, e->stream, e->dev, tiles, ntiles, u, du); break;
+    case 1: hipLaunchKernelGGL((k_grp_gemm<1, false>), dim3(grid), dim3(256), 0, e->stream, e->dev, tiles, ntiles, u, du); break;
+    case 2: hipLaunchKernelGGL((k_grp_gemm<2, true>), dim3(grid), dim3(256), 0, e->stream, e->dev, tiles, ntiles, u, du); break;
+    case 3: hipLaunchKernelGGL((k_grp_gemm<2, false>), dim3(grid), dim3(256), 0, e->stream, e->dev, tiles, ntiles, u, du); break;
+    case 4: hipLaunchKernelGGL((k_grp_gemm<4, true>), dim3(grid), dim3(256), 0, e->stream, e->dev, tiles, ntiles, u, du); break;
+    default: hipLaunchKernelGGL((k_grp_gemm<4, false>), dim3(grid), dim3(256), 0, e->stream, e->dev, tiles, ntiles, u, du); break;
+    }
+    OQ_HIP(hipGetLastError());
+    return OQ_OK;
+}
+
+static int grp_rhs(oq_group *e, const double *u, double *du)
+{
+    hipLaunchKernelGGL(k_grp_prologue, dim3(gbk(e->nprolog, 256)), dim3(256), 0, e->stream, e->dev,
+                       u, du);
+    OQ_HIP(hipGetLastError());
+    for (int s = 0; s < kGShapes; ++s) {
+        int rc = launch_gemm(e, s, e->dtiles[s].as<int2>(), e->ntiles[s], u, du);
+        if (rc) return rc;
+    }
+    return OQ_OK;
+}
+
+static int grp_step(oq_group *e)
+{
+    const int64_t nt = e->ntot;
+    const int GM = (int)(e->G * e->M);
+    double *u = e->u.as<double>(), *y = e->y.as<double>(), *k = e->k.as<double>();
+    int rc;
+    hipLaunchKernelGGL(k_grp_begin, dim3(gbk(GM, 256)), dim3(256), 0, e->stream, e->ctrl);
+    for (int s = 1; s < 7; ++s) {
+        hipLaunchKernelGGL(k_grp_stage, dim3(gbk(nt, 256)), dim3(256), 0, e->stream, e->ctrl, s, nt,
+                           u, k, y);
+        if ((rc = grp_rhs(e, y, k + s * nt))) return rc;
+    }
+    hipLaunchKernelGGL(k_grp_err, dim3(gbk(GM, 256), kGChunks), dim3(256), 0, e->stream, e->ctrl, nt,
+                       u, y, k, e->part.as<double>());
+    hipLaunchKernelGGL(k_grp_controller, dim3(gbk(GM, 256)), dim3(256), 0, e->stream, e->ctrl,
+                       e->part.as<double>());
+    hipLaunchKernelGGL(k_grp_commit, dim3(gbk(nt, 256)), dim3(256), 0, e->stream, e->ctrl, nt, u, y,
+                       k);
+    OQ_HIP(hipGetLastError());
+    return OQ_OK;
+}
+
+static void grp_destroy(oq_group *e)
+{
+    if (!e) return;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    if (e->exec) (void)hipGraphExecDestroy(e->exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    for (oq_rhs *h : e->rhs)
+        if (h) oq_rhs_destroy(h);
+    if (e->ev0) (void)hipEventDestroy(e->ev0);
+    if (e->ev1) (void)hipEventDestroy(e->ev1);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+// host member-major states (geometry g: M members of n_g entries) <-> device
+// entry-major segments
+static void to_entry_major(const oq_group *e, const double *src, std::vector<double> &dst)
+{
+    dst.resize((size_t)e->ntot);
+    const int64_t M = e->M;
+    int64_t hoff = 0;
+    for (int64_t g = 0; g < e->G; ++g) {
+        const int64_t n = e->nst[g], base = e->soffs[g];
+        for (int64_t m = 0; m < M; ++m)
+            for (int64_t i = 0; i < n; ++i) dst[(size_t)(base + i * M + m)] = src[hoff + m * n + i];
+        hoff += n * M;
+    }
+}
+
+static void from_entry_major(const oq_group *e, const std::vector<double> &src, double *dst)
+{
+    const int64_t M = e->M;
+    int64_t hoff = 0;
+    for (int64_t g = 0; g < e->G; ++g) {
+        const int64_t n = e->nst[g], base = e->soffs[g];
+        for (int64_t m = 0; m < M; ++m)
+            for (int64_t i = 0; i < n; ++i) dst[hoff + m * n + i] = src[(size_t)(base + i * M + m)];
+        hoff += n * M;
+    }
+}
+
+}  // namespace oq
+
+using namespace oq;
+
+extern "C" {
+
+int oq_group_create(int device, int64_t ngeo, const oq_fault_mesh *mf,
+                    const oq_hex8_mesh *const *ma, const oq_visc_props *const *visc,
+                    double lambda, double mu, int ftype, int64_t nrept, double buffer_ratio,
+                    const oq_quadrature *quad, int64_t nmembers, const oq_rs_props *members,
+                    int kdtype, oq_group **out)
+{
+    OQ_REQUIRE(out != nullptr && mf != nullptr && members != nullptr, "NULL argument");
+    *out = nullptr;
+    OQ_REQUIRE(ngeo >= 1 && ngeo <= (1 << 20), "ngeo must be in [1, 2^20]");
+    OQ_REQUIRE(nmembers >= 1 && nmembers <= kGM, "nmembers must be in [1, 16]");
+    OQ_REQUIRE(kdtype == OQ_K_FP32, "the grouped operator is FP32");
+    for (int64_t q = 0; q < ngeo * nmembers; ++q) {
+        const oq_rs_props &p = members[q];
+        OQ_REQUIRE(p.a && p.b && p.L && p.sigma, "member property array is NULL");
+        OQ_REQUIRE(p.f0 > 0.0 && p.v0 > 0.0 && p.eta > 0.0 && p.vpl > 0.0,
+                   "RateStateQuasiDynamicProperty requires f₀ > 0, v₀ > 0, η > 0, vpl > 0");
+    }
+    for (int64_t g = 0; g < ngeo; ++g)
+        if (ma && ma[g]) OQ_REQUIRE(visc && visc[g], "a geometry with a mantle needs its viscosity");
+    oq_group *e = new oq_group();
+    e->device = device;
+    e->G = ngeo;
+    e->M = nmembers;
+    for (int b = 0; b < kGBuckets; ++b) e->bucket_shape[b] = 1;  // RB = 1, plain
+    int rc = OQ_OK;
+    for (int64_t g = 0; g < ngeo && !rc; ++g) {
+        oq_rhs *h = nullptr;
+        const oq_hex8_mesh *mg = ma ? ma[g] : nullptr;
+        rc = oq_rhs_create_from_mesh(device, &mf[g], mg, lambda, mu, ftype, nrept, buffer_ratio,
+                                     mg ? quad : nullptr, &members[g * nmembers],
+                                     mg ? visc[g] : nullptr, OQ_K_FP32, &h);
+        e->rhs.push_back(h);
+    }
+    if (rc) {
+        grp_destroy(e);
+        return rc;
+    }
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&e->stream) != hipSuccess ||
+        hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess) {
+        set_error("stream/event creation failed");
+        grp_destroy(e);
+        return OQ_ERR_HIP;
+    }
+    const int64_t M = nmembers;
+    int64_t soff = 0, foff = 0, Soff = 0, poff = 0;
+    std::vector<int64_t> poffs;
+    for (int64_t g = 0; g < ngeo; ++g) {
+        const oq_rhs *h = e->rhs[g];
+        GrpGeo q{};
+        q.K = static_cast<const float *>(h->K.p);
+        q.N = h->N; q.ld = h->ld; q.nf = h->nf; q.ne = h->ne; q.n = h->lay.n;
+        q.soff = soff; q.foff = foff; q.Soff = Soff;
+        q.lay = h->lay;
+        q.gamma = h->dev.gamma; q.nexp = h->dev.nexp; q.nlaws = h->dev.nlaws;
+        for (int k = 0; k < 6; ++k) q.deps0[k] = h->dev.deps0[k];
+        e->geo.push_back(q);
+        e->soffs.push_back(soff);
+        e->nst.push_back(h->lay.n);
+        e->geo_bucket.push_back(bucket_of(h->N));
+        poffs.push_back(poff);
+        soff += h->lay.n * M;
+        foff += h->nf * M;
+        Soff += (int64_t)kGM * h->ld;
+        poff += (h->nf + h->ne) * M;
+    }
+    e->soffs.push_back(soff);
+    poffs.push_back(poff);
+    e->ntot = soff;
+    e->nprolog = poff;
+    // per-member friction, entry-major per geometry
+    std::vector<double> va((size_t)foff), vb((size_t)foff), vL((size_t)foff), vs((size_t)foff);
+    std::vector<double> eta(ngeo * M), vpl(ngeo * M), f0(ngeo * M), v0(ngeo * M);
+    for (int64_t g = 0; g < ngeo; ++g) {
+        const int64_t nf = e->geo[g].nf, fo = e->geo[g].foff;
+        for (int64_t m = 0; m < M; ++m) {
+            const oq_rs_props &p = members[g * M + m];
+            for (int64_t i = 0; i < nf; ++i) {
+                const size_t ix = (size_t)(fo + i * M + m);
+                va[ix] = p.a[i]; vb[ix] = p.b[i]; vL[ix] = p.L[i]; vs[ix] = p.sigma[i];
+            }
+            eta[g * M + m] = p.eta; vpl[g * M + m] = p.vpl; f0[g * M + m] = p.f0; v0[g * M + m] = p.v0;
+        }
+    }
+    const size_t f8 = (size_t)foff * 8, m8 = (size_t)(ngeo * M) * 8;
+    if (!rc) rc = e->dgeo.upload(e->geo.data(), e->geo.size() * sizeof(GrpGeo), e->stream);
+    if (!rc) rc = e->dsoffs.upload(e->soffs.data(), e->soffs.size() * 8, e->stream);
+    if (!rc) rc = e->dpoffs.upload(poffs.data(), poffs.size() * 8, e->stream);
+    if (!rc) rc = e->dnst.upload(e->nst.data(), e->nst.size() * 8, e->stream);
+    if (!rc) rc = e->a.upload(va.data(), f8, e->stream);
+    if (!rc) rc = e->b.upload(vb.data(), f8, e->stream);
+    if (!rc) rc = e->L.upload(vL.data(), f8, e->stream);
+    if (!rc) rc = e->sigma.upload(vs.data(), f8, e->stream);
+    if (!rc) rc = e->eta.upload(eta.data(), m8, e->stream);
+    if (!rc) rc = e->vpl.upload(vpl.data(), m8, e->stream);
+    if (!rc) rc = e->f0.upload(f0.data(), m8, e->stream);
+    if (!rc) rc = e->v0.upload(v0.data(), m8, e->stream);
+    if (!rc) rc = e->S.alloc((size_t)Soff * 4);
+    if (!rc && hipMemsetAsync(e->S.p, 0, (size_t)Soff * 4, e->stream) != hipSuccess) rc = OQ_ERR_HIP;
+    const int64_t GM = ngeo * M;
+    if (!rc) rc = e->u.alloc((size_t)soff * 8);
+    if (!rc) rc = e->y.alloc((size_t)soff * 8);
+    if (!rc) rc = e->k.alloc((size_t)(7 * soff) * 8);
+    if (!rc) rc = e->part.alloc((size_t)(kGChunks * GM) * 8);
+    DevBuf *cb8[] = {&e->ct, &e->cdt, &e->cdtcur, &e->cqold, &e->cna, &e->cnr, &e->citer};
+    for (DevBuf *bp : cb8)
+        if (!rc) rc = bp->alloc((size_t)GM * 8);
+    DevBuf *cb4[] = {&e->cdone, &e->cacc, &e->cret};
+    for (DevBuf *bp : cb4)
+        if (!rc) rc = bp->alloc((size_t)GM * 4);
+    if (!rc) {
+        int gr[kGShapes];
+        rc = occupancy_grid<1, true>(&gr[0]);
+        if (!rc) rc = occupancy_grid<1, false>(&gr[1]);
+        if (!rc) rc = occupancy_grid<2, true>(&gr[2]);
+        if (!rc) rc = occupancy_grid<2, false>(&gr[3]);
+        if (!rc) rc = occupancy_grid<4, true>(&gr[4]);
+        if (!rc) rc = occupancy_grid<4, false>(&gr[5]);
+        for (int s = 0; s < kGShapes; ++s) e->grid[s] = gr[s];
+    }
+    if (!rc) rc = upload_tiles(e);
+    if (!rc && hipStreamSynchronize(e->stream) != hipSuccess) rc = OQ_ERR_HIP;
+    if (rc) {
+        grp_destroy(e);
+        return rc;
+    }
+    GrpDev &d = e->dev;
+    d.geo = e->dgeo.as<GrpGeo>(); d.soffs = e->dsoffs.as<int64_t>(); d.poffs = e->dpoffs.as<int64_t>();
+    d.G = (int)ngeo; d.M = (int)M;
+    d.a = e->a.as<double>(); d.b = e->b.as<double>(); d.L = e->L.as<double>(); d.sigma = e->sigma.as<double>();
+    d.eta = e->eta.as<double>(); d.vpl = e->vpl.as<double>(); d.f0 = e->f0.as<double>(); d.v0 = e->v0.as<double>();
+    d.S = e->S.as<float>();
+    d.nprolog = e->nprolog;
+    d.ntot = e->ntot;
+    GrpCtrl &c = e->ctrl;
+    c.t = e->ct.as<double>(); c.dt = e->cdt.as<double>(); c.dtcur = e->cdtcur.as<double>();
+    c.qold = e->cqold.as<double>(); c.naccept = e->cna.as<long long>(); c.nreject = e->cnr.as<long long>();
+    c.iter = e->citer.as<long long>(); c.done = e->cdone.as<int>(); c.accept = e->cacc.as<int>();
+    c.retcode = e->cret.as<int>();
+    c.soffs = e->dsoffs.as<int64_t>(); c.nst = e->dnst.as<int64_t>();
+    c.G = (int)ngeo; c.M = (int)M;
+    *out = e;
+    return OQ_OK;
+}
+
+int oq_group_info(oq_group *e, int64_t *ntot, int64_t *nstate, int64_t *kbytes)
+{
+    OQ_REQUIRE(e != nullptr, "NULL argument");
+    if (ntot) *ntot = e->ntot;
+    int64_t kb = 0;
+    for (int64_t g = 0; g < e->G; ++g) {
+        if (nstate) nstate[g] = e->nst[g];
+        kb += e->geo[g].N * e->geo[g].N * 4;
+    }
+    if (kbytes) *kbytes = kb;
+    return OQ_OK;
+}
+
+int oq_group_autotune(oq_group *e, int iters, int *shape_of_bucket, double *ms_of_bucket,
+                      double *ms_table)
+{
+    OQ_REQUIRE(e != nullptr && iters > 0, "invalid argument");
+    OQ_HIP(hipSetDevice(e->device));
+    // S = f(state 0) is irrelevant to the timing; the output goes to scratch
+    double *u = e->y.as<double>(), *du = e->k.as<double>();
+    for (int b = 0; b < kGBuckets; ++b) {
+        std::vector<int> sel;
+        for (int g = 0; g < (int)e->G; ++g)
+            if (e->geo_bucket[g] == b) sel.push_back(g);
+        if (ms_of_bucket) ms_of_bucket[b] = 0.0;
+        if (shape_of_bucket) shape_of_bucket[b] = -1;
+        if (ms_table)
+            for (int s = 0; s < kGShapes; ++s) ms_table[b * kGShapes + s] = 0.0;
+        if (sel.empty()) continue;
+        double best = 1e300;
+        int bs = e->bucket_shape[b];
+        for (int s = 0; s < kGShapes; ++s) {
+            const std::vector<int2> t = build_tiles(e, s, sel);
+            DevBuf dt;
+            int rc = dt.upload(t.data(), t.size() * sizeof(int2), e->stream);
+            if (rc) return rc;
+            if ((rc = launch_gemm(e, s, dt.as<int2>(), (int)t.size(), u, du))) return rc;
+            OQ_HIP(hipEventRecord(e->ev0, e->stream));
+            for (int i = 0; i < iters; ++i)
+                if ((rc = launch_gemm(e, s, dt.as<int2>(), (int)t.size(), u, du))) return rc;
+            OQ_HIP(hipEventRecord(e->ev1, e->stream));
+            OQ_HIP(hipEventSynchronize(e->ev1));
+            float ms = 0.f;
+            OQ_HIP(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+            if (ms_table) ms_table[b * kGShapes + s] = ms / iters;
+            if (ms / iters < best) {
+                best = ms / iters;
+                bs = s;
+            }
+        }
+        e->bucket_shape[b] = bs;
+        if (shape_of_bucket) shape_of_bucket[b] = bs;
+        if (ms_of_bucket) ms_of_bucket[b] = best;
+    }
+    int rc = upload_tiles(e);
+    if (rc) return rc;
+    e->stepping = false;  // a captured step holds the old tile tables
+    return OQ_OK;
+}
+
+int oq_group_eval(oq_group *e, const double *u, double *du)
+{
+    OQ_REQUIRE(e && u && du, "NULL argument");
+    OQ_HIP(hipSetDevice(e->device));
+    std::vector<double> ui;
+    to_entry_major(e, u, ui);
+    OQ_HIP(hipMemcpyAsync(e->y.p, ui.data(), ui.size() * 8, hipMemcpyHostToDevice, e->stream));
+    int rc = grp_rhs(e, e->y.as<double>(), e->k.as<double>());
+    if (rc) return rc;
+    std::vector<double> o((size_t)e->ntot);
+    OQ_HIP(hipMemcpyAsync(o.data(), e->k.p, o.size() * 8, hipMemcpyDeviceToHost, e->stream));
+    OQ_HIP(hipStreamSynchronize(e->stream));
+    from_entry_major(e, o, du);
+    return OQ_OK;
+}
+
+int oq_group_time(oq_group *e, int iters, double *ms_eval)
+{
+    OQ_REQUIRE(e && iters > 0, "invalid argument");
+    OQ_HIP(hipSetDevice(e->device));
+    int rc;
+    OQ_HIP(hipEventRecord(e->ev0, e->stream));
+    for (int i = 0; i < iters; ++i)
+        if ((rc = grp_rhs(e, e->y.as<double>(), e->k.as<double>()))) return rc;
+    OQ_HIP(hipEventRecord(e->ev1, e->stream));
+    OQ_HIP(hipEventSynchronize(e->ev1));
+    float ms = 0.f;
+    OQ_HIP(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+    if (ms_eval) *ms_eval = (double)ms / iters;
+    return OQ_OK;
+}
+
+int oq_group_start(oq_group *e, const double *u0, double t0, double t1, const oq_solver_opts *o)
+{
+    OQ_REQUIRE(e && u0 && o, "NULL argument");
+    OQ_REQUIRE(t1 > t0, "tspan must satisfy t1 > t0");
+    OQ_REQUIRE(o->dt0 > 0.0, "the grouped stepper needs an initial step dt0 > 0");
+    OQ_HIP(hipSetDevice(e->device));
+    const int64_t GM = e->G * e->M;
+    std::vector<double> ui;
+    to_entry_major(e, u0, ui);
+    OQ_HIP(hipMemcpyAsync(e->u.p, ui.data(), ui.size() * 8, hipMemcpyHostToDevice, e->stream));
+    std::vector<double> tv(GM, t0), dtv(GM, o->dt0), qv(GM, 1e-4);
+    std::vector<long long> zl(GM, 0);
+    std::vector<int> zi(GM, 0);
+    OQ_HIP(hipMemcpyAsync(e->ct.p, tv.data(), GM * 8, hipMemcpyHostToDevice, e->stream));
+    OQ_HIP(hipMemcpyAsync(e->cdt.p, dtv.data(), GM * 8, hipMemcpyHostToDevice, e->stream));
+    OQ_HIP(hipMemcpyAsync(e->cqold.p, qv.data(), GM * 8, hipMemcpyHostToDevice, e->stream));
+    OQ_HIP(hipMemcpyAsync(e->cna.p, zl.data(), GM * 8, hipMemcpyHostToDevice, e->stream));
+    OQ_HIP(hipMemcpyAsync(e->cnr.p, zl.data(), GM * 8, hipMemcpyHostToDevice, e->stream));
+    OQ_HIP(hipMemcpyAsync(e->citer.p, zl.data(), GM * 8, hipMemcpyHostToDevice, e->stream));
+    OQ_HIP(hipMemcpyAsync(e->cdone.p, zi.data(), GM * 4, hipMemcpyHostToDevice, e->stream));
+    OQ_HIP(hipMemcpyAsync(e->cacc.p, zi.data(), GM * 4, hipMemcpyHostToDevice, e->stream));
+    OQ_HIP(hipMemcpyAsync(e->cret.p, zi.data(), GM * 4, hipMemcpyHostToDevice, e->stream));
+    GrpCtrl &c = e->ctrl;
+    c.t1 = t1;
+    c.reltol = o->reltol;
+    c.abstol = o->abstol;
+    c.dtmax = o->dtmax > 0.0 ? o->dtmax : INFINITY;
+    c.dtmin = o->dtmin;
+    c.maxiters = o->maxiters > 0 ? o->maxiters : (long long)1e18;
+    // FSAL seed: k1 = f(u0)
+    int rc = grp_rhs(e, e->u.as<double>(), e->k.as<double>());
+    if (rc) return rc;
+    if (e->exec) { (void)hipGraphExecDestroy(e->exec); e->exec = nullptr; }
+    if (e->graph) { (void)hipGraphDestroy(e->graph); e->graph = nullptr; }
+    OQ_HIP(hipStreamSynchronize(e->stream));
+    OQ_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    rc = grp_step(e);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(e->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    OQ_HIP(ce);
+    e->graph = g;
+    OQ_HIP(hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0));
+    e->stepping = true;
+    return OQ_OK;
+}
+
+int oq_group_run(oq_group *e, int64_t max_steps, int64_t *nrunning)
+{
+    OQ_REQUIRE(e && max_steps > 0, "invalid argument");
+    OQ_REQUIRE(e->stepping, "call oq_group_start first");
+    OQ_HIP(hipSetDevice(e->device));
+    const int64_t GM = e->G * e->M;
+    std::vector<int> done(GM);
+    int64_t left = GM;
+    for (int64_t s = 0; s < max_steps; s += 8) {
+        const int64_t m = (max_steps - s) < 8 ? (max_steps - s) : 8;
+        for (int64_t i = 0; i < m; ++i) OQ_HIP(hipGraphLaunch(e->exec, e->stream));
+        OQ_HIP(hipMemcpyAsync(done.data(), e->cdone.p, GM * 4, hipMemcpyDeviceToHost, e->stream));
+        OQ_HIP(hipStreamSynchronize(e->stream));
+        left = 0;
+        for (int v : done) left += v ? 0 : 1;
+        if (left == 0) break;
+    }
+    if (nrunning) *nrunning = left;
+    return OQ_OK;
+}
+
+int oq_group_state(oq_group *e, double *t, double *u, int64_t *naccept, int64_t *nreject,
+                   int *retcode)
+{
+    OQ_REQUIRE(e != nullptr, "NULL argument");
+    OQ_HIP(hipSetDevice(e->device));
+    const int64_t GM = e->G * e->M;
+    if (t) OQ_HIP(hipMemcpyAsync(t, e->ct.p, GM * 8, hipMemcpyDeviceToHost, e->stream));
+    if (naccept) OQ_HIP(hipMemcpyAsync(naccept, e->cna.p, GM * 8, hipMemcpyDeviceToHost, e->stream));
+    if (nreject) OQ_HIP(hipMemcpyAsync(nreject, e->cnr.p, GM * 8, hipMemcpyDeviceToHost, e->stream));
+    if (retcode) OQ_HIP(hipMemcpyAsync(retcode, e->cret.p, GM * 4, hipMemcpyDeviceToHost, e->stream));
+    std::vector<double> o;
+    if (u) {
+        o.resize((size_t)e->ntot);
+        OQ_HIP(hipMemcpyAsync(o.data(), e->u.p, o.size() * 8, hipMemcpyDeviceToHost, e->stream));
+    }
+    OQ_HIP(hipStreamSynchronize(e->stream));
+    if (u) from_entry_major(e, o, u);
+    return OQ_OK;
+}
+
+int oq_group_destroy(oq_group *e)
+{
+    if (e) {
+        (void)hipSetDevice(e->device);
+        grp_destroy(e);
+    }
+    return OQ_OK;
+}
+
+}  // extern "C"

@@ -101,6 +101,18 @@ PROTOTYPES = {
     "oq_ensemble_drain": (_int, [_vp, ctypes.POINTER(_i64), _pd, _pd]),
     "oq_ensemble_time": (_int, [_vp, _int, ctypes.POINTER(_d)]),
     "oq_ensemble_destroy": (_int, [_vp]),
+    "oq_group_create": (_int, [_int, _i64, ctypes.POINTER(FaultMesh), _vp, _vp, _d, _d, _int, _i64,
+                               _d, ctypes.POINTER(Quadrature), _i64, ctypes.POINTER(RSProps), _int,
+                               ctypes.POINTER(_vp)]),
+    "oq_group_info": (_int, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
+    "oq_group_autotune": (_int, [_vp, _int, ctypes.POINTER(_int), _pd, _pd]),
+    "oq_group_eval": (_int, [_vp, _pd, _pd]),
+    "oq_group_time": (_int, [_vp, _int, ctypes.POINTER(_d)]),
+    "oq_group_start": (_int, [_vp, _pd, _d, _d, ctypes.POINTER(SolverOpts)]),
+    "oq_group_run": (_int, [_vp, _i64, ctypes.POINTER(_i64)]),
+    "oq_group_state": (_int, [_vp, _pd, _pd, ctypes.POINTER(_i64), ctypes.POINTER(_i64),
+                              ctypes.POINTER(_int)]),
+    "oq_group_destroy": (_int, [_vp]),
 }
 
 

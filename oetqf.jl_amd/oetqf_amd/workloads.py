@@ -116,3 +116,46 @@ def precision_scale_65536(seed=20261202):
     w = transform_fault_composite(nx=256, nxi=64, mx=32, my=32, mz=8, seed=seed)
     w["tspan"] = (0.0, 200 * YEAR)
     return w
+
+
+def grouped_batch(G=24, M=16, seed=20261204, pmin=1024, pmax=8192):
+    """G synthetic faults of uneven size: patch counts log-uniform in
+    [pmin, pmax] at aspect ratio 4:1 (80 x 20 km; nxi = round(sqrt(P/4)),
+    nx = 4 nxi), planar (uniform rows) and transfinite (rows graded x1.05)
+    alternating; half of them, drawn at random, coupled to a hex8 mantle
+    block below 10 km (80 x 40 x 30 km) of 512-4096 cells (log-uniform,
+    m x m x 8), linear Maxwell viscosity 1e19 Pa s.  M frictional
+    realizations per geometry drawn as ensemble_members.  Returns a list of
+    workload dicts (each with "members" and "U0")."""
+    rng = np.random.default_rng(seed)
+    coupled = set(rng.permutation(G)[:G // 2].tolist())
+    out = []
+    for g in range(G):
+        P = float(np.exp(rng.uniform(np.log(pmin), np.log(pmax))))
+        nxi = max(4, int(round(np.sqrt(P / 4.0))))
+        nx = 4 * nxi
+        grading = 1.0 if g % 2 == 0 else 1.05
+        mf = gen_mesh_graded(80e3, 20e3, 80e3 / nx, nxi, 90.0, ratio=grading)
+        mu = 30e9
+        shp = (mf.nx, mf.nxi)
+        d = np.broadcast_to(-mf.z, shp)
+        ma = pa = None
+        if g in coupled:
+            ne = float(np.exp(rng.uniform(np.log(512), np.log(4096))))
+            m = max(2, int(round(np.sqrt(ne / 8.0))))
+            ma = hex8_box(-40e3, -20e3, -10e3, 80e3, 40e3, -30e3, m, m, 8)
+            pa = PowerLawViscosityProperty(np.full(ma.ne, 1.0 / (2 * 1e19)), np.zeros(ma.ne),
+                                           np.zeros(6))
+        b = _b_profile(d, bottom=8e3) if ma is not None else _b_profile(d)
+        pf = RateStateQuasiDynamicProperty(np.full(shp, 0.015), b, np.full(shp, 0.012),
+                                           np.full(shp, 50e6), mu / (2 * 3040.0), 1e-9, 0.6, 1e-6)
+        parts = [np.zeros(shp), np.zeros(shp)]
+        if ma is not None:
+            parts += [np.zeros((ma.ne, 6)), np.zeros((ma.ne, 6))]
+        parts.append(np.zeros(shp))
+        w = dict(name="fault %dx%d%s" % (nx, nxi, " + mantle %d cells" % ma.ne if ma is not None else ""),
+                 mf=mf, ma=ma, lam=mu, mu=mu, pf=pf, pa=pa, u0=ArrayPartition(*parts),
+                 tspan=(0.0, 300 * YEAR), nrept=2, buffer_ratio=0.0, reltol=1e-6, abstol=1e-8)
+        w["members"], w["U0"] = ensemble_members(w, M, seed=seed + 1 + g)
+        out.append(w)
+    return out
