@@ -359,13 +359,13 @@ def main():
         w0 = strike_slip_512()
         p0 = oq.assemble_from_mesh(w0["mf"], None, w0["lam"], w0["mu"], w0["pf"], None, w0["u0"],
                                    w0["tspan"])
-        r0 = run_cycle(p0, reltol=w0["reltol"], abstol=w0["abstol"],
-                       progress=lambda t: None)
+        r0 = run_cycle(p0, reltol=w0["reltol"], abstol=w0["abstol"], persistent="auto")
         ev = events(r0["t"], r0["vmax"], 1e-3, merge_gap=365 * 86400.0)
         yrs = (w0["tspan"][1] - w0["tspan"][0]) / YEAR
         cycle512 = dict(workload=w0["name"], sim_years=yrs, wall_s=r0["wall"],
                         sim_yr_per_hr=yrs / (r0["wall"] / 3600.0), accepted=int(r0["naccept"]),
                         rejected=int(r0["nreject"]), events=len(ev), retcode=r0["retcode"],
+                        stepper=r0["mode"],
                         event_def="max v > 1e-3 m/s, crossings within 1 yr merged",
                         reltol=w0["reltol"], abstol=w0["abstol"])
         log("512-patch cycle: %.0f yr in %.1f s (%.0f yr/hr), %d events" %

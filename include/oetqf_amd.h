@@ -220,6 +220,13 @@ int oq_stepper_record(oq_stepper *s, int64_t stride, int64_t snap_capacity,
 int oq_stepper_drain(oq_stepper *s, int64_t *nsteps, double *step_t, double *step_vmax,
                      int64_t *nsnaps, double *snap_t, double *snap_u);
 int oq_stepper_destroy(oq_stepper *s);
+/* Persistent mode (no counterpart in the reference; the graph stepper's
+ * semantics are kept): for fault-only problems with an FP64 operator whose
+ * rows fit in LDS of at most 128 workgroups (N <= 1024), oq_stepper_run executes all of
+ * its steps in one cooperative launch -- rows of K resident in LDS, one grid
+ * barrier per RHS -- instead of replaying the per-step graph.  on = 0 returns
+ * to the graph.  OQ_ERR_ARG if the problem does not qualify. */
+int oq_stepper_set_persistent(oq_stepper *s, int on);
 
 /* ---- ensembles of frictional-parameter realizations sharing one
  * geometry.  No entry point of the reference corresponds: a parameter study
@@ -285,11 +292,12 @@ int oq_group_create(int device, int64_t ngeo, const oq_fault_mesh *mf,
 /* Total state entries, per-geometry state size nstate[ngeo] and the bytes
  * of all operators (any pointer may be NULL). */
 int oq_group_info(oq_group *e, int64_t *ntot, int64_t *nstate, int64_t *kbytes);
-/* Times every tile shape (6: rows per wavefront 16/32/64, each with and
- * without a software-pipelined K stream) on the geometries of each size
- * bucket (bucket b: N < 2048 << b; 8 buckets) and keeps the fastest;
- * shape_of_bucket[8] and ms_of_bucket[8] receive the choice (-1 for an
- * empty bucket), ms_table[8*6] (may be NULL) every timing.  Call before
+/* Times every tile shape (8: register-streaming kernels with 16/32/64 rows
+ * per wavefront, each with and without a software-pipelined K stream, and
+ * LDS-staged kernels with 16/32 rows per wavefront) on the geometries of
+ * each size bucket (bucket b: N < 2048 << b; 8 buckets) and keeps the
+ * fastest; shape_of_bucket[8] and ms_of_bucket[8] receive the choice (-1
+ * for an empty bucket), ms_table[8*8] (may be NULL) every timing.  Call before
  * oq_group_start (a running stepper must be started again). */
 int oq_group_autotune(oq_group *e, int iters, int *shape_of_bucket, double *ms_of_bucket,
                       double *ms_table);

@@ -13,9 +13,11 @@
 //     lane streams 16 consecutive columns of one row (64 contiguous bytes),
 //     and the source block S_g is stored member-major so that the matching
 //     B operands are contiguous too.  FP32 MFMA partials are folded into an
-//     FP32 (hi, lo) two-sum pair every 256 columns; the fused epilogue is the
-//     rate-and-state update (fault rows) or the stress-rate store (mantle
-//     rows), per (row, member);
+//     FP32 (hi, lo) two-sum pair every 256 columns and the row sums stored in
+//     FP64; an element-wise epilogue then applies the rate-and-state update
+//     (fault rows) or the stress-rate store (mantle rows) per (row, member);
+//   - an LDS-staged variant stages 32 KB slices of K (row-contiguous global
+//     loads) and the shared S slice through LDS;
 //   - the grid is persistent (at most the resident workgroups) and walks the
 //     tile table largest geometry first; the table is static, so every
 //     workgroup ends after its last tile;
@@ -54,6 +56,7 @@ struct GrpGeo {
     int64_t soff;  // state entries before this geometry
     int64_t foff;  // fault-parameter entries before this geometry
     int64_t Soff;  // S floats before this geometry (S[Soff + m*ld + c])
+    int64_t Yoff;  // row-sum entries before this geometry (Y[Yoff + row*M + m])
     StateLayout lay;
     const double *gamma, *nexp;
     int nlaws;
@@ -64,11 +67,13 @@ struct GrpDev {
     const GrpGeo *geo;
     const int64_t *soffs;  // G+1 state offsets
     const int64_t *poffs;  // G+1 prologue offsets ((nf+ne)*M per geometry)
+    const int64_t *yoffs;  // G+1 row-sum offsets (N*M per geometry)
     int G, M;
     const double *a, *b, *L, *sigma;    // per geometry nf x M, entry-major
     const double *eta, *vpl, *f0, *v0;  // G*M
     float *S;
-    int64_t nprolog, ntot;
+    double *Y;  // K S row sums, FP64, entry-major per geometry
+    int64_t nprolog, ntot, nrows;
 };
 
 // segment g with off[g] <= x < off[g+1] (off ascending, off[0] = 0)
@@ -118,6 +123,14 @@ __global__ void __launch_bounds__(256) k_grp_prologue(GrpDev d, const double *__
     }
 }
 
+// Global-address-space view of a device pointer: loads through it compile
+// to global (not flat) loads.
+typedef const __attribute__((address_space(1))) f4x gf4x;
+__device__ __forceinline__ f4x gload4(const float *p)
+{
+    return *(gf4x *)p;
+}
+
 __device__ __forceinline__ void grp_two_sum(float &hi, float &lo, float x)
 {
     const float s = hi + x;
@@ -125,6 +138,59 @@ __device__ __forceinline__ void grp_two_sum(float &hi, float &lo, float x)
     const float err = (hi - (s - bp)) + (x - bp);
     hi = s;
     lo += err;
+}
+
+// Row sums of one wavefront: element v of row block rb is row
+// row0 + 16 rb + 4 lq + v, member lr (the 16x16x4 MFMA output layout).  The
+// rate-and-state / stress-rate epilogue runs in k_grp_finish: its FP64
+// transcendentals would otherwise set the register budget (and halve the
+// occupancy) of the streaming loop.
+template <int RB>
+__device__ __forceinline__ void grp_store_rows(const GrpDev &d, const GrpGeo &G, int64_t row0,
+                                               int lr, int lq, const float (&hi)[RB][4],
+                                               const float (&lo)[RB][4])
+{
+    const int64_t M = d.M, N = G.N;
+    if (lr >= M) return;
+    double *Y = d.Y + G.Yoff + lr;
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int64_t row = row0 + rb * 16 + 4 * lq + v;
+            if (row < N) Y[row * M] = (double)hi[rb][v] + (double)lo[rb][v];
+        }
+    }
+}
+
+// ---------------- epilogue: du from the row sums, per (row, member) ----------------
+__global__ void __launch_bounds__(256) k_grp_finish(GrpDev d, const double *__restrict__ u,
+                                                    double *__restrict__ du)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= d.nrows) return;
+    const int g = seg_find(d.yoffs, d.G, t);
+    const GrpGeo &G = d.geo[g];
+    const int64_t M = d.M, loc = t - G.Yoff;
+    const int64_t row = loc / M, m = loc % M, gm = (int64_t)g * M + m;
+    const double y = d.Y[t];
+    const double *ug = u + G.soff;
+    double *dug = du + G.soff;
+    if (row < G.nf) {
+        // update_fault! (equation.jl:237-245), member m's parameters
+        const int64_t ix = G.foff + row * M + m;
+        const double vv = ug[(G.lay.v + row) * M + m], th = ug[(G.lay.th + row) * M + m];
+        const double a = d.a[ix], b = d.b[ix], L = d.L[ix];
+        const double thp = th > 0.0 ? th : 0.0;
+        const double psi1 = exp((d.f0[gm] + b * log(d.v0[gm] * thp / L)) / a) / (2.0 * d.v0[gm]);
+        const double psi2 = d.sigma[ix] * psi1 / hypot(1.0, vv * psi1);
+        const double dth = 1.0 - vv * th / L;
+        dug[(G.lay.th + row) * M + m] = dth;
+        dug[(G.lay.v + row) * M + m] = (y - b / th * vv * psi2 * dth) / (a * psi2 + d.eta[gm]);
+        dug[(G.lay.del + row) * M + m] = vv;
+    } else {
+        dug[(G.lay.sig + row - G.nf) * M + m] = y;
+    }
 }
 
 // ---------------- grouped GEMM + fused epilogue ----------------
@@ -170,31 +236,31 @@ __global__ void __launch_bounds__(256) k_grp_gemm(GrpDev d, const int2 *__restri
         f4x sv[4], kv[RB][4];
         if (PIPE) {
 #pragma unroll
-            for (int t = 0; t < 4; ++t) sv[t] = *reinterpret_cast<const f4x *>(sp + p * kGStep + 16 * t);
+            for (int t = 0; t < 4; ++t) sv[t] = gload4(sp + p * kGStep + 16 * t);
 #pragma unroll
             for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-                for (int t = 0; t < 4; ++t) kv[rb][t] = *reinterpret_cast<const f4x *>(kp[rb] + p * kGStep + 16 * t);
+                for (int t = 0; t < 4; ++t) kv[rb][t] = gload4(kp[rb] + p * kGStep + 16 * t);
         }
         for (int64_t it = 0; it < nstep; ++it) {
             const int64_t pn = (p + 1 == nstep) ? 0 : p + 1;
             const bool more = it + 1 < nstep;
             if (!PIPE) {
 #pragma unroll
-                for (int t = 0; t < 4; ++t) sv[t] = *reinterpret_cast<const f4x *>(sp + p * kGStep + 16 * t);
+                for (int t = 0; t < 4; ++t) sv[t] = gload4(sp + p * kGStep + 16 * t);
 #pragma unroll
                 for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-                    for (int t = 0; t < 4; ++t) kv[rb][t] = *reinterpret_cast<const f4x *>(kp[rb] + p * kGStep + 16 * t);
+                    for (int t = 0; t < 4; ++t) kv[rb][t] = gload4(kp[rb] + p * kGStep + 16 * t);
             }
             f4x sn[4], kn[RB][4];
             if (PIPE && more) {
 #pragma unroll
-                for (int t = 0; t < 4; ++t) sn[t] = *reinterpret_cast<const f4x *>(sp + pn * kGStep + 16 * t);
+                for (int t = 0; t < 4; ++t) sn[t] = gload4(sp + pn * kGStep + 16 * t);
 #pragma unroll
                 for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-                    for (int t = 0; t < 4; ++t) kn[rb][t] = *reinterpret_cast<const f4x *>(kp[rb] + pn * kGStep + 16 * t);
+                    for (int t = 0; t < 4; ++t) kn[rb][t] = gload4(kp[rb] + pn * kGStep + 16 * t);
             }
             __builtin_amdgcn_sched_barrier(0);
             // lane (row lr, group lq) holds columns c0 + 16t + 4lq + e; MFMA
@@ -229,36 +295,123 @@ __global__ void __launch_bounds__(256) k_grp_gemm(GrpDev d, const int2 *__restri
             }
             p = pn;
         }
-        // element v of row block rb: row row0 + 16 rb + 4 lq + v, member lr
-        const int64_t M = d.M;
-        if (lr >= M) continue;
-        const int64_t m = lr, gm = (int64_t)tl.x * M + m;
-        const double *ug = u + G.soff;
-        double *dug = du + G.soff;
+        grp_store_rows<RB>(d, G, row0, lr, lq, hi, lo);
+    }
+}
+
+// LDS-staged variant: the workgroup stages a 64*RB-row x BK-column slice of
+// K (32 KB, BK = 128/RB) and the matching 16 x BK slice of S through LDS.
+// The global loads are row-contiguous (one wavefront instruction covers
+// 2-4 rows x 256-512 bytes, whole cache lines) and the next slice is in
+// flight in registers while the current one is reduced; the four
+// wavefronts share the S slice.  Rows are padded by 4 floats so that the
+// MFMA-layout reads (16 rows x 16 bytes per quarter wavefront) hit distinct
+// banks.
+template <int RB>
+__global__ void __launch_bounds__(256) k_grp_gemm_lds(GrpDev d, const int2 *__restrict__ tiles,
+                                                      int ntiles, const double *__restrict__ u,
+                                                      double *__restrict__ du)
+{
+    constexpr int BR = 64 * RB, BK = 128 / RB, PAD = BK + 4, C4 = BK / 4;
+    constexpr int KV = BR * C4 / 256;  // float4 of K per thread per slice (8)
+    constexpr int SV = 16 * C4 / 256;  // float4 of S per thread per slice (2 or 1)
+    constexpr int NC = 4 / RB;
+    __shared__ __attribute__((aligned(16))) float As[BR * PAD];
+    __shared__ __attribute__((aligned(16))) float Bs[16 * PAD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lr = lane & 15, lq = lane >> 4;
+    const int trow = tid / C4, tc4 = tid % C4;  // this thread's slot in a slice
+    for (int tix = blockIdx.x; tix < ntiles; tix += gridDim.x) {
+        const int2 tl = tiles[tix];
+        const GrpGeo &G = d.geo[tl.x];
+        const int64_t N = G.N, ld = G.ld, r0 = tl.y;
+        const int64_t nch = ld / BK;
+        const float *kp[KV];
+#pragma unroll
+        for (int j = 0; j < KV; ++j) {
+            const int64_t r = r0 + trow + j * (256 / C4);
+            kp[j] = G.K + (r < N ? r : N - 1) * ld + 4 * tc4;
+        }
+        const float *sp[SV];
+#pragma unroll
+        for (int j = 0; j < SV; ++j)
+            sp[j] = d.S + G.Soff + (int64_t)(trow + j * (256 / C4)) * ld + 4 * tc4;
+        f4x acc[RB][NC];
+        float hi[RB][4], lo[RB][4];
 #pragma unroll
         for (int rb = 0; rb < RB; ++rb) {
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int64_t row = row0 + rb * 16 + 4 * lq + v;
-                if (row >= N) continue;
-                const double y = (double)hi[rb][v] + (double)lo[rb][v];
-                if (row < G.nf) {
-                    // update_fault! (equation.jl:237-245), member m's parameters
-                    const int64_t ix = G.foff + row * M + m;
-                    const double vv = ug[(G.lay.v + row) * M + m], th = ug[(G.lay.th + row) * M + m];
-                    const double a = d.a[ix], b = d.b[ix], L = d.L[ix];
-                    const double thp = th > 0.0 ? th : 0.0;
-                    const double psi1 = exp((d.f0[gm] + b * log(d.v0[gm] * thp / L)) / a) / (2.0 * d.v0[gm]);
-                    const double psi2 = d.sigma[ix] * psi1 / hypot(1.0, vv * psi1);
-                    const double dth = 1.0 - vv * th / L;
-                    dug[(G.lay.th + row) * M + m] = dth;
-                    dug[(G.lay.v + row) * M + m] = (y - b / th * vv * psi2 * dth) / (a * psi2 + d.eta[gm]);
-                    dug[(G.lay.del + row) * M + m] = vv;
-                } else {
-                    dug[(G.lay.sig + row - G.nf) * M + m] = y;
+            for (int c = 0; c < NC; ++c) acc[rb][c] = (f4x){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int v = 0; v < 4; ++v) hi[rb][v] = lo[rb][v] = 0.f;
+        }
+        int64_t p = ((int64_t)tix * 13) % nch;
+        f4x kr[KV], sr[SV];
+#pragma unroll
+        for (int j = 0; j < KV; ++j) kr[j] = gload4(kp[j] + p * BK);
+#pragma unroll
+        for (int j = 0; j < SV; ++j) sr[j] = gload4(sp[j] + p * BK);
+        __syncthreads();  // the previous tile's reads of As/Bs are done
+#pragma unroll
+        for (int j = 0; j < KV; ++j)
+            *reinterpret_cast<f4x *>(&As[(trow + j * (256 / C4)) * PAD + 4 * tc4]) = kr[j];
+#pragma unroll
+        for (int j = 0; j < SV; ++j)
+            *reinterpret_cast<f4x *>(&Bs[(trow + j * (256 / C4)) * PAD + 4 * tc4]) = sr[j];
+        __syncthreads();
+        for (int64_t it = 0; it < nch; ++it) {
+            const int64_t pn = (p + 1 == nch) ? 0 : p + 1;
+            const bool more = it + 1 < nch;
+            if (more) {
+#pragma unroll
+                for (int j = 0; j < KV; ++j) kr[j] = gload4(kp[j] + pn * BK);
+#pragma unroll
+                for (int j = 0; j < SV; ++j) sr[j] = gload4(sp[j] + pn * BK);
+            }
+#pragma unroll
+            for (int sub = 0; sub < BK / 64; ++sub) {
+                f4x a[RB][4], b[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    b[t] = *reinterpret_cast<const f4x *>(&Bs[lr * PAD + sub * 64 + 16 * t + 4 * lq]);
+#pragma unroll
+                    for (int rb = 0; rb < RB; ++rb)
+                        a[rb][t] = *reinterpret_cast<const f4x *>(
+                            &As[(wave * 16 * RB + rb * 16 + lr) * PAD + sub * 64 + 16 * t + 4 * lq]);
+                }
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+#pragma unroll
+                        for (int rb = 0; rb < RB; ++rb)
+                            acc[rb][t % NC] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                                a[rb][t][e], b[t][e], acc[rb][t % NC], 0, 0, 0);
+            }
+            if (((it + 1) * BK) % 256 == 0 || !more) {
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb) {
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) grp_two_sum(hi[rb][v], lo[rb][v], acc[rb][c][v]);
+                        acc[rb][c] = (f4x){0.f, 0.f, 0.f, 0.f};
+                    }
                 }
             }
+            __syncthreads();
+            if (more) {
+#pragma unroll
+                for (int j = 0; j < KV; ++j)
+                    *reinterpret_cast<f4x *>(&As[(trow + j * (256 / C4)) * PAD + 4 * tc4]) = kr[j];
+#pragma unroll
+                for (int j = 0; j < SV; ++j)
+                    *reinterpret_cast<f4x *>(&Bs[(trow + j * (256 / C4)) * PAD + 4 * tc4]) = sr[j];
+            }
+            __syncthreads();
+            p = pn;
         }
+        grp_store_rows<RB>(d, G, r0 + (int64_t)wave * 16 * RB, lr, lq, hi, lo);
     }
 }
 
@@ -401,9 +554,10 @@ __global__ void __launch_bounds__(256) k_grp_commit(GrpCtrl c, int64_t ntot,
     k[i] = k[6 * ntot + i];
 }
 
-// tile shapes: RB in {1, 2, 4} x {software-pipelined, plain}
-static constexpr int kGShapes = 6;
-static inline int shape_rb(int s) { return s / 2 == 0 ? 1 : (s / 2 == 1 ? 2 : 4); }
+// tile shapes: RB in {1, 2, 4} x {software-pipelined, plain} register
+// kernels, and the LDS-staged kernel with RB in {1, 2}
+static constexpr int kGShapes = 8;
+static inline int shape_rb(int s) { return s < 6 ? (s / 2 == 0 ? 1 : (s / 2 == 1 ? 2 : 4)) : s - 5; }
 
 }  // namespace oq
 
@@ -418,7 +572,8 @@ struct oq_group {
     int bucket_shape[oq::kGBuckets];
     std::vector<int> geo_bucket;
     std::vector<std::vector<int2>> tiles_of_shape;  // host tables per shape
-    oq::DevBuf dgeo, dsoffs, dpoffs, dnst, a, b, L, sigma, eta, vpl, f0, v0, S;
+    int64_t nrows = 0;
+    oq::DevBuf dgeo, dsoffs, dpoffs, dyoffs, dnst, a, b, L, sigma, eta, vpl, f0, v0, S, Y;
     oq::DevBuf dtiles[oq::kGShapes];
     int ntiles[oq::kGShapes] = {0};
     int grid[oq::kGShapes] = {0};
@@ -442,14 +597,14 @@ static int bucket_of(int64_t N)
     return b;
 }
 
-template <int RB, bool PIPE>
-static int occupancy_grid(int *grid)
+template <class F>
+static int occupancy_grid(F kern, int *grid)
 {
     int nb = 0, dev = 0;
     OQ_HIP(hipGetDevice(&dev));
     hipDeviceProp_t prop;
     OQ_HIP(hipGetDeviceProperties(&prop, dev));
-    OQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_grp_gemm<RB, PIPE>, 256, 0));
+    OQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, 256, 0));
     *grid = (nb > 0 ? nb : 1) * prop.multiProcessorCount;
     return OQ_OK;
 }
@@ -496,7 +651,9 @@ static int launch_gemm(oq_group *e, int s, const int2 *tiles, int ntiles, const 
     case 2: hipLaunchKernelGGL((k_grp_gemm<2, true>), dim3(grid), dim3(256), 0, e->stream, e->dev, tiles, ntiles, u, du); break;
     case 3: hipLaunchKernelGGL((k_grp_gemm<2, false>), dim3(grid), dim3(256), 0, e->stream, e->dev, tiles, ntiles, u, du); break;
     case 4: hipLaunchKernelGGL((k_grp_gemm<4, true>), dim3(grid), dim3(256), 0, e->stream, e->dev, tiles, ntiles, u, du); break;
-    default: hipLaunchKernelGGL((k_grp_gemm<4, false>), dim3(grid), dim3(256), 0, e->stream, e->dev, tiles, ntiles, u, du); break;
+    case 5: hipLaunchKernelGGL((k_grp_gemm<4, false>), dim3(grid), dim3(256), 0, e->stream, e->dev, tiles, ntiles, u, du); break;
+    case 6: hipLaunchKernelGGL((k_grp_gemm_lds<1>), dim3(grid), dim3(256), 0, e->stream, e->dev, tiles, ntiles, u, du); break;
+    default: hipLaunchKernelGGL((k_grp_gemm_lds<2>), dim3(grid), dim3(256), 0, e->stream, e->dev, tiles, ntiles, u, du); break;
     }
     OQ_HIP(hipGetLastError());
     return OQ_OK;
@@ -511,6 +668,9 @@ static int grp_rhs(oq_group *e, const double *u, double *du)
         int rc = launch_gemm(e, s, e->dtiles[s].as<int2>(), e->ntiles[s], u, du);
         if (rc) return rc;
     }
+    hipLaunchKernelGGL(k_grp_finish, dim3(gbk(e->nrows, 256)), dim3(256), 0, e->stream, e->dev, u,
+                       du);
+    OQ_HIP(hipGetLastError());
     return OQ_OK;
 }
 
@@ -627,14 +787,14 @@ int oq_group_create(int device, int64_t ngeo, const oq_fault_mesh *mf,
         return OQ_ERR_HIP;
     }
     const int64_t M = nmembers;
-    int64_t soff = 0, foff = 0, Soff = 0, poff = 0;
-    std::vector<int64_t> poffs;
+    int64_t soff = 0, foff = 0, Soff = 0, poff = 0, yoff = 0;
+    std::vector<int64_t> poffs, yoffs;
     for (int64_t g = 0; g < ngeo; ++g) {
         const oq_rhs *h = e->rhs[g];
         GrpGeo q{};
         q.K = static_cast<const float *>(h->K.p);
         q.N = h->N; q.ld = h->ld; q.nf = h->nf; q.ne = h->ne; q.n = h->lay.n;
-        q.soff = soff; q.foff = foff; q.Soff = Soff;
+        q.soff = soff; q.foff = foff; q.Soff = Soff; q.Yoff = yoff;
         q.lay = h->lay;
         q.gamma = h->dev.gamma; q.nexp = h->dev.nexp; q.nlaws = h->dev.nlaws;
         for (int k = 0; k < 6; ++k) q.deps0[k] = h->dev.deps0[k];
@@ -643,6 +803,8 @@ int oq_group_create(int device, int64_t ngeo, const oq_fault_mesh *mf,
         e->nst.push_back(h->lay.n);
         e->geo_bucket.push_back(bucket_of(h->N));
         poffs.push_back(poff);
+        yoffs.push_back(yoff);
+        yoff += h->N * M;
         soff += h->lay.n * M;
         foff += h->nf * M;
         Soff += (int64_t)kGM * h->ld;
@@ -650,8 +812,10 @@ int oq_group_create(int device, int64_t ngeo, const oq_fault_mesh *mf,
     }
     e->soffs.push_back(soff);
     poffs.push_back(poff);
+    yoffs.push_back(yoff);
     e->ntot = soff;
     e->nprolog = poff;
+    e->nrows = yoff;
     // per-member friction, entry-major per geometry
     std::vector<double> va((size_t)foff), vb((size_t)foff), vL((size_t)foff), vs((size_t)foff);
     std::vector<double> eta(ngeo * M), vpl(ngeo * M), f0(ngeo * M), v0(ngeo * M);
@@ -670,6 +834,8 @@ int oq_group_create(int device, int64_t ngeo, const oq_fault_mesh *mf,
     if (!rc) rc = e->dgeo.upload(e->geo.data(), e->geo.size() * sizeof(GrpGeo), e->stream);
     if (!rc) rc = e->dsoffs.upload(e->soffs.data(), e->soffs.size() * 8, e->stream);
     if (!rc) rc = e->dpoffs.upload(poffs.data(), poffs.size() * 8, e->stream);
+    if (!rc) rc = e->dyoffs.upload(yoffs.data(), yoffs.size() * 8, e->stream);
+    if (!rc) rc = e->Y.alloc((size_t)yoff * 8);
     if (!rc) rc = e->dnst.upload(e->nst.data(), e->nst.size() * 8, e->stream);
     if (!rc) rc = e->a.upload(va.data(), f8, e->stream);
     if (!rc) rc = e->b.upload(vb.data(), f8, e->stream);
@@ -694,12 +860,14 @@ int oq_group_create(int device, int64_t ngeo, const oq_fault_mesh *mf,
         if (!rc) rc = bp->alloc((size_t)GM * 4);
     if (!rc) {
         int gr[kGShapes];
-        rc = occupancy_grid<1, true>(&gr[0]);
-        if (!rc) rc = occupancy_grid<1, false>(&gr[1]);
-        if (!rc) rc = occupancy_grid<2, true>(&gr[2]);
-        if (!rc) rc = occupancy_grid<2, false>(&gr[3]);
-        if (!rc) rc = occupancy_grid<4, true>(&gr[4]);
-        if (!rc) rc = occupancy_grid<4, false>(&gr[5]);
+        rc = occupancy_grid(k_grp_gemm<1, true>, &gr[0]);
+        if (!rc) rc = occupancy_grid(k_grp_gemm<1, false>, &gr[1]);
+        if (!rc) rc = occupancy_grid(k_grp_gemm<2, true>, &gr[2]);
+        if (!rc) rc = occupancy_grid(k_grp_gemm<2, false>, &gr[3]);
+        if (!rc) rc = occupancy_grid(k_grp_gemm<4, true>, &gr[4]);
+        if (!rc) rc = occupancy_grid(k_grp_gemm<4, false>, &gr[5]);
+        if (!rc) rc = occupancy_grid(k_grp_gemm_lds<1>, &gr[6]);
+        if (!rc) rc = occupancy_grid(k_grp_gemm_lds<2>, &gr[7]);
         for (int s = 0; s < kGShapes; ++s) e->grid[s] = gr[s];
     }
     if (!rc) rc = upload_tiles(e);
@@ -714,6 +882,9 @@ int oq_group_create(int device, int64_t ngeo, const oq_fault_mesh *mf,
     d.a = e->a.as<double>(); d.b = e->b.as<double>(); d.L = e->L.as<double>(); d.sigma = e->sigma.as<double>();
     d.eta = e->eta.as<double>(); d.vpl = e->vpl.as<double>(); d.f0 = e->f0.as<double>(); d.v0 = e->v0.as<double>();
     d.S = e->S.as<float>();
+    d.Y = e->Y.as<double>();
+    d.yoffs = e->dyoffs.as<int64_t>();
+    d.nrows = e->nrows;
     d.nprolog = e->nprolog;
     d.ntot = e->ntot;
     GrpCtrl &c = e->ctrl;

@@ -171,24 +171,6 @@ __device__ __forceinline__ double wave_sum(double x)
     return x;
 }
 
-// Rate-and-state update of one fault patch (update_fault! with the aging
-// law, equation.jl:237-245 and :279).
-__device__ __forceinline__ void fault_update(const RhsDev &d, int64_t i, double dtau,
-                                            const double *__restrict__ u, double *__restrict__ du)
-{
-    const double v = u[d.lay.v + i], th = u[d.lay.th + i];
-    const double a = d.a[i], b = d.b[i], L = d.L[i];
-    const double thp = th > 0.0 ? th : 0.0;
-    const double psi1 = exp((d.f0 + b * log(d.v0 * thp / L)) / a) / (2.0 * d.v0);
-    const double psi2 = d.sigma[i] * psi1 / hypot(1.0, v * psi1);
-    const double dmu_dv = a * psi2;
-    const double dmu_dth = b / th * v * psi2;
-    const double dth = 1.0 - v * th / L;
-    du[d.lay.th + i] = dth;
-    du[d.lay.v + i] = (dtau - dmu_dth * dth) / (dmu_dv + d.eta);
-    du[d.lay.del + i] = v;
-}
-
 // One wavefront streams KTraits::kRows consecutive rows.  Each wavefront
 // starts its sweep over the columns at a different pass (rotation by a
 // multiple of its index): with all wavefronts reading the same column

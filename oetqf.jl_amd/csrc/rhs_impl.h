@@ -40,6 +40,34 @@ struct RhsDev {
     int64_t N, ld;
 };
 
+// Rate-and-state rates of fault patch i (update_fault! with the aging law,
+// equation.jl:237-245 and :279): dv/dt and dtheta/dt from v, theta and the
+// traction rate dtau.
+__device__ __forceinline__ void fault_rates(const RhsDev &d, int64_t i, double v, double th,
+                                            double dtau, double &dv, double &dth)
+{
+    const double a = d.a[i], b = d.b[i], L = d.L[i];
+    const double thp = th > 0.0 ? th : 0.0;
+    const double psi1 = exp((d.f0 + b * log(d.v0 * thp / L)) / a) / (2.0 * d.v0);
+    const double psi2 = d.sigma[i] * psi1 / hypot(1.0, v * psi1);
+    const double dmu_dv = a * psi2;
+    const double dmu_dth = b / th * v * psi2;
+    dth = 1.0 - v * th / L;
+    dv = (dtau - dmu_dth * dth) / (dmu_dv + d.eta);
+}
+
+// The same on the packed state vectors: du gets (dv, dtheta, ddelta = v).
+__device__ __forceinline__ void fault_update(const RhsDev &d, int64_t i, double dtau,
+                                            const double *__restrict__ u, double *__restrict__ du)
+{
+    const double v = u[d.lay.v + i], th = u[d.lay.th + i];
+    double dv, dth;
+    fault_rates(d, i, v, th, dtau, dv, dth);
+    du[d.lay.th + i] = dth;
+    du[d.lay.v + i] = dv;
+    du[d.lay.del + i] = v;
+}
+
 }  // namespace oq
 
 struct oq_rhs {

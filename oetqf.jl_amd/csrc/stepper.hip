@@ -213,6 +213,262 @@ __global__ void __launch_bounds__(256) k_record(const Ctrl *__restrict__ c, Reco
     }
 }
 
+// ---------------- persistent stepper for small fault-only problems ----------------
+// For a few hundred to a few thousand patches the graph above is bound by
+// its ~23 kernel launches per step, not by arithmetic.  Here one cooperative
+// launch runs up to max_steps attempted steps: workgroup w owns patches
+// [w R, w R + R) -- their rows of K live in its LDS, and so do their entries
+// of u, the stage state and the seven stages.  Each RHS needs the whole
+// source vector s = v - vpl, published through device memory and one grid
+// barrier (two buffers by stage parity, so a workgroup that is ahead never
+// overwrites a vector another is still reading); the error norm and max|v|
+// partials need one more.  Every workgroup then runs the same controller on
+// the same sums in the same order, so all copies of t, dt and the step
+// counters stay identical without further communication.  Barrier waits
+// are bounded by a wall-clock limit: a workgroup that times out raises the
+// abort flag and every workgroup leaves at its next barrier.
+struct PersistArgs {
+    RhsDev d;
+    const double *K;   // FP64 operator, N x ld
+    Ctrl *ctrl;
+    double *u, *k1;    // packed state and the FSAL stage
+    double *s2;        // 2 x N source vectors
+    double *part;      // 2 x nwg: squared-error sums, max |v|
+    unsigned *bar;     // barrier arrivals (zero at launch)
+    int *abort;
+    Recorder rec;
+    int recording, R, nwg;
+    long long max_steps, wait_ticks;
+};
+
+__device__ __forceinline__ double wave_sum64(double x)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+
+__device__ __forceinline__ bool grid_barrier(const PersistArgs &a, unsigned &epoch, int *sh_bad)
+{
+    __syncthreads();
+    ++epoch;
+    if (threadIdx.x == 0) {
+        const unsigned target = epoch * (unsigned)a.nwg;
+        __hip_atomic_fetch_add(a.bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        const long long t0 = (long long)wall_clock64();
+        int bad = 0;
+        while (__hip_atomic_load(a.bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(a.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                bad = 1;
+                break;
+            }
+            if ((long long)wall_clock64() - t0 > a.wait_ticks) {
+                __hip_atomic_store(a.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                bad = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        *sh_bad = bad;
+    }
+    __syncthreads();
+    return *sh_bad == 0;
+}
+
+__global__ void __launch_bounds__(256) k_persist(PersistArgs a)
+{
+    extern __shared__ double lds[];
+    __shared__ Ctrl c;
+    __shared__ double red[256], red2[256], tau[64];
+    __shared__ int bad, take;
+    __shared__ long long nstep, nsnap;
+    const int R = a.R, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t N = a.d.N, ld = a.d.ld, n = a.d.lay.n;
+    const int64_t p0 = (int64_t)blockIdx.x * R;
+    const int Rv = (int)((N - p0) < R ? (N - p0) : R);  // owned patches (>= 1)
+    const int E = 3 * R;                                 // owned entries: [v | th | del]
+    double *Ks = lds;                                    // R x ld
+    double *sv = Ks + (size_t)R * ld;                    // ld
+    double *uo = sv + ld, *yo = uo + E, *ko = yo + E;    // E, E, 7 x E
+    const int64_t comp_off[3] = {a.d.lay.v, a.d.lay.th, a.d.lay.del};
+
+    for (int64_t x = tid; x < (int64_t)Rv * ld; x += 256) Ks[x] = a.K[p0 * ld + x];
+    for (int64_t j = tid; j < ld; j += 256) sv[j] = 0.0;
+    for (int e = tid; e < E; e += 256) {
+        const int cpt = e / R, i = e % R;
+        uo[e] = i < Rv ? a.u[comp_off[cpt] + p0 + i] : 0.0;
+        ko[e] = i < Rv ? a.k1[comp_off[cpt] + p0 + i] : 0.0;
+    }
+    if (tid == 0) {
+        c = *a.ctrl;
+        nstep = a.recording ? *a.rec.nstep : 0;
+        nsnap = a.recording ? *a.rec.nsnap : 0;
+    }
+    __syncthreads();
+    unsigned epoch = 0;
+    for (long long it = 0; it < a.max_steps; ++it) {
+        if (tid == 0) {  // k_step_begin
+            c.accept = 0;
+            if (!c.done) {
+                if (!(c.t < c.t1)) {
+                    c.done = 1;
+                    c.retcode = OQ_RET_SUCCESS;
+                } else if (c.iter >= c.maxiters) {
+                    c.done = 1;
+                    c.retcode = OQ_RET_MAXITERS;
+                } else {
+                    double dt = c.dt;
+                    if (dt > c.dtmax) dt = c.dtmax;
+                    if (c.t + dt > c.t1) dt = c.t1 - c.t;
+                    if (!(dt >= c.dtmin) || !isfinite(dt)) {
+                        c.done = 1;
+                        c.retcode = isfinite(dt) ? OQ_RET_DTMIN : OQ_RET_NONFINITE;
+                    } else {
+                        c.dtcur = dt;
+                        c.iter += 1;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (c.done) break;
+        const double dt = c.dtcur;
+        for (int st = 1; st < 7; ++st) {
+            for (int e = tid; e < E; e += 256) {
+                double acc = 0.0;
+                for (int j = 0; j < st; ++j) acc += kTsA[st][j] * ko[j * E + e];
+                yo[e] = uo[e] + dt * acc;
+            }
+            __syncthreads();
+            double *sg = a.s2 + (st & 1) * N;
+            for (int i = tid; i < Rv; i += 256) sg[p0 + i] = yo[i] - a.d.vpl;
+            if (!grid_barrier(a, epoch, &bad)) return;
+            for (int64_t j = tid; j < N; j += 256) sv[j] = sg[j];
+            __syncthreads();
+            for (int r = wave; r < Rv; r += 4) {
+                const double *kr = Ks + (size_t)r * ld;
+                double acc = 0.0;
+                for (int64_t j = lane; j < ld; j += 64) acc += kr[j] * sv[j];
+                acc = wave_sum64(acc);
+                if (lane == 0) tau[r] = acc;
+            }
+            __syncthreads();
+            for (int r = tid; r < Rv; r += 256) {  // one patch per thread
+                double dv, dth;
+                fault_rates(a.d, p0 + r, yo[r], yo[R + r], tau[r], dv, dth);
+                ko[st * E + r] = dv;
+                ko[st * E + R + r] = dth;
+                ko[st * E + 2 * R + r] = yo[r];
+            }
+            __syncthreads();
+        }
+        // squared scaled error and max |v| of the owned entries
+        double acc = 0.0, vm = 0.0;
+        for (int e = tid; e < E; e += 256) {
+            if (e % R >= Rv) continue;
+            double er = 0.0;
+#pragma unroll
+            for (int j = 0; j < 7; ++j) er += kTsBt[j] * ko[j * E + e];
+            const double w = dt * er / (c.abstol + fmax(fabs(uo[e]), fabs(yo[e])) * c.reltol);
+            acc += w * w;
+            if (e < R) vm = fmax(vm, fabs(yo[e]));
+        }
+        red[tid] = acc;
+        red2[tid] = vm;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if (tid < o) {
+                red[tid] += red[tid + o];
+                red2[tid] = fmax(red2[tid], red2[tid + o]);
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            a.part[2 * blockIdx.x] = red[0];
+            a.part[2 * blockIdx.x + 1] = red2[0];
+        }
+        if (!grid_barrier(a, epoch, &bad)) return;
+        for (int w = tid; w < 2 * a.nwg; w += 256) red[w] = a.part[w];  // all loads in flight at once
+        __syncthreads();
+        if (tid == 0) {  // k_controller, on the same sums in every workgroup
+            double sum = 0.0, vmax = 0.0;
+            for (int w = 0; w < a.nwg; ++w) {
+                sum += red[2 * w];
+                vmax = fmax(vmax, red[2 * w + 1]);
+            }
+            const double beta1 = 7.0 / 50.0, beta2 = 2.0 / 25.0, qmin = 0.2, qmax = 10.0,
+                         gamma = 0.9, qoldinit = 1e-4;
+            const double eest = sqrt(sum / (double)n);
+            double q11, q;
+            c.eest = eest;
+            if (eest == 0.0) {
+                q11 = 0.0;
+                q = 1.0 / qmax;
+            } else {
+                q11 = pow(eest, beta1);
+                q = q11 / pow(c.qold, beta2);
+                q = fmax(1.0 / qmax, fmin(1.0 / qmin, q / gamma));
+            }
+            take = 0;
+            if (eest <= 1.0) {
+                c.t = (c.t1 - (c.t + c.dtcur) <= 0.0) ? c.t1 : c.t + c.dtcur;
+                c.qold = fmax(eest, qoldinit);
+                c.dt = c.dtcur / q;
+                c.naccept += 1;
+                c.accept = 1;
+                if (a.recording) {  // k_record
+                    if (blockIdx.x == 0 && a.rec.step_cap > 0) {
+                        a.rec.step_t[nstep % a.rec.step_cap] = c.t;
+                        a.rec.step_vmax[nstep % a.rec.step_cap] = vmax;
+                    }
+                    nstep += 1;
+                    take = (a.rec.snap_cap > 0) && (c.naccept % a.rec.stride == 0);
+                }
+            } else {
+                c.nreject += 1;
+                if (!isfinite(eest)) {
+                    c.done = 1;
+                    c.retcode = OQ_RET_NONFINITE;
+                } else {
+                    c.dt = c.dtcur / fmin(1.0 / qmin, q11 / gamma);
+                }
+            }
+        }
+        __syncthreads();
+        if (c.accept) {  // k_commit (FSAL: k1 <- k7)
+            for (int e = tid; e < E; e += 256) {
+                uo[e] = yo[e];
+                ko[e] = ko[6 * E + e];
+            }
+            if (take) {
+                const long long js = nsnap % a.rec.snap_cap;
+                for (int e = tid; e < E; e += 256)
+                    if (e % R < Rv) a.rec.snap_u[js * n + comp_off[e / R] + p0 + e % R] = yo[e];
+                if (blockIdx.x == 0 && tid == 0) a.rec.snap_t[js] = c.t;
+            }
+            __syncthreads();
+            if (tid == 0 && take) nsnap += 1;
+        }
+        __syncthreads();
+    }
+    // hand the state back for the next launch (or the graph stepper)
+    for (int e = tid; e < E; e += 256) {
+        const int cpt = e / R, i = e % R;
+        if (i < Rv) {
+            a.u[comp_off[cpt] + p0 + i] = uo[e];
+            a.k1[comp_off[cpt] + p0 + i] = ko[e];
+        }
+    }
+    if (blockIdx.x == 0 && tid == 0) {
+        *a.ctrl = c;
+        if (a.recording) {
+            *a.rec.nstep = nstep;
+            *a.rec.nsnap = nsnap;
+        }
+    }
+}
+
 }  // namespace oq
 
 struct oq_stepper {
@@ -226,6 +482,12 @@ struct oq_stepper {
     oq::DevBuf rcnt, rstep_t, rstep_v, rsnap_t, rsnap_u;
     oq::Recorder rec{};
     bool recording = false;
+    // persistent mode (oq_stepper_set_persistent)
+    bool persistent = false;
+    int p_R = 0, p_nwg = 0;
+    size_t p_lds = 0;
+    long long p_wait = 0;
+    oq::DevBuf p_s2, p_part, p_bar, p_abort;
 };
 
 namespace oq {
@@ -402,6 +664,40 @@ int oq_stepper_run(oq_stepper *st, int64_t max_steps, int *retcode)
     OQ_REQUIRE(st != nullptr && max_steps > 0, "invalid argument");
     oq_rhs *h = st->h;
     OQ_HIP(hipSetDevice(h->device));
+    if (st->persistent) {
+        PersistArgs a{};
+        a.d = h->dev;
+        a.K = h->K.as<double>();
+        a.ctrl = st->ctrl.as<Ctrl>();
+        a.u = st->u.as<double>();
+        a.k1 = st->k.as<double>();
+        a.s2 = st->p_s2.as<double>();
+        a.part = st->p_part.as<double>();
+        a.bar = st->p_bar.as<unsigned>();
+        a.abort = st->p_abort.as<int>();
+        a.rec = st->rec;
+        a.recording = st->recording ? 1 : 0;
+        a.R = st->p_R;
+        a.nwg = st->p_nwg;
+        a.max_steps = max_steps;
+        a.wait_ticks = st->p_wait;
+        OQ_HIP(hipMemsetAsync(st->p_bar.p, 0, sizeof(unsigned), h->stream));
+        void *args[] = {&a};
+        OQ_HIP(hipLaunchCooperativeKernel((const void *)k_persist, dim3(st->p_nwg), dim3(256), args,
+                                          st->p_lds, h->stream));
+        int ab = 0;
+        OQ_HIP(hipMemcpyAsync(&ab, st->p_abort.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+        OQ_HIP(hipMemcpyAsync(st->ctrl_host, st->ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost,
+                              h->stream));
+        OQ_HIP(hipStreamSynchronize(h->stream));
+        if (ab) {
+            st->persistent = false;
+            set_error("persistent stepper: a grid barrier timed out; the state is not valid");
+            return OQ_ERR_STATE;
+        }
+        if (retcode) *retcode = st->ctrl_host->done ? st->ctrl_host->retcode : OQ_RET_RUNNING;
+        return OQ_OK;
+    }
     const int64_t chunk = 16;
     int64_t done_steps = 0;
     while (done_steps < max_steps) {
@@ -414,6 +710,53 @@ int oq_stepper_run(oq_stepper *st, int64_t max_steps, int *retcode)
         if (st->ctrl_host->done) break;
     }
     if (retcode) *retcode = st->ctrl_host->done ? st->ctrl_host->retcode : OQ_RET_RUNNING;
+    return OQ_OK;
+}
+
+int oq_stepper_set_persistent(oq_stepper *st, int on)
+{
+    OQ_REQUIRE(st != nullptr, "stepper is NULL");
+    oq_rhs *h = st->h;
+    OQ_HIP(hipSetDevice(h->device));
+    if (!on) {
+        st->persistent = false;
+        return OQ_OK;
+    }
+    OQ_REQUIRE(h->ne == 0, "the persistent stepper takes fault-only problems");
+    OQ_REQUIRE(h->kdtype == OQ_K_FP64, "the persistent stepper takes an FP64 operator");
+    int coop = 0, cus = 0, maxlds = 0, khz = 0;
+    OQ_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, h->device));
+    OQ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
+    OQ_HIP(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device));
+    OQ_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, h->device));
+    OQ_REQUIRE(coop != 0, "the device has no cooperative launch");
+    // rows of K per workgroup: as many as fit in 128 KB of LDS
+    const int64_t rows = (int64_t)(131072 / (h->ld * 8));
+    OQ_REQUIRE(rows >= 1, "the operator rows are too long for the persistent stepper (N > 16384)");
+    const int R = (int)(rows < 64 ? rows : 64);
+    const int nwg = (int)((h->N + R - 1) / R);
+    const size_t lds = ((size_t)R * h->ld + h->ld + (size_t)9 * 3 * R) * 8;
+    OQ_REQUIRE((int64_t)lds <= (int64_t)maxlds, "the persistent stepper does not fit in LDS");
+    OQ_HIP(hipFuncSetAttribute((const void *)k_persist, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds));
+    int per_cu = 0;
+    OQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_persist, 256, lds));
+    OQ_REQUIRE(per_cu >= 1 && (int64_t)per_cu * cus >= nwg,
+               "the persistent stepper needs more co-resident workgroups than the device holds");
+    OQ_REQUIRE(nwg <= 128, "the persistent stepper takes at most 128 workgroups (N <= 1024)");
+    int rc = st->p_s2.alloc((size_t)(2 * h->N) * 8);
+    if (!rc) rc = st->p_part.alloc((size_t)(2 * nwg) * 8);
+    if (!rc) rc = st->p_bar.alloc(sizeof(unsigned));
+    if (!rc) rc = st->p_abort.alloc(sizeof(int));
+    if (rc) return rc;
+    OQ_HIP(hipMemsetAsync(st->p_abort.p, 0, sizeof(int), h->stream));
+    OQ_HIP(hipStreamSynchronize(h->stream));
+    st->p_R = R;
+    st->p_nwg = nwg;
+    st->p_lds = lds;
+    // a barrier wait longer than 5 s means a workgroup is gone: abort
+    st->p_wait = (long long)(khz > 0 ? khz : 100000) * 5000;
+    st->persistent = true;
     return OQ_OK;
 }
 

@@ -12,6 +12,7 @@ module OetqfAMD
 
 using Oetqf
 using RecursiveArrayTools: ArrayPartition
+import HDF5
 
 const libpath = joinpath(@__DIR__, "..", "lib", "liboetqf_amd.so")
 
@@ -225,6 +226,59 @@ function solve_gpu(prob; reltol=1e-6, abstol=1e-8, dt=0.0, dtmax=0.0, dtmin=0.0,
     ccall((:oq_stepper_destroy, libpath), Cint, (Ptr{Cvoid},), s[])
     out = similar(prob.u0); copyto!(out, u)
     return t[], out, (:Running, :Success, :MaxIters, :DtLessThanMin, :NonFinite)[rc[] + 1]
+end
+
+"""wsolve on the device (io.jl:118-133 semantics): every `stride`-th accepted
+state is recorded on the GPU, drained every `nstep` snapshots and appended
+through `getu(u, t, nothing)` to HDF5 datasets `ustrs` and `tstr` (time is the
+trailing, extendible dimension)."""
+function wsolve_gpu(prob, file, nstep, getu, ustrs, tstr; stride=1, append=false, force=false,
+                    reltol=1e-6, abstol=1e-8, dt=0.0, dtmax=0.0, dtmin=0.0, maxiters=10^9)
+    if isfile(file) && !force && !append
+        @info "Overwrite existing file $(file) must set `force = true`."
+        return nothing
+    end
+    @assert tstr ∉ ustrs "Duplicate name of $tstr in $ustrs."
+    f = prob.p[end]::GPURHS
+    u0 = collect(Float64, prob.u0)
+    s = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:oq_stepper_create, libpath), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Float64, Float64, Ref{CSolverOpts}, Ref{Ptr{Cvoid}}),
+        f.h, u0, prob.tspan[1], prob.tspan[2], Ref(CSolverOpts(reltol, abstol, dt, dtmax, dtmin, maxiters)), s))
+    check(ccall((:oq_stepper_record, libpath), Cint, (Ptr{Cvoid}, Int64, Int64, Int64), s[], stride, nstep, 0))
+    h5 = HDF5.h5open(file, append ? "r+" : "w")
+    function save(u, t)
+        xs = getu(u, t, nothing)
+        for (name, x) in zip(ustrs, xs)
+            if !haskey(h5, name)
+                HDF5.create_dataset(h5, name, Float64, ((size(x)..., 0), (size(x)..., -1)); chunk=(size(x)..., nstep))
+            end
+            ds = h5[name]; n = size(ds)[end] + 1
+            HDF5.set_extent_dims(ds, (size(x)..., n)); ds[ntuple(_ -> Colon(), ndims(x))..., n] = x
+        end
+        haskey(h5, tstr) || HDF5.create_dataset(h5, tstr, Float64, ((0,), (-1,)); chunk=(nstep,))
+        dt_ = h5[tstr]; n = length(dt_) + 1
+        HDF5.set_extent_dims(dt_, (n,)); dt_[n] = t
+    end
+    save(prob.u0, prob.tspan[1])
+    rc = Ref{Cint}(0)
+    nsn = Ref{Int64}(0); nst = Ref{Int64}(0)
+    snap_t = Vector{Float64}(undef, nstep); snap_u = Vector{Float64}(undef, nstep * length(u0))
+    ubuf = similar(prob.u0)
+    while rc[] == 0
+        check(ccall((:oq_stepper_run, libpath), Cint, (Ptr{Cvoid}, Int64, Ref{Cint}), s[], nstep, rc))
+        check(ccall((:oq_stepper_drain, libpath), Cint,
+            (Ptr{Cvoid}, Ref{Int64}, Ptr{Float64}, Ptr{Float64}, Ref{Int64}, Ptr{Float64}, Ptr{Float64}),
+            s[], nst, C_NULL, C_NULL, nsn, snap_t, snap_u))
+        for k in 1:nsn[]
+            copyto!(ubuf, view(snap_u, (k - 1) * length(u0) + 1:k * length(u0)))
+            save(ubuf, snap_t[k])
+        end
+        flush(h5)
+    end
+    close(h5)
+    ccall((:oq_stepper_destroy, libpath), Cint, (Ptr{Cvoid},), s[])
+    return (:Running, :Success, :MaxIters, :DtLessThanMin, :NonFinite)[rc[] + 1]
 end
 
 end # module

@@ -86,6 +86,7 @@ PROTOTYPES = {
     "oq_stepper_state": (_int, [_vp, ctypes.POINTER(_d), ctypes.POINTER(_d), _pd,
                                 ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
     "oq_stepper_destroy": (_int, [_vp]),
+    "oq_stepper_set_persistent": (_int, [_vp, _int]),
     "oq_stepper_record": (_int, [_vp, _i64, _i64, _i64]),
     "oq_stepper_drain": (_int, [_vp, ctypes.POINTER(_i64), _pd, _pd, ctypes.POINTER(_i64), _pd, _pd]),
     "oq_ensemble_create": (_int, [_int, ctypes.POINTER(FaultMesh), ctypes.POINTER(Hex8Mesh), _d, _d,

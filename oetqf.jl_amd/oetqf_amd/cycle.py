@@ -13,11 +13,22 @@ from .equation import Integrator
 
 
 def run_cycle(prob, reltol=1e-6, abstol=1e-8, dt=0.0, dtmax=0.0, chunk=4096, wall_limit=None,
-              progress=None):
+              progress=None, persistent=False):
     """Integrates prob over its tspan on the device.  Returns a dict with the
-    per-step record (t, vmax), the final state and counters."""
+    per-step record (t, vmax), the final state and counters.  persistent:
+    True = one cooperative launch per chunk of steps (small fault-only
+    problems; ValueError otherwise), "auto" = that where the problem
+    qualifies, else the graph stepper."""
     it = Integrator(prob, reltol=reltol, abstol=abstol, dt=dt, dtmax=dtmax)
     it.record(stride=1, snap_capacity=0, step_capacity=chunk)
+    mode = "graph"
+    if persistent:
+        try:
+            it.set_persistent(True)
+            mode = "persistent"
+        except ValueError:
+            if persistent != "auto":
+                raise
     ts, vs = [], []
     t0 = time.perf_counter()
     ret = "Running"
@@ -34,7 +45,7 @@ def run_cycle(prob, reltol=1e-6, abstol=1e-8, dt=0.0, dtmax=0.0, chunk=4096, wal
     t, dtl, u, na, nr = it.state()
     it.close()
     return dict(t=np.concatenate(ts), vmax=np.concatenate(vs), retcode=ret, t_end=t, u=u,
-                naccept=na, nreject=nr, wall=wall)
+                naccept=na, nreject=nr, wall=wall, mode=mode)
 
 
 def events(step_t, step_vmax, vthresh=1e-3, merge_gap=0.0):

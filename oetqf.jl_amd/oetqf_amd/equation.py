@@ -243,6 +243,11 @@ class Integrator:
                                                 ctypes.byref(nr)))
         return t.value, dt.value, self.u0.similar().unpack(flat), na.value, nr.value
 
+    def set_persistent(self, on=True):
+        """Runs whole chunks of steps in one cooperative launch (fault-only
+        problems whose operator rows fit in LDS); ValueError otherwise."""
+        _lib.check(_lib.load().oq_stepper_set_persistent(self.s, 1 if on else 0))
+
     def record(self, stride=1, snap_capacity=0, step_capacity=4096):
         """Enables the device recorder: per accepted step (t, max|v|), and a
         state snapshot every `stride` accepted steps."""
