@@ -74,8 +74,11 @@ def scale_leg(oq, args):
     """The precision variant at scale: 256x64 fault + 32x32x8 mantle
     (N = 65536, K 34.4 GB in FP64).  Per storage type: device assembly time,
     RHS rate and HBM rate, RHS error against FP64 on a perturbed state, and
-    a trajectory of `scale_years` from the interseismic start whose end
-    state (slip rate, slip) is compared with the FP64 trajectory."""
+    a trajectory of `scale_years` from the interseismic start with its
+    event catalogue (max V > 1e-3 m/s, crossings within 1 yr merged); when
+    FP64 is among the types, the others' end state and catalogue are
+    compared with it."""
+    from oetqf_amd.cycle import events, recurrence, run_cycle
     from oetqf_amd.workloads import YEAR, precision_scale_65536
 
     w = precision_scale_65536()
@@ -90,7 +93,9 @@ def scale_leg(oq, args):
     out = dict(workload=w["name"], N=nf + 6 * ne, k_bytes_fp64=8 * (nf + 6 * ne) ** 2,
                trajectory_years=args.scale_years, reltol=w["reltol"], abstol=w["abstol"])
     ref = None
-    for kd, esz in (("fp64", 8), ("fp32", 4), ("bf16", 2)):
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    esz = {"fp64": 8, "fp32": 4, "bf16": 2}
+    for kd in args.scale_kdtypes.split(","):
         t0 = time.perf_counter()
         p = oq.assemble_from_mesh(mf, ma, w["lam"], w["mu"], w["pf"], w["pa"], w["u0"], span,
                                   nrept=w["nrept"], buffer_ratio=w["buffer_ratio"], kdtype=kd)
@@ -99,28 +104,37 @@ def scale_leg(oq, args):
         p.f(du, ug, None, 0.0)
         p.f.time(3)
         me, mm = p.f.time(20)
-        it = oq.Integrator(p, reltol=w["reltol"], abstol=w["abstol"])
-        t0 = time.perf_counter()
-        while it.step(256) == "Running":
-            pass
-        tw = time.perf_counter() - t0
-        t, _, ue, na, nr = it.state()
-        it.close()
+        last = [time.perf_counter()]
+
+        def prog(t, kd=kd):
+            if time.perf_counter() - last[0] > 30 and t is not None:
+                last[0] = time.perf_counter()
+                log("N=65536 %s: t = %.2f yr" % (kd, t / YEAR))
+        r = run_cycle(p, reltol=w["reltol"], abstol=w["abstol"], chunk=2048, progress=prog)
         p.f.close()
-        r = dict(assembly_s=ta, ms_per_rhs=me, ms_matvec=mm, rhs_per_s=1000.0 / me,
-                 hbm_GBps=algorithmic_bytes(nf, ne, esz) / (mm * 1e-3) / 1e9,
-                 trajectory_wall_s=tw, accepted=na, rejected=nr)
-        if ref is None:
-            ref = (du, ue)
-        else:
-            rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
-            r.update(rel_err_dv=rel(du.x[0], ref[0].x[0]), rel_err_dsigma=rel(du.x[3], ref[0].x[3]),
-                     traj_rel_err_v=rel(ue.x[0], ref[1].x[0]),
-                     traj_rel_err_slip=rel(ue.x[4], ref[1].x[4]))
-        out[kd] = r
-        log("N=65536 %s: assembly %.1f s, %.3f ms/RHS (%.0f GB/s), %.1f yr in %.1f s (%d+%d)%s" % (
-            kd, ta, me, r["hbm_GBps"], t / YEAR, tw, na, nr,
-            "" if kd == "fp64" else ", err dv %.1e traj v %.1e" % (r["rel_err_dv"], r["traj_rel_err_v"])))
+        ev = events(r["t"], r["vmax"], 1e-3, merge_gap=YEAR)
+        res = dict(assembly_s=ta, ms_per_rhs=me, ms_matvec=mm, rhs_per_s=1000.0 / me,
+                   hbm_GBps=algorithmic_bytes(nf, ne, esz[kd]) / (mm * 1e-3) / 1e9,
+                   trajectory_wall_s=r["wall"], sim_yr_per_hr=args.scale_years / (r["wall"] / 3600.0),
+                   accepted=int(r["naccept"]), rejected=int(r["nreject"]), retcode=r["retcode"],
+                   events=len(ev), onsets_yr=[e["onset"] / YEAR for e in ev],
+                   peak_v=[e["peak_v"] for e in ev],
+                   recurrence_yr=[float(x) / YEAR for x in recurrence(ev)])
+        if kd == "fp64":
+            ref = (du, r["u"], ev)
+        elif ref is not None:
+            res.update(rel_err_dv=rel(du.x[0], ref[0].x[0]), rel_err_dsigma=rel(du.x[3], ref[0].x[3]),
+                       traj_rel_err_v=rel(r["u"].x[0], ref[1].x[0]),
+                       traj_rel_err_slip=rel(r["u"].x[4], ref[1].x[4]))
+            if len(ev) == len(ref[2]) and len(ev) > 0:
+                res["onset_rel_err_max"] = max(abs(a["onset"] - b["onset"]) / b["onset"]
+                                               for a, b in zip(ev, ref[2]))
+                if len(ev) > 1:
+                    rg, rc = recurrence(ev), recurrence(ref[2])
+                    res["recurrence_rel_err_max"] = float(np.max(np.abs(rg - rc) / rc))
+        out[kd] = res
+        log("N=65536 %s: assembly %.1f s, %.3f ms/RHS (%.0f GB/s), %.1f yr in %.1f s (%d+%d), %d events" % (
+            kd, ta, me, res["hbm_GBps"], args.scale_years, r["wall"], r["naccept"], r["nreject"], len(ev)))
     return out
 
 
@@ -212,6 +226,8 @@ def main():
                     help="skip the uneven-size grouped batch leg")
     ap.add_argument("--scale-years", type=float, default=1.0,
                     help="trajectory length of the N = 65536 leg")
+    ap.add_argument("--scale-kdtypes", default="fp64,fp32,bf16",
+                    help="operator storage types of the N = 65536 leg (fp64 first: the reference)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (known traffic) for counter scaling")
     args = ap.parse_args()

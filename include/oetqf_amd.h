@@ -292,12 +292,13 @@ int oq_group_create(int device, int64_t ngeo, const oq_fault_mesh *mf,
 /* Total state entries, per-geometry state size nstate[ngeo] and the bytes
  * of all operators (any pointer may be NULL). */
 int oq_group_info(oq_group *e, int64_t *ntot, int64_t *nstate, int64_t *kbytes);
-/* Times every tile shape (8: register-streaming kernels with 16/32/64 rows
+/* Times every tile shape (10: register-streaming kernels with 16/32/64 rows
  * per wavefront, each with and without a software-pipelined K stream, and
- * LDS-staged kernels with 16/32 rows per wavefront) on the geometries of
- * each size bucket (bucket b: N < 2048 << b; 8 buckets) and keeps the
- * fastest; shape_of_bucket[8] and ms_of_bucket[8] receive the choice (-1
- * for an empty bucket), ms_table[8*8] (may be NULL) every timing.  Call before
+ * LDS-staged kernels with 16/32 rows per wavefront and one or two slices in
+ * flight) on the geometries of each size bucket (bucket b: N < 2048 << b;
+ * 8 buckets) and keeps the fastest; shape_of_bucket[8] and ms_of_bucket[8]
+ * receive the choice (-1 for an empty bucket), ms_table[8*10] (may be NULL)
+ * every timing.  Call before
  * oq_group_start (a running stepper must be started again). */
 int oq_group_autotune(oq_group *e, int iters, int *shape_of_bucket, double *ms_of_bucket,
                       double *ms_table);

@@ -415,6 +415,117 @@ __global__ void __launch_bounds__(256) k_grp_gemm_lds(GrpDev d, const int2 *__re
     }
 }
 
+// Depth-2 variant: two register sets alternate, so each workgroup has the
+// next two 32 KB slices in flight while it reduces the one in LDS.
+template <int RB>
+__global__ void __launch_bounds__(256) k_grp_gemm_lds2(GrpDev d, const int2 *__restrict__ tiles,
+                                                       int ntiles, const double *__restrict__ u,
+                                                       double *__restrict__ du)
+{
+    constexpr int BR = 64 * RB, BK = 128 / RB, PAD = BK + 4, C4 = BK / 4;
+    constexpr int KV = BR * C4 / 256, SV = 16 * C4 / 256, NC = 4 / RB, RS = 256 / C4;
+    __shared__ __attribute__((aligned(16))) float As[BR * PAD];
+    __shared__ __attribute__((aligned(16))) float Bs[16 * PAD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lr = lane & 15, lq = lane >> 4;
+    const int trow = tid / C4, tc4 = tid % C4;
+    for (int tix = blockIdx.x; tix < ntiles; tix += gridDim.x) {
+        const int2 tl = tiles[tix];
+        const GrpGeo &G = d.geo[tl.x];
+        const int64_t N = G.N, ld = G.ld, r0 = tl.y;
+        const int64_t nch = ld / BK;
+        const int64_t pstart = ((int64_t)tix * 13) % nch;
+        const float *kp[KV];
+#pragma unroll
+        for (int j = 0; j < KV; ++j) {
+            const int64_t r = r0 + trow + j * RS;
+            kp[j] = G.K + (r < N ? r : N - 1) * ld + 4 * tc4;
+        }
+        const float *sp[SV];
+#pragma unroll
+        for (int j = 0; j < SV; ++j) sp[j] = d.S + G.Soff + (int64_t)(trow + j * RS) * ld + 4 * tc4;
+        f4x acc[RB][NC];
+        float hi[RB][4], lo[RB][4];
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) acc[rb][c] = (f4x){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int v = 0; v < 4; ++v) hi[rb][v] = lo[rb][v] = 0.f;
+        }
+        auto col = [&](int64_t k) { const int64_t q = pstart + k; return (q >= nch ? q - nch : q) * BK; };
+        f4x ka[KV], sa[SV], kb[KV], sb[SV];
+        auto load = [&](f4x (&kr)[KV], f4x (&sr)[SV], int64_t k) {
+            const int64_t c0 = col(k);
+#pragma unroll
+            for (int j = 0; j < KV; ++j) kr[j] = gload4(kp[j] + c0);
+#pragma unroll
+            for (int j = 0; j < SV; ++j) sr[j] = gload4(sp[j] + c0);
+        };
+        auto store = [&](const f4x (&kr)[KV], const f4x (&sr)[SV]) {
+#pragma unroll
+            for (int j = 0; j < KV; ++j) *reinterpret_cast<f4x *>(&As[(trow + j * RS) * PAD + 4 * tc4]) = kr[j];
+#pragma unroll
+            for (int j = 0; j < SV; ++j) *reinterpret_cast<f4x *>(&Bs[(trow + j * RS) * PAD + 4 * tc4]) = sr[j];
+        };
+        auto reduce = [&](int64_t k) {
+#pragma unroll
+            for (int sub = 0; sub < BK / 64; ++sub) {
+                f4x a[RB][4], b[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    b[t] = *reinterpret_cast<const f4x *>(&Bs[lr * PAD + sub * 64 + 16 * t + 4 * lq]);
+#pragma unroll
+                    for (int rb = 0; rb < RB; ++rb)
+                        a[rb][t] = *reinterpret_cast<const f4x *>(
+                            &As[(wave * 16 * RB + rb * 16 + lr) * PAD + sub * 64 + 16 * t + 4 * lq]);
+                }
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+#pragma unroll
+                        for (int rb = 0; rb < RB; ++rb)
+                            acc[rb][t % NC] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                                a[rb][t][e], b[t][e], acc[rb][t % NC], 0, 0, 0);
+            }
+            if (((k + 1) * BK) % 256 == 0 || k + 1 == nch) {
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb) {
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) grp_two_sum(hi[rb][v], lo[rb][v], acc[rb][c][v]);
+                        acc[rb][c] = (f4x){0.f, 0.f, 0.f, 0.f};
+                    }
+                }
+            }
+        };
+        load(ka, sa, 0);
+        if (nch > 1) load(kb, sb, 1);
+        __syncthreads();  // the previous tile's reads of As/Bs are done
+        store(ka, sa);
+        __syncthreads();
+        if (nch > 2) load(ka, sa, 2);
+        for (int64_t k = 0; k < nch; k += 2) {
+            // LDS: slice k; kb: slice k+1; ka: slice k+2
+            reduce(k);
+            __syncthreads();
+            if (k + 1 < nch) store(kb, sb);
+            __syncthreads();
+            if (k + 3 < nch) load(kb, sb, k + 3);
+            if (k + 1 >= nch) break;
+            // LDS: slice k+1; ka: slice k+2; kb: slice k+3
+            reduce(k + 1);
+            __syncthreads();
+            if (k + 2 < nch) store(ka, sa);
+            __syncthreads();
+            if (k + 4 < nch) load(ka, sa, k + 4);
+        }
+        grp_store_rows<RB>(d, G, r0 + (int64_t)wave * 16 * RB, lr, lq, hi, lo);
+    }
+}
+
 // ---------------- lockstep Tsit5 over all members ----------------
 __constant__ double kGA[7][6] = {
     {0, 0, 0, 0, 0, 0},
@@ -555,9 +666,13 @@ __global__ void __launch_bounds__(256) k_grp_commit(GrpCtrl c, int64_t ntot,
 }
 
 // tile shapes: RB in {1, 2, 4} x {software-pipelined, plain} register
-// kernels, and the LDS-staged kernel with RB in {1, 2}
-static constexpr int kGShapes = 8;
-static inline int shape_rb(int s) { return s < 6 ? (s / 2 == 0 ? 1 : (s / 2 == 1 ? 2 : 4)) : s - 5; }
+// kernels, and the LDS-staged kernels with RB in {1, 2}, one or two slices
+// in flight
+static constexpr int kGShapes = 10;
+static inline int shape_rb(int s)
+{
+    return s < 6 ? (s / 2 == 0 ? 1 : (s / 2 == 1 ? 2 : 4)) : (s == 6 || s == 8 ? 1 : 2);
+}
 
 }  // namespace oq
 
@@ -653,7 +768,9 @@ static int launch_gemm(oq_group *e, int s, const int2 *tiles, int ntiles, const 
     case 4: hipLaunchKernelGGL((k_grp_gemm<4, true>), dim3(grid), dim3(256), 0, e->stream, e->dev, tiles, ntiles, u, du); break;
     case 5: hipLaunchKernelGGL((k_grp_gemm<4, false>), dim3(grid), dim3(256), 0, e->stream, e->dev, tiles, ntiles, u, du); break;
     case 6: hipLaunchKernelGGL((k_grp_gemm_lds<1>), dim3(grid), dim3(256), 0, e->stream, e->dev, tiles, ntiles, u, du); break;
-    default: hipLaunchKernelGGL((k_grp_gemm_lds<2>), dim3(grid), dim3(256), 0, e->stream, e->dev, tiles, ntiles, u, du); break;
+    case 7: hipLaunchKernelGGL((k_grp_gemm_lds<2>), dim3(grid), dim3(256), 0, e->stream, e->dev, tiles, ntiles, u, du); break;
+    case 8: hipLaunchKernelGGL((k_grp_gemm_lds2<1>), dim3(grid), dim3(256), 0, e->stream, e->dev, tiles, ntiles, u, du); break;
+    default: hipLaunchKernelGGL((k_grp_gemm_lds2<2>), dim3(grid), dim3(256), 0, e->stream, e->dev, tiles, ntiles, u, du); break;
     }
     OQ_HIP(hipGetLastError());
     return OQ_OK;
@@ -868,6 +985,8 @@ int oq_group_create(int device, int64_t ngeo, const oq_fault_mesh *mf,
         if (!rc) rc = occupancy_grid(k_grp_gemm<4, false>, &gr[5]);
         if (!rc) rc = occupancy_grid(k_grp_gemm_lds<1>, &gr[6]);
         if (!rc) rc = occupancy_grid(k_grp_gemm_lds<2>, &gr[7]);
+        if (!rc) rc = occupancy_grid(k_grp_gemm_lds2<1>, &gr[8]);
+        if (!rc) rc = occupancy_grid(k_grp_gemm_lds2<2>, &gr[9]);
         for (int s = 0; s < kGShapes; ++s) e->grid[s] = gr[s];
     }
     if (!rc) rc = upload_tiles(e);

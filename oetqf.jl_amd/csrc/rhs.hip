@@ -164,13 +164,6 @@ __device__ __forceinline__ double dot16(const u32x4 kv, const SVec<OQ_K_BF16> &s
     return (double)r + (double)q;
 }
 
-__device__ __forceinline__ double wave_sum(double x)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-    return x;
-}
-
 // One wavefront streams KTraits::kRows consecutive rows.  Each wavefront
 // starts its sweep over the columns at a different pass (rotation by a
 // multiple of its index): with all wavefronts reading the same column

@@ -40,6 +40,49 @@ struct RhsDev {
     int64_t N, ld;
 };
 
+// Cross-lane moves of a double as two 32-bit halves.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+__device__ __forceinline__ double swizzle_xor16_f64(double x)
+{
+    // ds_swizzle bit-mask mode: and 0x1F, or 0, xor 0x10 (lane ^ 16 within 32)
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_ds_swizzle((int)b, 0x401F);
+    const int hi = __builtin_amdgcn_ds_swizzle((int)(b >> 32), 0x401F);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+__device__ __forceinline__ double readlane_f64(double x, int lane)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Sum over the 64 lanes of a wavefront, the same value in every lane, in a
+// fixed order: DPP quad permutations (lane ^ 1, lane ^ 2), then the
+// half-row and row mirrors (lane <-> 7 - lane, 15 - lane: after the quad
+// steps all lanes of a quad hold the same value, so a mirror pairs whole
+// groups as an xor would), ds_swizzle across the rows of each 32-lane half,
+// and the two halves read out as scalars.
+__device__ __forceinline__ double wave_sum(double x)
+{
+    x += dpp_f64<0xB1>(x);   // quad_perm [1,0,3,2]
+    x += dpp_f64<0x4E>(x);   // quad_perm [2,3,0,1]
+    x += dpp_f64<0x141>(x);  // row_half_mirror
+    x += dpp_f64<0x140>(x);  // row_mirror
+    x += swizzle_xor16_f64(x);
+    return readlane_f64(x, 0) + readlane_f64(x, 32);
+}
+
 // Rate-and-state rates of fault patch i (update_fault! with the aging law,
 // equation.jl:237-245 and :279): dv/dt and dtheta/dt from v, theta and the
 // traction rate dtau.

@@ -241,13 +241,6 @@ struct PersistArgs {
     long long max_steps, wait_ticks;
 };
 
-__device__ __forceinline__ double wave_sum64(double x)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-    return x;
-}
-
 __device__ __forceinline__ bool grid_barrier(const PersistArgs &a, unsigned &epoch, int *sh_bad)
 {
     __syncthreads();
@@ -350,7 +343,7 @@ __global__ void __launch_bounds__(256) k_persist(PersistArgs a)
                 const double *kr = Ks + (size_t)r * ld;
                 double acc = 0.0;
                 for (int64_t j = lane; j < ld; j += 64) acc += kr[j] * sv[j];
-                acc = wave_sum64(acc);
+                acc = wave_sum(acc);
                 if (lane == 0) tau[r] = acc;
             }
             __syncthreads();
