@@ -70,6 +70,68 @@ def cpu_baseline(w, u0, sample_seconds):
                 sample="%d RHS evaluations of the same workload (%.1f s)" % (n, el)), out
 
 
+def cpu_extras(w_c1, seconds):
+    """More of the CPU restatement beside the GPU figures (bounded samples):
+    the 512-patch fault's RHS (Toeplitz DFT, the reference algorithm) and a
+    70-year stretch of its cycle (C Tsit5 over the dense operator), Green's
+    assembly of the 512-patch fault, and the composite model's assembly
+    extrapolated from a 64-receiver column sample of each block."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    from types import SimpleNamespace
+    from oetqf_amd.workloads import YEAR, strike_slip_512
+
+    out = {}
+    w = strike_slip_512()
+    mf, pf, u0 = w["mf"], w["pf"], w["u0"]
+    t0 = time.perf_counter()
+    st = oracle.gf_ff(mf, w["lam"], w["mu"])
+    out["c0_assembly_s"] = time.perf_counter() - t0
+    n, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < min(3.0, seconds):
+        oracle.rhs_fault(pf, st, u0.x[0], u0.x[1])
+        n += 1
+    out["c0_rhs_per_s"] = n / (time.perf_counter() - t0)
+    nf = mf.nx * mf.nxi
+    K = np.zeros((nf, nf))
+    for j in range(mf.nxi):
+        for l in range(mf.nxi):
+            blk = st[np.abs(np.arange(mf.nx)[:, None] - np.arange(mf.nx)[None, :]), j, l]
+            K[j * mf.nx:(j + 1) * mf.nx, l * mf.nx:(l + 1) * mf.nx] = blk
+    yrs = 70.0
+    t0 = time.perf_counter()
+    _, stt, _, _ = oracle.cycle_dense(pf, None, K, nf, 0, u0.pack(), 0.0, yrs * YEAR)
+    el = time.perf_counter() - t0
+    out["c0_cycle"] = dict(sim_years=yrs, wall_s=el, sim_yr_per_hr=yrs / (el / 3600.0),
+                           accepted=int(stt.naccept),
+                           note="first 70 years (two events) of the 300-year run")
+    # composite assembly: time every block on 64 receivers, scale by rows
+    mf1, ma1 = w_c1["mf"], w_c1["ma"]
+    sel = np.linspace(0, ma1.ne - 1, 64).astype(int)
+    msub = SimpleNamespace(**{k: getattr(ma1, k)[sel] for k in
+                              ("cx", "cy", "cz", "qx", "qy", "qz", "dx", "dy", "dz")})
+    I, J = np.linspace(0, mf1.nx - 1, 16).astype(int), np.linspace(0, mf1.nxi - 1, 4).astype(int)
+    fsub = SimpleNamespace(x=mf1.x[I], ax=mf1.ax[I], y=mf1.y[J], z=mf1.z[J], axi=mf1.axi[J],
+                           nx=len(I), nxi=len(J), dx=mf1.dx, dep=mf1.dep, dip=mf1.dip)
+    t0 = time.perf_counter()
+    oracle.gf_ff(mf1, w_c1["lam"], w_c1["mu"])
+    t_ff = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oracle.gf_mf(mf1, msub, w_c1["lam"], w_c1["mu"])
+    t_mf = (time.perf_counter() - t0) * ma1.ne / 64
+    t0 = time.perf_counter()
+    oracle.gf_fm(ma1, fsub, w_c1["lam"], w_c1["mu"])
+    t_fm = (time.perf_counter() - t0) * (mf1.nx * mf1.nxi) / 64
+    t0 = time.perf_counter()
+    oracle.gf_mm_rows(ma1, msub, w_c1["lam"], w_c1["mu"])
+    t_mm = (time.perf_counter() - t0) * ma1.ne / 64
+    out["c1_assembly_s_extrapolated"] = dict(ff=t_ff, mf=t_mf, fm=t_fm, mm=t_mm,
+                                             total=t_ff + t_mf + t_fm + t_mm,
+                                             note="K_ff complete; the other blocks timed on 64 "
+                                                  "receivers and scaled by the receiver count")
+    return out
+
+
 def scale_leg(oq, args):
     """The precision variant at scale: 256x64 fault + 32x32x8 mantle
     (N = 65536, K 34.4 GB in FP64).  Per storage type: device assembly time,
@@ -135,6 +197,42 @@ def scale_leg(oq, args):
         out[kd] = res
         log("N=65536 %s: assembly %.1f s, %.3f ms/RHS (%.0f GB/s), %.1f yr in %.1f s (%d+%d), %d events" % (
             kd, ta, me, res["hbm_GBps"], args.scale_years, r["wall"], r["naccept"], r["nreject"], len(ev)))
+    return out
+
+
+def toeplitz_leg(oq, args):
+    """Fault-only 256x64 = 16384 patches (80 x 20 km, the N = 65536 fault
+    without its mantle): the RHS with K_ff kept as its Toeplitz kernel
+    against the dense FP64 operator."""
+    from oetqf_amd.workloads import precision_scale_65536
+
+    w = precision_scale_65536()
+    mf, pf = w["mf"], w["pf"]
+    nf = mf.nx * mf.nxi
+    rng = np.random.default_rng(13)
+    u = oq.ArrayPartition(pf.vpl * np.exp(0.5 * rng.standard_normal((mf.nx, mf.nxi))),
+                          np.zeros((mf.nx, mf.nxi)), np.zeros((mf.nx, mf.nxi)))
+    u.x[1][:] = pf.L / u.x[0]
+    out = dict(workload="fault only %dx%d = %d patches" % (mf.nx, mf.nxi, nf))
+    ref = None
+    for kd in ("fp64", "toeplitz"):
+        p = oq.assemble_from_mesh(mf, None, w["lam"], w["mu"], pf, None, u, (0.0, 1.0), kdtype=kd)
+        du = oq.similar(u)
+        p.f(du, u, None, 0.0)
+        p.f.time(3)
+        me, mm = p.f.time(args.steps)
+        p.f.close()
+        kb = 8 * nf * nf if kd == "fp64" else 8 * mf.nx * mf.nxi * mf.nxi
+        out[kd] = dict(ms_per_rhs=me, rhs_per_s=1000.0 / me, operator_bytes=kb)
+        if ref is None:
+            ref = du
+        else:
+            out[kd]["rel_err_dv_vs_dense"] = float(np.linalg.norm(du.x[0] - ref.x[0]) /
+                                                   np.linalg.norm(ref.x[0]))
+    out["speedup_vs_dense"] = out["fp64"]["ms_per_rhs"] / out["toeplitz"]["ms_per_rhs"]
+    log("fault-only %d patches: dense FP64 %.4f ms, Toeplitz %.4f ms per RHS (%.1fx), err %.1e" % (
+        nf, out["fp64"]["ms_per_rhs"], out["toeplitz"]["ms_per_rhs"], out["speedup_vs_dense"],
+        out["toeplitz"]["rel_err_dv_vs_dense"]))
     return out
 
 
@@ -296,6 +394,9 @@ def main():
             log("%s: %.4f ms/eval (%.2fx), err dv %.2e dsigma %.2e" % (kd, me, ms_eval / me, err_tau, err_sig))
             p.f.close()
 
+    if not args.no_variants:
+        variants["toeplitz_fault_only"] = toeplitz_leg(oq, args)
+
     ensemble = None
     if args.ensemble > 0:
         from oetqf_amd.workloads import ensemble_members
@@ -400,6 +501,9 @@ def main():
     if not args.no_cpu:
         log("CPU baseline (oracle, reference algorithm)")
         cpu, ref = cpu_baseline(w, u0, args.cpu_seconds)
+        cpu["extra"] = cpu_extras(w, args.cpu_seconds)
+        log("CPU extras: %s" % json.dumps({k: (v if not isinstance(v, dict) else v.get("total", v.get("sim_yr_per_hr")))
+                                           for k, v in cpu["extra"].items()}))
         cpu["rhs_rel_err_gpu_vs_cpu"] = float(np.linalg.norm(du64.x[0] - ref[0]) / np.linalg.norm(ref[0]))
 
     if ensemble and ensemble.get("cycle") and cycle:

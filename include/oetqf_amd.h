@@ -40,6 +40,11 @@ extern "C" {
 #define OQ_K_FP64 0
 #define OQ_K_FP32 1
 #define OQ_K_BF16 2
+/* Fault-only problems (ne = 0): K_ff stays its Toeplitz kernel st
+ * (nx*nxi*nxi FP64, GF.jl:41-58) instead of the dense (nx*nxi)^2 operator,
+ * and is applied along strike directly (the reference applies it by FFT
+ * convolution, equation.jl:44-61). */
+#define OQ_K_TOEPLITZ 3
 
 /* RectOkadaMesh (src/BEM/mesh.jl:5-21).  Only x, ax, y, z, axi, nx, nxi, dx,
  * dep and dip enter the Green's functions (GF.jl:38,49-53,150-160). */
@@ -85,6 +90,14 @@ typedef struct oq_visc_props {
     int64_t nlaws;
     double deps0[6];
 } oq_visc_props;
+
+/* DilatancyProperty (property.jl:27-32): per patch, nx*nxi each. */
+typedef struct oq_dila_props {
+    const double *tp;   /* characteristic diffusion time */
+    const double *eps;  /* dilatancy coefficient */
+    const double *beta; /* gouge bulk compressibility */
+    const double *p0;   /* ambient pore pressure */
+} oq_dila_props;
 
 /* Solver options (the reference passes these to OrdinaryDiffEq.solve:
  * examples/otf-with-mantle.jl:160-161). dt0 <= 0 selects the automatic
@@ -169,6 +182,16 @@ int oq_rhs_create_from_mesh(int device, const oq_fault_mesh *mf,
                             const oq_rs_props *fault,
                             const oq_visc_props *visc, int kdtype,
                             oq_rhs **out);
+
+/* Replaces assemble(gf, p, dila, u0, tspan) (equation.jl:108-117) on a
+ * fault-only handle: the RHS becomes the dilatancy `ode` (equation.jl:168-183,
+ * update_fault_with_dilatancy! :248-276) and the packed state [v theta
+ * delta p] (4 nx*nxi).  Call before creating a stepper on the handle. */
+int oq_rhs_set_dilatancy(oq_rhs *h, const oq_dila_props *dila);
+/* The dilatancy `ode(du,u,p,t)` on host arrays (u = (v, theta, delta, p)). */
+int oq_rhs_eval_dila(oq_rhs *h, double t, const double *v, const double *theta,
+                     const double *delta, const double *p, double *dv, double *dtheta,
+                     double *ddelta, double *dp);
 
 /* The RHS `ode(du,u,p,t)` (equation.jl:156-166 fault only, :185-205 coupled)
  * on host arrays: the ArrayPartition components are separate pointers.

@@ -246,8 +246,80 @@ template <int KT> static void launch_matvec(oq_rhs *h, const double *s, const do
                        h->stream, h->dev, h->K.p, s, u, du);
 }
 
+// ---------------- Toeplitz K_ff (fault-only problems, OQ_K_TOEPLITZ) ----------------
+// The reference keeps K_ff as its Toeplitz kernel st[|i-k|, j, l] (GF.jl:41-58)
+// and applies it by FFT convolution along strike (equation.jl:44-61).  Here
+// the same kernel (nx*nxi^2 doubles instead of nf^2) is applied directly:
+// block (j, c) forms, for every strike index i of dip row j, the partial sum
+// over the dip rows l of chunk c of  sum_k st[|i-k|, j, l] s[k, l],  with
+// st[:, j, l] and s[:, l] staged in LDS; k_toep_finish adds the chunks in a
+// fixed order and applies the rate-and-state update.
+__global__ void __launch_bounds__(256) k_toep_partial(const double *__restrict__ st,
+                                                      const double *__restrict__ s, int64_t nx,
+                                                      int64_t nxi, int lchunk,
+                                                      double *__restrict__ part)
+{
+    extern __shared__ double tsh[];  // T[lchunk][nx], S[lchunk][nx]
+    const int64_t j = blockIdx.x, c = blockIdx.y, nf = nx * nxi;
+    const int64_t l0 = c * lchunk, l1 = (l0 + lchunk < nxi) ? l0 + lchunk : nxi;
+    const int64_t nl = l1 - l0;
+    double *T = tsh, *S = tsh + (size_t)lchunk * nx;
+    for (int64_t x = threadIdx.x; x < nl * nx; x += blockDim.x) {
+        const int64_t l = l0 + x / nx, i = x % nx;
+        T[x] = st[i + nx * j + nf * l];
+        S[x] = s[i + nx * l];
+    }
+    __syncthreads();
+    for (int64_t i = threadIdx.x; i < nx; i += blockDim.x) {
+        double acc = 0.0;
+        for (int64_t l = 0; l < nl; ++l) {
+            const double *Tl = T + l * nx, *Sl = S + l * nx;
+            for (int64_t k = 0; k <= i; ++k) acc += Tl[i - k] * Sl[k];
+            for (int64_t k = i + 1; k < nx; ++k) acc += Tl[k - i] * Sl[k];
+        }
+        part[c * nf + i + nx * j] = acc;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_toep_finish(RhsDev d, const double *__restrict__ part,
+                                                     int nchunk, const double *__restrict__ u,
+                                                     double *__restrict__ du)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nf = d.lay.nf;
+    if (q >= nf) return;
+    double tau = 0.0;
+    for (int c = 0; c < nchunk; ++c) tau += part[c * nf + q];
+    fault_update(d, q, tau, u, du);
+}
+
+// chunking of the dip rows: enough blocks to fill the device
+static void toep_shape(const oq_rhs *h, int *lchunk, int *nchunk, int *threads)
+{
+    int nc = (int)((1024 + h->nxi - 1) / h->nxi);
+    if (nc > h->nxi) nc = (int)h->nxi;
+    int lc = (int)((h->nxi + nc - 1) / nc);
+    // LDS: 2 * lchunk * nx doubles within 64 KB
+    while (lc > 1 && (int64_t)2 * lc * h->nx * 8 > 65536) --lc;
+    *lchunk = lc;
+    *nchunk = (int)((h->nxi + lc - 1) / lc);
+    const int64_t t = (h->nx + 63) / 64 * 64;
+    *threads = (int)(t < 256 ? t : 256);
+}
+
 int rhs_enqueue_matvec(oq_rhs *h, const double *u, double *du)
 {
+    if (h->kdtype == OQ_K_TOEPLITZ) {
+        int lc, nc, nt;
+        toep_shape(h, &lc, &nc, &nt);
+        hipLaunchKernelGGL(k_toep_partial, dim3((unsigned)h->nxi, (unsigned)nc), dim3(nt),
+                           (size_t)2 * lc * h->nx * 8, h->stream, h->K.as<double>(),
+                           h->s64.as<double>(), h->nx, h->nxi, lc, h->tpart.as<double>());
+        hipLaunchKernelGGL(k_toep_finish, dim3(nblk(h->nf, 256)), dim3(256), 0, h->stream, h->dev,
+                           h->tpart.as<double>(), nc, u, du);
+        OQ_HIP(hipGetLastError());
+        return OQ_OK;
+    }
     switch (h->kdtype) {
     case OQ_K_FP64: launch_matvec<OQ_K_FP64>(h, h->s64.as<double>(), u, du); break;
     case OQ_K_FP32: launch_matvec<OQ_K_FP32>(h, h->s64.as<double>(), u, du); break;
@@ -332,8 +404,12 @@ __global__ void __launch_bounds__(256) k_from_lowp(const void *src, int kt, doub
 static int alloc_common(oq_rhs *h, int device, int64_t nx, int64_t nxi, int64_t ne, int kdtype)
 {
     OQ_REQUIRE(nx > 0 && nxi > 0 && ne >= 0, "sizes must be positive");
-    OQ_REQUIRE(kdtype == OQ_K_FP64 || kdtype == OQ_K_FP32 || kdtype == OQ_K_BF16,
-               "kdtype must be OQ_K_FP64, OQ_K_FP32 or OQ_K_BF16");
+    OQ_REQUIRE(kdtype == OQ_K_FP64 || kdtype == OQ_K_FP32 || kdtype == OQ_K_BF16 ||
+                   kdtype == OQ_K_TOEPLITZ,
+               "kdtype must be OQ_K_FP64, OQ_K_FP32, OQ_K_BF16 or OQ_K_TOEPLITZ");
+    OQ_REQUIRE(kdtype != OQ_K_TOEPLITZ || ne == 0, "the Toeplitz operator is for fault-only problems");
+    OQ_REQUIRE(kdtype != OQ_K_TOEPLITZ || nx <= 4096,
+               "the Toeplitz operator takes nx <= 4096 (one strike row of K and s in 64 KB of LDS)");
     int ndev = 0;
     OQ_HIP(hipGetDeviceCount(&ndev));
     OQ_REQUIRE(device >= 0 && device < ndev, "invalid device index");
@@ -400,7 +476,22 @@ static int upload_props(oq_rhs *h, const oq_rs_props *f, const oq_visc_props *v)
     d.lay = h->lay;
     d.N = h->N;
     d.ld = h->ld;
+    d.tp = d.deps_d = d.beta = d.p0 = nullptr;
+    d.dila = 0;
     return OQ_OK;
+}
+
+// The Toeplitz operator: h->K holds st (nx*nxi*nxi doubles, st[i + nx j +
+// nf l]) already on the device in `st`; allocate the chunk partials.
+static int finalize_toeplitz(oq_rhs *h, DevBuf &st)
+{
+    h->K.p = st.p;
+    h->K.bytes = st.bytes;
+    st.p = nullptr;
+    st.bytes = 0;
+    int lc, nc, nt;
+    toep_shape(h, &lc, &nc, &nt);
+    return h->tpart.alloc((size_t)(nc * h->nf) * 8);
 }
 
 // Converts the FP64 operator in k64 (N x ld) to h->K in h->kdtype.
@@ -471,6 +562,23 @@ int oq_rhs_create(int device, int64_t nx, int64_t nxi, int64_t ne, const oq_rs_p
     }
     const int64_t nf = h->nf, m6 = 6 * ne, ld = h->ld;
     DevBuf k64, tmp;
+    if (kdtype == OQ_K_TOEPLITZ) {
+        std::vector<double> st;
+        const double *stp = gf11;
+        if (gf11_is_dft) {
+            st.resize((size_t)(nf * nxi));
+            toeplitz_idft(gf11, nx, nxi, st.data());
+            stp = st.data();
+        }
+        rc = tmp.upload(stp, (size_t)(nf * nxi) * 8, h->stream);
+        if (!rc) rc = finalize_toeplitz(h, tmp);
+        if (rc) {
+            destroy(h);
+            return rc;
+        }
+        *out = h;
+        return OQ_OK;
+    }
     rc = k64.alloc((size_t)(h->N * ld) * 8);
     if (!rc && hipMemsetAsync(k64.p, 0, (size_t)(h->N * ld) * 8, h->stream) != hipSuccess) {
         set_error("hipMemsetAsync failed");
@@ -546,6 +654,21 @@ int oq_rhs_create_from_mesh(int device, const oq_fault_mesh *mf, const oq_hex8_m
     if (!rc && ma) rc = hu.load(ma);
     if (!rc && ma) rc = qu.load(quad);
     const int64_t nf = h->nf, ld = h->ld;
+    if (!rc && kdtype == OQ_K_TOEPLITZ) {
+        rc = st.alloc((size_t)(nf * mf->nxi) * 8);
+        if (!rc) rc = launch_gf_ff(fu.dev, lambda, mu, ftype, (int)nrept, st.as<double>(), h->stream);
+        if (!rc && hipStreamSynchronize(h->stream) != hipSuccess) {
+            set_error("device assembly failed");
+            rc = OQ_ERR_HIP;
+        }
+        if (!rc) rc = finalize_toeplitz(h, st);
+        if (rc) {
+            destroy(h);
+            return rc;
+        }
+        *out = h;
+        return OQ_OK;
+    }
     if (!rc) rc = k64.alloc((size_t)(h->N * ld) * 8);
     if (!rc && hipMemsetAsync(k64.p, 0, (size_t)(h->N * ld) * 8, h->stream) != hipSuccess) {
         set_error("hipMemsetAsync failed");
@@ -585,6 +708,7 @@ int oq_rhs_eval(oq_rhs *h, double t, const double *v, const double *theta, const
     OQ_REQUIRE(h != nullptr, "handle is NULL");
     OQ_REQUIRE(v && theta && delta && dv && dtheta && ddelta, "state pointer is NULL");
     OQ_REQUIRE(h->ne == 0 || (eps && sigma && deps && dsigma), "mantle state pointer is NULL");
+    OQ_REQUIRE(!h->dev.dila, "a dilatancy handle takes oq_rhs_eval_dila");
     OQ_HIP(hipSetDevice(h->device));
     const StateLayout &L = h->lay;
     double *u = h->u.as<double>(), *du = h->du.as<double>();
@@ -605,6 +729,60 @@ int oq_rhs_eval(oq_rhs *h, double t, const double *v, const double *theta, const
         OQ_HIP(hipMemcpyAsync(deps, du + L.eps, m8, hipMemcpyDeviceToHost, h->stream));
         OQ_HIP(hipMemcpyAsync(dsigma, du + L.sig, m8, hipMemcpyDeviceToHost, h->stream));
     }
+    OQ_HIP(hipStreamSynchronize(h->stream));
+    return OQ_OK;
+}
+
+int oq_rhs_set_dilatancy(oq_rhs *h, const oq_dila_props *dila)
+{
+    OQ_REQUIRE(h != nullptr && dila != nullptr, "NULL argument");
+    OQ_REQUIRE(dila->tp && dila->eps && dila->beta && dila->p0, "dilatancy property array is NULL");
+    OQ_REQUIRE(h->ne == 0, "dilatancy is a fault-only model (equation.jl:108-117)");
+    OQ_HIP(hipSetDevice(h->device));
+    const size_t nf8 = (size_t)h->nf * 8;
+    int rc;
+    if ((rc = h->tp.upload(dila->tp, nf8, h->stream))) return rc;
+    if ((rc = h->deps_d.upload(dila->eps, nf8, h->stream))) return rc;
+    if ((rc = h->beta.upload(dila->beta, nf8, h->stream))) return rc;
+    if ((rc = h->p0.upload(dila->p0, nf8, h->stream))) return rc;
+    h->lay.set_dilatancy(h->nf);
+    if ((rc = h->u.alloc((size_t)h->lay.n * 8))) return rc;
+    if ((rc = h->du.alloc((size_t)h->lay.n * 8))) return rc;
+    OQ_HIP(hipMemsetAsync(h->u.p, 0, (size_t)h->lay.n * 8, h->stream));
+    OQ_HIP(hipMemsetAsync(h->du.p, 0, (size_t)h->lay.n * 8, h->stream));
+    OQ_HIP(hipStreamSynchronize(h->stream));
+    RhsDev &d = h->dev;
+    d.lay = h->lay;
+    d.tp = h->tp.as<double>();
+    d.deps_d = h->deps_d.as<double>();
+    d.beta = h->beta.as<double>();
+    d.p0 = h->p0.as<double>();
+    d.dila = 1;
+    return OQ_OK;
+}
+
+int oq_rhs_eval_dila(oq_rhs *h, double t, const double *v, const double *theta,
+                     const double *delta, const double *p, double *dv, double *dtheta,
+                     double *ddelta, double *dp)
+{
+    (void)t;
+    OQ_REQUIRE(h != nullptr, "handle is NULL");
+    OQ_REQUIRE(h->dev.dila, "the handle has no dilatancy (oq_rhs_set_dilatancy)");
+    OQ_REQUIRE(v && theta && delta && p && dv && dtheta && ddelta && dp, "state pointer is NULL");
+    OQ_HIP(hipSetDevice(h->device));
+    const StateLayout &L = h->lay;
+    double *u = h->u.as<double>(), *du = h->du.as<double>();
+    const size_t f8 = (size_t)L.nf * 8;
+    OQ_HIP(hipMemcpyAsync(u + L.v, v, f8, hipMemcpyHostToDevice, h->stream));
+    OQ_HIP(hipMemcpyAsync(u + L.th, theta, f8, hipMemcpyHostToDevice, h->stream));
+    OQ_HIP(hipMemcpyAsync(u + L.del, delta, f8, hipMemcpyHostToDevice, h->stream));
+    OQ_HIP(hipMemcpyAsync(u + L.pp, p, f8, hipMemcpyHostToDevice, h->stream));
+    int rc = rhs_enqueue(h, u, du);
+    if (rc) return rc;
+    OQ_HIP(hipMemcpyAsync(dv, du + L.v, f8, hipMemcpyDeviceToHost, h->stream));
+    OQ_HIP(hipMemcpyAsync(dtheta, du + L.th, f8, hipMemcpyDeviceToHost, h->stream));
+    OQ_HIP(hipMemcpyAsync(ddelta, du + L.del, f8, hipMemcpyDeviceToHost, h->stream));
+    OQ_HIP(hipMemcpyAsync(dp, du + L.pp, f8, hipMemcpyDeviceToHost, h->stream));
     OQ_HIP(hipStreamSynchronize(h->stream));
     return OQ_OK;
 }
@@ -633,7 +811,15 @@ int oq_rhs_get_k(oq_rhs *h, double *k)
     const int64_t n = h->N * h->ld;
     DevBuf tmp;
     const double *src = h->K.as<double>();
-    if (h->kdtype != OQ_K_FP64) {
+    if (h->kdtype == OQ_K_TOEPLITZ) {
+        int rc = tmp.alloc((size_t)n * 8);
+        if (rc) return rc;
+        OQ_HIP(hipMemsetAsync(tmp.p, 0, (size_t)n * 8, h->stream));
+        if ((rc = launch_expand_toeplitz(h->K.as<double>(), h->nx, h->nxi, tmp.as<double>(), h->ld,
+                                         h->stream)))
+            return rc;
+        src = tmp.as<double>();
+    } else if (h->kdtype != OQ_K_FP64) {
         int rc = tmp.alloc((size_t)n * 8);
         if (rc) return rc;
         hipLaunchKernelGGL(k_from_lowp, dim3(grid_capped(n, 256)), dim3(256), 0, h->stream, h->K.p,

@@ -10,6 +10,14 @@ namespace oq {
 struct StateLayout {
     int64_t nf = 0, ne = 0;
     int64_t v = 0, th = 0, eps = 0, sig = 0, del = 0, n = 0;
+    int64_t pp = 0;  // pore pressure (dilatancy: [v th del p], equation.jl:172-173)
+    // The dilatancy model (fault only): u = (v, theta, delta, p).
+    void set_dilatancy(int64_t nf_)
+    {
+        set(nf_, 0);
+        pp = 3 * nf_;
+        n = 4 * nf_;
+    }
     void set(int64_t nf_, int64_t ne_)
     {
         nf = nf_;
@@ -38,6 +46,9 @@ struct RhsDev {
     double deps0[6];
     StateLayout lay;
     int64_t N, ld;
+    // DilatancyProperty (property.jl:27-32), per patch; dila = 0: off
+    const double *tp, *deps_d, *beta, *p0;
+    int dila;
 };
 
 // Cross-lane moves of a double as two 32-bit halves.
@@ -99,13 +110,39 @@ __device__ __forceinline__ void fault_rates(const RhsDev &d, int64_t i, double v
     dv = (dtau - dmu_dth * dth) / (dmu_dv + d.eta);
 }
 
-// The same on the packed state vectors: du gets (dv, dtheta, ddelta = v).
+// Rates with dilatancy (update_fault_with_dilatancy!, equation.jl:248-276;
+// aging law :279, pore-pressure law :282): power-law friction
+// f0 (sigma - p) (v/v0)^(a/f0) (theta v0/L)^(b/f0), no radiation damping.
+__device__ __forceinline__ void fault_rates_dila(const RhsDev &d, int64_t i, double v, double th,
+                                                 double pp, double dtau, double &dv, double &dth,
+                                                 double &dp)
+{
+    const double L = d.L[i], af = d.a[i] / d.f0, bf = d.b[i] / d.f0;
+    dth = 1.0 - v * th / L;
+    dp = -(pp - d.p0[i]) / d.tp[i] + d.deps_d[i] / d.beta[i] / th * dth;
+    const double vf = v / d.v0 > 0.0 ? v / d.v0 : 0.0;
+    const double tf0 = th * d.v0 / L, thf = tf0 > 0.0 ? tf0 : 0.0;
+    const double va = pow(vf, af), vam1 = pow(vf, af - 1.0);
+    const double tb = pow(thf, bf), tbm1 = pow(thf, bf - 1.0);
+    const double sg = d.sigma[i] - pp;
+    dv = (dtau + d.f0 * dp * va * tb - d.f0 * sg * va * tbm1 * bf * d.v0 / L * dth) /
+         (d.f0 * sg * vam1 * tb * af / d.v0);
+}
+
+// The same on the packed state vectors: du gets (dv, dtheta, ddelta = v)
+// and, with dilatancy, dp.
 __device__ __forceinline__ void fault_update(const RhsDev &d, int64_t i, double dtau,
                                             const double *__restrict__ u, double *__restrict__ du)
 {
     const double v = u[d.lay.v + i], th = u[d.lay.th + i];
     double dv, dth;
-    fault_rates(d, i, v, th, dtau, dv, dth);
+    if (d.dila) {
+        double dp;
+        fault_rates_dila(d, i, v, th, u[d.lay.pp + i], dtau, dv, dth, dp);
+        du[d.lay.pp + i] = dp;
+    } else {
+        fault_rates(d, i, v, th, dtau, dv, dth);
+    }
     du[d.lay.th + i] = dth;
     du[d.lay.v + i] = dv;
     du[d.lay.del + i] = v;
@@ -121,9 +158,11 @@ struct oq_rhs {
     oq::StateLayout lay;
     oq::DevBuf K;                       // N x ld, kdtype
     oq::DevBuf a, b, L, sigma, gamma, nexp;
+    oq::DevBuf tp, deps_d, beta, p0;    // dilatancy (oq_rhs_set_dilatancy)
     oq::DevBuf s64;                     // source vector, ld doubles (zero padded)
     oq::DevBuf s32;                     // FP32 copy of s (BF16 operator)
     oq::DevBuf u, du;                   // packed state scratch for the host API
+    oq::DevBuf tpart;                   // Toeplitz operator: per-chunk partial sums
     oq::RhsDev dev{};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };

@@ -715,7 +715,8 @@ int oq_stepper_set_persistent(oq_stepper *st, int on)
         st->persistent = false;
         return OQ_OK;
     }
-    OQ_REQUIRE(h->ne == 0, "the persistent stepper takes fault-only problems");
+    OQ_REQUIRE(h->ne == 0 && !h->dev.dila,
+               "the persistent stepper takes fault-only problems without dilatancy");
     OQ_REQUIRE(h->kdtype == OQ_K_FP64, "the persistent stepper takes an FP64 operator");
     int coop = 0, cus = 0, maxlds = 0, khz = 0;
     OQ_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, h->device));

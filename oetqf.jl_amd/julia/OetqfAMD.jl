@@ -148,7 +148,12 @@ function cvisc(p::CompositePowerLawViscosityProperty, ne)
     return CViscProps(pointer(γ), pointer(n), length(p.piter), Tuple(Float64.(p.dϵ₀))), (γ, n)
 end
 
-kcode(T) = T === Float64 ? Cint(0) : T === Float32 ? Cint(1) : Cint(2)  # :bf16 -> 2
+# Kstorage: Float64, Float32, :bf16, or :toeplitz (fault only: K_ff kept as st)
+kcode(T) = T === Float64 ? Cint(0) : T === Float32 ? Cint(1) : T === :toeplitz ? Cint(3) : Cint(2)
+
+struct CDilaProps
+    tp::Ptr{Float64}; eps::Ptr{Float64}; beta::Ptr{Float64}; p0::Ptr{Float64}
+end
 
 """assemble(gf, p, u0, tspan; Kstorage=Float64): fault only (equation.jl:81-89)."""
 function assemble(gf::AbstractArray, p::RateStateQuasiDynamicProperty, u0::ArrayPartition, tspan::NTuple{2};
@@ -166,6 +171,21 @@ function assemble(gf::AbstractArray, p::RateStateQuasiDynamicProperty, u0::Array
     end
     f = GPURHS(h[], 0, (p,))
     return Oetqf.ODEProblem{true}((du, u, pp, t) -> rhs!(f, du, u, t), u0, tspan, (p, f))
+end
+
+"""assemble(gf, p, dila, u0, tspan; Kstorage=Float64): fault only with
+dilatancy (equation.jl:108-117), u0 = (v, θ, δ, 𝓅)."""
+function assemble(gf::AbstractArray, p::RateStateQuasiDynamicProperty, dila::DilatancyProperty,
+        u0::ArrayPartition, tspan::NTuple{2}; Kstorage=Float64, device::Integer=0, kwargs...)
+    prob = assemble(gf, p, ArrayPartition(u0.x[1], u0.x[2], u0.x[3]), tspan; Kstorage=Kstorage,
+                    device=device)
+    f = prob.p[end]
+    GC.@preserve dila begin
+        d = CDilaProps(pointer(dila.tₚ), pointer(dila.ϵ), pointer(dila.β), pointer(dila.p₀))
+        check(ccall((:oq_rhs_set_dilatancy, libpath), Cint, (Ptr{Cvoid}, Ref{CDilaProps}), f.h, Ref(d)))
+    end
+    f.keep = (f.keep, dila)
+    return Oetqf.ODEProblem{true}((du, u, pp, t) -> rhs!(f, du, u, t), u0, tspan, (p, dila, f))
 end
 
 """assemble(gf₁₁, gf₁₂, gf₂₁, gf₂₂, pf, pa, u0, tspan) (equation.jl:141-154)."""
@@ -190,7 +210,13 @@ end
 
 """The `ode(du,u,p,t)` contract on host arrays (equation.jl:156-166, :185-205)."""
 function rhs!(f::GPURHS, du::ArrayPartition, u::ArrayPartition, t)
-    if f.ne == 0
+    if length(u.x) == 4   # dilatancy (equation.jl:168-183)
+        v, θ, δ, 𝓅 = u.x; dv, dθ, dδ, d𝓅 = du.x
+        check(ccall((:oq_rhs_eval_dila, libpath), Cint,
+            (Ptr{Cvoid}, Float64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+             Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+            f.h, t, v, θ, δ, 𝓅, dv, dθ, dδ, d𝓅))
+    elseif f.ne == 0
         v, θ, δ = u.x; dv, dθ, dδ = du.x
         check(ccall((:oq_rhs_eval, libpath), Cint,
             (Ptr{Cvoid}, Float64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},

@@ -15,5 +15,5 @@ from .greens import (DipSlip, StrikeSlip, get_quadrature, gmsh_quadrature,  # no
 from .mesh import (BEMHex8Mesh, RectOkadaMesh, cosd, gen_gmsh_mesh, gen_mesh,  # noqa: F401
                    gen_mesh_graded, hex8_box, sind)
 from .pref import get_matvecmul, set_matvecmul  # noqa: F401
-from .property import (CompositePowerLawViscosityProperty, DieterichStateLaw,  # noqa: F401
+from .property import (CompositePowerLawViscosityProperty, DieterichStateLaw, DilatancyProperty,  # noqa: F401,E501
                        PowerLawViscosityProperty, RateStateQuasiDynamicProperty, dϵ_dt)

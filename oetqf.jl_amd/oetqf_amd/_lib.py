@@ -18,7 +18,7 @@ _vp = ctypes.c_void_p
 
 OQ_OK = 0
 OQ_ERR_ARG = 1
-OQ_K_FP64, OQ_K_FP32, OQ_K_BF16 = 0, 1, 2
+OQ_K_FP64, OQ_K_FP32, OQ_K_BF16, OQ_K_TOEPLITZ = 0, 1, 2, 3
 RETCODES = {0: "Running", 1: "Success", 2: "MaxIters", 3: "DtLessThanMin", 4: "NonFinite"}
 
 
@@ -43,6 +43,10 @@ class RSProps(ctypes.Structure):
 
 class ViscProps(ctypes.Structure):
     _fields_ = [("gamma", _pd), ("n", _pd), ("nlaws", _i64), ("deps0", _d * 6)]
+
+
+class DilaProps(ctypes.Structure):
+    _fields_ = [("tp", _pd), ("eps", _pd), ("beta", _pd), ("p0", _pd)]
 
 
 class SolverOpts(ctypes.Structure):
@@ -77,6 +81,8 @@ PROTOTYPES = {
                                        ctypes.POINTER(_vp)]),
     "oq_rhs_eval": (_int, [_vp, _d] + [_pd] * 10),
     "oq_rhs_eval_device": (_int, [_vp, _d, _vp, _vp]),
+    "oq_rhs_set_dilatancy": (_int, [_vp, ctypes.POINTER(DilaProps)]),
+    "oq_rhs_eval_dila": (_int, [_vp, _d] + [_pd] * 8),
     "oq_rhs_info": (_int, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
     "oq_rhs_get_k": (_int, [_vp, _pd]),
     "oq_rhs_time": (_int, [_vp, _int, ctypes.POINTER(_d), ctypes.POINTER(_d)]),

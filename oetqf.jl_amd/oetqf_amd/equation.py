@@ -64,13 +64,23 @@ def _visc_struct(pa, ne, keep):
     return v
 
 
+def _dila_struct(d, keep):
+    return _lib.DilaProps(*[_lib.ptr(keep.arr(np.asarray(getattr(d, k)).reshape(-1, order="F")))
+                            for k in ("tp", "eps", "beta", "p0")])
+
+
 class RHS:
     """Owns an oq_rhs handle; callable as f(du, u, p, t)."""
 
-    def __init__(self, handle, nx, nxi, ne, keep):
+    def __init__(self, handle, nx, nxi, ne, keep, dila=None):
         self.h = handle
         self.nx, self.nxi, self.ne = nx, nxi, ne
         self._keep = keep
+        self.dila = dila is not None
+        if self.dila:
+            ds = _dila_struct(dila, keep)
+            keep.append(ds)
+            _lib.check(_lib.load().oq_rhs_set_dilatancy(handle, ctypes.byref(ds)))
 
     def __call__(self, du, u, p=None, t=0.0):
         L = _lib.load()
@@ -79,7 +89,12 @@ class RHS:
             if not a.flags.f_contiguous or a.dtype != np.float64:
                 raise TypeError("ArrayPartition components must be Fortran-ordered float64")
         P = _lib.ptr
-        if self.ne == 0:
+        if self.dila:
+            v, th, dl, pp = ux
+            dv, dth, ddl, dpp = dux
+            _lib.check(L.oq_rhs_eval_dila(self.h, float(t), P(v), P(th), P(dl), P(pp),
+                                          P(dv), P(dth), P(ddl), P(dpp)))
+        elif self.ne == 0:
             v, th, dl = ux
             dv, dth, ddl = dux
             _lib.check(L.oq_rhs_eval(self.h, float(t), P(v), P(th), None, None, P(dl),
@@ -127,11 +142,13 @@ class ODEProblem:
         self.f, self.u0, self.tspan, self.p = f, u0, (float(tspan[0]), float(tspan[1])), p
 
 
-_KDTYPES = {"fp64": _lib.OQ_K_FP64, "fp32": _lib.OQ_K_FP32, "bf16": _lib.OQ_K_BF16}
+_KDTYPES = {"fp64": _lib.OQ_K_FP64, "fp32": _lib.OQ_K_FP32, "bf16": _lib.OQ_K_BF16,
+            "toeplitz": _lib.OQ_K_TOEPLITZ}
 
 
 def assemble(*args, se=DieterichStateLaw(), kdtype="fp64", device=0, **kwargs):
-    """assemble(gf, p, u0, tspan) (equation.jl:81-89) or
+    """assemble(gf, p, u0, tspan) (equation.jl:81-89),
+    assemble(gf, p, dila, u0, tspan) (equation.jl:108-117, u0 = (v, θ, δ, 𝓅)) or
     assemble(gf₁₁, gf₁₂, gf₂₁, gf₂₂, pf, pa, u0, tspan) (equation.jl:141-154).
     gf/gf₁₁ may be the real Toeplitz kernel or its DFT (the reference
     builder's default, fourier=true)."""
@@ -143,14 +160,19 @@ def assemble(*args, se=DieterichStateLaw(), kdtype="fp64", device=0, **kwargs):
     keep = _Keep()
     L = _lib.load()
     h = ctypes.c_void_p()
+    dila = None
     if len(args) == 4:
         gf, pf, u0, tspan = args
+        pa, gf12, gf21, gf22, ne = None, None, None, None, 0
+    elif len(args) == 5:
+        gf, pf, dila, u0, tspan = args
         pa, gf12, gf21, gf22, ne = None, None, None, None, 0
     elif len(args) == 8:
         gf, gf12, gf21, gf22, pf, pa, u0, tspan = args
         ne = u0.x[2].shape[0]
     else:
-        raise TypeError("assemble takes (gf, p, u0, tspan) or (gf₁₁, gf₁₂, gf₂₁, gf₂₂, pf, pa, u0, tspan)")
+        raise TypeError("assemble takes (gf, p, u0, tspan), (gf, p, dila, u0, tspan) or "
+                        "(gf₁₁, gf₁₂, gf₂₁, gf₂₂, pf, pa, u0, tspan)")
     nx, nxi = u0.x[0].shape
     is_dft = np.iscomplexobj(gf)
     if is_dft:
@@ -170,12 +192,12 @@ def assemble(*args, se=DieterichStateLaw(), kdtype="fp64", device=0, **kwargs):
     else:
         _lib.check(L.oq_rhs_create(device, nx, nxi, 0, ctypes.byref(rs), None, _lib.ptr(g11),
                                    int(is_dft), None, None, None, kdt, ctypes.byref(h)))
-    f = RHS(h, nx, nxi, ne, keep)
-    return ODEProblem(f, u0, tspan, (pf, pa, f, se))
+    f = RHS(h, nx, nxi, ne, keep, dila=dila)
+    return ODEProblem(f, u0, tspan, (pf, dila if dila is not None else pa, f, se))
 
 
 def assemble_from_mesh(mf, ma, lam, mu, pf, pa, u0, tspan, ftype=StrikeSlip(), nrept=2,
-                       buffer_ratio=0.0, qtype="Gauss1", kdtype="fp64", device=0):
+                       buffer_ratio=0.0, qtype="Gauss1", kdtype="fp64", device=0, dila=None):
     """The four stress_greens_function calls followed by assemble, with the
     Green's functions built straight into the device operator (no host copy
     of K).  ma = None gives the fault-only problem."""
@@ -201,8 +223,8 @@ def assemble_from_mesh(mf, ma, lam, mu, pf, pa, u0, tspan, ftype=StrikeSlip(), n
                                              ctypes.byref(rs), None, _KDTYPES[kdtype],
                                              ctypes.byref(h)))
         ne = 0
-    f = RHS(h, mf.nx, mf.nxi, ne, keep)
-    return ODEProblem(f, u0, tspan, (pf, pa, f, DieterichStateLaw()))
+    f = RHS(h, mf.nx, mf.nxi, ne, keep, dila=dila)
+    return ODEProblem(f, u0, tspan, (pf, dila if dila is not None else pa, f, DieterichStateLaw()))
 
 
 class Tsit5:

@@ -28,6 +28,22 @@ class RateStateQuasiDynamicProperty:
     eta = property(lambda s: s.η)
 
 
+class DilatancyProperty:
+    """property.jl:27-32: per-patch diffusion time tₚ, dilatancy coefficient
+    ϵ, gouge compressibility β and ambient pore pressure p₀ (positional order
+    as the reference; the Julia field names are aliases)."""
+
+    def __init__(self, tp, eps, beta, p0):
+        self.tp, self.eps, self.beta, self.p0 = _arr(tp), _arr(eps), _arr(beta), _arr(p0)
+        assert self.tp.shape == self.eps.shape == self.beta.shape == self.p0.shape, \
+            "dilatancy property arrays must have the same shape"
+        setattr(self, "p\u2080", self.p0)
+
+    # (tₚ is tp after Python's identifier normalization)
+    ϵ = property(lambda s: s.eps)
+    β = property(lambda s: s.beta)
+
+
 class PowerLawViscosityProperty:
     """property.jl:34-41: dϵ/dt = γ σ τ^n with n = power - 1."""
 

@@ -103,10 +103,25 @@ typedef struct {
     double deps0[6];
 } oq_visc_t;
 
+/* DilatancyProperty (property.jl:27-32): per patch. */
+typedef struct {
+    const double *tp, *eps, *beta, *p0;
+} oq_dila_t;
+
 /* update_fault! (equation.jl:233-246) given dtau. */
 void oq_update_fault(const oq_rs_t *p, int64_t nf, const double *v,
                      const double *theta, const double *dtau, double *dv,
                      double *dtheta, double *ddelta);
+/* update_fault_with_dilatancy! (equation.jl:248-276) given dtau. */
+void oq_update_fault_dila(const oq_rs_t *p, const oq_dila_t *d, int64_t nf,
+                          const double *v, const double *theta, const double *pp,
+                          const double *dtau, double *dv, double *dtheta,
+                          double *ddelta, double *dpp);
+/* Fault-only RHS with dilatancy (equation.jl:168-183). */
+void oq_rhs_fault_dila(const oq_rs_t *p, const oq_dila_t *d, const double *st,
+                       int64_t nx, int64_t nxi, const double *v, const double *theta,
+                       const double *pp, double *dv, double *dtheta, double *ddelta,
+                       double *dpp);
 /* update_strain_rate! (equation.jl:207-222). */
 void oq_update_strain_rate(const oq_visc_t *p, int64_t ne, const double *sigma,
                            double *deps);

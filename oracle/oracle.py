@@ -264,6 +264,43 @@ def update_fault(p, v, theta, dtau):
     return dv, dth, dd
 
 
+class DilaC(ctypes.Structure):
+    _fields_ = [("tp", _pd), ("eps", _pd), ("beta", _pd), ("p0", _pd)]
+
+
+def _dila(d, keep):
+    return DilaC(*[_p(keep.arr(_fortran_flat(getattr(d, k)))) for k in ("tp", "eps", "beta", "p0")])
+
+
+def update_fault_dila(p, d, v, theta, pp, dtau):
+    """update_fault_with_dilatancy! (equation.jl:248-276)."""
+    keep = _Keep()
+    c, cd = _rs(p, keep), _dila(d, keep)
+    n = v.size
+    out = [np.zeros(n) for _ in range(4)]
+    L = lib()
+    L.oq_update_fault_dila.argtypes = [ctypes.POINTER(RSC), ctypes.POINTER(DilaC), _i64] + [_pd] * 8
+    L.oq_update_fault_dila(ctypes.byref(c), ctypes.byref(cd), n,
+                           *[_p(keep.arr(x.reshape(-1, order="F"))) for x in (v, theta, pp, dtau)],
+                           *[_p(o) for o in out])
+    return tuple(out)  # dv, dtheta, ddelta, dp
+
+
+def rhs_fault_dila(pf, d, st, v, theta, pp):
+    """ode with dilatancy (equation.jl:168-183)."""
+    keep = _Keep()
+    c, cd = _rs(pf, keep), _dila(d, keep)
+    nx, nxi = v.shape
+    out = [np.zeros(nx * nxi) for _ in range(4)]
+    L = lib()
+    L.oq_rhs_fault_dila.argtypes = [ctypes.POINTER(RSC), ctypes.POINTER(DilaC), _pd, _i64, _i64] + [_pd] * 7
+    L.oq_rhs_fault_dila(ctypes.byref(c), ctypes.byref(cd), _p(keep.arr(_st_flat(st))), nx, nxi,
+                        *[_p(keep.arr(x.reshape(-1, order="F"))) for x in (v, theta, pp)],
+                        *[_p(o) for o in out])
+    sh = (nx, nxi)
+    return tuple(o.reshape(sh, order="F") for o in out)  # dv, dtheta, ddelta, dp
+
+
 def update_strain_rate(p, sigma):
     """update_strain_rate! (equation.jl:207-222); sigma (ne, 6)."""
     keep = _Keep()

@@ -34,6 +34,57 @@ void oq_update_fault(const oq_rs_t *p, int64_t nf, const double *v,
     }
 }
 
+/* update_fault_with_dilatancy! (equation.jl:248-276) with the aging law
+ * (:279) and the pore-pressure law d𝓅_dt (:282): power-law friction
+ * mu = f0 vf^(a/f0) thf^(b/f0) on the effective normal stress sigma - p,
+ * vf = max(0, v/v0), thf = max(0, theta v0/L), no radiation damping. */
+void oq_update_fault_dila(const oq_rs_t *p, const oq_dila_t *d, int64_t nf,
+                          const double *v, const double *theta, const double *pp,
+                          const double *dtau, double *dv, double *dtheta,
+                          double *ddelta, double *dpp)
+{
+    int64_t i;
+#pragma omp parallel for
+    for (i = 0; i < nf; ++i) {
+        double dth = 1.0 - v[i] * theta[i] / p->L[i];
+        double dp = -(pp[i] - d->p0[i]) / d->tp[i] + d->eps[i] / d->beta[i] / theta[i] * dth;
+        double af = p->a[i] / p->f0, bf = p->b[i] / p->f0;
+        double vf = v[i] / p->v0 > 0.0 ? v[i] / p->v0 : 0.0;
+        double thf = theta[i] * p->v0 / p->L[i];
+        double sig = p->sigma[i] - pp[i];
+        double va, vam1, tb, tbm1;
+        thf = thf > 0.0 ? thf : 0.0;
+        va = pow(vf, af);
+        vam1 = pow(vf, af - 1.0);
+        tb = pow(thf, bf);
+        tbm1 = pow(thf, bf - 1.0);
+        dtheta[i] = dth;
+        dpp[i] = dp;
+        dv[i] = (dtau[i] + p->f0 * dp * va * tb -
+                 p->f0 * sig * va * tbm1 * bf * p->v0 / p->L[i] * dth) /
+                (p->f0 * sig * vam1 * tb * af / p->v0);
+        ddelta[i] = v[i];
+    }
+}
+
+/* ode with dilatancy (equation.jl:168-183): Toeplitz-by-DFT dtau, then
+ * update_fault_with_dilatancy!. */
+void oq_rhs_fault_dila(const oq_rs_t *p, const oq_dila_t *d, const double *st,
+                       int64_t nx, int64_t nxi, const double *v, const double *theta,
+                       const double *pp, double *dv, double *dtheta, double *ddelta,
+                       double *dpp)
+{
+    const int64_t nf = nx * nxi;
+    double *s = (double *)malloc(sizeof(double) * (size_t)nf);
+    double *dtau = (double *)malloc(sizeof(double) * (size_t)nf);
+    int64_t i;
+    for (i = 0; i < nf; ++i) s[i] = v[i] - p->vpl;
+    oq_dtau_fft(st, nx, nxi, s, dtau);
+    oq_update_fault_dila(p, d, nf, v, theta, pp, dtau, dv, dtheta, ddelta, dpp);
+    free(s);
+    free(dtau);
+}
+
 /* update_strain_rate! with dϵ_dt = γ σ τ^n (equation.jl:207-222, :285). */
 void oq_update_strain_rate(const oq_visc_t *p, int64_t ne, const double *sg,
                            double *de)

@@ -144,3 +144,85 @@ def test_coupled_rhs_equals_dense_operator_rhs(oracle, rng):
     z = oracle.rhs_coupled(pf, visc_props(ma, rng, 0.0).__class__(np.zeros(ne), np.zeros(ne), np.zeros(6)),
                            st, g12, g21, g22, np.full((nx, nxi), pf.vpl), th, sig)
     assert np.abs(z[3]).max() == 0.0
+
+
+def _dila_props(mf, rng):
+    from types import SimpleNamespace
+    shp = (mf.nx, mf.nxi)
+    return SimpleNamespace(tp=10 ** rng.uniform(5, 7, shp), eps=10 ** rng.uniform(-5, -3, shp),
+                           beta=10 ** rng.uniform(-10, -9, shp), p0=rng.uniform(1e6, 5e6, shp))
+
+
+def _tau_f_dila(a, b, L, sigma, f0, v0, v, th, pp):
+    """Power-law rate-and-state friction on the effective normal stress
+    (Segall & Rice, the form of equation.jl:248-276): f0 (sigma - p)
+    (v/v0)^(a/f0) (theta v0/L)^(b/f0)."""
+    return f0 * (sigma - pp) * max(0.0, v / v0) ** (a / f0) * max(0.0, th * v0 / L) ** (b / f0)
+
+
+def test_dilatancy_update_is_quasi_static_balance(oracle, rng):
+    """The rates of v, theta and p keep the friction equal to the stress:
+    d/dt tau_f(v, theta, p) = dtau (no radiation damping in this law); theta
+    follows the aging law; delta' = v."""
+    mf = small_fault(90.0)
+    p = rs_props(mf, rng)
+    d = _dila_props(mf, rng)
+    nf = mf.nx * mf.nxi
+    flat = lambda x: np.asarray(x).reshape(-1, order="F")
+    v = p.vpl * np.exp(2 * rng.standard_normal(nf))
+    th = flat(p.L) / v * np.exp(rng.standard_normal(nf))
+    pp = flat(d.p0) * (1 + 0.5 * rng.standard_normal(nf))
+    dtau = 1e-3 * rng.standard_normal(nf)
+    dv, dth, dd, dp = oracle.update_fault_dila(p, d, v, th, pp, dtau)
+    a, b, L, sg = flat(p.a), flat(p.b), flat(p.L), flat(p.σ)
+    for i in range(0, nf, 5):
+        h = 1e-3 * min(abs(v[i] / dv[i]), abs(th[i] / dth[i]), abs(pp[i] / dp[i]))
+        f = lambda s: _tau_f_dila(a[i], b[i], L[i], sg[i], p.f0, p.v0, v[i] + s * dv[i],
+                                  th[i] + s * dth[i], pp[i] + s * dp[i])
+        rate = (8 * (f(h) - f(-h)) - (f(2 * h) - f(-2 * h))) / (12 * h)
+        assert rate == pytest.approx(dtau[i], rel=1e-6, abs=1e-9 * abs(dtau).max())
+        assert dth[i] == pytest.approx(1 - v[i] * th[i] / L[i], rel=1e-15)
+    assert np.array_equal(dd, v)
+
+
+def test_dilatancy_pressure_law(oracle, rng):
+    """At a steady state variable the pore pressure relaxes linearly toward
+    p0 on the time scale tp; at p = p0 it follows (eps/beta) d(ln theta)/dt."""
+    mf = small_fault(90.0)
+    p = rs_props(mf, rng)
+    d = _dila_props(mf, rng)
+    nf = mf.nx * mf.nxi
+    flat = lambda x: np.asarray(x).reshape(-1, order="F")
+    v = p.vpl * np.exp(rng.standard_normal(nf))
+    th_ss = flat(p.L) / v
+    p0, tp = flat(d.p0), flat(d.tp)
+    z = np.zeros(nf)
+    _, dth, _, dp1 = oracle.update_fault_dila(p, d, v, th_ss, p0 + 1e5, z)
+    _, _, _, dp2 = oracle.update_fault_dila(p, d, v, th_ss, p0 + 3e5, z)
+    assert np.abs(dth).max() <= 4e-16
+    assert np.allclose(dp1, -1e5 / tp, rtol=1e-12) and np.allclose(dp2, 3 * dp1, rtol=1e-12)
+    th = th_ss * np.exp(rng.standard_normal(nf))
+    _, dth, _, dp = oracle.update_fault_dila(p, d, v, th, p0, z)
+    ratio = flat(d.eps) / flat(d.beta)
+    for i in range(0, nf, 5):
+        h = 1e-4 * th[i] / max(abs(dth[i]), 1e-300)
+        g = lambda s: ratio[i] * np.log(th[i] + s * dth[i])
+        assert (g(h) - g(-h)) / (2 * h) == pytest.approx(dp[i], rel=1e-6)
+
+
+def test_dilatancy_rhs_uses_fft_traction_rate(oracle, rng):
+    """The dilatancy ode (equation.jl:168-183) = FFT traction rate + the
+    pointwise update."""
+    mf = small_fault(60.0)
+    p = rs_props(mf, rng)
+    d = _dila_props(mf, rng)
+    st = oracle.gf_ff(mf, 3e10, 3e10, buffer_ratio=1.0)
+    v = p.vpl * np.exp(rng.standard_normal((mf.nx, mf.nxi)))
+    th = np.asarray(p.L) / v
+    pp = np.asarray(d.p0) * 1.1
+    out = oracle.rhs_fault_dila(p, d, st, v, th, pp)
+    dtau = oracle.dtau_fft(st, v - p.vpl)
+    ref = oracle.update_fault_dila(p, d, v.reshape(-1, order="F"), th.reshape(-1, order="F"),
+                                   pp.reshape(-1, order="F"), np.asarray(dtau).reshape(-1, order="F"))
+    for k in range(4):
+        assert np.array_equal(out[k].reshape(-1, order="F"), ref[k])
