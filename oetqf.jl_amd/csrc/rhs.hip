@@ -250,33 +250,66 @@ template <int KT> static void launch_matvec(oq_rhs *h, const double *s, const do
 // The reference keeps K_ff as its Toeplitz kernel st[|i-k|, j, l] (GF.jl:41-58)
 // and applies it by FFT convolution along strike (equation.jl:44-61).  Here
 // the same kernel (nx*nxi^2 doubles instead of nf^2) is applied directly:
-// block (j, c) forms, for every strike index i of dip row j, the partial sum
-// over the dip rows l of chunk c of  sum_k st[|i-k|, j, l] s[k, l],  with
-// st[:, j, l] and s[:, l] staged in LDS; k_toep_finish adds the chunks in a
-// fixed order and applies the rate-and-state update.
+// block (j, c) stages, for the dip rows l of chunk c, the reflected kernel
+// T2[m] = st[|m|, j, l] (m in [-(nx-1), nx-1], zero-padded above) and s[:, l]
+// in LDS; thread (i-block, l) forms kTR consecutive outputs
+// out[i + r] = sum_k T2[i + r - k] s[k, l] with a register window over T2
+// that slides by one element per column (one LDS load and one broadcast
+// load per kTR multiply-adds); the l partials of the block are added in a
+// fixed order, and k_toep_finish adds the chunks and applies the
+// rate-and-state update.
+static constexpr int kTR = 8;
+
 __global__ void __launch_bounds__(256) k_toep_partial(const double *__restrict__ st,
                                                       const double *__restrict__ s, int64_t nx,
                                                       int64_t nxi, int lchunk,
                                                       double *__restrict__ part)
 {
-    extern __shared__ double tsh[];  // T[lchunk][nx], S[lchunk][nx]
+    extern __shared__ double tsh[];
     const int64_t j = blockIdx.x, c = blockIdx.y, nf = nx * nxi;
     const int64_t l0 = c * lchunk, l1 = (l0 + lchunk < nxi) ? l0 + lchunk : nxi;
-    const int64_t nl = l1 - l0;
-    double *T = tsh, *S = tsh + (size_t)lchunk * nx;
-    for (int64_t x = threadIdx.x; x < nl * nx; x += blockDim.x) {
-        const int64_t l = l0 + x / nx, i = x % nx;
-        T[x] = st[i + nx * j + nf * l];
-        S[x] = s[i + nx * l];
+    const int64_t nl = l1 - l0, W = 2 * nx - 1 + kTR;
+    double *T2 = tsh;                       // lchunk x W
+    double *S = T2 + (size_t)lchunk * W;    // lchunk x nx
+    double *Rp = S + (size_t)lchunk * nx;   // lchunk x nx partial outputs
+    for (int64_t x = threadIdx.x; x < nl * W; x += blockDim.x) {
+        const int64_t l = x / W, m = x % W - (nx - 1);
+        const int64_t am = m < 0 ? -m : m;
+        T2[x] = am < nx ? st[am + nx * j + nf * (l0 + l)] : 0.0;
+    }
+    for (int64_t x = threadIdx.x; x < nl * nx; x += blockDim.x) S[x] = s[x % nx + nx * (l0 + x / nx)];
+    __syncthreads();
+    const int64_t nib = (nx + kTR - 1) / kTR;
+    for (int64_t t = threadIdx.x; t < nib * nl; t += blockDim.x) {
+        const int64_t l = t / nib, i = (t % nib) * kTR;
+        const double *T = T2 + l * W + (nx - 1);  // T[m], m in [-(nx-1), nx-1+kTR]
+        const double *Sl = S + l * nx;
+        double w[kTR], acc[kTR];
+#pragma unroll
+        for (int r = 0; r < kTR; ++r) {
+            w[r] = T[i + r];  // window for column k = 0: slot r holds T[i + r]
+            acc[r] = 0.0;
+        }
+        for (int64_t k = 0; k < nx; k += kTR) {
+#pragma unroll
+            for (int kk = 0; kk < kTR; ++kk) {
+                // column k + kk: slot (r - kk) mod kTR holds T[i + r - k - kk]
+                if (k + kk < nx) {
+                    const double sv = Sl[k + kk];
+#pragma unroll
+                    for (int r = 0; r < kTR; ++r) acc[r] += w[(r - kk + kTR) % kTR] * sv;
+                    if (k + kk + 1 < nx) w[(kTR - 1 - kk) % kTR] = T[i - (k + kk) - 1];
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kTR; ++r)
+            if (i + r < nx) Rp[l * nx + i + r] = acc[r];
     }
     __syncthreads();
     for (int64_t i = threadIdx.x; i < nx; i += blockDim.x) {
         double acc = 0.0;
-        for (int64_t l = 0; l < nl; ++l) {
-            const double *Tl = T + l * nx, *Sl = S + l * nx;
-            for (int64_t k = 0; k <= i; ++k) acc += Tl[i - k] * Sl[k];
-            for (int64_t k = i + 1; k < nx; ++k) acc += Tl[k - i] * Sl[k];
-        }
+        for (int64_t l = 0; l < nl; ++l) acc += Rp[l * nx + i];
         part[c * nf + i + nx * j] = acc;
     }
 }
@@ -293,18 +326,24 @@ __global__ void __launch_bounds__(256) k_toep_finish(RhsDev d, const double *__r
     fault_update(d, q, tau, u, du);
 }
 
-// chunking of the dip rows: enough blocks to fill the device
+// LDS bytes of a block with lchunk dip rows
+static inline int64_t toep_lds(const oq_rhs *h, int64_t lc)
+{
+    return lc * (2 * h->nx - 1 + kTR + 2 * h->nx) * 8;
+}
+
+// chunking of the dip rows: enough blocks to fill the device, LDS <= 64 KB
 static void toep_shape(const oq_rhs *h, int *lchunk, int *nchunk, int *threads)
 {
     int nc = (int)((1024 + h->nxi - 1) / h->nxi);
     if (nc > h->nxi) nc = (int)h->nxi;
     int lc = (int)((h->nxi + nc - 1) / nc);
-    // LDS: 2 * lchunk * nx doubles within 64 KB
-    while (lc > 1 && (int64_t)2 * lc * h->nx * 8 > 65536) --lc;
+    while (lc > 1 && toep_lds(h, lc) > 65536) --lc;
     *lchunk = lc;
     *nchunk = (int)((h->nxi + lc - 1) / lc);
-    const int64_t t = (h->nx + 63) / 64 * 64;
-    *threads = (int)(t < 256 ? t : 256);
+    const int64_t t = ((h->nx + kTR - 1) / kTR) * lc;
+    const int64_t t64 = (t + 63) / 64 * 64;
+    *threads = (int)(t64 < 64 ? 64 : (t64 > 256 ? 256 : t64));
 }
 
 int rhs_enqueue_matvec(oq_rhs *h, const double *u, double *du)
@@ -313,7 +352,7 @@ int rhs_enqueue_matvec(oq_rhs *h, const double *u, double *du)
         int lc, nc, nt;
         toep_shape(h, &lc, &nc, &nt);
         hipLaunchKernelGGL(k_toep_partial, dim3((unsigned)h->nxi, (unsigned)nc), dim3(nt),
-                           (size_t)2 * lc * h->nx * 8, h->stream, h->K.as<double>(),
+                           (size_t)toep_lds(h, lc), h->stream, h->K.as<double>(),
                            h->s64.as<double>(), h->nx, h->nxi, lc, h->tpart.as<double>());
         hipLaunchKernelGGL(k_toep_finish, dim3(nblk(h->nf, 256)), dim3(256), 0, h->stream, h->dev,
                            h->tpart.as<double>(), nc, u, du);
@@ -408,8 +447,8 @@ static int alloc_common(oq_rhs *h, int device, int64_t nx, int64_t nxi, int64_t 
                    kdtype == OQ_K_TOEPLITZ,
                "kdtype must be OQ_K_FP64, OQ_K_FP32, OQ_K_BF16 or OQ_K_TOEPLITZ");
     OQ_REQUIRE(kdtype != OQ_K_TOEPLITZ || ne == 0, "the Toeplitz operator is for fault-only problems");
-    OQ_REQUIRE(kdtype != OQ_K_TOEPLITZ || nx <= 4096,
-               "the Toeplitz operator takes nx <= 4096 (one strike row of K and s in 64 KB of LDS)");
+    OQ_REQUIRE(kdtype != OQ_K_TOEPLITZ || nx <= 2040,
+               "the Toeplitz operator takes nx <= 2040 (one dip row of the kernel and of s in 64 KB of LDS)");
     int ndev = 0;
     OQ_HIP(hipGetDeviceCount(&ndev));
     OQ_REQUIRE(device >= 0 && device < ndev, "invalid device index");
