@@ -243,6 +243,14 @@ int oq_stepper_record(oq_stepper *s, int64_t stride, int64_t snap_capacity,
 int oq_stepper_drain(oq_stepper *s, int64_t *nsteps, double *step_t, double *step_vmax,
                      int64_t *nsnaps, double *snap_t, double *snap_u);
 int oq_stepper_destroy(oq_stepper *s);
+/* As oq_stepper_create, with VCABM5 instead of Tsit5 (the solver of the
+ * reference's example, examples/otf-with-mantle.jl:160,167-168): variable-
+ * coefficient Adams-Bashforth-Moulton of order 5 in divided-difference form
+ * (Hairer, Norsett & Wanner I, Sec. III.5), PECE, two RHS per step; the
+ * order ramps from 1 to 5 over the first steps.  run/state/record/drain as
+ * for Tsit5. */
+int oq_stepper_create_vcabm5(oq_rhs *h, const double *u0, double t0, double t1,
+                             const oq_solver_opts *opts, oq_stepper **out);
 /* Persistent mode (no counterpart in the reference; the graph stepper's
  * semantics are kept): for fault-only problems with an FP64 operator whose
  * rows fit in LDS of at most 128 workgroups (N <= 1024), oq_stepper_run executes all of

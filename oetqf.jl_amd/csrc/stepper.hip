@@ -213,6 +213,175 @@ __global__ void __launch_bounds__(256) k_record(const Ctrl *__restrict__ c, Reco
     }
 }
 
+// ---------------- VCABM5: variable-coefficient Adams-Bashforth-Moulton ----------------
+// The solver of the reference's example (VCABM5, examples/otf-with-mantle.jl:
+// 160,167-168), restated from Hairer, Norsett & Wanner I, Sec. III.5 (variable
+// step size Adams methods in divided-difference form), PECE:
+//   predictor p = y_n + h sum_{j<k} g_j beta_j Phi_j, fp = f(p),
+//   corrector y_{n+1} = p + h g_k Phi^p_k, Phi^p_k = fp - sum_{j<k} beta_j Phi_j,
+//   error h (g_k - g_{k-1}) Phi^p_k, then f_{n+1} = f(y_{n+1}) and
+//   Phi_0 = f_{n+1}, Phi_{j+1} = Phi_j(n+1) - beta_j Phi_j(n).
+// Two RHS per step; k = 1 at the start, + 1 per accepted step up to 5; step
+// factor h_new/h = 1/clamp(err^(1/(k+1))/0.9, 1/2, 5).
+static constexpr int kAbK = 5;
+
+struct AbCtrl {
+    Ctrl c;  // t, dt (next step), dtcur, eest, tolerances, counters, flags
+    int k, kcur;
+    double hist[kAbK + 1];  // past steps, most recent first
+    double g[kAbK + 1], beta[kAbK + 1];
+};
+
+__device__ void ab_coeffs(int k, double h, const double *hist, double *g, double *beta)
+{
+    double pn[kAbK + 1], po[kAbK + 1], c[kAbK + 2];
+    pn[0] = h;
+    po[0] = hist[0];
+    for (int i = 1; i <= k; ++i) {
+        pn[i] = pn[i - 1] + hist[i - 1];
+        po[i] = po[i - 1] + hist[i];
+    }
+    beta[0] = 1.0;
+    for (int j = 1; j < k; ++j) beta[j] = beta[j - 1] * pn[j - 1] / po[j - 1];
+    for (int q = 1; q <= k + 1; ++q) c[q] = 1.0 / q;
+    g[0] = c[1];
+    for (int j = 1; j <= k; ++j) {
+        for (int q = 1; q <= k + 1 - j; ++q) c[q] = c[q] - c[q + 1] * h / pn[j - 1];
+        g[j] = c[1];
+    }
+}
+
+__global__ void k_ab_begin(AbCtrl *a)
+{
+    Ctrl *c = &a->c;
+    c->accept = 0;
+    if (c->done) return;
+    if (!(c->t < c->t1)) {
+        c->done = 1;
+        c->retcode = OQ_RET_SUCCESS;
+        return;
+    }
+    if (c->iter >= c->maxiters) {
+        c->done = 1;
+        c->retcode = OQ_RET_MAXITERS;
+        return;
+    }
+    double dt = c->dt;
+    if (dt > c->dtmax) dt = c->dtmax;
+    if (c->t + dt > c->t1) dt = c->t1 - c->t;
+    if (!(dt >= c->dtmin) || !isfinite(dt)) {
+        c->done = 1;
+        c->retcode = isfinite(dt) ? OQ_RET_DTMIN : OQ_RET_NONFINITE;
+        return;
+    }
+    c->dtcur = dt;
+    c->iter += 1;
+    a->kcur = a->k;
+    ab_coeffs(a->k, dt, a->hist, a->g, a->beta);
+}
+
+__global__ void __launch_bounds__(256) k_ab_predict(const AbCtrl *__restrict__ a, int64_t n,
+                                                    const double *__restrict__ u,
+                                                    const double *__restrict__ phi,
+                                                    double *__restrict__ p)
+{
+    if (a->c.done) return;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int k = a->kcur;
+    double acc = 0.0;
+    for (int j = 0; j < k; ++j) acc += a->g[j] * a->beta[j] * phi[j * n + i];
+    p[i] = u[i] + a->c.dtcur * acc;
+}
+
+// corrector and per-block sums of the squared scaled error
+__global__ void __launch_bounds__(256) k_ab_correct(const AbCtrl *__restrict__ a, int64_t n,
+                                                    const double *__restrict__ u,
+                                                    const double *__restrict__ p,
+                                                    const double *__restrict__ fp,
+                                                    const double *__restrict__ phi,
+                                                    double *__restrict__ ynew,
+                                                    double *__restrict__ part)
+{
+    __shared__ double red[256];
+    if (a->c.done) return;
+    const int k = a->kcur;
+    const double h = a->c.dtcur, gk = a->g[k], dg = a->g[k] - a->g[k - 1];
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        double ph = fp[i];
+        for (int j = 0; j < k; ++j) ph -= a->beta[j] * phi[j * n + i];
+        const double yn = p[i] + h * gk * ph;
+        ynew[i] = yn;
+        const double w = h * dg * ph / (a->c.abstol + fmax(fabs(u[i]), fabs(yn)) * a->c.reltol);
+        acc += w * w;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(256) k_ab_controller(AbCtrl *a, int64_t n, const double *part)
+{
+    __shared__ double red[256];
+    Ctrl *c = &a->c;
+    if (c->done) return;
+    red[threadIdx.x] = part[threadIdx.x];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const double eest = sqrt(red[0] / (double)n);
+    const int k = a->kcur;
+    const double q = (eest == 0.0) ? 0.5 : fmax(0.5, fmin(5.0, pow(eest, 1.0 / (k + 1)) / 0.9));
+    c->eest = eest;
+    if (eest <= 1.0) {
+        c->t = (c->t1 - (c->t + c->dtcur) <= 0.0) ? c->t1 : c->t + c->dtcur;
+        for (int j = kAbK; j > 0; --j) a->hist[j] = a->hist[j - 1];
+        a->hist[0] = c->dtcur;
+        if (a->k < kAbK) a->k += 1;
+        c->dt = c->dtcur / q;
+        c->naccept += 1;
+        c->accept = 1;
+    } else {
+        c->nreject += 1;
+        if (!isfinite(eest)) {
+            c->done = 1;
+            c->retcode = OQ_RET_NONFINITE;
+            return;
+        }
+        c->dt = c->dtcur / q;
+    }
+}
+
+// accepted step: u <- y_{n+1}; divided differences of the new derivative
+__global__ void __launch_bounds__(256) k_ab_commit(const AbCtrl *__restrict__ a, int64_t n,
+                                                   double *__restrict__ u,
+                                                   const double *__restrict__ ynew,
+                                                   const double *__restrict__ fnew,
+                                                   double *__restrict__ phi)
+{
+    if (!a->c.accept) return;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int k = a->kcur;
+    double tmp = fnew[i];
+    for (int j = 0; j < k; ++j) {
+        const double old = phi[j * n + i];
+        phi[j * n + i] = tmp;
+        tmp -= a->beta[j] * old;
+    }
+    phi[k * n + i] = tmp;
+    u[i] = ynew[i];
+}
+
 // ---------------- persistent stepper for small fault-only problems ----------------
 // For a few hundred to a few thousand patches the graph above is bound by
 // its ~23 kernel launches per step, not by arithmetic.  Here one cooperative
@@ -475,6 +644,9 @@ struct oq_stepper {
     oq::DevBuf rcnt, rstep_t, rstep_v, rsnap_t, rsnap_u;
     oq::Recorder rec{};
     bool recording = false;
+    // algorithm: 0 Tsit5, 1 VCABM5 (oq_stepper_create_vcabm5)
+    int alg = 0;
+    oq::DevBuf ab;  // VCABM5: y_{n+1} and f(y_{n+1}), 2n
     // persistent mode (oq_stepper_set_persistent)
     bool persistent = false;
     int p_R = 0, p_nwg = 0;
@@ -487,9 +659,34 @@ namespace oq {
 
 static inline unsigned nb(int64_t n) { return (unsigned)((n + 255) / 256); }
 
+// One attempted VCABM5 step: two RHS evaluations (the second also runs on a
+// rejected step; its result is not used then).
+static int enqueue_step_ab(oq_stepper *st)
+{
+    oq_rhs *h = st->h;
+    const int64_t n = st->n;
+    AbCtrl *a = st->ctrl.as<AbCtrl>();
+    double *u = st->u.as<double>(), *p = st->y.as<double>(), *phi = st->k.as<double>();
+    double *fp = phi + 6 * n, *ynew = st->ab.as<double>(), *fnew = ynew + n;
+    int rc;
+    hipLaunchKernelGGL(k_ab_begin, dim3(1), dim3(1), 0, h->stream, a);
+    hipLaunchKernelGGL(k_ab_predict, dim3(nb(n)), dim3(256), 0, h->stream, a, n, u, phi, p);
+    if ((rc = rhs_enqueue(h, p, fp))) return rc;
+    hipLaunchKernelGGL(k_ab_correct, dim3(kErrBlocks), dim3(256), 0, h->stream, a, n, u, p, fp, phi,
+                       ynew, st->part.as<double>());
+    hipLaunchKernelGGL(k_ab_controller, dim3(1), dim3(256), 0, h->stream, a, n, st->part.as<double>());
+    if ((rc = rhs_enqueue(h, ynew, fnew))) return rc;
+    hipLaunchKernelGGL(k_ab_commit, dim3(nb(n)), dim3(256), 0, h->stream, a, n, u, ynew, fnew, phi);
+    if (st->recording)
+        hipLaunchKernelGGL(k_record, dim3(1), dim3(256), 0, h->stream, &a->c, st->rec, u);
+    OQ_HIP(hipGetLastError());
+    return OQ_OK;
+}
+
 // The launch sequence of one attempted step (captured into a graph).
 static int enqueue_step(oq_stepper *st)
 {
+    if (st->alg == 1) return enqueue_step_ab(st);
     oq_rhs *h = st->h;
     const int64_t n = st->n;
     Ctrl *c = st->ctrl.as<Ctrl>();
@@ -559,8 +756,8 @@ using namespace oq;
 
 extern "C" {
 
-int oq_stepper_create(oq_rhs *h, const double *u0, double t0, double t1,
-                      const oq_solver_opts *o, oq_stepper **out)
+static int stepper_create(oq_rhs *h, const double *u0, double t0, double t1,
+                          const oq_solver_opts *o, int alg, oq_stepper **out)
 {
     OQ_REQUIRE(h && u0 && o && out, "NULL argument");
     OQ_REQUIRE(t1 > t0, "tspan must satisfy t1 > t0");
@@ -570,12 +767,14 @@ int oq_stepper_create(oq_rhs *h, const double *u0, double t0, double t1,
     oq_stepper *st = new oq_stepper();
     st->h = h;
     st->n = h->lay.n;
+    st->alg = alg;
     const int64_t n = st->n;
     int rc = st->u.alloc((size_t)n * 8);
     if (!rc) rc = st->y.alloc((size_t)n * 8);
     if (!rc) rc = st->k.alloc((size_t)(7 * n) * 8);
     if (!rc) rc = st->part.alloc(kErrBlocks * 8);
-    if (!rc) rc = st->ctrl.alloc(sizeof(Ctrl));
+    if (!rc) rc = st->ctrl.alloc(alg == 1 ? sizeof(AbCtrl) : sizeof(Ctrl));
+    if (!rc && alg == 1) rc = st->ab.alloc((size_t)(2 * n) * 8);
     if (!rc && hipHostMalloc((void **)&st->ctrl_host, sizeof(Ctrl), hipHostMallocDefault) != hipSuccess) {
         set_error("hipHostMalloc failed");
         rc = OQ_ERR_NOMEM;
@@ -637,7 +836,13 @@ int oq_stepper_create(oq_rhs *h, const double *u0, double t0, double t1,
     c.dtmin = o->dtmin;
     c.maxiters = o->maxiters > 0 ? o->maxiters : (long long)1e18;
     *st->ctrl_host = c;
-    if (hipMemcpyAsync(st->ctrl.p, st->ctrl_host, sizeof(Ctrl), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+    AbCtrl ab;
+    memset(&ab, 0, sizeof(ab));
+    ab.c = c;
+    ab.k = 1;
+    const void *cblk = alg == 1 ? (const void *)&ab : (const void *)st->ctrl_host;
+    if (hipMemcpyAsync(st->ctrl.p, cblk, alg == 1 ? sizeof(AbCtrl) : sizeof(Ctrl), hipMemcpyHostToDevice,
+                       h->stream) != hipSuccess ||
         hipStreamSynchronize(h->stream) != hipSuccess) {
         set_error("control block upload failed");
         destroy_stepper(st);
@@ -650,6 +855,18 @@ int oq_stepper_create(oq_rhs *h, const double *u0, double t0, double t1,
     }
     *out = st;
     return OQ_OK;
+}
+
+int oq_stepper_create(oq_rhs *h, const double *u0, double t0, double t1,
+                      const oq_solver_opts *o, oq_stepper **out)
+{
+    return stepper_create(h, u0, t0, t1, o, 0, out);
+}
+
+int oq_stepper_create_vcabm5(oq_rhs *h, const double *u0, double t0, double t1,
+                             const oq_solver_opts *o, oq_stepper **out)
+{
+    return stepper_create(h, u0, t0, t1, o, 1, out);
 }
 
 int oq_stepper_run(oq_stepper *st, int64_t max_steps, int *retcode)
@@ -717,6 +934,7 @@ int oq_stepper_set_persistent(oq_stepper *st, int on)
     }
     OQ_REQUIRE(h->ne == 0 && !h->dev.dila,
                "the persistent stepper takes fault-only problems without dilatancy");
+    OQ_REQUIRE(st->alg == 0, "the persistent stepper runs Tsit5");
     OQ_REQUIRE(h->kdtype == OQ_K_FP64, "the persistent stepper takes an FP64 operator");
     int coop = 0, cus = 0, maxlds = 0, khz = 0;
     OQ_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, h->device));

@@ -7,7 +7,7 @@ generation, stress Green's functions, properties, `assemble` and a Tsit5
 from ._lib import OetqfError, device_count, load as load_library  # noqa: F401
 from .ensemble import Ensemble  # noqa: F401
 from .group import Group  # noqa: F401
-from .equation import (ArrayPartition, Integrator, ODEProblem, ODESolution, Tsit5,  # noqa: F401
+from .equation import (ArrayPartition, Integrator, ODEProblem, ODESolution, Tsit5, VCABM5,  # noqa: F401,E501
                        assemble, assemble_from_mesh, similar, solve)
 from .io import wread, wsolve  # noqa: F401
 from .greens import (DipSlip, StrikeSlip, get_quadrature, gmsh_quadrature,  # noqa: F401

@@ -88,6 +88,8 @@ PROTOTYPES = {
     "oq_rhs_time": (_int, [_vp, _int, ctypes.POINTER(_d), ctypes.POINTER(_d)]),
     "oq_rhs_destroy": (_int, [_vp]),
     "oq_stepper_create": (_int, [_vp, _pd, _d, _d, ctypes.POINTER(SolverOpts), ctypes.POINTER(_vp)]),
+    "oq_stepper_create_vcabm5": (_int, [_vp, _pd, _d, _d, ctypes.POINTER(SolverOpts),
+                                        ctypes.POINTER(_vp)]),
     "oq_stepper_run": (_int, [_vp, _i64, ctypes.POINTER(_int)]),
     "oq_stepper_state": (_int, [_vp, ctypes.POINTER(_d), ctypes.POINTER(_d), _pd,
                                 ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),

@@ -13,13 +13,13 @@ from .equation import Integrator
 
 
 def run_cycle(prob, reltol=1e-6, abstol=1e-8, dt=0.0, dtmax=0.0, chunk=4096, wall_limit=None,
-              progress=None, persistent=False):
+              progress=None, persistent=False, alg=None):
     """Integrates prob over its tspan on the device.  Returns a dict with the
     per-step record (t, vmax), the final state and counters.  persistent:
     True = one cooperative launch per chunk of steps (small fault-only
     problems; ValueError otherwise), "auto" = that where the problem
     qualifies, else the graph stepper."""
-    it = Integrator(prob, reltol=reltol, abstol=abstol, dt=dt, dtmax=dtmax)
+    it = Integrator(prob, reltol=reltol, abstol=abstol, dt=dt, dtmax=dtmax, alg=alg)
     it.record(stride=1, snap_capacity=0, step_capacity=chunk)
     mode = "graph"
     if persistent:

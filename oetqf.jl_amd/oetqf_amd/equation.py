@@ -231,6 +231,11 @@ class Tsit5:
     """Tsitouras 5(4) (the algorithm object passed to solve)."""
 
 
+class VCABM5:
+    """Variable-coefficient Adams-Bashforth-Moulton of order 5, PECE (the
+    solver of the reference's example, examples/otf-with-mantle.jl:160)."""
+
+
 class ODESolution:
     def __init__(self, t, u, retcode, naccept, nreject):
         self.t, self.u, self.retcode = t, u, retcode
@@ -241,15 +246,18 @@ class Integrator:
     """Device-resident Tsit5 integration of an assembled problem."""
 
     def __init__(self, prob, reltol=1e-6, abstol=1e-8, dt=0.0, dtmax=0.0, dtmin=0.0,
-                 maxiters=10**9):
+                 maxiters=10**9, alg=None):
         self.prob = prob
         self.u0 = prob.u0
         self._u0flat = np.ascontiguousarray(prob.u0.pack())
         o = _lib.SolverOpts(reltol, abstol, dt, dtmax, dtmin, int(maxiters))
         self.s = ctypes.c_void_p()
-        _lib.check(_lib.load().oq_stepper_create(prob.f.h, _lib.ptr(self._u0flat), prob.tspan[0],
-                                                 prob.tspan[1], ctypes.byref(o),
-                                                 ctypes.byref(self.s)))
+        if alg is not None and not isinstance(alg, (Tsit5, VCABM5)):
+            raise NotImplementedError("the device steppers are Tsit5 and VCABM5")
+        create = (_lib.load().oq_stepper_create_vcabm5 if isinstance(alg, VCABM5)
+                  else _lib.load().oq_stepper_create)
+        _lib.check(create(prob.f.h, _lib.ptr(self._u0flat), prob.tspan[0], prob.tspan[1],
+                          ctypes.byref(o), ctypes.byref(self.s)))
 
     def step(self, nsteps):
         rc = ctypes.c_int()
@@ -304,11 +312,9 @@ class Integrator:
 
 def solve(prob, alg=None, reltol=1e-6, abstol=1e-8, dt=0.0, dtmax=0.0, dtmin=0.0,
           maxiters=10**9, chunk=4096):
-    """solve(prob, Tsit5(); reltol, abstol, dt, dtmax, maxiters): integrates
-    to tspan[2] on the device and returns the final state."""
-    if alg is not None and not isinstance(alg, Tsit5):
-        raise NotImplementedError("only Tsit5 is provided")
-    it = Integrator(prob, reltol, abstol, dt, dtmax, dtmin, maxiters)
+    """solve(prob, Tsit5() or VCABM5(); reltol, abstol, dt, dtmax, maxiters):
+    integrates to tspan[2] on the device and returns the final state."""
+    it = Integrator(prob, reltol, abstol, dt, dtmax, dtmin, maxiters, alg=alg)
     ret = "Running"
     while ret == "Running":
         ret = it.step(chunk)

@@ -175,12 +175,28 @@ void oq_tsit5_record(oq_rhs_fn f, void *ctx, int64_t n, double *u, double t0,
                      int64_t *nrec);
 const double *oq_tsit5_tableau(void); /* c[7], a[7][7], b[7], btilde[7] */
 
+/* VCABM5 (variable-coefficient Adams-Bashforth-Moulton, order 5, PECE),
+ * vcabm.c; same interface and recorder as oq_tsit5_record. */
+void oq_vcabm5(oq_rhs_fn f, void *ctx, int64_t n, double *u, double t0,
+               double t1, const oq_ode_opts_t *o, oq_ode_stats_t *st);
+void oq_vcabm5_record(oq_rhs_fn f, void *ctx, int64_t n, double *u, double t0,
+                      double t1, const oq_ode_opts_t *o, oq_ode_stats_t *st,
+                      int64_t nrec_n, double *step_t, double *step_v, int64_t cap,
+                      int64_t *nrec);
+/* g[0..k] and beta[0..k-1] of a step h with past steps hist[0..k]. */
+void oq_vcabm_coeffs(int k, double h, const double *hist, double *g, double *beta);
+
 /* A whole earthquake cycle in C: Tsit5 over oq_rhs_dense (packed state
  * [v θ δ] or [v θ ϵ σ δ]), recording (t, max v) per accepted step. */
 void oq_cycle_dense(const oq_rs_t *pf, const oq_visc_t *pa, const double *K,
                     int64_t nf, int64_t ne, double *u, double t0, double t1,
                     const oq_ode_opts_t *o, oq_ode_stats_t *st, double *step_t,
                     double *step_v, int64_t cap, int64_t *nrec);
+/* The same with alg = 0 (Tsit5) or 1 (VCABM5). */
+void oq_cycle_dense_alg(const oq_rs_t *pf, const oq_visc_t *pa, const double *K,
+                        int64_t nf, int64_t ne, double *u, double t0, double t1,
+                        const oq_ode_opts_t *o, oq_ode_stats_t *st, double *step_t,
+                        double *step_v, int64_t cap, int64_t *nrec, int alg);
 
 #ifdef __cplusplus
 }

@@ -425,6 +425,39 @@ def tsit5(f, u0, t0, t1, reltol=1e-6, abstol=1e-8, dtmax=np.inf, dtmin=0.0, dt0=
     return u, st
 
 
+def vcabm5(f, u0, t0, t1, reltol=1e-6, abstol=1e-8, dtmax=np.inf, dtmin=0.0, dt0=0.0,
+           maxiters=10**9):
+    """VCABM5 (oracle/vcabm.c).  f(t, u) -> du on flat numpy arrays.
+    Returns (u, stats)."""
+    u = np.array(u0, dtype=np.float64, copy=True)
+    n = u.size
+
+    def cb(_ctx, t, up, dup):
+        uu = np.ctypeslib.as_array(up, shape=(n,))
+        du = np.ctypeslib.as_array(dup, shape=(n,))
+        du[:] = f(t, uu)
+
+    fn = RHSFN(cb)
+    o = OdeOpts(reltol, abstol, dtmax, dtmin, dt0, maxiters)
+    st = OdeStats()
+    L = lib()
+    L.oq_vcabm5.argtypes = [RHSFN, ctypes.c_void_p, _i64, _pd, _d, _d, ctypes.POINTER(OdeOpts),
+                            ctypes.POINTER(OdeStats)]
+    L.oq_vcabm5(fn, None, n, _p(u), t0, t1, ctypes.byref(o), ctypes.byref(st))
+    return u, st
+
+
+def vcabm_coeffs(k, h, hist):
+    """(g[0..k], beta[0..k-1]) of the VCABM step (oracle/vcabm.c)."""
+    hs = np.zeros(6)
+    hs[:len(hist)] = hist
+    g, beta = np.zeros(6), np.zeros(6)
+    L = lib()
+    L.oq_vcabm_coeffs.argtypes = [ctypes.c_int, _d, _pd, _pd, _pd]
+    L.oq_vcabm_coeffs(int(k), float(h), _p(hs), _p(g), _p(beta))
+    return g[:k + 1].copy(), beta[:k].copy()
+
+
 def tsit5_tableau():
     p = lib().oq_tsit5_tableau()
     a = np.ctypeslib.as_array(p, shape=(70,)).copy()
@@ -432,9 +465,10 @@ def tsit5_tableau():
 
 
 def cycle_dense(pf, pa, K, nf, ne, u0, t0, t1, reltol=1e-6, abstol=1e-8, dtmax=np.inf, dtmin=0.0,
-                dt0=0.0, maxiters=10**9, cap=2_000_000):
-    """A whole cycle in C (oq_cycle_dense): Tsit5 over the dense RHS; returns
-    (u, stats, step_t, step_vmax) for every accepted step."""
+                dt0=0.0, maxiters=10**9, cap=2_000_000, alg="tsit5"):
+    """A whole cycle in C (oq_cycle_dense_alg): Tsit5 or VCABM5 over the
+    dense RHS; returns (u, stats, step_t, step_vmax) for every accepted
+    step."""
     keep = _Keep()
     cf = _rs(pf, keep)
     ca = _visc(pa, keep) if ne > 0 else ViscC()
@@ -444,11 +478,12 @@ def cycle_dense(pf, pa, K, nf, ne, u0, t0, t1, reltol=1e-6, abstol=1e-8, dtmax=n
     o = OdeOpts(reltol, abstol, dtmax, dtmin, dt0, maxiters)
     st = OdeStats()
     L = lib()
-    L.oq_cycle_dense.argtypes = [ctypes.POINTER(RSC), ctypes.POINTER(ViscC), _pd, _i64, _i64, _pd,
-                                 _d, _d, ctypes.POINTER(OdeOpts), ctypes.POINTER(OdeStats), _pd, _pd,
-                                 _i64, ctypes.POINTER(_i64)]
+    L.oq_cycle_dense_alg.argtypes = [ctypes.POINTER(RSC), ctypes.POINTER(ViscC), _pd, _i64, _i64,
+                                     _pd, _d, _d, ctypes.POINTER(OdeOpts), ctypes.POINTER(OdeStats),
+                                     _pd, _pd, _i64, ctypes.POINTER(_i64), ctypes.c_int]
     Kc = keep.arr(np.asarray(K).reshape(-1))
-    L.oq_cycle_dense(ctypes.byref(cf), ctypes.byref(ca), _p(Kc), nf, ne, _p(u), t0, t1,
-                     ctypes.byref(o), ctypes.byref(st), _p(st_t), _p(st_v), cap, ctypes.byref(nrec))
+    L.oq_cycle_dense_alg(ctypes.byref(cf), ctypes.byref(ca), _p(Kc), nf, ne, _p(u), t0, t1,
+                         ctypes.byref(o), ctypes.byref(st), _p(st_t), _p(st_v), cap,
+                         ctypes.byref(nrec), 1 if alg == "vcabm5" else 0)
     k = min(nrec.value, cap)
     return u, st, st_t[:k].copy(), st_v[:k].copy()

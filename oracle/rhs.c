@@ -325,7 +325,18 @@ void oq_cycle_dense(const oq_rs_t *pf, const oq_visc_t *pa, const double *K,
                     const oq_ode_opts_t *o, oq_ode_stats_t *st, double *step_t,
                     double *step_v, int64_t cap, int64_t *nrec)
 {
+    oq_cycle_dense_alg(pf, pa, K, nf, ne, u, t0, t1, o, st, step_t, step_v, cap, nrec, 0);
+}
+
+void oq_cycle_dense_alg(const oq_rs_t *pf, const oq_visc_t *pa, const double *K,
+                        int64_t nf, int64_t ne, double *u, double t0, double t1,
+                        const oq_ode_opts_t *o, oq_ode_stats_t *st, double *step_t,
+                        double *step_v, int64_t cap, int64_t *nrec, int alg)
+{
     dense_ctx_t c = {pf, pa, K, nf, ne};
     const int64_t n = ne > 0 ? 3 * nf + 12 * ne : 3 * nf;
-    oq_tsit5_record(dense_packed, &c, n, u, t0, t1, o, st, nf, step_t, step_v, cap, nrec);
+    if (alg == 1)
+        oq_vcabm5_record(dense_packed, &c, n, u, t0, t1, o, st, nf, step_t, step_v, cap, nrec);
+    else
+        oq_tsit5_record(dense_packed, &c, n, u, t0, t1, o, st, nf, step_t, step_v, cap, nrec);
 }
