@@ -631,6 +631,213 @@ __global__ void __launch_bounds__(256) k_persist(PersistArgs a)
     }
 }
 
+// Persistent VCABM5: as k_persist, with the method of enqueue_step_ab.  A
+// step needs the predicted source vector (one barrier), the error partials
+// (one barrier) and, when the step is accepted, the corrected source vector
+// (one barrier): three grid barriers instead of Tsit5's seven.  Phi_0..Phi_5
+// of the owned entries stay in LDS across the steps of a launch.
+__global__ void __launch_bounds__(256) k_persist_ab(PersistArgs a, AbCtrl *actrl, double *phig)
+{
+    extern __shared__ double lds[];
+    __shared__ AbCtrl ac;
+    __shared__ double red[256], red2[256], tau[64];
+    __shared__ int bad, take;
+    __shared__ long long nstep, nsnap;
+    const int R = a.R, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t N = a.d.N, ld = a.d.ld, n = a.d.lay.n;
+    const int64_t p0 = (int64_t)blockIdx.x * R;
+    const int Rv = (int)((N - p0) < R ? (N - p0) : R);
+    const int E = 3 * R;
+    double *Ks = lds, *sv = Ks + (size_t)R * ld;
+    double *uo = sv + ld, *po = uo + E, *fpo = po + E, *yo = fpo + E, *fo = yo + E, *ph = fo + E;  // ph: 6E
+    const int64_t comp_off[3] = {a.d.lay.v, a.d.lay.th, a.d.lay.del};
+
+    for (int64_t x = tid; x < (int64_t)Rv * ld; x += 256) Ks[x] = a.K[p0 * ld + x];
+    for (int64_t j = tid; j < ld; j += 256) sv[j] = 0.0;
+    for (int e = tid; e < E; e += 256) {
+        const int cpt = e / R, i = e % R;
+        const int64_t gi = comp_off[cpt] + p0 + i;
+        uo[e] = i < Rv ? a.u[gi] : 0.0;
+        po[e] = fpo[e] = yo[e] = fo[e] = 0.0;  // padding entries stay zero
+        for (int j = 0; j <= kAbK; ++j) ph[j * E + e] = i < Rv ? phig[j * n + gi] : 0.0;
+    }
+    if (tid == 0) {
+        ac = *actrl;
+        nstep = a.recording ? *a.rec.nstep : 0;
+        nsnap = a.recording ? *a.rec.nsnap : 0;
+    }
+    __syncthreads();
+    unsigned epoch = 0;
+    // s <- x[v] - vpl (owned), barrier, s -> LDS, owned rows of K s, rates -> out
+    auto rhs = [&](const double *x, double *out, int buf) -> bool {
+        double *sg = a.s2 + buf * N;
+        for (int i = tid; i < Rv; i += 256) sg[p0 + i] = x[i] - a.d.vpl;
+        if (!grid_barrier(a, epoch, &bad)) return false;
+        for (int64_t j = tid; j < N; j += 256) sv[j] = sg[j];
+        __syncthreads();
+        for (int r = wave; r < Rv; r += 4) {
+            const double *kr = Ks + (size_t)r * ld;
+            double acc = 0.0;
+            for (int64_t j = lane; j < ld; j += 64) acc += kr[j] * sv[j];
+            acc = wave_sum(acc);
+            if (lane == 0) tau[r] = acc;
+        }
+        __syncthreads();
+        for (int r = tid; r < Rv; r += 256) {
+            double dv, dth;
+            fault_rates(a.d, p0 + r, x[r], x[R + r], tau[r], dv, dth);
+            out[r] = dv;
+            out[R + r] = dth;
+            out[2 * R + r] = x[r];
+        }
+        __syncthreads();
+        return true;
+    };
+    for (long long it = 0; it < a.max_steps; ++it) {
+        if (tid == 0) {  // k_ab_begin
+            Ctrl &c = ac.c;
+            c.accept = 0;
+            if (!c.done) {
+                if (!(c.t < c.t1)) {
+                    c.done = 1;
+                    c.retcode = OQ_RET_SUCCESS;
+                } else if (c.iter >= c.maxiters) {
+                    c.done = 1;
+                    c.retcode = OQ_RET_MAXITERS;
+                } else {
+                    double dt = c.dt;
+                    if (dt > c.dtmax) dt = c.dtmax;
+                    if (c.t + dt > c.t1) dt = c.t1 - c.t;
+                    if (!(dt >= c.dtmin) || !isfinite(dt)) {
+                        c.done = 1;
+                        c.retcode = isfinite(dt) ? OQ_RET_DTMIN : OQ_RET_NONFINITE;
+                    } else {
+                        c.dtcur = dt;
+                        c.iter += 1;
+                        ac.kcur = ac.k;
+                        ab_coeffs(ac.k, dt, ac.hist, ac.g, ac.beta);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (ac.c.done) break;
+        const int k = ac.kcur;
+        const double h = ac.c.dtcur;
+        for (int e = tid; e < E; e += 256) {  // predictor
+            double acc = 0.0;
+            for (int j = 0; j < k; ++j) acc += ac.g[j] * ac.beta[j] * ph[j * E + e];
+            po[e] = uo[e] + h * acc;
+        }
+        __syncthreads();
+        if (!rhs(po, fpo, 0)) return;
+        double acc2 = 0.0, vm = 0.0;  // corrector, error and max |v| partials
+        for (int e = tid; e < E; e += 256) {
+            if (e % R >= Rv) continue;  // padding: yo stays zero
+            double phk = fpo[e];
+            for (int j = 0; j < k; ++j) phk -= ac.beta[j] * ph[j * E + e];
+            const double yn = po[e] + h * ac.g[k] * phk;
+            yo[e] = yn;
+            const double w = h * (ac.g[k] - ac.g[k - 1]) * phk /
+                             (ac.c.abstol + fmax(fabs(uo[e]), fabs(yn)) * ac.c.reltol);
+            acc2 += w * w;
+            if (e < R) vm = fmax(vm, fabs(yn));
+        }
+        red[tid] = acc2;
+        red2[tid] = vm;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if (tid < o) {
+                red[tid] += red[tid + o];
+                red2[tid] = fmax(red2[tid], red2[tid + o]);
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            a.part[2 * blockIdx.x] = red[0];
+            a.part[2 * blockIdx.x + 1] = red2[0];
+        }
+        if (!grid_barrier(a, epoch, &bad)) return;
+        for (int w = tid; w < 2 * a.nwg; w += 256) red[w] = a.part[w];
+        __syncthreads();
+        if (tid == 0) {  // k_ab_controller, on the same sums in every workgroup
+            Ctrl &c = ac.c;
+            double sum = 0.0, vmax = 0.0;
+            for (int w = 0; w < a.nwg; ++w) {
+                sum += red[2 * w];
+                vmax = fmax(vmax, red[2 * w + 1]);
+            }
+            const double eest = sqrt(sum / (double)n);
+            const double q = (eest == 0.0) ? 0.5 : fmax(0.5, fmin(5.0, pow(eest, 1.0 / (k + 1)) / 0.9));
+            c.eest = eest;
+            take = 0;
+            if (eest <= 1.0) {
+                c.t = (c.t1 - (c.t + c.dtcur) <= 0.0) ? c.t1 : c.t + c.dtcur;
+                for (int j = kAbK; j > 0; --j) ac.hist[j] = ac.hist[j - 1];
+                ac.hist[0] = c.dtcur;
+                if (ac.k < kAbK) ac.k += 1;
+                c.dt = c.dtcur / q;
+                c.naccept += 1;
+                c.accept = 1;
+                if (a.recording) {
+                    if (blockIdx.x == 0 && a.rec.step_cap > 0) {
+                        a.rec.step_t[nstep % a.rec.step_cap] = c.t;
+                        a.rec.step_vmax[nstep % a.rec.step_cap] = vmax;
+                    }
+                    nstep += 1;
+                    take = (a.rec.snap_cap > 0) && (c.naccept % a.rec.stride == 0);
+                }
+            } else {
+                c.nreject += 1;
+                if (!isfinite(eest)) {
+                    c.done = 1;
+                    c.retcode = OQ_RET_NONFINITE;
+                } else {
+                    c.dt = c.dtcur / q;
+                }
+            }
+        }
+        __syncthreads();
+        if (ac.c.accept) {
+            if (!rhs(yo, fo, 1)) return;
+            for (int e = tid; e < E; e += 256) {  // k_ab_commit
+                double tmp = fo[e];
+                for (int j = 0; j < k; ++j) {
+                    const double old = ph[j * E + e];
+                    ph[j * E + e] = tmp;
+                    tmp -= ac.beta[j] * old;
+                }
+                ph[k * E + e] = tmp;
+                uo[e] = yo[e];
+            }
+            if (take) {
+                const long long js = nsnap % a.rec.snap_cap;
+                for (int e = tid; e < E; e += 256)
+                    if (e % R < Rv) a.rec.snap_u[js * n + comp_off[e / R] + p0 + e % R] = yo[e];
+                if (blockIdx.x == 0 && tid == 0) a.rec.snap_t[js] = ac.c.t;
+            }
+            __syncthreads();
+            if (tid == 0 && take) nsnap += 1;
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < E; e += 256) {
+        const int cpt = e / R, i = e % R;
+        if (i < Rv) {
+            const int64_t gi = comp_off[cpt] + p0 + i;
+            a.u[gi] = uo[e];
+            for (int j = 0; j <= kAbK; ++j) phig[j * n + gi] = ph[j * E + e];
+        }
+    }
+    if (blockIdx.x == 0 && tid == 0) {
+        *actrl = ac;  // a.ctrl points at actrl->c
+        if (a.recording) {
+            *a.rec.nstep = nstep;
+            *a.rec.nsnap = nsnap;
+        }
+    }
+}
+
 }  // namespace oq
 
 struct oq_stepper {
@@ -892,9 +1099,17 @@ int oq_stepper_run(oq_stepper *st, int64_t max_steps, int *retcode)
         a.max_steps = max_steps;
         a.wait_ticks = st->p_wait;
         OQ_HIP(hipMemsetAsync(st->p_bar.p, 0, sizeof(unsigned), h->stream));
-        void *args[] = {&a};
-        OQ_HIP(hipLaunchCooperativeKernel((const void *)k_persist, dim3(st->p_nwg), dim3(256), args,
-                                          st->p_lds, h->stream));
+        if (st->alg == 1) {
+            AbCtrl *actrl = st->ctrl.as<AbCtrl>();
+            double *phig = st->k.as<double>();
+            void *args[] = {&a, &actrl, &phig};
+            OQ_HIP(hipLaunchCooperativeKernel((const void *)k_persist_ab, dim3(st->p_nwg), dim3(256),
+                                              args, st->p_lds, h->stream));
+        } else {
+            void *args[] = {&a};
+            OQ_HIP(hipLaunchCooperativeKernel((const void *)k_persist, dim3(st->p_nwg), dim3(256),
+                                              args, st->p_lds, h->stream));
+        }
         int ab = 0;
         OQ_HIP(hipMemcpyAsync(&ab, st->p_abort.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
         OQ_HIP(hipMemcpyAsync(st->ctrl_host, st->ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost,
@@ -934,7 +1149,6 @@ int oq_stepper_set_persistent(oq_stepper *st, int on)
     }
     OQ_REQUIRE(h->ne == 0 && !h->dev.dila,
                "the persistent stepper takes fault-only problems without dilatancy");
-    OQ_REQUIRE(st->alg == 0, "the persistent stepper runs Tsit5");
     OQ_REQUIRE(h->kdtype == OQ_K_FP64, "the persistent stepper takes an FP64 operator");
     int coop = 0, cus = 0, maxlds = 0, khz = 0;
     OQ_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, h->device));
@@ -947,12 +1161,15 @@ int oq_stepper_set_persistent(oq_stepper *st, int on)
     OQ_REQUIRE(rows >= 1, "the operator rows are too long for the persistent stepper (N > 16384)");
     const int R = (int)(rows < 64 ? rows : 64);
     const int nwg = (int)((h->N + R - 1) / R);
-    const size_t lds = ((size_t)R * h->ld + h->ld + (size_t)9 * 3 * R) * 8;
+    // per owned entry: Tsit5 keeps u, the stage state and seven stages; VCABM5
+    // keeps u, p, f(p), y_{n+1}, f(y_{n+1}) and Phi_0..Phi_5
+    const size_t per_entry = st->alg == 1 ? 11 : 9;
+    const size_t lds = ((size_t)R * h->ld + h->ld + per_entry * 3 * R) * 8;
     OQ_REQUIRE((int64_t)lds <= (int64_t)maxlds, "the persistent stepper does not fit in LDS");
-    OQ_HIP(hipFuncSetAttribute((const void *)k_persist, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)lds));
+    const void *kfn = st->alg == 1 ? (const void *)k_persist_ab : (const void *)k_persist;
+    OQ_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int per_cu = 0;
-    OQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_persist, 256, lds));
+    OQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, 256, lds));
     OQ_REQUIRE(per_cu >= 1 && (int64_t)per_cu * cus >= nwg,
                "the persistent stepper needs more co-resident workgroups than the device holds");
     OQ_REQUIRE(nwg <= 128, "the persistent stepper takes at most 128 workgroups (N <= 1024)");

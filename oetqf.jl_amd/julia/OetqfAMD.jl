@@ -232,15 +232,33 @@ function rhs!(f::GPURHS, du::ArrayPartition, u::ArrayPartition, t)
     return nothing
 end
 
-"""solve on the device (Tsit5 with OrdinaryDiffEq's controller); returns
-(t, u::ArrayPartition, retcode::Symbol)."""
-function solve_gpu(prob; reltol=1e-6, abstol=1e-8, dt=0.0, dtmax=0.0, dtmin=0.0, maxiters=10^9)
+"""solve on the device; returns (t, u::ArrayPartition, retcode::Symbol).
+alg = :Tsit5 (OrdinaryDiffEq's controller) or :VCABM5 (Adams PECE, order <= 5,
+the solver of examples/otf-with-mantle.jl:160).  persistent = true runs
+whole chunks of steps in one cooperative launch (fault-only, N <= 1024)."""
+function solve_gpu(prob; alg=:Tsit5, persistent=false, reltol=1e-6, abstol=1e-8, dt=0.0, dtmax=0.0,
+                   dtmin=0.0, maxiters=10^9)
     f = prob.p[end]::GPURHS
     u0 = collect(Float64, prob.u0)                     # packed ArrayPartition order
     s = Ref{Ptr{Cvoid}}(C_NULL)
-    check(ccall((:oq_stepper_create, libpath), Cint,
-        (Ptr{Cvoid}, Ptr{Float64}, Float64, Float64, Ref{CSolverOpts}, Ref{Ptr{Cvoid}}),
-        f.h, u0, prob.tspan[1], prob.tspan[2], Ref(CSolverOpts(reltol, abstol, dt, dtmax, dtmin, maxiters)), s))
+    alg in (:Tsit5, :VCABM5) || throw(ArgumentError("alg must be :Tsit5 or :VCABM5"))
+    if alg == :VCABM5
+        check(ccall((:oq_stepper_create_vcabm5, libpath), Cint,
+            (Ptr{Cvoid}, Ptr{Float64}, Float64, Float64, Ref{CSolverOpts}, Ref{Ptr{Cvoid}}),
+            f.h, u0, prob.tspan[1], prob.tspan[2], Ref(CSolverOpts(reltol, abstol, dt, dtmax, dtmin, maxiters)), s))
+    else
+        check(ccall((:oq_stepper_create, libpath), Cint,
+            (Ptr{Cvoid}, Ptr{Float64}, Float64, Float64, Ref{CSolverOpts}, Ref{Ptr{Cvoid}}),
+            f.h, u0, prob.tspan[1], prob.tspan[2], Ref(CSolverOpts(reltol, abstol, dt, dtmax, dtmin, maxiters)), s))
+    end
+    if persistent
+        st = ccall((:oq_stepper_set_persistent, libpath), Cint, (Ptr{Cvoid}, Cint), s[], 1)
+        if st != 0
+            msg = lasterror()
+            ccall((:oq_stepper_destroy, libpath), Cint, (Ptr{Cvoid},), s[])
+            throw(st == 1 ? ArgumentError(msg) : ErrorException(msg))
+        end
+    end
     rc = Ref{Cint}(0)
     while rc[] == 0
         check(ccall((:oq_stepper_run, libpath), Cint, (Ptr{Cvoid}, Int64, Ref{Cint}), s[], 4096, rc))

@@ -1,7 +1,8 @@
 """The persistent stepper (one cooperative launch per chunk of steps, rows
 of K resident in LDS, one grid barrier per RHS) against the graph stepper:
-same controller and tableau, so the same accepted steps and the same state
-up to FP64 summation order; the recorder gives the same catalogue."""
+same controller and method (Tsit5, or VCABM5 with its divided differences
+kept in LDS), so the same accepted steps and the same state up to FP64
+summation order; the recorder gives the same catalogue."""
 import numpy as np
 import pytest
 
@@ -13,8 +14,8 @@ from oetqf_amd.workloads import YEAR, strike_slip_512
 pytestmark = pytest.mark.gpu
 
 
-def _run(prob, persistent, chunk):
-    it = oq.Integrator(prob, reltol=1e-6, abstol=1e-8)
+def _run(prob, persistent, chunk, alg=None):
+    it = oq.Integrator(prob, reltol=1e-6, abstol=1e-8, alg=alg)
     it.record(stride=7, snap_capacity=chunk, step_capacity=chunk)
     if persistent:
         it.set_persistent(True)
@@ -30,16 +31,18 @@ def _run(prob, persistent, chunk):
     return ret, out, np.concatenate(ts), np.concatenate(snaps)
 
 
+@pytest.mark.parametrize("alg", ["tsit5", "vcabm5"])
 @pytest.mark.parametrize("dip", [90.0, 45.0])
-def test_persistent_matches_graph_small(oracle, rng, dip):
+def test_persistent_matches_graph_small(oracle, rng, dip, alg):
     mf = small_fault(dip)
     st = oracle.gf_ff(mf, 3e10, 3e10, buffer_ratio=1.0)
     pf = rs_props(mf, rng)
     v = pf.vpl * np.exp(0.3 * rng.standard_normal((mf.nx, mf.nxi)))
     u0 = oq.ArrayPartition(v, pf.L / v, np.zeros_like(v))
     prob = oq.assemble(st, pf, u0, (0.0, 0.5 * YEAR))
-    rg, (tg, _, ug, nag, nrg), stg, sng = _run(prob, False, 300)
-    rp, (tp, _, up, nap, nrp), stp, snp = _run(prob, True, 300)
+    a = oq.VCABM5() if alg == "vcabm5" else None
+    rg, (tg, _, ug, nag, nrg), stg, sng = _run(prob, False, 300, a)
+    rp, (tp, _, up, nap, nrp), stp, snp = _run(prob, True, 300, a)
     assert rg == rp == "Success" and tg == tp
     assert abs(nag - nap) <= 1 and abs(nrg - nrp) <= 1
     assert relerr(up.pack(), ug.pack()) <= 1e-8
@@ -48,16 +51,18 @@ def test_persistent_matches_graph_small(oracle, rng, dip):
     assert sng.shape[1] == snp.shape[1] and abs(len(sng) - len(snp)) <= 1
 
 
-def test_persistent_cycle_512(oq):
+@pytest.mark.parametrize("alg", ["tsit5", "vcabm5"])
+def test_persistent_cycle_512(oq, alg):
     """Two earthquakes of the 512-patch fault (70 years): the persistent run
     gives the graph run's catalogue."""
     w = strike_slip_512()
     prob = oq.assemble_from_mesh(w["mf"], None, w["lam"], w["mu"], w["pf"], None, w["u0"],
                                  (0.0, 70 * YEAR))
-    rg = run_cycle(prob, persistent=False)
-    rp = run_cycle(prob, persistent=True)
-    print("graph %.2f s, persistent %.2f s, steps %d / %d" % (rg["wall"], rp["wall"], rg["naccept"],
-                                                             rp["naccept"]), flush=True)
+    a = oq.VCABM5() if alg == "vcabm5" else None
+    rg = run_cycle(prob, persistent=False, alg=a)
+    rp = run_cycle(prob, persistent=True, alg=a)
+    print("%s: graph %.2f s, persistent %.2f s, steps %d / %d" % (alg, rg["wall"], rp["wall"], rg["naccept"],
+                                                                 rp["naccept"]), flush=True)
     assert rp["retcode"] == "Success"
     eg = events(rg["t"], rg["vmax"], 1e-3, merge_gap=YEAR)
     ep = events(rp["t"], rp["vmax"], 1e-3, merge_gap=YEAR)
