@@ -45,6 +45,12 @@ extern "C" {
  * and is applied along strike directly (the reference applies it by FFT
  * convolution, equation.jl:44-61). */
 #define OQ_K_TOEPLITZ 3
+/* Ensembles only: each row of K as two FP16 halves (hi + lo, 22 significant
+ * bits) times a power-of-two row scale, multiplied on the FP16 matrix cores
+ * with the members' source vectors split the same way (three products per
+ * pair, FP32 chunk sums, FP64 totals).  Same bytes as FP32, FP32-class
+ * error, eight times the FP32 matrix rate. */
+#define OQ_K_FP16X2 4
 
 /* RectOkadaMesh (src/BEM/mesh.jl:5-21).  Only x, ax, y, z, axi, nx, nxi, dx,
  * dep and dip enter the Green's functions (GF.jl:38,49-53,150-160). */
@@ -270,7 +276,7 @@ int oq_stepper_set_persistent(oq_stepper *s, int on);
 typedef struct oq_ensemble oq_ensemble;
 
 /* members: nmembers oq_rs_props (per-member a, b, L, sigma arrays and
- * scalars); visc is shared.  kdtype: OQ_K_FP32 or OQ_K_BF16. */
+ * scalars); visc is shared.  kdtype: OQ_K_FP32, OQ_K_BF16 or OQ_K_FP16X2. */
 int oq_ensemble_create(int device, const oq_fault_mesh *mf, const oq_hex8_mesh *ma,
                        double lambda, double mu, int ftype, int64_t nrept,
                        double buffer_ratio, const oq_quadrature *quad,

@@ -81,6 +81,29 @@ __device__ __forceinline__ void two_sum(float &hi, float &lo, float x)
     lo += err;
 }
 
+// Epilogue of one (row, member) of Y = K S: update_fault! (equation.jl:
+// 237-245) with member m's parameters on fault rows, the stress rate on
+// mantle rows.
+__device__ __forceinline__ void ens_epilogue(const EnsDev &d, int64_t row, int64_t m, double y,
+                                             const double *__restrict__ u, double *__restrict__ du)
+{
+    const int64_t M = d.M;
+    if (row < d.nf) {
+        const int64_t ix = row * M + m;
+        const double v = u[(d.lay.v + row) * M + m], th = u[(d.lay.th + row) * M + m];
+        const double a = d.a[ix], b = d.b[ix], L = d.L[ix];
+        const double thp = th > 0.0 ? th : 0.0;
+        const double psi1 = exp((d.f0[m] + b * log(d.v0[m] * thp / L)) / a) / (2.0 * d.v0[m]);
+        const double psi2 = d.sigma[ix] * psi1 / hypot(1.0, v * psi1);
+        const double dth = 1.0 - v * th / L;
+        du[(d.lay.th + row) * M + m] = dth;
+        du[(d.lay.v + row) * M + m] = (y - b / th * v * psi2 * dth) / (a * psi2 + d.eta[m]);
+        du[(d.lay.del + row) * M + m] = v;
+    } else {
+        du[(d.lay.sig + row - d.nf) * M + m] = y;
+    }
+}
+
 // ---------------- GEMM + fused epilogue ----------------
 // Block: 256 threads = 4 wavefronts; tile 128 rows x 64 members; each
 // wavefront owns 32 rows x 64 members = two 32x32 MFMA tiles.  K-slices of
@@ -159,23 +182,317 @@ __global__ void __launch_bounds__(256) k_ens_gemm(EnsDev d, const void *__restri
             const int64_t row = rb + wave * 32 + (i / 4) * 8 + (lane >> 5) * 4 + (i % 4);
             if (row >= d.N) continue;
             const double y = t == 0 ? (double)hi0[i] + (double)lo0[i] : (double)hi1[i] + (double)lo1[i];
-            if (row < d.nf) {
-                // update_fault! (equation.jl:237-245), member m's parameters
-                const int64_t ix = row * M + m;
-                const double v = u[(d.lay.v + row) * M + m], th = u[(d.lay.th + row) * M + m];
-                const double a = d.a[ix], b = d.b[ix], L = d.L[ix];
-                const double thp = th > 0.0 ? th : 0.0;
-                const double psi1 = exp((d.f0[m] + b * log(d.v0[m] * thp / L)) / a) / (2.0 * d.v0[m]);
-                const double psi2 = d.sigma[ix] * psi1 / hypot(1.0, v * psi1);
-                const double dth = 1.0 - v * th / L;
-                du[(d.lay.th + row) * M + m] = dth;
-                du[(d.lay.v + row) * M + m] = (y - b / th * v * psi2 * dth) / (a * psi2 + d.eta[m]);
-                du[(d.lay.del + row) * M + m] = v;
-            } else {
-                du[(d.lay.sig + row - d.nf) * M + m] = y;
-            }
+            ens_epilogue(d, row, m, y, u, du);
         }
     }
+}
+
+// ---------------- split-FP16 GEMM (OQ_K_FP16X2) ----------------
+// FP32-class products on the FP16 matrix cores.  Each row r of K is kept as
+// K[r, j] = (Kh + 2^-11 Kl) * rsc[r] with Kh = fp16(x), Kl = fp16(2^11 (x -
+// Kh)), x = K sigma_r and sigma_r the power of two that brings max_j |K[r, j]|
+// into [2^14, 2^15): 22 significant bits per entry.  Kl carries the 2^11 so
+// that it stays a normal FP16 number wherever Kh is (a bare remainder would
+// fall into the subnormal range for entries below 2^-18 of the row maximum).
+// Each member's source vector is split the same way per chunk of kHCh
+// columns, with its own power-of-two scale per (chunk, member), so that
+// fault slip rates and mantle strain rates (many decades apart) each keep 22
+// bits.  Per chunk, Kh Sh accumulates in one FP32 accumulator and Kh Sl + Kl
+// Sh in another (the dropped Kl Sl is 2^-22 relative), on
+// v_mfma_f32_32x32x16_f16; the chunk sum (acc_h + 2^-11 acc_x) is scaled and
+// added into FP64.  The column space puts the mantle columns at the next
+// chunk boundary after the fault columns, so no chunk mixes the two.
+//
+// Operand layout (global and LDS): per row of K (per member of S), every 8
+// consecutive columns are stored as 8 hi halves then 8 lo halves (32 B).  In
+// one MFMA k-step of 16 columns, lane l supplies columns 8 (l / 32) + 0..7 of
+// row (member) l % 32 for both A and B, so a lane reads one 16-byte hi and
+// one 16-byte lo vector.
+typedef _Float16 h8x __attribute__((ext_vector_type(8)));
+
+static constexpr int kHCh = 512;               // columns per chunk
+static constexpr int kHBM = 64, kHBN = 256;    // block tile: rows x members
+static constexpr int kHKS = 32;                // columns per LDS stage
+static constexpr int kHRow = 144;              // LDS bytes per row/member (128 + 16 pad)
+static constexpr int kHStage = (kHBM + kHBN) * kHRow;
+static constexpr size_t kHLds = 2 * (size_t)kHStage;  // double-buffered
+
+struct EnsH3 {
+    int64_t nfp, ldc, nchunk;  // padded fault columns, columns, chunks
+    const _Float16 *K;         // N x 2 ldc halves
+    const double *rsc;         // N: K = (hi + lo) rsc
+    _Float16 *S;               // Mp x 2 ldc halves (member-major)
+    double *ssc;               // nchunk x Mp: S = (hi + lo) ssc
+    double *Y;                 // N x Mp
+};
+
+// column of the split layout -> column of K (or -1 for padding)
+__device__ __forceinline__ int64_t h3_col(const EnsDev &d, const EnsH3 &h, int64_t jc)
+{
+    if (jc < d.nf) return jc;
+    if (jc >= h.nfp && jc - h.nfp < d.N - d.nf) return d.nf + (jc - h.nfp);
+    return -1;
+}
+
+// power of two s with mx s in [2^14, 2^15) (1 for mx = 0)
+__device__ __forceinline__ double h3_scale(double mx)
+{
+    if (!(mx > 0.0)) return 1.0;
+    int e;
+    (void)frexp(mx, &e);  // mx in [2^(e-1), 2^e)
+    return ldexp(1.0, 15 - e);
+}
+
+__device__ __forceinline__ void h3_split8(const double *x, double sc, h8x &hi, h8x &lo)
+{
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const double v = x[e] * sc;
+        const _Float16 a = (_Float16)v;
+        hi[e] = a;
+        lo[e] = (_Float16)((v - (double)a) * 2048.0);
+    }
+}
+
+// one-time: FP64 operator (N x ld) -> split rows and row scales
+__global__ void __launch_bounds__(256) k_h3_split_k(EnsDev d, EnsH3 h, const double *__restrict__ K64,
+                                                    _Float16 *__restrict__ Kh, double *__restrict__ rsc)
+{
+    __shared__ double red[256];
+    for (int64_t row = blockIdx.x; row < d.N; row += gridDim.x) {
+        const double *kr = K64 + row * d.ld;
+        double mx = 0.0;
+        for (int64_t j = threadIdx.x; j < d.N; j += 256) mx = fmax(mx, fabs(kr[j]));
+        red[threadIdx.x] = mx;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+            __syncthreads();
+        }
+        const double sc = h3_scale(red[0]);
+        if (threadIdx.x == 0) rsc[row] = 1.0 / sc;
+        _Float16 *out = Kh + row * 2 * h.ldc;
+        for (int64_t g = threadIdx.x; g < h.ldc / 8; g += 256) {
+            double x[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int64_t j = h3_col(d, h, 8 * g + e);
+                x[e] = j >= 0 ? kr[j] : 0.0;
+            }
+            h8x hi, lo;
+            h3_split8(x, sc, hi, lo);
+            *reinterpret_cast<h8x *>(out + 16 * g) = hi;
+            *reinterpret_cast<h8x *>(out + 16 * g + 8) = lo;
+        }
+        __syncthreads();  // red is reused by the next row
+    }
+}
+
+// per RHS, step 1: the maximum |S| of each member over each 64-column
+// sub-block of the split column space.  Block: 64 columns x 64 members.
+__global__ void __launch_bounds__(256) k_h3_smax(EnsDev d, EnsH3 h, const float *__restrict__ S32,
+                                                 float *__restrict__ pmax)
+{
+    __shared__ float red[4][64];
+    const int tid = threadIdx.x, ml = tid & 63, q = tid >> 6;
+    const int64_t mb = (int64_t)blockIdx.y * 64, c0 = (int64_t)blockIdx.x * 64;
+    float mx = 0.f;
+#pragma unroll 4
+    for (int jj = q; jj < 64; jj += 4) {
+        const int64_t j = h3_col(d, h, c0 + jj);
+        if (j >= 0) mx = fmaxf(mx, fabsf(S32[j * d.Mp + mb + ml]));
+    }
+    red[q][ml] = mx;
+    __syncthreads();
+    if (q == 0)
+        pmax[blockIdx.x * d.Mp + mb + ml] =
+            fmaxf(fmaxf(red[0][ml], red[1][ml]), fmaxf(red[2][ml], red[3][ml]));
+}
+
+// per RHS, step 2: split member-major rows and per-(chunk, member) scales
+// (the scale of a chunk from the maxima of its kHCh / 64 sub-blocks).
+// Block: 64 columns x 64 members, written through an LDS transpose: four
+// threads per member write 128 contiguous bytes.
+__global__ void __launch_bounds__(256) k_h3_split_s(EnsDev d, EnsH3 h, const float *__restrict__ S32,
+                                                    const float *__restrict__ pmax)
+{
+    constexpr int kSub = kHCh / 64;
+    __shared__ float tile[64][65];
+    const int tid = threadIdx.x, ml = tid & 63, q = tid >> 6;
+    const int64_t mb = (int64_t)blockIdx.y * 64, cb = (int64_t)blockIdx.x * 64;
+    const int64_t chunk = blockIdx.x / kSub;
+    for (int jj = q; jj < 64; jj += 4) {
+        const int64_t j = h3_col(d, h, cb + jj);
+        tile[jj][ml] = j >= 0 ? S32[j * d.Mp + mb + ml] : 0.f;
+    }
+    const int mm = tid >> 2, g2 = tid & 3;  // output member and group lane
+    float mx = 0.f;
+#pragma unroll
+    for (int k = 0; k < kSub; ++k) mx = fmaxf(mx, pmax[(chunk * kSub + k) * d.Mp + mb + mm]);
+    const double sc = h3_scale((double)mx);
+    if (g2 == 0 && blockIdx.x % kSub == 0) h.ssc[chunk * d.Mp + mb + mm] = 1.0 / sc;
+    __syncthreads();
+    _Float16 *out = h.S + (mb + mm) * 2 * h.ldc + 2 * cb;
+#pragma unroll
+    for (int gg = g2; gg < 8; gg += 4) {
+        double x[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) x[e] = (double)tile[8 * gg + e][mm];
+        h8x hi, lo;
+        h3_split8(x, sc, hi, lo);
+        *reinterpret_cast<h8x *>(out + 16 * gg) = hi;
+        *reinterpret_cast<h8x *>(out + 16 * gg + 8) = lo;
+    }
+}
+
+// Y = K S for a 64-row x 256-member block tile, 8 wavefronts.  Wave w owns
+// rows [32 (w % 2), +32) x members [64 (w / 2), +64): two 32 x 32 MFMA
+// tiles.  Stages of 32 columns of K (64 rows) and S (256 members) go through
+// double-buffered LDS; global loads run two stages ahead in registers.
+// (Measured alternatives: 4 waves with 2 x 2 tiles spilled into AGPR
+// copies; 128-member blocks two per CU streamed K twice and ran 20 %
+// slower.)
+__global__ void __launch_bounds__(512) k_ens_gemm_h3(EnsDev d, EnsH3 h)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char hlds[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave & 1, wm = wave >> 1;
+    const int64_t rb = (int64_t)blockIdx.x * kHBM, m0 = (int64_t)blockIdx.y * kHBN;
+    const int64_t rowb = 4 * h.ldc;  // bytes per split row
+    // global -> LDS pieces of 16 B: K 64 rows x 8 (one per thread), S 256
+    // members x 8 (four per thread)
+    const unsigned char *kg, *sg[4];
+    int kl, sl[4];
+    {
+        const int r = tid >> 3, pc = tid & 7;
+        const int64_t row = rb + r < d.N ? rb + r : d.N - 1;
+        kg = reinterpret_cast<const unsigned char *>(h.K) + row * rowb + 16 * pc;
+        kl = r * kHRow + 16 * pc;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int p = tid + 512 * i, mm = p >> 3, qc = p & 7;
+            sg[i] = reinterpret_cast<const unsigned char *>(h.S) + (m0 + mm) * rowb + 16 * qc;
+            sl[i] = (kHBM + mm) * kHRow + 16 * qc;
+        }
+    }
+    u4x pa[5], pb[5];
+    auto gload = [&](u4x *pr, int64_t k0) {
+        pr[0] = __builtin_nontemporal_load(reinterpret_cast<const u4x *>(kg + 4 * k0));
+#pragma unroll
+        for (int i = 0; i < 4; ++i) pr[1 + i] = *reinterpret_cast<const u4x *>(sg[i] + 4 * k0);
+    };
+    auto lstore = [&](const u4x *pr, unsigned char *st) {
+        *reinterpret_cast<u4x *>(st + kl) = pr[0];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) *reinterpret_cast<u4x *>(st + sl[i]) = pr[1 + i];
+    };
+    f16v acc[2], acx[2];
+    double y[2][16];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+        acc[b] = f16v{0};
+        acx[b] = f16v{0};
+#pragma unroll
+        for (int i = 0; i < 16; ++i) y[b][i] = 0.0;
+    }
+    const int lr = lane & 31, lg = lane >> 5;
+    auto compute = [&](const unsigned char *cur) {
+        // both k-steps' fragments first: the second step's LDS latency hides
+        // behind the first step's MFMAs
+        h8x ah[2], al[2], bh[2][2], bl[2][2];
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            const int go = (2 * ks + lg) * 32;
+            const unsigned char *pA = cur + (wr * 32 + lr) * kHRow + go;
+            ah[ks] = *reinterpret_cast<const h8x *>(pA);
+            al[ks] = *reinterpret_cast<const h8x *>(pA + 16);
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const unsigned char *pB = cur + (kHBM + wm * 64 + b * 32 + lr) * kHRow + go;
+                bh[ks][b] = *reinterpret_cast<const h8x *>(pB);
+                bl[ks][b] = *reinterpret_cast<const h8x *>(pB + 16);
+            }
+        }
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+                acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[ks], bh[ks][b], acc[b], 0, 0, 0);
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+                acx[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[ks], bl[ks][b], acx[b], 0, 0, 0);
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+                acx[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[ks], bh[ks][b], acx[b], 0, 0, 0);
+        }
+    };
+    // chunk scales of this lane's two members, loaded one chunk ahead so the
+    // fold does not wait on the stage prefetches
+    const double *scp = h.ssc + m0 + wm * 64 + lr;
+    double inv[2] = {scp[0], scp[32]};
+    auto fold = [&](int64_t st) {  // chunk end: scale by the member's chunk scale, add into FP64
+        if (((st + 1) * kHKS) % kHCh != 0) return;
+        const int64_t cn = (st + 1) * kHKS / kHCh;  // next chunk
+        const int64_t cl = cn < h.nchunk ? cn : h.nchunk - 1;
+        const double nx0 = scp[cl * d.Mp], nx1 = scp[cl * d.Mp + 32];
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+                y[b][i] += ((double)acc[b][i] + (double)acx[b][i] * (1.0 / 2048.0)) * inv[b];
+            acc[b] = f16v{0};
+            acx[b] = f16v{0};
+        }
+        inv[0] = nx0;
+        inv[1] = nx1;
+    };
+    // Pipeline: at the top of stage st, registers hold stage st + 1 (set
+    // (st + 1) % 2) and stage st + 2 is issued into the other set; stage
+    // st + 1 goes to LDS after stage st's MFMAs.  nst is even (ldc is a
+    // multiple of kHCh), so the loop runs in pairs with static register sets.
+    // The body has no data-dependent branches (past the end, the last stage
+    // is loaded again and stored into the buffer nobody reads), so the
+    // compiler's wait counts keep both prefetched stages in flight.
+    const int64_t nst = h.ldc / kHKS;
+    auto kof = [&](int64_t st) { return (st < nst ? st : nst - 1) * kHKS; };
+    gload(pa, 0);
+    gload(pb, kof(1));
+    lstore(pa, hlds);
+    __syncthreads();
+    for (int64_t st = 0; st < nst; st += 2) {
+        // even stage st: pb holds st + 1, pa is free
+        gload(pa, kof(st + 2));
+        compute(hlds);
+        lstore(pb, hlds + kHStage);
+        fold(st);
+        __syncthreads();
+        // odd stage st + 1: pa holds st + 2, pb is free
+        gload(pb, kof(st + 3));
+        compute(hlds + kHStage);
+        lstore(pa, hlds);
+        fold(st + 1);
+        __syncthreads();
+    }
+    // element i of tile b: row 32 wr + (i / 4) 8 + (lane / 32) 4 + i % 4,
+    // member 64 wm + 32 b + lane % 32
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int64_t row = rb + wr * 32 + (i / 4) * 8 + lg * 4 + (i % 4);
+        if (row >= d.N) continue;
+        const double rs = h.rsc[row];
+#pragma unroll
+        for (int b = 0; b < 2; ++b) h.Y[row * d.Mp + m0 + wm * 64 + b * 32 + lr] = y[b][i] * rs;
+    }
+}
+
+// epilogue of the split-FP16 GEMM: one thread per (row, member)
+__global__ void __launch_bounds__(256) k_ens_finish(EnsDev d, const double *__restrict__ Y,
+                                                    const double *__restrict__ u, double *__restrict__ du)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= d.N * d.M) return;
+    const int64_t row = t / d.M, m = t % d.M;
+    ens_epilogue(d, row, m, Y[row * d.Mp + m], u, du);
 }
 
 // ---------------- batched Tsit5 ----------------
@@ -327,7 +644,10 @@ __global__ void __launch_bounds__(256) k_ens_commit(EnsCtrl c, int64_t nM, int64
 
 struct oq_ensemble {
     oq_rhs *h = nullptr;  // owns K (FP32/BF16), stream, sizes, shared viscosity
+    int kdtype = OQ_K_FP32;
     int64_t M = 0, Mp = 0, n = 0;
+    oq::DevBuf hK, hrsc, hS, hssc, hY, hpmax;  // OQ_K_FP16X2: split operator, split S, scales, Y
+    oq::EnsH3 h3{};
     oq::DevBuf a, b, L, sigma, eta, vpl, f0, v0, S;
     oq::DevBuf u, y, k, part;
     oq::DevBuf ct, cdt, cdtcur, cqold, cna, cnr, citer, cdone, cacc, cret;
@@ -351,6 +671,19 @@ static int ens_rhs(oq_ensemble *e, const double *u, double *du)
     const int64_t rows = (h->nf + h->ne) * e->M;
     hipLaunchKernelGGL(k_ens_prologue, dim3(nbk(rows, 256)), dim3(256), 0, h->stream, e->dev, u, du,
                        e->S.as<float>());
+    if (e->kdtype == OQ_K_FP16X2) {
+        const dim3 gs((unsigned)(e->h3.ldc / 64), (unsigned)(e->Mp / 64));
+        hipLaunchKernelGGL(k_h3_smax, gs, dim3(256), 0, h->stream, e->dev, e->h3, e->S.as<float>(),
+                           e->hpmax.as<float>());
+        hipLaunchKernelGGL(k_h3_split_s, gs, dim3(256), 0, h->stream, e->dev, e->h3, e->S.as<float>(),
+                           e->hpmax.as<float>());
+        hipLaunchKernelGGL(k_ens_gemm_h3, dim3(nbk(h->N, kHBM), (unsigned)(e->Mp / kHBN)), dim3(512), kHLds,
+                           h->stream, e->dev, e->h3);
+        hipLaunchKernelGGL(k_ens_finish, dim3(nbk(h->N * e->M, 256)), dim3(256), 0, h->stream, e->dev,
+                           e->h3.Y, u, du);
+        OQ_HIP(hipGetLastError());
+        return OQ_OK;
+    }
     const dim3 grid(nbk(h->N, EBM), (unsigned)(e->Mp / EBN));
     if (h->kdtype == OQ_K_FP32)
         hipLaunchKernelGGL(k_ens_gemm<OQ_K_FP32>, grid, dim3(256), 0, h->stream, e->dev, h->K.p,
@@ -394,6 +727,39 @@ static void ens_destroy(oq_ensemble *e)
     delete e;
 }
 
+// OQ_K_FP16X2: split the FP64 operator, allocate the per-RHS buffers and
+// drop the FP64 copy.
+static int build_h3(oq_ensemble *e)
+{
+    oq_rhs *h = e->h;
+    EnsH3 &s = e->h3;
+    const int64_t nm = 6 * h->ne;
+    s.nfp = h->ne > 0 ? (h->nf + kHCh - 1) / kHCh * kHCh : h->nf;
+    s.ldc = (s.nfp + nm + kHCh - 1) / kHCh * kHCh;
+    s.nchunk = s.ldc / kHCh;
+    int rc = e->hK.alloc((size_t)(h->N * s.ldc) * 4);
+    if (!rc) rc = e->hrsc.alloc((size_t)h->N * 8);
+    if (!rc) rc = e->hS.alloc((size_t)(e->Mp * s.ldc) * 4);
+    if (!rc) rc = e->hssc.alloc((size_t)(s.nchunk * e->Mp) * 8);
+    if (!rc) rc = e->hY.alloc((size_t)(h->N * e->Mp) * 8);
+    if (!rc) rc = e->hpmax.alloc((size_t)(s.ldc / 64 * e->Mp) * 4);
+    if (rc) return rc;
+    s.K = e->hK.as<_Float16>();
+    s.rsc = e->hrsc.as<double>();
+    s.S = e->hS.as<_Float16>();
+    s.ssc = e->hssc.as<double>();
+    s.Y = e->hY.as<double>();
+    const unsigned g = (unsigned)(h->N < 65536 ? h->N : 65536);
+    hipLaunchKernelGGL(k_h3_split_k, dim3(g), dim3(256), 0, h->stream, e->dev, s, h->K.as<double>(),
+                       e->hK.as<_Float16>(), e->hrsc.as<double>());
+    OQ_HIP(hipGetLastError());
+    OQ_HIP(hipStreamSynchronize(h->stream));
+    h->K.release();
+    OQ_HIP(hipFuncSetAttribute((const void *)k_ens_gemm_h3, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)kHLds));
+    return OQ_OK;
+}
+
 // entry-major copy of M member arrays (each length n): out[i*M + m]
 static std::vector<double> interleave(const double *const *src, int64_t n, int64_t M)
 {
@@ -417,7 +783,8 @@ int oq_ensemble_create(int device, const oq_fault_mesh *mf, const oq_hex8_mesh *
     OQ_REQUIRE(out != nullptr && members != nullptr, "NULL argument");
     *out = nullptr;
     OQ_REQUIRE(nmembers >= 1, "nmembers must be >= 1");
-    OQ_REQUIRE(kdtype == OQ_K_FP32 || kdtype == OQ_K_BF16, "ensemble operator is FP32 or BF16");
+    OQ_REQUIRE(kdtype == OQ_K_FP32 || kdtype == OQ_K_BF16 || kdtype == OQ_K_FP16X2,
+               "ensemble operator is FP32, BF16 or FP16X2");
     for (int64_t m = 0; m < nmembers; ++m) {
         const oq_rs_props &p = members[m];
         OQ_REQUIRE(p.a && p.b && p.L && p.sigma, "member property array is NULL");
@@ -425,8 +792,11 @@ int oq_ensemble_create(int device, const oq_fault_mesh *mf, const oq_hex8_mesh *
                    "RateStateQuasiDynamicProperty requires f₀ > 0, v₀ > 0, η > 0, vpl > 0");
     }
     oq_ensemble *e = new oq_ensemble();
+    e->kdtype = kdtype;
+    // the split operator is made from the FP64 assembly
     int rc = oq_rhs_create_from_mesh(device, mf, ma, lambda, mu, ftype, nrept, buffer_ratio, quad,
-                                     &members[0], visc, kdtype, &e->h);
+                                     &members[0], visc, kdtype == OQ_K_FP16X2 ? OQ_K_FP64 : kdtype,
+                                     &e->h);
     if (rc) {
         ens_destroy(e);
         return rc;
@@ -434,7 +804,8 @@ int oq_ensemble_create(int device, const oq_fault_mesh *mf, const oq_hex8_mesh *
     oq_rhs *h = e->h;
     const int64_t M = nmembers, nf = h->nf;
     e->M = M;
-    e->Mp = (M + EBN - 1) / EBN * EBN;
+    const int64_t mb = kdtype == OQ_K_FP16X2 ? kHBN : EBN;
+    e->Mp = (M + mb - 1) / mb * mb;
     e->n = h->lay.n;
     std::vector<const double *> pa(M), pb(M), pL(M), ps(M);
     std::vector<double> eta(M), vpl(M), f0(M), v0(M);
@@ -482,6 +853,10 @@ int oq_ensemble_create(int device, const oq_fault_mesh *mf, const oq_hex8_mesh *
     c.iter = e->citer.as<long long>(); c.done = e->cdone.as<int>(); c.accept = e->cacc.as<int>();
     c.retcode = e->cret.as<int>(); c.M = (int)M;
     c.nf = nf; c.rec_cap = 0; c.vpart = c.rec_t = c.rec_v = nullptr;
+    if (kdtype == OQ_K_FP16X2 && (rc = build_h3(e))) {
+        ens_destroy(e);
+        return rc;
+    }
     *out = e;
     return OQ_OK;
 }

@@ -14,11 +14,14 @@ from .equation import _KDTYPES, _rs_struct, _visc_struct
 from .greens import StrikeSlip, _Keep, _ftype, fault_struct, get_quadrature, hex8_struct, quad_struct
 
 
+_ENS_KDTYPES = dict(_KDTYPES, fp16x2=_lib.OQ_K_FP16X2)
+
+
 class Ensemble:
     def __init__(self, mf, ma, lam, mu, members, pa, ftype=StrikeSlip(), nrept=2, buffer_ratio=0.0,
                  qtype="Gauss1", kdtype="fp32", device=0):
-        if kdtype not in ("fp32", "bf16"):
-            raise ValueError("the ensemble operator is fp32 or bf16")
+        if kdtype not in ("fp32", "bf16", "fp16x2"):
+            raise ValueError("the ensemble operator is fp32, bf16 or fp16x2")
         keep = _Keep()
         self._keep = keep
         M = len(members)
@@ -33,13 +36,13 @@ class Ensemble:
             vs = _visc_struct(pa, ma.ne, keep)
             _lib.check(L.oq_ensemble_create(device, ctypes.byref(a), ctypes.byref(b), lam, mu,
                                             _ftype(ftype), nrept, buffer_ratio, ctypes.byref(q), M,
-                                            arr, ctypes.byref(vs), _KDTYPES[kdtype],
+                                            arr, ctypes.byref(vs), _ENS_KDTYPES[kdtype],
                                             ctypes.byref(self.h)))
             self.ne = ma.ne
         else:
             _lib.check(L.oq_ensemble_create(device, ctypes.byref(a), None, lam, mu, _ftype(ftype),
                                             nrept, buffer_ratio, None, M, arr, None,
-                                            _KDTYPES[kdtype], ctypes.byref(self.h)))
+                                            _ENS_KDTYPES[kdtype], ctypes.byref(self.h)))
             self.ne = 0
         self.M = M
         self._rec_cap = 0

@@ -1,5 +1,6 @@
-"""Ensemble RHS (one MFMA GEMM over a shared FP32/BF16 operator) and the
-lockstep ensemble Tsit5, against the FP64 oracle member by member."""
+"""Ensemble RHS (one MFMA GEMM over a shared FP32/BF16 operator, or the
+split-FP16 operator on the FP16 matrix cores) and the lockstep ensemble
+Tsit5, against the FP64 oracle member by member."""
 import numpy as np
 import pytest
 
@@ -28,11 +29,13 @@ def _state(mf, ma, p, rng):
                              1e6 * rng.standard_normal((ma.ne, 6)), rng.random((mf.nx, mf.nxi)))
 
 
-@pytest.mark.parametrize("kdtype,tol", [("fp32", 1e-5), ("bf16", 2e-2)])
-def test_ensemble_rhs_matches_fp64_oracle(oq, oracle, rng, kdtype, tol):
+@pytest.mark.parametrize("kdtype,tol,M", [("fp32", 1e-5, 70), ("bf16", 2e-2, 70), ("fp16x2", 1e-5, 70),
+                                          ("fp16x2", 1e-5, 260)])
+def test_ensemble_rhs_matches_fp64_oracle(oq, oracle, rng, kdtype, tol, M):
+    """M = 70 spans two member tiles of 64 (FP32/BF16); M = 260 two of 256
+    (split FP16)."""
     mf, ma = small_fault(90.0), small_mantle()
     lam = mu = 3e10
-    M = 70  # spans two member tiles of 64
     members = _members(mf, rng, M)
     pa = visc_props(ma, rng, 0.0)
     states = [_state(mf, ma, p, rng) for p in members]
@@ -43,7 +46,7 @@ def test_ensemble_rhs_matches_fp64_oracle(oq, oracle, rng, kdtype, tol):
     g21 = oracle.gf_fm(ma, mf, lam, mu)
     g22 = oracle.gf_mm(ma, lam, mu)
     worst = 0.0
-    for m in (0, 1, 33, 64, 69):
+    for m in sorted({0, 1, 33, 64, 69, M - 5, M - 1}):
         s = states[m]
         ref = oracle.rhs_coupled(members[m], pa, st, g12, g21, g22, s.x[0], s.x[1], s.x[3])
         got = s.similar().unpack(dU[m])
@@ -54,7 +57,8 @@ def test_ensemble_rhs_matches_fp64_oracle(oq, oracle, rng, kdtype, tol):
     print("%s ensemble RHS worst rel err vs FP64 oracle: %.2e" % (kdtype, worst))
 
 
-def test_ensemble_stepper_matches_single_member(oq, rng):
+@pytest.mark.parametrize("kdtype", ["fp32", "fp16x2"])
+def test_ensemble_stepper_matches_single_member(oq, rng, kdtype):
     mf, ma = small_fault(90.0), small_mantle()
     members = _members(mf, rng, 3)
     pa = visc_props(ma, rng, 0.0)
@@ -64,7 +68,7 @@ def test_ensemble_stepper_matches_single_member(oq, rng):
         U0.append(oq.ArrayPartition(v, p.L / v, np.zeros((ma.ne, 6)), 1e5 * np.ones((ma.ne, 6)),
                                     np.zeros((mf.nx, mf.nxi))))
     T = 0.2 * 365 * 86400.0
-    ens = oq.Ensemble(mf, ma, 3e10, 3e10, members, pa, buffer_ratio=1.0, kdtype="fp32")
+    ens = oq.Ensemble(mf, ma, 3e10, 3e10, members, pa, buffer_ratio=1.0, kdtype=kdtype)
     ens.start(np.stack([u.pack() for u in U0]), (0.0, T), reltol=1e-6, abstol=1e-8, dt0=1.0)
     while ens.run(256) > 0:
         pass
