@@ -227,17 +227,28 @@ void oq_rhs_fault(const oq_rs_t *p, const double *st, int64_t nx, int64_t nxi,
     free(dtau);
 }
 
-/* y += A x with A (m x n) column-major, OpenMP over rows. */
+/* y += A x with A (m x n) column-major.  OpenMP over blocks of 512 rows;
+ * each block sweeps the columns in order, reading 4 KB contiguous per
+ * column (the access pattern of a BLAS column-major dgemv), with one
+ * accumulator per row: row i's sum is formed in column order and then
+ * added to y[i], as a row-by-row loop would form it. */
 static void gemv_colmajor(const double *A, int64_t m, int64_t n,
                           const double *x, double *y)
 {
-    int64_t i;
-#pragma omp parallel for
-    for (i = 0; i < m; ++i) {
-        double acc = 0.0;
-        int64_t j;
-        for (j = 0; j < n; ++j) acc += A[i + m * j] * x[j];
-        y[i] += acc;
+    enum { RB = 512 };
+    int64_t b;
+#pragma omp parallel for schedule(static)
+    for (b = 0; b < m; b += RB) {
+        double acc[RB];
+        const int64_t ib = m - b < RB ? m - b : RB;
+        int64_t i, j;
+        for (i = 0; i < ib; ++i) acc[i] = 0.0;
+        for (j = 0; j < n; ++j) {
+            const double *col = A + b + m * j;
+            const double xj = x[j];
+            for (i = 0; i < ib; ++i) acc[i] += col[i] * xj;
+        }
+        for (i = 0; i < ib; ++i) y[b + i] += acc[i];
     }
 }
 
