@@ -51,6 +51,12 @@ extern "C" {
  * pair, FP32 chunk sums, FP64 totals).  Same bytes as FP32, FP32-class
  * error, eight times the FP32 matrix rate. */
 #define OQ_K_FP16X2 4
+/* Fault-only problems (ne = 0): K_ff applied as the reference applies it,
+ * by DFT convolution along strike (equation.jl:44-61): transform of the
+ * zero-padded source (length 2nx-1), per-frequency contraction with the
+ * DFT of the symmetric circulant of st (GF.jl:60-67, real), inverse
+ * transform.  st stays resident as well (operator expansion). */
+#define OQ_K_FFT 5
 
 /* RectOkadaMesh (src/BEM/mesh.jl:5-21).  Only x, ax, y, z, axi, nx, nxi, dx,
  * dep and dip enter the Green's functions (GF.jl:38,49-53,150-160). */

@@ -326,6 +326,143 @@ __global__ void __launch_bounds__(256) k_toep_finish(RhsDev d, const double *__r
     fault_update(d, q, tau, u, du);
 }
 
+// ---------------- DFT K_ff (fault-only problems, OQ_K_FFT) ----------------
+// The reference's own algorithm (equation.jl:44-61): the source s(:, l) is
+// zero-padded to length M = 2nx - 1 and transformed along strike, each
+// frequency f < nx is contracted over the dip rows with the DFT of the
+// circulant [st; reverse(st[2:end])] (GF.jl:60-67), and the inverse real DFT
+// gives the first nx rows.  That circulant is symmetric (x_n = x_{M-n}), so
+// its DFT is real: G[f, j, l] = sum_{n<M} x_n cos(2 pi f n / M).  The
+// transforms are direct sums over an exact twiddle index (f n mod M) read
+// from a table: O(nx^2 nxi) work per transform against the O(nx^2 nxi^2) of
+// the direct Toeplitz product, and each launch is a few microseconds.
+// Layouts: G[f][l][j] (j fastest), S^[f][l] and T^[j][f] complex
+// (interleaved re, im); the twiddle table holds cos then sin, M each.
+
+// one-time: G from st (grid-stride over nx * nxi^2 entries)
+__global__ void __launch_bounds__(256) k_fft_kernel(const double *__restrict__ st, int64_t nx,
+                                                    int64_t nxi, const double *__restrict__ tw,
+                                                    double *__restrict__ G)
+{
+    const int64_t M = 2 * nx - 1, nf = nx * nxi, tot = nx * nxi * nxi;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < tot;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = t % nxi, l = (t / nxi) % nxi, f = t / (nxi * nxi);
+        const double *col = st + nx * j + nf * l;
+        double acc = 0.0;
+        int64_t k = 0;
+        for (int64_t n = 0; n < M; ++n) {
+            acc += (n < nx ? col[n] : col[M - n]) * tw[k];
+            k += f;
+            if (k >= M) k -= M;
+        }
+        G[t] = acc;
+    }
+}
+
+// forward DFT of s(:, l): block (l, 64 frequencies), four interleaved
+// partial sums over n per frequency
+__global__ void __launch_bounds__(256) k_fft_fwd(const double *__restrict__ s, int64_t nx, int64_t nxi,
+                                                 const double *__restrict__ tw, double *__restrict__ Sh)
+{
+    extern __shared__ double fsm[];
+    const int64_t l = blockIdx.x, M = 2 * nx - 1;
+    double *sc = fsm, *cs = sc + nx, *sn = cs + M, *red = sn + M;  // red: 2 x 256
+    const int tid = threadIdx.x;
+    for (int64_t x = tid; x < nx; x += 256) sc[x] = s[x + nx * l];
+    for (int64_t x = tid; x < 2 * M; x += 256) cs[x] = tw[x];  // cs then sn
+    __syncthreads();
+    const int fi = tid & 63, q = tid >> 6;
+    const int64_t f = (int64_t)blockIdx.y * 64 + fi;
+    double re = 0.0, im = 0.0;
+    if (f < nx) {
+        int64_t k = (f * q) % M;
+        const int64_t dk = (4 * f) % M;
+        for (int64_t n = q; n < nx; n += 4) {
+            re += sc[n] * cs[k];
+            im -= sc[n] * sn[k];
+            k += dk;
+            if (k >= M) k -= M;
+        }
+    }
+    red[tid] = re;
+    red[256 + tid] = im;
+    __syncthreads();
+    if (q == 0 && f < nx) {
+        Sh[2 * (l + nxi * f)] = (red[fi] + red[64 + fi]) + (red[128 + fi] + red[192 + fi]);
+        Sh[2 * (l + nxi * f) + 1] =
+            (red[256 + fi] + red[320 + fi]) + (red[384 + fi] + red[448 + fi]);
+    }
+}
+
+// per-frequency contraction T^[j][f] = sum_l G[f][l][j] S^[f][l]: block f
+__global__ void __launch_bounds__(256) k_fft_mid(const double *__restrict__ G,
+                                                 const double *__restrict__ Sh, int64_t nx,
+                                                 int64_t nxi, double *__restrict__ Th)
+{
+    __shared__ double red[2][256];
+    const int64_t f = blockIdx.x;
+    const int tid = threadIdx.x, ji = tid & 63, q = tid >> 6;
+    const double *Gf = G + nxi * nxi * f;
+    const double *Sf = Sh + 2 * nxi * f;
+    for (int64_t jb = 0; jb < nxi; jb += 64) {
+        const int64_t j = jb + ji;
+        double re = 0.0, im = 0.0;
+        if (j < nxi) {
+            for (int64_t l = q; l < nxi; l += 4) {
+                const double g = Gf[j + nxi * l];
+                re += g * Sf[2 * l];
+                im += g * Sf[2 * l + 1];
+            }
+        }
+        red[0][tid] = re;
+        red[1][tid] = im;
+        __syncthreads();
+        if (q == 0 && j < nxi) {
+            Th[2 * (f + nx * j)] = (red[0][ji] + red[0][64 + ji]) + (red[0][128 + ji] + red[0][192 + ji]);
+            Th[2 * (f + nx * j) + 1] =
+                (red[1][ji] + red[1][64 + ji]) + (red[1][128 + ji] + red[1][192 + ji]);
+        }
+        __syncthreads();
+    }
+}
+
+// inverse real DFT of T^[j][:], first nx rows, and the rate-and-state
+// update: block (j, 64 rows), four interleaved partial sums over f
+__global__ void __launch_bounds__(256) k_fft_inv(RhsDev d, const double *__restrict__ Th, int64_t nx,
+                                                 const double *__restrict__ tw,
+                                                 const double *__restrict__ u, double *__restrict__ du)
+{
+    extern __shared__ double fsm[];
+    const int64_t j = blockIdx.x, M = 2 * nx - 1;
+    double *tr = fsm, *cs = tr + 2 * nx, *sn = cs + M, *red = sn + M;  // red: 256
+    const int tid = threadIdx.x;
+    for (int64_t x = tid; x < 2 * nx; x += 256) tr[x] = Th[2 * nx * j + x];
+    for (int64_t x = tid; x < 2 * M; x += 256) cs[x] = tw[x];
+    __syncthreads();
+    const int ni = tid & 63, q = tid >> 6;
+    const int64_t n = (int64_t)blockIdx.y * 64 + ni;
+    double acc = 0.0;
+    if (n < nx) {
+        int64_t f = 1 + q, k = ((1 + q) * n) % M;
+        const int64_t dk = (4 * n) % M;
+        for (; f < nx; f += 4) {
+            acc += tr[2 * f] * cs[k] - tr[2 * f + 1] * sn[k];
+            k += dk;
+            if (k >= M) k -= M;
+        }
+    }
+    red[tid] = acc;
+    __syncthreads();
+    if (q == 0 && n < nx) {
+        const double sum = (red[ni] + red[64 + ni]) + (red[128 + ni] + red[192 + ni]);
+        fault_update(d, n + nx * j, (tr[0] + 2.0 * sum) / (double)M, u, du);
+    }
+}
+
+static inline size_t fft_fwd_lds(int64_t nx) { return (size_t)(nx + 2 * (2 * nx - 1) + 512) * 8; }
+static inline size_t fft_inv_lds(int64_t nx) { return (size_t)(2 * nx + 2 * (2 * nx - 1) + 256) * 8; }
+
 // LDS bytes of a block with lchunk dip rows
 static inline int64_t toep_lds(const oq_rhs *h, int64_t lc)
 {
@@ -348,6 +485,17 @@ static void toep_shape(const oq_rhs *h, int *lchunk, int *nchunk, int *threads)
 
 int rhs_enqueue_matvec(oq_rhs *h, const double *u, double *du)
 {
+    if (h->kdtype == OQ_K_FFT) {
+        const unsigned fb = (unsigned)((h->nx + 63) / 64);
+        hipLaunchKernelGGL(k_fft_fwd, dim3((unsigned)h->nxi, fb), dim3(256), fft_fwd_lds(h->nx), h->stream,
+                           h->s64.as<double>(), h->nx, h->nxi, h->ftw.as<double>(), h->fsh.as<double>());
+        hipLaunchKernelGGL(k_fft_mid, dim3((unsigned)h->nx), dim3(256), 0, h->stream, h->fg.as<double>(),
+                           h->fsh.as<double>(), h->nx, h->nxi, h->fth.as<double>());
+        hipLaunchKernelGGL(k_fft_inv, dim3((unsigned)h->nxi, fb), dim3(256), fft_inv_lds(h->nx), h->stream,
+                           h->dev, h->fth.as<double>(), h->nx, h->ftw.as<double>(), u, du);
+        OQ_HIP(hipGetLastError());
+        return OQ_OK;
+    }
     if (h->kdtype == OQ_K_TOEPLITZ) {
         int lc, nc, nt;
         toep_shape(h, &lc, &nc, &nt);
@@ -444,11 +592,13 @@ static int alloc_common(oq_rhs *h, int device, int64_t nx, int64_t nxi, int64_t 
 {
     OQ_REQUIRE(nx > 0 && nxi > 0 && ne >= 0, "sizes must be positive");
     OQ_REQUIRE(kdtype == OQ_K_FP64 || kdtype == OQ_K_FP32 || kdtype == OQ_K_BF16 ||
-                   kdtype == OQ_K_TOEPLITZ,
-               "kdtype must be OQ_K_FP64, OQ_K_FP32, OQ_K_BF16 or OQ_K_TOEPLITZ");
-    OQ_REQUIRE(kdtype != OQ_K_TOEPLITZ || ne == 0, "the Toeplitz operator is for fault-only problems");
-    OQ_REQUIRE(kdtype != OQ_K_TOEPLITZ || nx <= 2040,
-               "the Toeplitz operator takes nx <= 2040 (one dip row of the kernel and of s in 64 KB of LDS)");
+                   kdtype == OQ_K_TOEPLITZ || kdtype == OQ_K_FFT,
+               "kdtype must be OQ_K_FP64, OQ_K_FP32, OQ_K_BF16, OQ_K_TOEPLITZ or OQ_K_FFT");
+    OQ_REQUIRE((kdtype != OQ_K_TOEPLITZ && kdtype != OQ_K_FFT) || ne == 0,
+               "the Toeplitz and DFT operators are for fault-only problems");
+    OQ_REQUIRE((kdtype != OQ_K_TOEPLITZ && kdtype != OQ_K_FFT) || nx <= 2040,
+               "the Toeplitz and DFT operators take nx <= 2040 (a dip row of the kernel or the twiddle "
+               "table with one row of s in LDS)");
     int ndev = 0;
     OQ_HIP(hipGetDeviceCount(&ndev));
     OQ_REQUIRE(device >= 0 && device < ndev, "invalid device index");
@@ -520,14 +670,38 @@ static int upload_props(oq_rhs *h, const oq_rs_props *f, const oq_visc_props *v)
     return OQ_OK;
 }
 
-// The Toeplitz operator: h->K holds st (nx*nxi*nxi doubles, st[i + nx j +
-// nf l]) already on the device in `st`; allocate the chunk partials.
+// The Toeplitz and DFT operators: h->K holds st (nx*nxi*nxi doubles, st[i +
+// nx j + nf l]) already on the device in `st`.  Toeplitz: allocate the chunk
+// partials.  DFT: the twiddle table, G (from st, on the device) and the
+// transform buffers.
 static int finalize_toeplitz(oq_rhs *h, DevBuf &st)
 {
     h->K.p = st.p;
     h->K.bytes = st.bytes;
     st.p = nullptr;
     st.bytes = 0;
+    if (h->kdtype == OQ_K_FFT) {
+        const int64_t nx = h->nx, nxi = h->nxi, M = 2 * nx - 1;
+        std::vector<double> tw((size_t)(2 * M));
+        for (int64_t k = 0; k < M; ++k) {  // the oracle's table: cos, sin of 2 pi k / M
+            tw[(size_t)k] = cos(2.0 * M_PI * (double)k / (double)M);
+            tw[(size_t)(M + k)] = sin(2.0 * M_PI * (double)k / (double)M);
+        }
+        int rc = h->ftw.upload(tw.data(), tw.size() * 8, h->stream);
+        if (!rc) rc = h->fg.alloc((size_t)(nx * nxi * nxi) * 8);
+        if (!rc) rc = h->fsh.alloc((size_t)(2 * nx * nxi) * 8);
+        if (!rc) rc = h->fth.alloc((size_t)(2 * nx * nxi) * 8);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_fft_kernel, dim3(grid_capped(nx * nxi * nxi, 256)), dim3(256), 0, h->stream,
+                           h->K.as<double>(), nx, nxi, h->ftw.as<double>(), h->fg.as<double>());
+        OQ_HIP(hipGetLastError());
+        OQ_HIP(hipFuncSetAttribute((const void *)k_fft_fwd, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)fft_fwd_lds(nx)));
+        OQ_HIP(hipFuncSetAttribute((const void *)k_fft_inv, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)fft_inv_lds(nx)));
+        OQ_HIP(hipStreamSynchronize(h->stream));
+        return OQ_OK;
+    }
     int lc, nc, nt;
     toep_shape(h, &lc, &nc, &nt);
     return h->tpart.alloc((size_t)(nc * h->nf) * 8);
@@ -601,7 +775,7 @@ int oq_rhs_create(int device, int64_t nx, int64_t nxi, int64_t ne, const oq_rs_p
     }
     const int64_t nf = h->nf, m6 = 6 * ne, ld = h->ld;
     DevBuf k64, tmp;
-    if (kdtype == OQ_K_TOEPLITZ) {
+    if (kdtype == OQ_K_TOEPLITZ || kdtype == OQ_K_FFT) {
         std::vector<double> st;
         const double *stp = gf11;
         if (gf11_is_dft) {
@@ -693,7 +867,7 @@ int oq_rhs_create_from_mesh(int device, const oq_fault_mesh *mf, const oq_hex8_m
     if (!rc && ma) rc = hu.load(ma);
     if (!rc && ma) rc = qu.load(quad);
     const int64_t nf = h->nf, ld = h->ld;
-    if (!rc && kdtype == OQ_K_TOEPLITZ) {
+    if (!rc && (kdtype == OQ_K_TOEPLITZ || kdtype == OQ_K_FFT)) {
         rc = st.alloc((size_t)(nf * mf->nxi) * 8);
         if (!rc) rc = launch_gf_ff(fu.dev, lambda, mu, ftype, (int)nrept, st.as<double>(), h->stream);
         if (!rc && hipStreamSynchronize(h->stream) != hipSuccess) {
@@ -850,7 +1024,7 @@ int oq_rhs_get_k(oq_rhs *h, double *k)
     const int64_t n = h->N * h->ld;
     DevBuf tmp;
     const double *src = h->K.as<double>();
-    if (h->kdtype == OQ_K_TOEPLITZ) {
+    if (h->kdtype == OQ_K_TOEPLITZ || h->kdtype == OQ_K_FFT) {
         int rc = tmp.alloc((size_t)n * 8);
         if (rc) return rc;
         OQ_HIP(hipMemsetAsync(tmp.p, 0, (size_t)n * 8, h->stream));

@@ -163,6 +163,7 @@ struct oq_rhs {
     oq::DevBuf s32;                     // FP32 copy of s (BF16 operator)
     oq::DevBuf u, du;                   // packed state scratch for the host API
     oq::DevBuf tpart;                   // Toeplitz operator: per-chunk partial sums
+    oq::DevBuf ftw, fg, fsh, fth;       // DFT operator: twiddles, kernel DFT, S^, T^
     oq::RhsDev dev{};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };

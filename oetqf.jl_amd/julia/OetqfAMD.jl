@@ -148,8 +148,10 @@ function cvisc(p::CompositePowerLawViscosityProperty, ne)
     return CViscProps(pointer(γ), pointer(n), length(p.piter), Tuple(Float64.(p.dϵ₀))), (γ, n)
 end
 
-# Kstorage: Float64, Float32, :bf16, or :toeplitz (fault only: K_ff kept as st)
-kcode(T) = T === Float64 ? Cint(0) : T === Float32 ? Cint(1) : T === :toeplitz ? Cint(3) : Cint(2)
+# Kstorage: Float64, Float32, :bf16, :toeplitz (fault only: K_ff kept as st)
+# or :fft (fault only: DFT convolution along strike, equation.jl:44-61)
+kcode(T) = T === Float64 ? Cint(0) : T === Float32 ? Cint(1) : T === :toeplitz ? Cint(3) :
+           T === :fft ? Cint(5) : Cint(2)
 
 struct CDilaProps
     tp::Ptr{Float64}; eps::Ptr{Float64}; beta::Ptr{Float64}; p0::Ptr{Float64}

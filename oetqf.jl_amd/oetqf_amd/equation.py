@@ -143,7 +143,7 @@ class ODEProblem:
 
 
 _KDTYPES = {"fp64": _lib.OQ_K_FP64, "fp32": _lib.OQ_K_FP32, "bf16": _lib.OQ_K_BF16,
-            "toeplitz": _lib.OQ_K_TOEPLITZ}
+            "toeplitz": _lib.OQ_K_TOEPLITZ, "fft": _lib.OQ_K_FFT}
 
 
 def assemble(*args, se=DieterichStateLaw(), kdtype="fp64", device=0, **kwargs):

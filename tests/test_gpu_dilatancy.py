@@ -20,7 +20,7 @@ def _physical(mf, rng):
 
 
 @pytest.mark.parametrize("case,dip,kdtype", [("random", 90.0, "fp64"), ("physical", 45.0, "fp64"),
-                                             ("physical", 90.0, "toeplitz")])
+                                             ("physical", 90.0, "toeplitz"), ("physical", 60.0, "fft")])
 def test_dilatancy_rhs_matches_oracle(oracle, rng, case, dip, kdtype):
     mf = small_fault(dip)
     pf = rs_props(mf, rng)

@@ -1,7 +1,9 @@
-"""Fault-only problems with K_ff kept as its Toeplitz kernel (OQ_K_TOEPLITZ):
-the RHS against the oracle's FFT convolution (the reference algorithm,
-equation.jl:44-61) and against the dense FP64 operator; the operator it
-expands to is the dense one; trajectories agree with the dense stepper."""
+"""Fault-only problems with K_ff kept as its Toeplitz kernel, applied
+directly (OQ_K_TOEPLITZ) or by DFT convolution along strike as the
+reference applies it (OQ_K_FFT, equation.jl:44-61): the RHS against the
+oracle's DFT convolution and against the dense FP64 operator; the operator
+it expands to is the dense one; trajectories agree with the dense
+stepper."""
 import numpy as np
 import pytest
 
@@ -18,14 +20,15 @@ def _state(mf, pf, rng):
     return oq.ArrayPartition(v, th, rng.random((mf.nx, mf.nxi)))
 
 
+@pytest.mark.parametrize("kdtype", ["toeplitz", "fft"])
 @pytest.mark.parametrize("dip,fourier", [(90.0, False), (45.0, True)])
-def test_toeplitz_rhs_matches_oracle_fft(oracle, rng, dip, fourier):
+def test_toeplitz_rhs_matches_oracle_fft(oracle, rng, dip, fourier, kdtype):
     mf = small_fault(dip, x=40e3, xi=15e3, d=1.25e3)
     pf = rs_props(mf, rng)
     u = _state(mf, pf, rng)
     st = oracle.gf_ff(mf, 3e10, 3e10, buffer_ratio=1.0)
     gf = oq.stress_greens_function(mf, 3e10, 3e10, buffer_ratio=1.0, fourier=fourier)
-    prob = oq.assemble(gf, pf, u, (0.0, 1.0), kdtype="toeplitz")
+    prob = oq.assemble(gf, pf, u, (0.0, 1.0), kdtype=kdtype)
     du = oq.similar(u)
     prob.f(du, u, None, 0.0)
     ref = oracle.rhs_fault(pf, st, u.x[0], u.x[1])
@@ -34,30 +37,32 @@ def test_toeplitz_rhs_matches_oracle_fft(oracle, rng, dip, fourier):
     prob.f.close()
 
 
-def test_toeplitz_from_mesh_equals_dense(rng):
+@pytest.mark.parametrize("kdtype", ["toeplitz", "fft"])
+def test_toeplitz_from_mesh_equals_dense(rng, kdtype):
     mf = oq.gen_mesh("RectOkada", 80e3, 20e3, 312.5, 312.5, 90.0)  # 256 x 64 = 16384 patches
     pf = rs_props(mf, rng)
     u = _state(mf, pf, rng)
-    pt = oq.assemble_from_mesh(mf, None, 3e10, 3e10, pf, None, u, (0.0, 1.0), kdtype="toeplitz")
+    pt = oq.assemble_from_mesh(mf, None, 3e10, 3e10, pf, None, u, (0.0, 1.0), kdtype=kdtype)
     pd = oq.assemble_from_mesh(mf, None, 3e10, 3e10, pf, None, u, (0.0, 1.0))
     dt_, dd = oq.similar(u), oq.similar(u)
     pt.f(dt_, u, None, 0.0)
     pd.f(dd, u, None, 0.0)
     e = relerr(dt_.x[0], dd.x[0])
     mt, md = pt.f.time(20)[0], pd.f.time(20)[0]
-    print("16384 patches: Toeplitz %.4f ms, dense FP64 %.4f ms per RHS, rel diff %.2e" % (mt, md, e))
+    print("16384 patches: %s %.4f ms, dense FP64 %.4f ms per RHS, rel diff %.2e" % (kdtype, mt, md, e))
     assert e <= 1e-12
     pt.f.close()
     pd.f.close()
 
 
-def test_toeplitz_operator_and_trajectory(oracle, rng):
+@pytest.mark.parametrize("kdtype", ["toeplitz", "fft"])
+def test_toeplitz_operator_and_trajectory(oracle, rng, kdtype):
     mf = small_fault(90.0)
     pf = rs_props(mf, rng)
     v = pf.vpl * np.exp(0.3 * rng.standard_normal((mf.nx, mf.nxi)))
     u0 = oq.ArrayPartition(v, pf.L / v, np.zeros_like(v))
     st = oracle.gf_ff(mf, 3e10, 3e10, buffer_ratio=1.0)
-    pt = oq.assemble(st, pf, u0, (0.0, 0.5 * YEAR), kdtype="toeplitz")
+    pt = oq.assemble(st, pf, u0, (0.0, 0.5 * YEAR), kdtype=kdtype)
     pd = oq.assemble(st, pf, u0, (0.0, 0.5 * YEAR))
     assert np.array_equal(pt.f.dense_operator(), pd.f.dense_operator())
     st_ = oq.solve(pt, oq.Tsit5(), reltol=1e-6, abstol=1e-8)
@@ -67,11 +72,12 @@ def test_toeplitz_operator_and_trajectory(oracle, rng):
     assert relerr(st_.u.pack(), sd.u.pack()) <= 1e-8
 
 
-def test_toeplitz_rejects_coupled(oq, oracle, rng):
+@pytest.mark.parametrize("kdtype", ["toeplitz", "fft"])
+def test_toeplitz_rejects_coupled(oq, oracle, rng, kdtype):
     from helpers import small_mantle, visc_props
     mf, ma = small_fault(90.0), small_mantle()
     pf, pa = rs_props(mf, rng), visc_props(ma, rng, 0.0)
     v = pf.vpl * np.ones((mf.nx, mf.nxi))
     u0 = oq.ArrayPartition(v, pf.L / v, np.zeros((ma.ne, 6)), np.zeros((ma.ne, 6)), np.zeros_like(v))
     with pytest.raises(ValueError):
-        oq.assemble_from_mesh(mf, ma, 3e10, 3e10, pf, pa, u0, (0.0, 1.0), kdtype="toeplitz")
+        oq.assemble_from_mesh(mf, ma, 3e10, 3e10, pf, pa, u0, (0.0, 1.0), kdtype=kdtype)
