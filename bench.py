@@ -373,6 +373,26 @@ def toeplitz_leg(oq, args):
                                                    np.linalg.norm(ref.x[0]))
     out["speedup_vs_dense"] = out["fp64"]["ms_per_rhs"] / out["toeplitz"]["ms_per_rhs"]
     out["fft_speedup_vs_dense"] = out["fp64"]["ms_per_rhs"] / out["fft"]["ms_per_rhs"]
+    if args.cycle_seconds > 0:
+        # the fault-only cycle on the DFT operator (Tsit5 graph stepper) from
+        # the workload's interseismic state, over a bounded wall-clock window
+        from oetqf_amd.workloads import YEAR
+        u0 = oq.ArrayPartition(w["u0"].x[0].copy(), w["u0"].x[1].copy(), w["u0"].x[4].copy())
+        for kd in ("fft", "fp64"):
+            p = oq.assemble_from_mesh(mf, None, w["lam"], w["mu"], pf, None, u0, w["tspan"], kdtype=kd)
+            it = oq.Integrator(p, reltol=w["reltol"], abstol=w["abstol"])
+            t0 = time.perf_counter()
+            while time.perf_counter() - t0 < min(args.cycle_seconds, 6.0):
+                if it.step(256) != "Running":
+                    break
+            el = time.perf_counter() - t0
+            t, _, _, na, nr = it.state()
+            it.close()
+            p.f.close()
+            out[kd]["cycle"] = dict(sim_years=t / YEAR, wall_s=el, sim_yr_per_hr=(t / YEAR) / (el / 3600.0),
+                                    accepted=int(na), rejected=int(nr))
+            log("fault-only %d patches, %s Tsit5: %.3f yr in %.1f s (%.0f yr/hr)" %
+                (nf, kd, t / YEAR, el, out[kd]["cycle"]["sim_yr_per_hr"]))
     log("fault-only %d patches: dense FP64 %.4f ms, Toeplitz %.4f ms (%.1fx, err %.1e), DFT %.4f ms "
         "(%.1fx, err %.1e) per RHS" % (
             nf, out["fp64"]["ms_per_rhs"], out["toeplitz"]["ms_per_rhs"], out["speedup_vs_dense"],
