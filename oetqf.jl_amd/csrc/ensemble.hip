@@ -351,7 +351,8 @@ __global__ void __launch_bounds__(256) k_h3_split_s(EnsDev d, EnsH3 h, const flo
 // double-buffered LDS; global loads run two stages ahead in registers.
 // (Measured alternatives: 4 waves with 2 x 2 tiles spilled into AGPR
 // copies; 128-member blocks two per CU streamed K twice and ran 20 %
-// slower.)
+// slower; three stages loaded straight into LDS with global_load_lds and
+// XOR-swizzled rows ran 12 % slower.)
 __global__ void __launch_bounds__(512) k_ens_gemm_h3(EnsDev d, EnsH3 h)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char hlds[];

@@ -253,8 +253,11 @@ __global__ void __launch_bounds__(256) k_grp_gemm(GrpDev d, const int2 *__restri
 #pragma unroll
                     for (int t = 0; t < 4; ++t) kv[rb][t] = gload4(kp[rb] + p * kGStep + 16 * t);
             }
+            // the prefetch is unconditional (pn wraps to the first pass after the
+            // last one): a conditional load would make the compiler's wait
+            // counts drain every load in flight at the branch join
             f4x sn[4], kn[RB][4];
-            if (PIPE && more) {
+            if (PIPE) {
 #pragma unroll
                 for (int t = 0; t < 4; ++t) sn[t] = gload4(sp + pn * kGStep + 16 * t);
 #pragma unroll
@@ -285,7 +288,7 @@ __global__ void __launch_bounds__(256) k_grp_gemm(GrpDev d, const int2 *__restri
                     }
                 }
             }
-            if (PIPE && more) {
+            if (PIPE) {
 #pragma unroll
                 for (int t = 0; t < 4; ++t) sv[t] = sn[t];
 #pragma unroll
@@ -362,12 +365,13 @@ __global__ void __launch_bounds__(256) k_grp_gemm_lds(GrpDev d, const int2 *__re
         for (int64_t it = 0; it < nch; ++it) {
             const int64_t pn = (p + 1 == nch) ? 0 : p + 1;
             const bool more = it + 1 < nch;
-            if (more) {
+            // unconditional prefetch and store (pn wraps; after the last
+            // slice they are redundant and the next tile re-stages after a
+            // barrier): conditional loads make the wait counts drain
 #pragma unroll
-                for (int j = 0; j < KV; ++j) kr[j] = gload4(kp[j] + pn * BK);
+            for (int j = 0; j < KV; ++j) kr[j] = gload4(kp[j] + pn * BK);
 #pragma unroll
-                for (int j = 0; j < SV; ++j) sr[j] = gload4(sp[j] + pn * BK);
-            }
+            for (int j = 0; j < SV; ++j) sr[j] = gload4(sp[j] + pn * BK);
 #pragma unroll
             for (int sub = 0; sub < BK / 64; ++sub) {
                 f4x a[RB][4], b[4];
@@ -400,14 +404,12 @@ __global__ void __launch_bounds__(256) k_grp_gemm_lds(GrpDev d, const int2 *__re
                 }
             }
             __syncthreads();
-            if (more) {
 #pragma unroll
-                for (int j = 0; j < KV; ++j)
-                    *reinterpret_cast<f4x *>(&As[(trow + j * (256 / C4)) * PAD + 4 * tc4]) = kr[j];
+            for (int j = 0; j < KV; ++j)
+                *reinterpret_cast<f4x *>(&As[(trow + j * (256 / C4)) * PAD + 4 * tc4]) = kr[j];
 #pragma unroll
-                for (int j = 0; j < SV; ++j)
-                    *reinterpret_cast<f4x *>(&Bs[(trow + j * (256 / C4)) * PAD + 4 * tc4]) = sr[j];
-            }
+            for (int j = 0; j < SV; ++j)
+                *reinterpret_cast<f4x *>(&Bs[(trow + j * (256 / C4)) * PAD + 4 * tc4]) = sr[j];
             __syncthreads();
             p = pn;
         }
@@ -501,26 +503,31 @@ __global__ void __launch_bounds__(256) k_grp_gemm_lds2(GrpDev d, const int2 *__r
                 }
             }
         };
+        // Loads and stores are unconditional (slice indices clamp to the last
+        // one; the redundant copies go to a buffer nobody reads and the next
+        // tile re-stages after a barrier): conditional loads would make the
+        // compiler's wait counts drain the younger slice too.  nch = ld / BK
+        // is even (ld is a multiple of 512 columns).
+        auto kc = [&](int64_t k) { return k < nch ? k : nch - 1; };
         load(ka, sa, 0);
-        if (nch > 1) load(kb, sb, 1);
+        load(kb, sb, kc(1));
         __syncthreads();  // the previous tile's reads of As/Bs are done
         store(ka, sa);
         __syncthreads();
-        if (nch > 2) load(ka, sa, 2);
+        load(ka, sa, kc(2));
         for (int64_t k = 0; k < nch; k += 2) {
             // LDS: slice k; kb: slice k+1; ka: slice k+2
             reduce(k);
             __syncthreads();
-            if (k + 1 < nch) store(kb, sb);
+            store(kb, sb);
             __syncthreads();
-            if (k + 3 < nch) load(kb, sb, k + 3);
-            if (k + 1 >= nch) break;
+            load(kb, sb, kc(k + 3));
             // LDS: slice k+1; ka: slice k+2; kb: slice k+3
             reduce(k + 1);
             __syncthreads();
-            if (k + 2 < nch) store(ka, sa);
+            store(ka, sa);
             __syncthreads();
-            if (k + 4 < nch) load(ka, sa, k + 4);
+            load(ka, sa, kc(k + 4));
         }
         grp_store_rows<RB>(d, G, r0 + (int64_t)wave * 16 * RB, lr, lq, hi, lo);
     }
