@@ -385,6 +385,7 @@ __global__ void __launch_bounds__(256) k_fft_fwd(RhsDev d, const double *__restr
     if (f < nx) {
         int64_t k = (f * q) % M;
         const int64_t dk = (4 * f) % M;
+#pragma unroll 8
         for (int64_t n = q; n < nx; n += 4) {
             re += sc[n] * cs[k];
             im -= sc[n] * sn[k];
@@ -453,6 +454,7 @@ __global__ void __launch_bounds__(256) k_fft_inv(RhsDev d, const double *__restr
     if (n < nx) {
         int64_t f = 1 + q, k = ((1 + q) * n) % M;
         const int64_t dk = (4 * n) % M;
+#pragma unroll 8
         for (; f < nx; f += 4) {
             acc += tr[2 * f] * cs[k] - tr[2 * f + 1] * sn[k];
             k += dk;
