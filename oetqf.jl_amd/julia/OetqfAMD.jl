@@ -327,4 +327,83 @@ function wsolve_gpu(prob, file, nstep, getu, ustrs, tstr; stride=1, append=false
     return (:Running, :Success, :MaxIters, :DtLessThanMin, :NonFinite)[rc[] + 1]
 end
 
+# ---- ensembles of frictional realizations on one geometry (no reference
+# counterpart: the reference solves one remade problem per realization,
+# examples/otf-with-mantle.jl:154-161) ----
+mutable struct GPUEnsemble
+    h::Ptr{Cvoid}
+    M::Int
+    n::Int
+    keep::Any
+    function GPUEnsemble(h, M, n, keep)
+        r = new(h, M, n, keep)
+        finalizer(x -> (x.h != C_NULL && ccall((:oq_ensemble_destroy, libpath), Cint, (Ptr{Cvoid},), x.h);
+                        x.h = C_NULL), r)
+        return r
+    end
+end
+
+"""ensemble_gpu(mf, ma, λ, μ, members, pa; Kstorage=:fp16x2): M members
+(RateStateQuasiDynamicProperty each, same mesh) sharing the device operator.
+Kstorage: Float32, :bf16 or :fp16x2 (FP16 hi/lo split on the FP16 matrix
+cores, FP32-class error).  ma = nothing for a fault-only model."""
+function ensemble_gpu(mf::Oetqf.RectOkadaMesh, ma, λ::Float64, μ::Float64,
+        members::AbstractVector{<:RateStateQuasiDynamicProperty}, pa; Kstorage=:fp16x2,
+        ftype=StrikeSlip(), nrept::Integer=2, buffer_ratio::Real=0.0, qtype="Gauss1", device::Integer=0)
+    kd = Kstorage === Float32 ? Cint(1) : Kstorage === :bf16 ? Cint(2) : Kstorage === :fp16x2 ? Cint(4) :
+         throw(ArgumentError("ensemble Kstorage must be Float32, :bf16 or :fp16x2"))
+    rs = [crs(p) for p in members]
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    ne = ma === nothing ? 0 : length(ma.cx)
+    if ne > 0
+        c, w = quadrature(qtype)
+        vs, keepv = cvisc(pa, ne)
+        GC.@preserve mf ma members c w keepv begin
+            check(ccall((:oq_ensemble_create, libpath), Cint,
+                (Cint, Ref{CFaultMesh}, Ref{CHex8Mesh}, Float64, Float64, Cint, Int64, Float64,
+                 Ref{CQuadrature}, Int64, Ptr{CRSProps}, Ref{CViscProps}, Cint, Ref{Ptr{Cvoid}}),
+                device, Ref(cmesh(mf)), Ref(cmesh(ma)), λ, μ, ftcode(ftype), nrept, buffer_ratio,
+                Ref(CQuadrature(pointer(c), pointer(w), length(w))), length(members), rs, Ref(vs), kd, h))
+        end
+        n = 3 * mf.nx * mf.nξ + 12 * ne
+    else
+        GC.@preserve mf members begin
+            check(ccall((:oq_ensemble_create, libpath), Cint,
+                (Cint, Ref{CFaultMesh}, Ptr{CHex8Mesh}, Float64, Float64, Cint, Int64, Float64,
+                 Ptr{CQuadrature}, Int64, Ptr{CRSProps}, Ptr{CViscProps}, Cint, Ref{Ptr{Cvoid}}),
+                device, Ref(cmesh(mf)), C_NULL, λ, μ, ftcode(ftype), nrept, buffer_ratio, C_NULL,
+                length(members), rs, C_NULL, kd, h))
+        end
+        n = 3 * mf.nx * mf.nξ
+    end
+    return GPUEnsemble(h[], length(members), n, (members, pa))
+end
+
+"""ensemble_rhs!(e, dU, U): the RHS of every member; U, dU are n x M
+matrices whose columns are packed ArrayPartition states."""
+function ensemble_rhs!(e::GPUEnsemble, dU::Matrix{Float64}, U::Matrix{Float64})
+    size(U) == (e.n, e.M) && size(dU) == (e.n, e.M) || throw(DimensionMismatch("U, dU must be n x M"))
+    check(ccall((:oq_ensemble_eval, libpath), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}), e.h, U, dU))
+    return dU
+end
+
+"""ensemble_solve(e, U0, tspan; reltol, abstol, dt0): lockstep Tsit5 of all
+members, each with its own step control.  Returns (t, U, naccept, nreject,
+retcodes) per member."""
+function ensemble_solve(e::GPUEnsemble, U0::Matrix{Float64}, tspan::NTuple{2};
+        reltol=1e-6, abstol=1e-8, dt0=1.0, dtmax=0.0, dtmin=0.0, maxiters=10^9, chunk::Integer=4096)
+    size(U0) == (e.n, e.M) || throw(DimensionMismatch("U0 must be n x M"))
+    opts = CSolverOpts(reltol, abstol, dt0, dtmax, dtmin, maxiters)
+    check(ccall((:oq_ensemble_start, libpath), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Float64, Float64, Ref{CSolverOpts}), e.h, U0, tspan[1], tspan[2], Ref(opts)))
+    left = Ref{Int64}(e.M)
+    while left[] > 0
+        check(ccall((:oq_ensemble_run, libpath), Cint, (Ptr{Cvoid}, Int64, Ref{Int64}), e.h, chunk, left))
+    end
+    t = zeros(e.M); U = similar(U0); na = zeros(Int64, e.M); nr = zeros(Int64, e.M); rc = zeros(Cint, e.M)
+    check(ccall((:oq_ensemble_state, libpath), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Int64}, Ptr{Int64}, Ptr{Cint}), e.h, t, U, na, nr, rc))
+    return t, U, na, nr, [(:Running, :Success, :MaxIters, :DtLessThanMin, :NonFinite)[r + 1] for r in rc]
+end
+
 end # module
