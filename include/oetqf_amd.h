@@ -346,6 +346,10 @@ int oq_group_info(oq_group *e, int64_t *ntot, int64_t *nstate, int64_t *kbytes);
  * oq_group_start (a running stepper must be started again). */
 int oq_group_autotune(oq_group *e, int iters, int *shape_of_bucket, double *ms_of_bucket,
                       double *ms_table);
+/* Pins the tile shape of each size bucket (shape_of_bucket[8]: 0..9, -1 =
+ * keep), e.g. to reuse an earlier autotuning.  Like oq_group_autotune, call
+ * before oq_group_start. */
+int oq_group_set_shapes(oq_group *e, const int *shape_of_bucket);
 /* du = f(u) for all members (host arrays of ntot entries). */
 int oq_group_eval(oq_group *e, const double *u, double *du);
 /* Mean time of one grouped RHS (all geometries and members). */

@@ -674,11 +674,12 @@ __global__ void __launch_bounds__(256) k_grp_commit(GrpCtrl c, int64_t ntot,
 
 // tile shapes: RB in {1, 2, 4} x {software-pipelined, plain} register
 // kernels, and the LDS-staged kernels with RB in {1, 2}, one or two slices
-// in flight
+// in flight (a third slice in flight measured the same as two: the kernel
+// is not limited by its prefetch depth)
 static constexpr int kGShapes = 10;
 static inline int shape_rb(int s)
 {
-    return s < 6 ? (s / 2 == 0 ? 1 : (s / 2 == 1 ? 2 : 4)) : (s == 6 || s == 8 ? 1 : 2);
+    return s < 6 ? (s / 2 == 0 ? 1 : (s / 2 == 1 ? 2 : 4)) : (s % 2 == 0 ? 1 : 2);
 }
 
 }  // namespace oq
@@ -1034,6 +1035,20 @@ int oq_group_info(oq_group *e, int64_t *ntot, int64_t *nstate, int64_t *kbytes)
         kb += e->geo[g].N * e->geo[g].N * 4;
     }
     if (kbytes) *kbytes = kb;
+    return OQ_OK;
+}
+
+int oq_group_set_shapes(oq_group *e, const int *shape_of_bucket)
+{
+    OQ_REQUIRE(e != nullptr && shape_of_bucket != nullptr, "NULL argument");
+    for (int b = 0; b < kGBuckets; ++b)
+        OQ_REQUIRE(shape_of_bucket[b] >= -1 && shape_of_bucket[b] < kGShapes, "tile shape out of range");
+    OQ_HIP(hipSetDevice(e->device));
+    for (int b = 0; b < kGBuckets; ++b)
+        if (shape_of_bucket[b] >= 0) e->bucket_shape[b] = shape_of_bucket[b];
+    int rc = upload_tiles(e);
+    if (rc) return rc;
+    e->stepping = false;  // a captured step holds the old tile tables
     return OQ_OK;
 }
 

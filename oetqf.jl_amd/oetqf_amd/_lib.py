@@ -115,6 +115,7 @@ PROTOTYPES = {
                                ctypes.POINTER(_vp)]),
     "oq_group_info": (_int, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
     "oq_group_autotune": (_int, [_vp, _int, ctypes.POINTER(_int), _pd, _pd]),
+    "oq_group_set_shapes": (_int, [_vp, ctypes.POINTER(_int)]),
     "oq_group_eval": (_int, [_vp, _pd, _pd]),
     "oq_group_time": (_int, [_vp, _int, ctypes.POINTER(_d)]),
     "oq_group_start": (_int, [_vp, _pd, _d, _d, ctypes.POINTER(SolverOpts)]),

@@ -87,6 +87,14 @@ class Group:
                            for b in range(8) if sh[b] >= 0}
         return {b: (SHAPES[sh[b]], float(ms[b])) for b in range(8) if sh[b] >= 0}
 
+    def set_shapes(self, shapes):
+        """Pins tile shapes: {bucket: shape name or index}; buckets left out
+        keep their shape."""
+        sh = np.full(8, -1, dtype=np.int32)
+        for b, s in shapes.items():
+            sh[int(b)] = SHAPES.index(s) if isinstance(s, str) else int(s)
+        _lib.check(_lib.load().oq_group_set_shapes(self.h, sh.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+
     def eval(self, U):
         U = np.ascontiguousarray(U, dtype=np.float64)
         assert U.shape == (self.ntot,)

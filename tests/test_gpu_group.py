@@ -111,6 +111,20 @@ def test_group_autotune_keeps_results(batch):
     assert relerr(after, before) <= 1e-6
 
 
+def test_group_every_tile_shape_gives_the_same_rhs(batch):
+    """Each of the tile shapes (register-streaming, LDS-staged with one or
+    two slices in flight), pinned for every bucket, reproduces the default
+    shape's RHS."""
+    from oetqf_amd.group import SHAPES
+    geoms, members, states, grp, U = batch
+    ref = grp.eval(U)
+    for s in SHAPES:
+        grp.set_shapes({b: s for b in range(8)})
+        e = relerr(grp.eval(U), ref)
+        print("%-11s rel diff %.2e" % (s, e))
+        assert e <= 1e-6, s
+
+
 def test_group_stepper_matches_single_members(oq):
     rng = np.random.default_rng(43)
     geoms = _geoms(rng)
