@@ -362,9 +362,14 @@ def toeplitz_leg(oq, args):
         p.f.time(3)
         me, mm = p.f.time(args.steps)
         p.f.close()
-        # bytes of the operator the RHS reads (the DFT operator reads G, the
-        # same size as st)
-        kb = 8 * nf * nf if kd == "fp64" else 8 * mf.nx * mf.nxi * mf.nxi
+        # bytes of the operator the RHS reads: dense K, st, or G (the real
+        # DFT of the circulant at the L/2 + 1 bins of the power-of-two length
+        # L >= 2nx - 1)
+        L = 2
+        while L < 2 * mf.nx - 1:
+            L *= 2
+        kb = {"fp64": 8 * nf * nf, "toeplitz": 8 * mf.nx * mf.nxi * mf.nxi,
+              "fft": 8 * (L // 2 + 1) * mf.nxi * mf.nxi}[kd]
         out[kd] = dict(ms_per_rhs=me, rhs_per_s=1000.0 / me, operator_bytes=kb)
         if ref is None:
             ref = du

@@ -52,10 +52,12 @@ extern "C" {
  * error, eight times the FP32 matrix rate. */
 #define OQ_K_FP16X2 4
 /* Fault-only problems (ne = 0): K_ff applied as the reference applies it,
- * by DFT convolution along strike (equation.jl:44-61): transform of the
- * zero-padded source (length 2nx-1), per-frequency contraction with the
- * DFT of the symmetric circulant of st (GF.jl:60-67, real), inverse
- * transform.  st stays resident as well (operator expansion). */
+ * by DFT convolution along strike (equation.jl:44-61): FFT of the
+ * zero-padded source, per-frequency contraction with the DFT of the
+ * symmetric circulant of st (GF.jl:60-67, real), inverse FFT.  The padded
+ * length is the power of two L >= 2nx-1 (the reference pads to 2nx-1; both
+ * give the same first nx rows of the circular convolution).  st stays
+ * resident as well (operator expansion). */
 #define OQ_K_FFT 5
 
 /* RectOkadaMesh (src/BEM/mesh.jl:5-21).  Only x, ax, y, z, axi, nx, nxi, dx,

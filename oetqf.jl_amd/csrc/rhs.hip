@@ -328,86 +328,180 @@ __global__ void __launch_bounds__(256) k_toep_finish(RhsDev d, const double *__r
 
 // ---------------- DFT K_ff (fault-only problems, OQ_K_FFT) ----------------
 // The reference's own algorithm (equation.jl:44-61): the source s(:, l) is
-// zero-padded to length M = 2nx - 1 and transformed along strike, each
-// frequency f < nx is contracted over the dip rows with the DFT of the
-// circulant [st; reverse(st[2:end])] (GF.jl:60-67), and the inverse real DFT
-// gives the first nx rows.  That circulant is symmetric (x_n = x_{M-n}), so
-// its DFT is real: G[f, j, l] = sum_{n<M} x_n cos(2 pi f n / M).  The
-// transforms are direct sums over an exact twiddle index (f n mod M) read
-// from a table: O(nx^2 nxi) work per transform against the O(nx^2 nxi^2) of
-// the direct Toeplitz product, and each launch is a few microseconds.
+// zero-padded and transformed along strike, each frequency is contracted
+// over the dip rows with the DFT of the circulant of st (GF.jl:60-67), and
+// the inverse transform gives the first nx rows.  The reference pads to
+// 2nx - 1; here the length is the power of two L >= 2nx - 1, where the
+// circular convolution of the first nx rows is the same (the padding adds
+// zeros between st[nx-1] and its mirror, so no product wraps around), and
+// the transforms are radix-2 FFTs in LDS.  The padded circulant is still
+// symmetric (x_m = x_{L-m}), so its DFT is real:
+//   G[f, j, l] = sum_{m<L} x_m cos(2 pi f m / L),  f = 0..L/2.
 // Layouts: G[f][l][j] (j fastest), S^[f][l] and T^[j][f] complex
-// (interleaved re, im); the twiddle table holds cos then sin, M each.
-// Three launches per RHS: the forward transform (which forms s = v - vpl
-// from the state itself, so no prologue runs), the per-frequency
-// contraction (one block per frequency), and the inverse transform with the
-// rate-and-state update.  (Fusing the contraction into the inverse, one
-// dip row per block, measured 30 us against 22 us at 256 x 64: each block
-// then streams its 128 KB of G with few loads in flight.)
+// (interleaved re, im), f = 0..L/2; the twiddle table holds cos(2 pi k / L)
+// then sin(2 pi k / L), k < L/2.  Three launches per RHS: the forward FFT
+// (which forms s = v - vpl from the state itself, so no prologue runs), the
+// per-frequency contraction, and the inverse FFT with the rate-and-state
+// update.
 
-// one-time: G from st (grid-stride over nx * nxi^2 entries)
+static inline int64_t fft_len(int64_t nx)
+{
+    int64_t L = 2;  // at least one butterfly, so the twiddle table is never empty
+    while (L < 2 * nx - 1) L <<= 1;
+    return L;
+}
+
+// one-time: G from st (grid-stride over (L/2 + 1) * nxi^2 entries)
 __global__ void __launch_bounds__(256) k_fft_kernel(const double *__restrict__ st, int64_t nx,
-                                                    int64_t nxi, const double *__restrict__ tw,
+                                                    int64_t nxi, int64_t L, const double *__restrict__ tw,
                                                     double *__restrict__ G)
 {
-    const int64_t M = 2 * nx - 1, nf = nx * nxi, tot = nx * nxi * nxi;
+    const int64_t nf = nx * nxi, h = L / 2, tot = (h + 1) * nxi * nxi;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < tot;
          t += (int64_t)gridDim.x * blockDim.x) {
         const int64_t j = t % nxi, l = (t / nxi) % nxi, f = t / (nxi * nxi);
         const double *col = st + nx * j + nf * l;
+        // x_0 + sum_{0<m<nx} x_m (cos(2 pi f m / L) + cos(2 pi f (L - m) / L)) = x_0 + 2 sum x_m cos
         double acc = 0.0;
         int64_t k = 0;
-        for (int64_t n = 0; n < M; ++n) {
-            acc += (n < nx ? col[n] : col[M - n]) * tw[k];
+        for (int64_t m = 1; m < nx; ++m) {
             k += f;
-            if (k >= M) k -= M;
+            if (k >= L) k -= L;
+            const int64_t kk = k < h ? k : L - k;  // cos(2 pi k / L) = cos(2 pi (L - k) / L)
+            acc += col[m] * (k < h ? tw[kk] : (kk == h ? -1.0 : tw[kk]));
         }
-        G[t] = acc;
+        G[t] = col[0] + 2.0 * acc;
     }
 }
 
-// relative_velocity! (equation.jl:35-42) and the forward DFT of s(:, l):
-// block (l, 64 frequencies), four interleaved partial sums over n
+// x1 <- x0 - w x1, x0 <- x0 + w x1 on registers
+__device__ __forceinline__ void bfly(double &r0, double &i0, double &r1, double &i1, double wr, double wi)
+{
+    const double xr = r1 * wr - i1 * wi, xi = r1 * wi + i1 * wr;
+    r1 = r0 - xr;
+    i1 = i0 - xi;
+    r0 += xr;
+    i0 += xi;
+}
+
+// In-place radix-2 FFT of re/im[0..L) (bit-reversed input order), sign -1
+// (forward) or +1 (inverse); the block's threads share the butterflies of
+// each stage.  Stages len, 2len and 4len run together: a thread that owns
+// the eight entries p + k len/2 (k < 8) of a 4len-block holds every pair of
+// the three stages, so three stages cost one barrier (3 for L = 512 instead
+// of 9); one or two stages left over run the same way with four or two
+// entries per thread.
+__device__ __forceinline__ void lds_fft(double *re, double *im, int64_t L, const double *cs,
+                                        const double *sn, double sign)
+{
+    int64_t len = 2;
+    for (; 4 * len <= L; len <<= 3) {
+        const int64_t half = len >> 1, s1 = L / len, s2 = s1 >> 1, s4 = s1 >> 2;
+        for (int64_t b = threadIdx.x; b < L / 8; b += blockDim.x) {
+            const int64_t p = b & (half - 1), a0 = (b - p) * 8 + p;
+            double r[8], i[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                r[k] = re[a0 + k * half];
+                i[k] = im[a0 + k * half];
+            }
+            const double w1r = cs[p * s1], w1i = sign * sn[p * s1];
+#pragma unroll
+            for (int k = 0; k < 8; k += 2) bfly(r[k], i[k], r[k + 1], i[k + 1], w1r, w1i);  // stage len
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {  // stage 2len: positions p, p + half
+                const int64_t x = (p + k * half) * s2;
+                bfly(r[k], i[k], r[k + 2], i[k + 2], cs[x], sign * sn[x]);
+                bfly(r[k + 4], i[k + 4], r[k + 6], i[k + 6], cs[x], sign * sn[x]);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {  // stage 4len: positions p + k half
+                const int64_t x = (p + k * half) * s4;
+                bfly(r[k], i[k], r[k + 4], i[k + 4], cs[x], sign * sn[x]);
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                re[a0 + k * half] = r[k];
+                im[a0 + k * half] = i[k];
+            }
+        }
+        __syncthreads();
+    }
+    if (2 * len <= L) {  // two stages left: len and 2len
+        const int64_t half = len >> 1, s1 = L / len, s2 = s1 >> 1;
+        for (int64_t b = threadIdx.x; b < L / 4; b += blockDim.x) {
+            const int64_t p = b & (half - 1), a0 = (b - p) * 4 + p, a1 = a0 + half, a2 = a0 + len,
+                          a3 = a2 + half;
+            double r0 = re[a0], i0 = im[a0], r1 = re[a1], i1 = im[a1];
+            double r2 = re[a2], i2 = im[a2], r3 = re[a3], i3 = im[a3];
+            const double w1r = cs[p * s1], w1i = sign * sn[p * s1];
+            bfly(r0, i0, r1, i1, w1r, w1i);
+            bfly(r2, i2, r3, i3, w1r, w1i);
+            bfly(r0, i0, r2, i2, cs[p * s2], sign * sn[p * s2]);
+            bfly(r1, i1, r3, i3, cs[(p + half) * s2], sign * sn[(p + half) * s2]);
+            re[a0] = r0;
+            im[a0] = i0;
+            re[a1] = r1;
+            im[a1] = i1;
+            re[a2] = r2;
+            im[a2] = i2;
+            re[a3] = r3;
+            im[a3] = i3;
+        }
+        __syncthreads();
+    } else if (len <= L) {  // one stage left (len == L)
+        const int64_t half = len >> 1, step = L / len;
+        for (int64_t b = threadIdx.x; b < L / 2; b += blockDim.x) {
+            const int64_t p = b & (half - 1), a0 = (b - p) * 2 + p, a1 = a0 + half;
+            double r0 = re[a0], i0 = im[a0], r1 = re[a1], i1 = im[a1];
+            bfly(r0, i0, r1, i1, cs[p * step], sign * sn[p * step]);
+            re[a0] = r0;
+            im[a0] = i0;
+            re[a1] = r1;
+            im[a1] = i1;
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ int64_t bitrev(int64_t x, int bits)
+{
+    return (int64_t)(__brev((unsigned)x) >> (32 - bits));  // bits >= 1 since L >= 2
+}
+
+// relative_velocity! (equation.jl:35-42) and the forward FFT of the padded
+// s(:, l): block l; bins 0..L/2 to S^[f][l]
 __global__ void __launch_bounds__(256) k_fft_fwd(RhsDev d, const double *__restrict__ u, int64_t nx,
-                                                 int64_t nxi, const double *__restrict__ tw,
-                                                 double *__restrict__ Sh)
+                                                 int64_t nxi, int64_t L, int bits,
+                                                 const double *__restrict__ tw, double *__restrict__ Sh)
 {
     extern __shared__ double fsm[];
-    const int64_t l = blockIdx.x, M = 2 * nx - 1;
-    double *sc = fsm, *cs = sc + nx, *sn = cs + M, *red = sn + M;  // red: 2 x 256
-    const int tid = threadIdx.x;
-    for (int64_t x = tid; x < nx; x += 256) sc[x] = u[d.lay.v + x + nx * l] - d.vpl;
-    for (int64_t x = tid; x < 2 * M; x += 256) cs[x] = tw[x];  // cs then sn
-    __syncthreads();
-    const int fi = tid & 63, q = tid >> 6;
-    const int64_t f = (int64_t)blockIdx.y * 64 + fi;
-    double re = 0.0, im = 0.0;
-    if (f < nx) {
-        int64_t k = (f * q) % M;
-        const int64_t dk = (4 * f) % M;
-#pragma unroll 8
-        for (int64_t n = q; n < nx; n += 4) {
-            re += sc[n] * cs[k];
-            im -= sc[n] * sn[k];
-            k += dk;
-            if (k >= M) k -= M;
-        }
+    const int64_t l = blockIdx.x, h = L / 2;
+    double *re = fsm, *im = re + L, *cs = im + L, *sn = cs + h;
+    for (int64_t x = threadIdx.x; x < h; x += blockDim.x) {
+        cs[x] = tw[x];
+        sn[x] = tw[h + x];
     }
-    red[tid] = re;
-    red[256 + tid] = im;
+    for (int64_t n = threadIdx.x; n < L; n += blockDim.x) {
+        const int64_t r = bitrev(n, bits);
+        re[r] = n < nx ? u[d.lay.v + n + nx * l] - d.vpl : 0.0;
+        im[r] = 0.0;
+    }
     __syncthreads();
-    if (q == 0 && f < nx) {
-        Sh[2 * (l + nxi * f)] = (red[fi] + red[64 + fi]) + (red[128 + fi] + red[192 + fi]);
-        Sh[2 * (l + nxi * f) + 1] = (red[256 + fi] + red[320 + fi]) + (red[384 + fi] + red[448 + fi]);
+    lds_fft(re, im, L, cs, sn, -1.0);
+    for (int64_t f = threadIdx.x; f <= h; f += blockDim.x) {
+        Sh[2 * (l + nxi * f)] = re[f];
+        Sh[2 * (l + nxi * f) + 1] = im[f];
     }
 }
 
-// per-frequency contraction T^[j][f] = sum_l G[f][l][j] S^[f][l]: block f
-__global__ void __launch_bounds__(256) k_fft_mid(const double *__restrict__ G,
-                                                 const double *__restrict__ Sh, int64_t nx,
+// per-frequency contraction T^[j][f] = sum_l G[f][l][j] S^[f][l]: block f,
+// eight l-partials per row so that a thread's loads go out together
+__global__ void __launch_bounds__(512) k_fft_mid(const double *__restrict__ G,
+                                                 const double *__restrict__ Sh, int64_t nbin,
                                                  int64_t nxi, double *__restrict__ Th)
 {
-    __shared__ double red[2][256];
+    __shared__ double red[2][512];
     const int64_t f = blockIdx.x;
     const int tid = threadIdx.x, ji = tid & 63, q = tid >> 6;
     const double *Gf = G + nxi * nxi * f;
@@ -416,8 +510,8 @@ __global__ void __launch_bounds__(256) k_fft_mid(const double *__restrict__ G,
         const int64_t j = jb + ji;
         double re = 0.0, im = 0.0;
         if (j < nxi) {
-#pragma unroll 4
-            for (int64_t l = q; l < nxi; l += 4) {
+#pragma unroll 8
+            for (int64_t l = q; l < nxi; l += 8) {
                 const double g = Gf[j + nxi * l];
                 re += g * Sf[2 * l];
                 im += g * Sf[2 * l + 1];
@@ -426,51 +520,45 @@ __global__ void __launch_bounds__(256) k_fft_mid(const double *__restrict__ G,
         red[0][tid] = re;
         red[1][tid] = im;
         __syncthreads();
-        if (q == 0 && j < nxi) {
-            Th[2 * (f + nx * j)] = (red[0][ji] + red[0][64 + ji]) + (red[0][128 + ji] + red[0][192 + ji]);
-            Th[2 * (f + nx * j) + 1] =
-                (red[1][ji] + red[1][64 + ji]) + (red[1][128 + ji] + red[1][192 + ji]);
+        if (q < 2 && j < nxi) {  // wave 0 the real parts, wave 1 the imaginary parts
+            const double *r = red[q] + ji;
+            Th[2 * (f + nbin * j) + q] = ((r[0] + r[64]) + (r[128] + r[192])) + ((r[256] + r[320]) + (r[384] + r[448]));
         }
         __syncthreads();
     }
 }
 
-// inverse real DFT of T^[j][:], first nx rows, and the rate-and-state
-// update: block (j, 64 rows), four interleaved partial sums over f
+// inverse FFT of the Hermitian spectrum of row j (bins 0..L/2 given), the
+// first nx rows, and the rate-and-state update: block j
 __global__ void __launch_bounds__(256) k_fft_inv(RhsDev d, const double *__restrict__ Th, int64_t nx,
-                                                 const double *__restrict__ tw,
+                                                 int64_t L, int bits, const double *__restrict__ tw,
                                                  const double *__restrict__ u, double *__restrict__ du)
 {
     extern __shared__ double fsm[];
-    const int64_t j = blockIdx.x, M = 2 * nx - 1;
-    double *tr = fsm, *cs = tr + 2 * nx, *sn = cs + M, *red = sn + M;  // red: 256
-    const int tid = threadIdx.x;
-    for (int64_t x = tid; x < 2 * nx; x += 256) tr[x] = Th[2 * nx * j + x];
-    for (int64_t x = tid; x < 2 * M; x += 256) cs[x] = tw[x];
-    __syncthreads();
-    const int ni = tid & 63, q = tid >> 6;
-    const int64_t n = (int64_t)blockIdx.y * 64 + ni;
-    double acc = 0.0;
-    if (n < nx) {
-        int64_t f = 1 + q, k = ((1 + q) * n) % M;
-        const int64_t dk = (4 * n) % M;
-#pragma unroll 8
-        for (; f < nx; f += 4) {
-            acc += tr[2 * f] * cs[k] - tr[2 * f + 1] * sn[k];
-            k += dk;
-            if (k >= M) k -= M;
+    const int64_t j = blockIdx.x, h = L / 2;
+    double *re = fsm, *im = re + L, *cs = im + L, *sn = cs + h;
+    for (int64_t x = threadIdx.x; x < h; x += blockDim.x) {
+        cs[x] = tw[x];
+        sn[x] = tw[h + x];
+    }
+    const double *T = Th + 2 * (h + 1) * j;
+    for (int64_t f = threadIdx.x; f < L; f += blockDim.x) {
+        const int64_t r = bitrev(f, bits);
+        if (f <= h) {
+            re[r] = T[2 * f];
+            im[r] = T[2 * f + 1];
+        } else {  // X[L - f] = conj(X[f])
+            re[r] = T[2 * (L - f)];
+            im[r] = -T[2 * (L - f) + 1];
         }
     }
-    red[tid] = acc;
     __syncthreads();
-    if (q == 0 && n < nx) {
-        const double sum = (red[ni] + red[64 + ni]) + (red[128 + ni] + red[192 + ni]);
-        fault_update(d, n + nx * j, (tr[0] + 2.0 * sum) / (double)M, u, du);
-    }
+    lds_fft(re, im, L, cs, sn, 1.0);
+    for (int64_t n = threadIdx.x; n < nx; n += blockDim.x)
+        fault_update(d, n + nx * j, re[n] / (double)L, u, du);
 }
 
-static inline size_t fft_fwd_lds(int64_t nx) { return (size_t)(nx + 2 * (2 * nx - 1) + 512) * 8; }
-static inline size_t fft_inv_lds(int64_t nx) { return (size_t)(2 * nx + 2 * (2 * nx - 1) + 256) * 8; }
+static inline size_t fft_lds(int64_t nx) { return (size_t)(3 * fft_len(nx)) * 8; }
 
 // LDS bytes of a block with lchunk dip rows
 static inline int64_t toep_lds(const oq_rhs *h, int64_t lc)
@@ -495,13 +583,15 @@ static void toep_shape(const oq_rhs *h, int *lchunk, int *nchunk, int *threads)
 int rhs_enqueue_matvec(oq_rhs *h, const double *u, double *du)
 {
     if (h->kdtype == OQ_K_FFT) {  // no separate prologue: k_fft_fwd forms s from u
-        const unsigned fb = (unsigned)((h->nx + 63) / 64);
-        hipLaunchKernelGGL(k_fft_fwd, dim3((unsigned)h->nxi, fb), dim3(256), fft_fwd_lds(h->nx), h->stream,
-                           h->dev, u, h->nx, h->nxi, h->ftw.as<double>(), h->fsh.as<double>());
-        hipLaunchKernelGGL(k_fft_mid, dim3((unsigned)h->nx), dim3(256), 0, h->stream, h->fg.as<double>(),
-                           h->fsh.as<double>(), h->nx, h->nxi, h->fth.as<double>());
-        hipLaunchKernelGGL(k_fft_inv, dim3((unsigned)h->nxi, fb), dim3(256), fft_inv_lds(h->nx), h->stream,
-                           h->dev, h->fth.as<double>(), h->nx, h->ftw.as<double>(), u, du);
+        const int64_t L = fft_len(h->nx);
+        int bits = 0;
+        while (((int64_t)1 << bits) < L) ++bits;
+        hipLaunchKernelGGL(k_fft_fwd, dim3((unsigned)h->nxi), dim3(256), fft_lds(h->nx), h->stream, h->dev, u,
+                           h->nx, h->nxi, L, bits, h->ftw.as<double>(), h->fsh.as<double>());
+        hipLaunchKernelGGL(k_fft_mid, dim3((unsigned)(L / 2 + 1)), dim3(512), 0, h->stream, h->fg.as<double>(),
+                           h->fsh.as<double>(), L / 2 + 1, h->nxi, h->fth.as<double>());
+        hipLaunchKernelGGL(k_fft_inv, dim3((unsigned)h->nxi), dim3(256), fft_lds(h->nx), h->stream, h->dev,
+                           h->fth.as<double>(), h->nx, L, bits, h->ftw.as<double>(), u, du);
         OQ_HIP(hipGetLastError());
         return OQ_OK;
     }
@@ -691,28 +781,27 @@ static int finalize_toeplitz(oq_rhs *h, DevBuf &st)
     st.p = nullptr;
     st.bytes = 0;
     if (h->kdtype == OQ_K_FFT) {
-        const int64_t nx = h->nx, nxi = h->nxi, M = 2 * nx - 1;
-        std::vector<double> tw((size_t)(2 * M));
-        for (int64_t k = 0; k < M; ++k) {  // the oracle's table: cos, sin of 2 pi k / M
-            tw[(size_t)k] = cos(2.0 * M_PI * (double)k / (double)M);
-            tw[(size_t)(M + k)] = sin(2.0 * M_PI * (double)k / (double)M);
+        const int64_t nx = h->nx, nxi = h->nxi, L = fft_len(nx), hl = L / 2;
+        std::vector<double> tw((size_t)(2 * hl));
+        for (int64_t k = 0; k < hl; ++k) {  // cos, sin of 2 pi k / L
+            tw[(size_t)k] = cos(2.0 * M_PI * (double)k / (double)L);
+            tw[(size_t)(hl + k)] = sin(2.0 * M_PI * (double)k / (double)L);
         }
         int rc = h->ftw.upload(tw.data(), tw.size() * 8, h->stream);
-        if (!rc) rc = h->fg.alloc((size_t)(nx * nxi * nxi) * 8);
-        if (!rc) rc = h->fsh.alloc((size_t)(2 * nx * nxi) * 8);
-        if (!rc) rc = h->fth.alloc((size_t)(2 * nx * nxi) * 8);
+        if (!rc) rc = h->fg.alloc((size_t)((hl + 1) * nxi * nxi) * 8);
+        if (!rc) rc = h->fsh.alloc((size_t)(2 * (hl + 1) * nxi) * 8);
+        if (!rc) rc = h->fth.alloc((size_t)(2 * (hl + 1) * nxi) * 8);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_fft_kernel, dim3(grid_capped(nx * nxi * nxi, 256)), dim3(256), 0, h->stream,
-                           h->K.as<double>(), nx, nxi, h->ftw.as<double>(), h->fg.as<double>());
+        hipLaunchKernelGGL(k_fft_kernel, dim3(grid_capped((hl + 1) * nxi * nxi, 256)), dim3(256), 0, h->stream,
+                           h->K.as<double>(), nx, nxi, L, h->ftw.as<double>(), h->fg.as<double>());
         OQ_HIP(hipGetLastError());
         OQ_HIP(hipFuncSetAttribute((const void *)k_fft_fwd, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)fft_fwd_lds(nx)));
+                                   (int)fft_lds(nx)));
         OQ_HIP(hipFuncSetAttribute((const void *)k_fft_inv, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)fft_inv_lds(nx)));
+                                   (int)fft_lds(nx)));
         OQ_HIP(hipStreamSynchronize(h->stream));
         return OQ_OK;
-    }
-    int lc, nc, nt;
+    }    int lc, nc, nt;
     toep_shape(h, &lc, &nc, &nt);
     return h->tpart.alloc((size_t)(nc * h->nf) * 8);
 }
