@@ -342,9 +342,11 @@ int oq_group_info(oq_group *e, int64_t *ntot, int64_t *nstate, int64_t *kbytes);
  * per wavefront, each with and without a software-pipelined K stream, and
  * LDS-staged kernels with 16/32 rows per wavefront and one or two slices in
  * flight) on the geometries of each size bucket (bucket b: N < 2048 << b;
- * 8 buckets) and keeps the fastest; shape_of_bucket[8] and ms_of_bucket[8]
- * receive the choice (-1 for an empty bucket), ms_table[8*10] (may be NULL)
- * every timing.  Call before
+ * 8 buckets) and keeps the fastest; then times the whole grouped GEMM (one
+ * launch per shape in use) with that per-bucket choice and with each single
+ * shape for all buckets (one launch), and keeps the faster configuration.
+ * shape_of_bucket[8] and ms_of_bucket[8] receive the choice (-1 for an empty
+ * bucket), ms_table[8*10] (may be NULL) every per-bucket timing.  Call before
  * oq_group_start (a running stepper must be started again). */
 int oq_group_autotune(oq_group *e, int iters, int *shape_of_bucket, double *ms_of_bucket,
                       double *ms_table);

@@ -1059,15 +1059,14 @@ int oq_group_autotune(oq_group *e, int iters, int *shape_of_bucket, double *ms_o
     OQ_HIP(hipSetDevice(e->device));
     // S = f(state 0) is irrelevant to the timing; the output goes to scratch
     double *u = e->y.as<double>(), *du = e->k.as<double>();
+    std::vector<double> tab((size_t)(kGBuckets * kGShapes), 0.0);
+    bool used[kGBuckets] = {false};
     for (int b = 0; b < kGBuckets; ++b) {
         std::vector<int> sel;
         for (int g = 0; g < (int)e->G; ++g)
             if (e->geo_bucket[g] == b) sel.push_back(g);
-        if (ms_of_bucket) ms_of_bucket[b] = 0.0;
-        if (shape_of_bucket) shape_of_bucket[b] = -1;
-        if (ms_table)
-            for (int s = 0; s < kGShapes; ++s) ms_table[b * kGShapes + s] = 0.0;
         if (sel.empty()) continue;
+        used[b] = true;
         double best = 1e300;
         int bs = e->bucket_shape[b];
         for (int s = 0; s < kGShapes; ++s) {
@@ -1083,17 +1082,55 @@ int oq_group_autotune(oq_group *e, int iters, int *shape_of_bucket, double *ms_o
             OQ_HIP(hipEventSynchronize(e->ev1));
             float ms = 0.f;
             OQ_HIP(hipEventElapsedTime(&ms, e->ev0, e->ev1));
-            if (ms_table) ms_table[b * kGShapes + s] = ms / iters;
+            tab[(size_t)(b * kGShapes + s)] = ms / iters;
             if (ms / iters < best) {
                 best = ms / iters;
                 bs = s;
             }
         }
         e->bucket_shape[b] = bs;
-        if (shape_of_bucket) shape_of_bucket[b] = bs;
-        if (ms_of_bucket) ms_of_bucket[b] = best;
     }
-    int rc = upload_tiles(e);
+    // Per-bucket timings miss what one launch per shape in use costs (each
+    // launch drains before the next starts, so every bucket pays its own
+    // tail).  The per-bucket choice is therefore compared with each single
+    // shape for all buckets (one launch, largest tiles first) on the whole
+    // set of GEMM launches, and the faster configuration is kept.
+    auto time_all = [&](double *ms_out) -> int {
+        int rc = upload_tiles(e);
+        for (int rep = 0; rep <= iters && !rc; ++rep) {  // rep 0 warms up
+            if (rep == 1) OQ_HIP(hipEventRecord(e->ev0, e->stream));
+            for (int s = 0; s < kGShapes && !rc; ++s)
+                rc = launch_gemm(e, s, e->dtiles[s].as<int2>(), e->ntiles[s], u, du);
+        }
+        if (rc) return rc;
+        OQ_HIP(hipEventRecord(e->ev1, e->stream));
+        OQ_HIP(hipEventSynchronize(e->ev1));
+        float ms = 0.f;
+        OQ_HIP(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+        *ms_out = ms / iters;
+        return OQ_OK;
+    };
+    int mix[kGBuckets];
+    memcpy(mix, e->bucket_shape, sizeof(mix));
+    double best = 0.0;
+    int rc = time_all(&best), pick = -1;
+    for (int s = 0; s < kGShapes && !rc; ++s) {
+        for (int b = 0; b < kGBuckets; ++b) e->bucket_shape[b] = s;
+        double ms = 0.0;
+        rc = time_all(&ms);
+        if (!rc && ms < best) {
+            best = ms;
+            pick = s;
+        }
+    }
+    if (rc) return rc;
+    for (int b = 0; b < kGBuckets; ++b) e->bucket_shape[b] = pick >= 0 ? pick : mix[b];
+    for (int b = 0; b < kGBuckets; ++b) {
+        if (shape_of_bucket) shape_of_bucket[b] = used[b] ? e->bucket_shape[b] : -1;
+        if (ms_of_bucket) ms_of_bucket[b] = used[b] ? tab[(size_t)(b * kGShapes + e->bucket_shape[b])] : 0.0;
+    }
+    if (ms_table) memcpy(ms_table, tab.data(), tab.size() * sizeof(double));
+    rc = upload_tiles(e);
     if (rc) return rc;
     e->stepping = false;  // a captured step holds the old tile tables
     return OQ_OK;

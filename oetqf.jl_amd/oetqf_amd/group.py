@@ -75,8 +75,10 @@ class Group:
         return out
 
     def autotune(self, iters=5):
-        """Times every tile shape per size bucket; returns {bucket: (shape,
-        ms)} and keeps the full table in self.tune_table {bucket: {shape: ms}}."""
+        """Times every tile shape per size bucket, then the whole grouped GEMM
+        with that choice against each single shape for all buckets, and keeps
+        the faster; returns {bucket: (shape, ms)} and keeps the per-bucket
+        table in self.tune_table {bucket: {shape: ms}}."""
         sh = np.zeros(8, dtype=np.int32)
         ms = np.zeros(8)
         tab = np.zeros(8 * len(SHAPES))
