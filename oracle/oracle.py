@@ -12,7 +12,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIBPATH = os.path.join(_HERE, "lib", "liboetqf_oracle.so")
+# OQ_ORACLE_LIB selects another build of the same sources (the sanitizer
+# build, lib/liboetqf_oracle_san.so)
+_LIBPATH = os.environ.get("OQ_ORACLE_LIB") or os.path.join(_HERE, "lib", "liboetqf_oracle.so")
 _lib = None
 
 _d = ctypes.c_double
