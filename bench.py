@@ -500,8 +500,9 @@ def main():
                     help="also run a plain 1 GiB device copy (known traffic) for counter scaling")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(requested --gpus %d)" % args.gpus, file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path; the sharding "
+              "design is row blocks of K with an all-gather of s (requested --gpus %d)" % args.gpus,
+              file=sys.stderr)
         sys.exit(2)
 
     import oetqf_amd as oq
