@@ -94,20 +94,27 @@ __device__ __forceinline__ double wave_sum(double x)
     return readlane_f64(x, 0) + readlane_f64(x, 32);
 }
 
-// Rate-and-state rates of fault patch i (update_fault! with the aging law,
-// equation.jl:237-245 and :279): dv/dt and dtheta/dt from v, theta and the
-// traction rate dtau.
-__device__ __forceinline__ void fault_rates(const RhsDev &d, int64_t i, double v, double th,
-                                            double dtau, double &dv, double &dth)
+// Rate-and-state rates of a fault patch with parameters a, b, L, sigma
+// (update_fault! with the aging law, equation.jl:237-245 and :279): dv/dt and
+// dtheta/dt from v, theta and the traction rate dtau.
+__device__ __forceinline__ void fault_rates_p(const RhsDev &d, double a, double b, double L, double sigma,
+                                              double v, double th, double dtau, double &dv, double &dth)
 {
-    const double a = d.a[i], b = d.b[i], L = d.L[i];
     const double thp = th > 0.0 ? th : 0.0;
     const double psi1 = exp((d.f0 + b * log(d.v0 * thp / L)) / a) / (2.0 * d.v0);
-    const double psi2 = d.sigma[i] * psi1 / hypot(1.0, v * psi1);
+    const double psi2 = sigma * psi1 / hypot(1.0, v * psi1);
     const double dmu_dv = a * psi2;
     const double dmu_dth = b / th * v * psi2;
     dth = 1.0 - v * th / L;
     dv = (dtau - dmu_dth * dth) / (dmu_dv + d.eta);
+}
+
+// The same for patch i, its parameters read from d (the persistent steppers
+// hold them in registers across the RHS evaluations of a launch instead).
+__device__ __forceinline__ void fault_rates(const RhsDev &d, int64_t i, double v, double th,
+                                            double dtau, double &dv, double &dth)
+{
+    fault_rates_p(d, d.a[i], d.b[i], d.L[i], d.sigma[i], v, th, dtau, dv, dth);
 }
 
 // Rates with dilatancy (update_fault_with_dilatancy!, equation.jl:248-276;

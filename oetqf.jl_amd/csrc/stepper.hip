@@ -410,6 +410,39 @@ struct PersistArgs {
     long long max_steps, wait_ticks;
 };
 
+// tau[r] = K row r . s for the owned rows r = wave, wave + 4, ... (K rows
+// and s in LDS), eight rows at a time: the eight sweeps share the loads of s
+// and their reductions overlap.  Rows past Rv repeat row Rv - 1 (in LDS) and
+// are not stored; each lane sums its columns j = lane + 64 t in order, as a
+// row-at-a-time loop would.
+__device__ __forceinline__ void lds_rows(const double *Ks, const double *sv, int64_t ld, int Rv, int wave,
+                                         int lane, double *tau)
+{
+    constexpr int G = 8;
+    for (int r0 = wave; r0 < Rv; r0 += 4 * G) {
+        const double *kr[G];
+#pragma unroll
+        for (int m = 0; m < G; ++m) {
+            const int r = r0 + 4 * m;
+            kr[m] = Ks + (size_t)(r < Rv ? r : Rv - 1) * ld;
+        }
+        double acc[G];
+#pragma unroll
+        for (int m = 0; m < G; ++m) acc[m] = 0.0;
+#pragma unroll 2
+        for (int64_t j = lane; j < ld; j += 64) {
+            const double sj = sv[j];
+#pragma unroll
+            for (int m = 0; m < G; ++m) acc[m] += kr[m][j] * sj;
+        }
+#pragma unroll
+        for (int m = 0; m < G; ++m) {
+            const double t = wave_sum(acc[m]);
+            if (lane == 0 && r0 + 4 * m < Rv) tau[r0 + 4 * m] = t;
+        }
+    }
+}
+
 __device__ __forceinline__ bool grid_barrier(const PersistArgs &a, unsigned &epoch, int *sh_bad)
 {
     __syncthreads();
@@ -467,6 +500,15 @@ __global__ void __launch_bounds__(256) k_persist(PersistArgs a)
         nstep = a.recording ? *a.rec.nstep : 0;
         nsnap = a.recording ? *a.rec.nsnap : 0;
     }
+    // thread r < Rv (R <= 256) keeps patch p0 + r's friction parameters in
+    // registers: no global load in the rates of a stage
+    double fa = 1.0, fb = 0.0, fL = 1.0, fs = 0.0;
+    if (tid < Rv) {
+        fa = a.d.a[p0 + tid];
+        fb = a.d.b[p0 + tid];
+        fL = a.d.L[p0 + tid];
+        fs = a.d.sigma[p0 + tid];
+    }
     __syncthreads();
     unsigned epoch = 0;
     for (long long it = 0; it < a.max_steps; ++it) {
@@ -508,17 +550,12 @@ __global__ void __launch_bounds__(256) k_persist(PersistArgs a)
             if (!grid_barrier(a, epoch, &bad)) return;
             for (int64_t j = tid; j < N; j += 256) sv[j] = sg[j];
             __syncthreads();
-            for (int r = wave; r < Rv; r += 4) {
-                const double *kr = Ks + (size_t)r * ld;
-                double acc = 0.0;
-                for (int64_t j = lane; j < ld; j += 64) acc += kr[j] * sv[j];
-                acc = wave_sum(acc);
-                if (lane == 0) tau[r] = acc;
-            }
+            lds_rows(Ks, sv, ld, Rv, wave, lane, tau);
             __syncthreads();
-            for (int r = tid; r < Rv; r += 256) {  // one patch per thread
+            if (tid < Rv) {  // one patch per thread
+                const int r = tid;
                 double dv, dth;
-                fault_rates(a.d, p0 + r, yo[r], yo[R + r], tau[r], dv, dth);
+                fault_rates_p(a.d, fa, fb, fL, fs, yo[r], yo[R + r], tau[r], dv, dth);
                 ko[st * E + r] = dv;
                 ko[st * E + R + r] = dth;
                 ko[st * E + 2 * R + r] = yo[r];
@@ -666,6 +703,13 @@ __global__ void __launch_bounds__(256) k_persist_ab(PersistArgs a, AbCtrl *actrl
         nstep = a.recording ? *a.rec.nstep : 0;
         nsnap = a.recording ? *a.rec.nsnap : 0;
     }
+    double fa = 1.0, fb = 0.0, fL = 1.0, fs = 0.0;  // patch p0 + tid's parameters (as in k_persist)
+    if (tid < Rv) {
+        fa = a.d.a[p0 + tid];
+        fb = a.d.b[p0 + tid];
+        fL = a.d.L[p0 + tid];
+        fs = a.d.sigma[p0 + tid];
+    }
     __syncthreads();
     unsigned epoch = 0;
     // s <- x[v] - vpl (owned), barrier, s -> LDS, owned rows of K s, rates -> out
@@ -675,17 +719,12 @@ __global__ void __launch_bounds__(256) k_persist_ab(PersistArgs a, AbCtrl *actrl
         if (!grid_barrier(a, epoch, &bad)) return false;
         for (int64_t j = tid; j < N; j += 256) sv[j] = sg[j];
         __syncthreads();
-        for (int r = wave; r < Rv; r += 4) {
-            const double *kr = Ks + (size_t)r * ld;
-            double acc = 0.0;
-            for (int64_t j = lane; j < ld; j += 64) acc += kr[j] * sv[j];
-            acc = wave_sum(acc);
-            if (lane == 0) tau[r] = acc;
-        }
+        lds_rows(Ks, sv, ld, Rv, wave, lane, tau);
         __syncthreads();
-        for (int r = tid; r < Rv; r += 256) {
+        if (tid < Rv) {
+            const int r = tid;
             double dv, dth;
-            fault_rates(a.d, p0 + r, x[r], x[R + r], tau[r], dv, dth);
+            fault_rates_p(a.d, fa, fb, fL, fs, x[r], x[R + r], tau[r], dv, dth);
             out[r] = dv;
             out[R + r] = dth;
             out[2 * R + r] = x[r];
