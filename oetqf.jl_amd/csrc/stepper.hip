@@ -669,10 +669,13 @@ __global__ void __launch_bounds__(256) k_persist(PersistArgs a)
 }
 
 // Persistent VCABM5: as k_persist, with the method of enqueue_step_ab.  A
-// step needs the predicted source vector (one barrier), the error partials
-// (one barrier) and, when the step is accepted, the corrected source vector
-// (one barrier): three grid barriers instead of Tsit5's seven.  Phi_0..Phi_5
-// of the owned entries stay in LDS across the steps of a launch.
+// step needs the predicted source vector (one barrier), then the error
+// partials and the corrected source vector, published together before one
+// more barrier (the corrected vector is used only when the step is
+// accepted): two grid barriers instead of Tsit5's seven.  Buffer 0 (predicted
+// s), buffer 1 (corrected s) and the partials are each rewritten only after
+// a barrier that every workgroup crosses once it has read them.
+// Phi_0..Phi_5 of the owned entries stay in LDS across the steps of a launch.
 __global__ void __launch_bounds__(256) k_persist_ab(PersistArgs a, AbCtrl *actrl, double *phig)
 {
     extern __shared__ double lds[];
@@ -713,10 +716,15 @@ __global__ void __launch_bounds__(256) k_persist_ab(PersistArgs a, AbCtrl *actrl
     __syncthreads();
     unsigned epoch = 0;
     // s <- x[v] - vpl (owned), barrier, s -> LDS, owned rows of K s, rates -> out
-    auto rhs = [&](const double *x, double *out, int buf) -> bool {
+    // s <- x[v] - vpl for the owned patches, into source buffer buf
+    auto publish = [&](const double *x, int buf) {
         double *sg = a.s2 + buf * N;
         for (int i = tid; i < Rv; i += 256) sg[p0 + i] = x[i] - a.d.vpl;
-        if (!grid_barrier(a, epoch, &bad)) return false;
+    };
+    // after a barrier that follows every workgroup's publish: s -> LDS, the
+    // owned rows of K s, rates -> out
+    auto consume = [&](const double *x, double *out, int buf) {
+        const double *sg = a.s2 + buf * N;
         for (int64_t j = tid; j < N; j += 256) sv[j] = sg[j];
         __syncthreads();
         lds_rows(Ks, sv, ld, Rv, wave, lane, tau);
@@ -730,7 +738,6 @@ __global__ void __launch_bounds__(256) k_persist_ab(PersistArgs a, AbCtrl *actrl
             out[2 * R + r] = x[r];
         }
         __syncthreads();
-        return true;
     };
     for (long long it = 0; it < a.max_steps; ++it) {
         if (tid == 0) {  // k_ab_begin
@@ -769,7 +776,9 @@ __global__ void __launch_bounds__(256) k_persist_ab(PersistArgs a, AbCtrl *actrl
             po[e] = uo[e] + h * acc;
         }
         __syncthreads();
-        if (!rhs(po, fpo, 0)) return;
+        publish(po, 0);
+        if (!grid_barrier(a, epoch, &bad)) return;
+        consume(po, fpo, 0);
         double acc2 = 0.0, vm = 0.0;  // corrector, error and max |v| partials
         for (int e = tid; e < E; e += 256) {
             if (e % R >= Rv) continue;  // padding: yo stays zero
@@ -796,6 +805,10 @@ __global__ void __launch_bounds__(256) k_persist_ab(PersistArgs a, AbCtrl *actrl
             a.part[2 * blockIdx.x] = red[0];
             a.part[2 * blockIdx.x + 1] = red2[0];
         }
+        // the corrected source vector crosses the same barrier as the error
+        // partials: an accepted step needs no third barrier (a rejected one
+        // leaves buffer 1 unread)
+        publish(yo, 1);
         if (!grid_barrier(a, epoch, &bad)) return;
         for (int w = tid; w < 2 * a.nwg; w += 256) red[w] = a.part[w];
         __syncthreads();
@@ -838,7 +851,7 @@ __global__ void __launch_bounds__(256) k_persist_ab(PersistArgs a, AbCtrl *actrl
         }
         __syncthreads();
         if (ac.c.accept) {
-            if (!rhs(yo, fo, 1)) return;
+            consume(yo, fo, 1);
             for (int e = tid; e < E; e += 256) {  // k_ab_commit
                 double tmp = fo[e];
                 for (int j = 0; j < k; ++j) {
