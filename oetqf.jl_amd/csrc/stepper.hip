@@ -391,9 +391,12 @@ __global__ void __launch_bounds__(256) k_ab_commit(const AbCtrl *__restrict__ a,
 // source vector s = v - vpl, published through device memory and one grid
 // barrier (two buffers by stage parity, so a workgroup that is ahead never
 // overwrites a vector another is still reading); the error norm and max|v|
-// partials need one more.  Every workgroup then runs the same controller on
-// the same sums in the same order, so all copies of t, dt and the step
-// counters stay identical without further communication.  Barrier waits
+// partials need one more.  The next step's first stage rides on that last
+// barrier (both candidates for its source vector are published with the
+// partials), so a step after the first of a launch costs six barriers.
+// Every workgroup then runs the same controller on the same sums in the same
+// order, so all copies of t, dt and the step counters stay identical without
+// further communication.  Barrier waits
 // are bounded by a wall-clock limit: a workgroup that times out raises the
 // abort flag and every workgroup leaves at its next barrier.
 struct PersistArgs {
@@ -401,7 +404,7 @@ struct PersistArgs {
     const double *K;   // FP64 operator, N x ld
     Ctrl *ctrl;
     double *u, *k1;    // packed state and the FSAL stage
-    double *s2;        // 2 x N source vectors
+    double *s2;        // 2 x N source vectors, then (Tsit5) 4 x N stage-1 candidates
     double *part;      // 2 x nwg: squared-error sums, max |v|
     unsigned *bar;     // barrier arrivals (zero at launch)
     int *abort;
@@ -511,6 +514,8 @@ __global__ void __launch_bounds__(256) k_persist(PersistArgs a)
     }
     __syncthreads();
     unsigned epoch = 0;
+    bool s1ready = false;  // stage 1's source vector crossed the last barrier
+    int s1acc = 0;
     for (long long it = 0; it < a.max_steps; ++it) {
         if (tid == 0) {  // k_step_begin
             c.accept = 0;
@@ -545,10 +550,18 @@ __global__ void __launch_bounds__(256) k_persist(PersistArgs a)
                 yo[e] = uo[e] + dt * acc;
             }
             __syncthreads();
-            double *sg = a.s2 + (st & 1) * N;
-            for (int i = tid; i < Rv; i += 256) sg[p0 + i] = yo[i] - a.d.vpl;
-            if (!grid_barrier(a, epoch, &bad)) return;
-            for (int64_t j = tid; j < N; j += 256) sv[j] = sg[j];
+            if (st == 1 && s1ready) {
+                // the previous step published both candidates for this stage's
+                // source vector with its error partials: s = (U + dt B) - vpl
+                // with the same operations as yo above
+                const double *U = a.s2 + (s1acc ? 2 : 4) * N, *B = U + N;
+                for (int64_t j = tid; j < N; j += 256) sv[j] = (U[j] + dt * B[j]) - a.d.vpl;
+            } else {
+                double *sg = a.s2 + (st & 1) * N;
+                for (int i = tid; i < Rv; i += 256) sg[p0 + i] = yo[i] - a.d.vpl;
+                if (!grid_barrier(a, epoch, &bad)) return;
+                for (int64_t j = tid; j < N; j += 256) sv[j] = sg[j];
+            }
             __syncthreads();
             lds_rows(Ks, sv, ld, Rv, wave, lane, tau);
             __syncthreads();
@@ -586,6 +599,23 @@ __global__ void __launch_bounds__(256) k_persist(PersistArgs a)
         if (tid == 0) {
             a.part[2 * blockIdx.x] = red[0];
             a.part[2 * blockIdx.x + 1] = red2[0];
+        }
+        // Stage 1 of the next attempt needs y = u + dt a21 k1 with (u, k1) =
+        // (y_{n+1}, k7) after an acceptance or (u_n, k1) after a rejection,
+        // and dt from the controller.  Both (u, a21 k1) candidates of the
+        // owned v entries cross this barrier with the partials, so the next
+        // step's first stage needs no barrier of its own.
+        {
+            double *C = a.s2 + 2 * N;  // [U_acc | B_acc | U_rej | B_rej]
+            for (int i = tid; i < Rv; i += 256) {
+                double b6 = 0.0, b1 = 0.0;
+                b6 += kTsA[1][0] * ko[6 * E + i];
+                b1 += kTsA[1][0] * ko[i];
+                C[p0 + i] = yo[i];
+                C[N + p0 + i] = b6;
+                C[2 * N + p0 + i] = uo[i];
+                C[3 * N + p0 + i] = b1;
+            }
         }
         if (!grid_barrier(a, epoch, &bad)) return;
         for (int w = tid; w < 2 * a.nwg; w += 256) red[w] = a.part[w];  // all loads in flight at once
@@ -635,6 +665,8 @@ __global__ void __launch_bounds__(256) k_persist(PersistArgs a)
             }
         }
         __syncthreads();
+        s1acc = c.accept;  // which candidate the next stage 1 takes
+        s1ready = true;
         if (c.accept) {  // k_commit (FSAL: k1 <- k7)
             for (int e = tid; e < E; e += 256) {
                 uo[e] = yo[e];
@@ -1225,7 +1257,8 @@ int oq_stepper_set_persistent(oq_stepper *st, int on)
     OQ_REQUIRE(per_cu >= 1 && (int64_t)per_cu * cus >= nwg,
                "the persistent stepper needs more co-resident workgroups than the device holds");
     OQ_REQUIRE(nwg <= 128, "the persistent stepper takes at most 128 workgroups (N <= 1024)");
-    int rc = st->p_s2.alloc((size_t)(2 * h->N) * 8);
+    // two source buffers and, for Tsit5, the four stage-1 candidate vectors
+    int rc = st->p_s2.alloc((size_t)(6 * h->N) * 8);
     if (!rc) rc = st->p_part.alloc((size_t)(2 * nwg) * 8);
     if (!rc) rc = st->p_bar.alloc(sizeof(unsigned));
     if (!rc) rc = st->p_abort.alloc(sizeof(int));
