@@ -413,36 +413,40 @@ struct PersistArgs {
     long long max_steps, wait_ticks;
 };
 
-// tau[r] = K row r . s for the owned rows r = wave, wave + 4, ... (K rows
-// and s in LDS), eight rows at a time: the eight sweeps share the loads of s
-// and their reductions overlap.  Rows past Rv repeat row Rv - 1 (in LDS) and
-// are not stored; each lane sums its columns j = lane + 64 t in order, as a
-// row-at-a-time loop would.
-__device__ __forceinline__ void lds_rows(const double *Ks, const double *sv, int64_t ld, int Rv, int wave,
-                                         int lane, double *tau)
+// tau[r] = K row r . s for the owned rows, K rows in LDS at a pitch of
+// ldp = ld + 2 doubles, s in LDS.  Eight lanes share a row: lane q of the
+// group sums the column pairs 16k + 2q, 16k + 2q + 1 (one 16-byte LDS read of
+// K and one of s per pair; the eight lanes of a row read 128 contiguous
+// bytes and the row groups of a wavefront read the same s) into two
+// accumulators, then three DPP steps sum the group.  32 rows per pass of the
+// 256 threads; rows past Rv repeat row Rv - 1 and are not stored.
+__device__ __forceinline__ void lds_rows(const double *Ks, const double *sv, int64_t ld, int64_t ldp, int Rv,
+                                         int tid, double *tau)
 {
-    constexpr int G = 8;
-    for (int r0 = wave; r0 < Rv; r0 += 4 * G) {
-        const double *kr[G];
+    const int q = tid & 7;
+    for (int r0 = 0; r0 < Rv; r0 += 32) {
+        const int r = r0 + (tid >> 3);
+        const double *kr = Ks + (size_t)(r < Rv ? r : Rv - 1) * ldp + 2 * q;
+        const double *sp = sv + 2 * q;
+        double a0 = 0.0, a1 = 0.0;
+        for (int64_t c = 0; c < ld; c += 64) {  // ld is a multiple of 512: four pairs per pass, loads first
+            double2 k2[4], s2[4];
 #pragma unroll
-        for (int m = 0; m < G; ++m) {
-            const int r = r0 + 4 * m;
-            kr[m] = Ks + (size_t)(r < Rv ? r : Rv - 1) * ld;
+            for (int u = 0; u < 4; ++u) {
+                k2[u] = *reinterpret_cast<const double2 *>(kr + c + 16 * u);
+                s2[u] = *reinterpret_cast<const double2 *>(sp + c + 16 * u);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                a0 += k2[u].x * s2[u].x;
+                a1 += k2[u].y * s2[u].y;
+            }
         }
-        double acc[G];
-#pragma unroll
-        for (int m = 0; m < G; ++m) acc[m] = 0.0;
-#pragma unroll 2
-        for (int64_t j = lane; j < ld; j += 64) {
-            const double sj = sv[j];
-#pragma unroll
-            for (int m = 0; m < G; ++m) acc[m] += kr[m][j] * sj;
-        }
-#pragma unroll
-        for (int m = 0; m < G; ++m) {
-            const double t = wave_sum(acc[m]);
-            if (lane == 0 && r0 + 4 * m < Rv) tau[r0 + 4 * m] = t;
-        }
+        double x = a0 + a1;
+        x += dpp_f64<0xB1>(x);   // quad_perm [1,0,3,2]
+        x += dpp_f64<0x4E>(x);   // quad_perm [2,3,0,1]
+        x += dpp_f64<0x141>(x);  // row_half_mirror: pairs the two quads of the 8 lanes
+        if (q == 0 && r < Rv) tau[r] = x;
     }
 }
 
@@ -476,22 +480,23 @@ __device__ __forceinline__ bool grid_barrier(const PersistArgs &a, unsigned &epo
 
 __global__ void __launch_bounds__(256) k_persist(PersistArgs a)
 {
-    extern __shared__ double lds[];
+    extern __shared__ __attribute__((aligned(16))) double lds[];  // 16-byte LDS reads of K and s
     __shared__ Ctrl c;
     __shared__ double red[256], red2[256], tau[64];
     __shared__ int bad, take;
     __shared__ long long nstep, nsnap;
-    const int R = a.R, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int R = a.R, tid = threadIdx.x;
     const int64_t N = a.d.N, ld = a.d.ld, n = a.d.lay.n;
     const int64_t p0 = (int64_t)blockIdx.x * R;
     const int Rv = (int)((N - p0) < R ? (N - p0) : R);  // owned patches (>= 1)
     const int E = 3 * R;                                 // owned entries: [v | th | del]
-    double *Ks = lds;                                    // R x ld
-    double *sv = Ks + (size_t)R * ld;                    // ld
+    const int64_t ldp = ld + 2;                          // K row pitch in LDS
+    double *Ks = lds;                                    // R x ldp
+    double *sv = Ks + (size_t)R * ldp;                   // ld
     double *uo = sv + ld, *yo = uo + E, *ko = yo + E;    // E, E, 7 x E
     const int64_t comp_off[3] = {a.d.lay.v, a.d.lay.th, a.d.lay.del};
 
-    for (int64_t x = tid; x < (int64_t)Rv * ld; x += 256) Ks[x] = a.K[p0 * ld + x];
+    for (int64_t x = tid; x < (int64_t)Rv * ld; x += 256) Ks[(x / ld) * ldp + x % ld] = a.K[p0 * ld + x];
     for (int64_t j = tid; j < ld; j += 256) sv[j] = 0.0;
     for (int e = tid; e < E; e += 256) {
         const int cpt = e / R, i = e % R;
@@ -563,7 +568,7 @@ __global__ void __launch_bounds__(256) k_persist(PersistArgs a)
                 for (int64_t j = tid; j < N; j += 256) sv[j] = sg[j];
             }
             __syncthreads();
-            lds_rows(Ks, sv, ld, Rv, wave, lane, tau);
+            lds_rows(Ks, sv, ld, ldp, Rv, tid, tau);
             __syncthreads();
             if (tid < Rv) {  // one patch per thread
                 const int r = tid;
@@ -710,21 +715,22 @@ __global__ void __launch_bounds__(256) k_persist(PersistArgs a)
 // Phi_0..Phi_5 of the owned entries stay in LDS across the steps of a launch.
 __global__ void __launch_bounds__(256) k_persist_ab(PersistArgs a, AbCtrl *actrl, double *phig)
 {
-    extern __shared__ double lds[];
+    extern __shared__ __attribute__((aligned(16))) double lds[];  // 16-byte LDS reads of K and s
     __shared__ AbCtrl ac;
     __shared__ double red[256], red2[256], tau[64];
     __shared__ int bad, take;
     __shared__ long long nstep, nsnap;
-    const int R = a.R, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int R = a.R, tid = threadIdx.x;
     const int64_t N = a.d.N, ld = a.d.ld, n = a.d.lay.n;
     const int64_t p0 = (int64_t)blockIdx.x * R;
     const int Rv = (int)((N - p0) < R ? (N - p0) : R);
     const int E = 3 * R;
-    double *Ks = lds, *sv = Ks + (size_t)R * ld;
+    const int64_t ldp = ld + 2;  // K row pitch in LDS
+    double *Ks = lds, *sv = Ks + (size_t)R * ldp;
     double *uo = sv + ld, *po = uo + E, *fpo = po + E, *yo = fpo + E, *fo = yo + E, *ph = fo + E;  // ph: 6E
     const int64_t comp_off[3] = {a.d.lay.v, a.d.lay.th, a.d.lay.del};
 
-    for (int64_t x = tid; x < (int64_t)Rv * ld; x += 256) Ks[x] = a.K[p0 * ld + x];
+    for (int64_t x = tid; x < (int64_t)Rv * ld; x += 256) Ks[(x / ld) * ldp + x % ld] = a.K[p0 * ld + x];
     for (int64_t j = tid; j < ld; j += 256) sv[j] = 0.0;
     for (int e = tid; e < E; e += 256) {
         const int cpt = e / R, i = e % R;
@@ -759,7 +765,7 @@ __global__ void __launch_bounds__(256) k_persist_ab(PersistArgs a, AbCtrl *actrl
         const double *sg = a.s2 + buf * N;
         for (int64_t j = tid; j < N; j += 256) sv[j] = sg[j];
         __syncthreads();
-        lds_rows(Ks, sv, ld, Rv, wave, lane, tau);
+        lds_rows(Ks, sv, ld, ldp, Rv, tid, tau);
         __syncthreads();
         if (tid < Rv) {
             const int r = tid;
@@ -1248,7 +1254,7 @@ int oq_stepper_set_persistent(oq_stepper *st, int on)
     // per owned entry: Tsit5 keeps u, the stage state and seven stages; VCABM5
     // keeps u, p, f(p), y_{n+1}, f(y_{n+1}) and Phi_0..Phi_5
     const size_t per_entry = st->alg == 1 ? 11 : 9;
-    const size_t lds = ((size_t)R * h->ld + h->ld + per_entry * 3 * R) * 8;
+    const size_t lds = ((size_t)R * (h->ld + 2) + h->ld + per_entry * 3 * R) * 8;  // K rows at pitch ld + 2
     OQ_REQUIRE((int64_t)lds <= (int64_t)maxlds, "the persistent stepper does not fit in LDS");
     const void *kfn = st->alg == 1 ? (const void *)k_persist_ab : (const void *)k_persist;
     OQ_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
