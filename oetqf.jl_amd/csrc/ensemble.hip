@@ -559,7 +559,7 @@ __global__ void __launch_bounds__(256) k_ens_stage(EnsCtrl c, int s, int64_t nM,
 }
 
 // per-(row chunk, member) partial sums of the squared scaled error
-static constexpr int kEnsChunks = 64;
+static constexpr int kEnsChunks = 512;
 __global__ void __launch_bounds__(256) k_ens_err(EnsCtrl c, int64_t n, int64_t M,
                                                  const double *__restrict__ u,
                                                  const double *__restrict__ y,
