@@ -47,7 +47,7 @@ namespace oq {
 static constexpr int kGM = 16;       // member slots per geometry (MFMA N)
 static constexpr int kGStep = 64;    // columns per wavefront pass (4 lane groups x 16)
 static constexpr int kGWaves = 4;    // wavefronts per workgroup
-static constexpr int kGChunks = 64;  // row chunks of the error norm
+static constexpr int kGChunks = 512;  // row chunks of the error norm (one workgroup row each)
 static constexpr int kGBuckets = 8;  // size buckets: N < 2048, < 4096, ... (log2)
 
 struct GrpGeo {
