@@ -495,6 +495,68 @@ __global__ void __launch_bounds__(256) k_fft_fwd(RhsDev d, const double *__restr
     }
 }
 
+// A Tsit5 stage fused with k_fft_fwd (fault-only state [v th del], graph
+// stepper): block l forms y = u + dt sum_{j<s} a_j k_j for the 3 nx entries
+// of dip row l, with the operations of the stepper's stage kernel, stores y
+// for the inverse pass and the rest of the step, and transforms s = y_v -
+// vpl.  *done set: y is left as it is (the stage kernel's skip) and s is
+// formed from it.
+__global__ void __launch_bounds__(256) k_fft_fwd_stage(RhsDev d, StageCoef sc, const double *__restrict__ dtp,
+                                                       const int *__restrict__ done, const double *__restrict__ u,
+                                                       const double *__restrict__ k, int64_t n,
+                                                       double *__restrict__ y, int64_t nx, int64_t nxi, int64_t L,
+                                                       int bits, const double *__restrict__ tw,
+                                                       double *__restrict__ Sh)
+{
+    extern __shared__ double fsm[];
+    const int64_t l = blockIdx.x, h = L / 2;
+    double *re = fsm, *im = re + L, *cs = im + L, *sn = cs + h, *yv = sn + h;  // yv: nx
+    for (int64_t x = threadIdx.x; x < h; x += blockDim.x) {
+        cs[x] = tw[x];
+        sn[x] = tw[h + x];
+    }
+    const bool skip = *done != 0;
+    const int64_t off[3] = {d.lay.v, d.lay.th, d.lay.del};
+    const double dt = *dtp;
+    for (int64_t x = threadIdx.x; x < nx; x += blockDim.x) {
+        // the three components of patch x: every load issued before the sums
+        // (k_j past the stage index s re-reads k_{s-1} and is not added)
+        double uv[3], kv[3][6];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int64_t i = off[c] + x + nx * l;
+            uv[c] = skip ? y[i] : u[i];
+#pragma unroll
+            for (int j = 0; j < 6; ++j) kv[c][j] = k[(j < sc.s ? j : sc.s - 1) * n + i];
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double yi = uv[c];
+            if (!skip) {
+                double acc = 0.0;
+#pragma unroll
+                for (int j = 0; j < 6; ++j)
+                    if (j < sc.s) acc += sc.a[j] * kv[c][j];
+                yi = uv[c] + dt * acc;
+                y[off[c] + x + nx * l] = yi;
+            }
+            if (c == 0) yv[x] = yi;
+        }
+    }
+    __syncthreads();
+    for (int64_t x = threadIdx.x; x < L; x += blockDim.x) {
+        const int64_t r = bitrev(x, bits);
+        re[r] = x < nx ? yv[x] - d.vpl : 0.0;
+        im[r] = 0.0;
+    }
+    __syncthreads();
+    lds_fft(re, im, L, cs, sn, -1.0);
+    for (int64_t f = threadIdx.x; f <= h; f += blockDim.x) {
+        Sh[2 * (l + nxi * f)] = re[f];
+        Sh[2 * (l + nxi * f) + 1] = im[f];
+    }
+}
+
 // per-frequency contraction T^[j][f] = sum_l G[f][l][j] S^[f][l]: block f,
 // eight l-partials per row so that a thread's loads go out together
 __global__ void __launch_bounds__(512) k_fft_mid(const double *__restrict__ G,
@@ -611,6 +673,23 @@ int rhs_enqueue_matvec(oq_rhs *h, const double *u, double *du)
     case OQ_K_FP32: launch_matvec<OQ_K_FP32>(h, h->s64.as<double>(), u, du); break;
     default: launch_matvec<OQ_K_BF16>(h, reinterpret_cast<const double *>(h->s32.p), u, du); break;
     }
+    OQ_HIP(hipGetLastError());
+    return OQ_OK;
+}
+
+int rhs_enqueue_fft_stage(oq_rhs *h, const StageCoef &sc, const double *dtp, const int *done, const double *u,
+                          const double *k, int64_t n, double *y, double *du)
+{
+    const int64_t L = fft_len(h->nx);
+    int bits = 0;
+    while (((int64_t)1 << bits) < L) ++bits;
+    hipLaunchKernelGGL(k_fft_fwd_stage, dim3((unsigned)h->nxi), dim3(256), fft_lds(h->nx) + (size_t)h->nx * 8,
+                       h->stream, h->dev, sc, dtp, done, u, k, n, y, h->nx, h->nxi, L, bits, h->ftw.as<double>(),
+                       h->fsh.as<double>());
+    hipLaunchKernelGGL(k_fft_mid, dim3((unsigned)(L / 2 + 1)), dim3(512), 0, h->stream, h->fg.as<double>(),
+                       h->fsh.as<double>(), L / 2 + 1, h->nxi, h->fth.as<double>());
+    hipLaunchKernelGGL(k_fft_inv, dim3((unsigned)h->nxi), dim3(256), fft_lds(h->nx), h->stream, h->dev,
+                       h->fth.as<double>(), h->nx, L, bits, h->ftw.as<double>(), y, du);
     OQ_HIP(hipGetLastError());
     return OQ_OK;
 }
@@ -799,6 +878,8 @@ static int finalize_toeplitz(oq_rhs *h, DevBuf &st)
                                    (int)fft_lds(nx)));
         OQ_HIP(hipFuncSetAttribute((const void *)k_fft_inv, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)fft_lds(nx)));
+        OQ_HIP(hipFuncSetAttribute((const void *)k_fft_fwd_stage, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(fft_lds(nx) + (size_t)nx * 8)));
         OQ_HIP(hipStreamSynchronize(h->stream));
         return OQ_OK;
     }    int lc, nc, nt;

@@ -180,4 +180,14 @@ namespace oq {
 int rhs_enqueue(oq_rhs *h, const double *u, double *du);
 // Enqueues only the K-streaming kernel (requires a prior prologue).
 int rhs_enqueue_matvec(oq_rhs *h, const double *u, double *du);
+// Stage coefficients of an explicit Runge-Kutta stage: y = u + dt sum_{j<s} a[j] k_j.
+struct StageCoef {
+    int s;
+    double a[6];
+};
+// DFT operator (OQ_K_FFT), fault-only without dilatancy: a Tsit5 stage and
+// the RHS at it in three launches (the stage fused into the forward FFT);
+// dtp and done point into the stepper's control block.
+int rhs_enqueue_fft_stage(oq_rhs *h, const StageCoef &sc, const double *dtp, const int *done, const double *u,
+                          const double *k, int64_t n, double *y, double *du);
 }  // namespace oq

@@ -21,16 +21,18 @@ namespace oq {
 
 // Tsitouras (2011) tableau, b = a[6][:] (FSAL), btilde = b - bhat.
 __constant__ double kTsC[7] = {0.0, 0.161, 0.327, 0.9, 0.9800255409045097, 1.0, 1.0};
-__constant__ double kTsA[7][6] = {
-    {0, 0, 0, 0, 0, 0},
-    {0.161, 0, 0, 0, 0, 0},
-    {-0.008480655492356989, 0.335480655492357, 0, 0, 0, 0},
-    {2.897153057105493, -6.359448489975075, 4.3622954328695815, 0, 0, 0},
-    {5.325864828439257, -11.748883564062828, 7.4955393428898365, -0.09249506636175525, 0, 0},
-    {5.86145544294642, -12.92096931784711, 8.159367898576159, -0.071584973281401,
-     -0.028269050394068383, 0},
-    {0.09646076681806523, 0.01, 0.4798896504144996, 1.379008574103742, -3.290069515436081,
-     2.324710524099774}};
+#define OQ_TSIT5_A                                                                                   \
+    {{0, 0, 0, 0, 0, 0},                                                                             \
+     {0.161, 0, 0, 0, 0, 0},                                                                         \
+     {-0.008480655492356989, 0.335480655492357, 0, 0, 0, 0},                                         \
+     {2.897153057105493, -6.359448489975075, 4.3622954328695815, 0, 0, 0},                           \
+     {5.325864828439257, -11.748883564062828, 7.4955393428898365, -0.09249506636175525, 0, 0},       \
+     {5.86145544294642, -12.92096931784711, 8.159367898576159, -0.071584973281401,                   \
+      -0.028269050394068383, 0},                                                                     \
+     {0.09646076681806523, 0.01, 0.4798896504144996, 1.379008574103742, -3.290069515436081,          \
+      2.324710524099774}}
+__constant__ double kTsA[7][6] = OQ_TSIT5_A;
+static const double kTsAHost[7][6] = OQ_TSIT5_A;  // the same literals, for stage arguments
 __constant__ double kTsBt[7] = {-0.00178001105222577714, -0.0008164344596567469,
                                 0.007880878010261995, -0.1447110071732629,
                                 0.5823571654525552, -0.45808210592918697,
@@ -990,7 +992,17 @@ static int enqueue_step(oq_stepper *st)
     double *u = st->u.as<double>(), *y = st->y.as<double>(), *k = st->k.as<double>();
     int rc;
     hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, h->stream, c);
+    // the DFT operator (fault only, no dilatancy) forms each stage inside
+    // its forward-FFT kernel: one launch less per stage
+    const bool fused = h->kdtype == OQ_K_FFT && !h->dev.dila;
     for (int s = 1; s < 7; ++s) {
+        if (fused) {
+            StageCoef sc{};
+            sc.s = s;
+            for (int j = 0; j < 6; ++j) sc.a[j] = kTsAHost[s][j];
+            if ((rc = rhs_enqueue_fft_stage(h, sc, &c->dtcur, &c->done, u, k, n, y, k + s * n))) return rc;
+            continue;
+        }
         hipLaunchKernelGGL(k_stage, dim3(nb(n)), dim3(256), 0, h->stream, c, s, n, u, k, y);
         // the RHS kernels have no done-check of their own: evaluating one
         // more RHS on a finished state is harmless and keeps them graph-safe
