@@ -268,10 +268,12 @@ int oq_stepper_create_vcabm5(oq_rhs *h, const double *u0, double t0, double t1,
 /* Persistent mode (no counterpart in the reference; the graph stepper's
  * semantics are kept): for fault-only problems with an FP64 operator whose
  * rows fit in LDS of at most 128 workgroups (N <= 1024), oq_stepper_run executes all of
- * its steps in one cooperative launch -- rows of K resident in LDS, one grid
- * barrier per RHS and one for the error norm -- instead of replaying the
- * per-step graph.  Tsit5 and VCABM5 steppers both qualify.  on = 0 returns
- * to the graph.  OQ_ERR_ARG if the problem does not qualify. */
+ * its steps in one cooperative launch -- rows of K resident in LDS, grid
+ * barriers between the RHS evaluations (Tsit5: 6 per step, the next step's
+ * first stage riding on the error-norm barrier; VCABM5: 2 per step) --
+ * instead of replaying the per-step graph.  Tsit5 and VCABM5 steppers both
+ * qualify.  on = 0 returns to the graph.  OQ_ERR_ARG if the problem does not
+ * qualify. */
 int oq_stepper_set_persistent(oq_stepper *s, int on);
 
 /* ---- ensembles of frictional-parameter realizations sharing one
