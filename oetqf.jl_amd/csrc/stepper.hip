@@ -253,6 +253,35 @@ __device__ void ab_coeffs(int k, double h, const double *hist, double *g, double
     }
 }
 
+// ab_coeffs on the lanes of one wavefront: lane q (1 <= q <= k + 1) holds
+// c_q through the j recursion (each update reads c_{q+1} before its own
+// update, so the lanes of a j step are independent) while lane 0 runs the
+// beta chain; the same operations in the same order per value, so the same
+// coefficients, with five dependent division steps instead of twenty-five.
+__device__ void ab_coeffs_wave(int k, double h, const double *hist, double *g, double *beta)
+{
+    const int lane = threadIdx.x & 63;
+    double pn[kAbK + 1], po[kAbK + 1];
+    pn[0] = h;
+    po[0] = hist[0];
+    for (int i = 1; i <= k; ++i) {
+        pn[i] = pn[i - 1] + hist[i - 1];
+        po[i] = po[i - 1] + hist[i];
+    }
+    double cq = (lane >= 1 && lane <= k + 1) ? 1.0 / lane : 0.0, b = 1.0;
+    if (lane == 0) beta[0] = 1.0;
+    if (lane == 1) g[0] = cq;
+    for (int j = 1; j <= k; ++j) {
+        const double cn = __shfl_down(cq, 1);  // c_{q+1} before this step's update
+        if (lane >= 1 && lane <= k + 1 - j) cq = cq - cn * h / pn[j - 1];
+        if (lane == 0 && j < k) {
+            b = b * pn[j - 1] / po[j - 1];
+            beta[j] = b;
+        }
+        if (lane == 1) g[j] = cq;
+    }
+}
+
 __global__ void k_ab_begin(AbCtrl *a)
 {
     Ctrl *c = &a->c;
@@ -801,13 +830,14 @@ __global__ void __launch_bounds__(256) k_persist_ab(PersistArgs a, AbCtrl *actrl
                         c.dtcur = dt;
                         c.iter += 1;
                         ac.kcur = ac.k;
-                        ab_coeffs(ac.k, dt, ac.hist, ac.g, ac.beta);
                     }
                 }
             }
         }
         __syncthreads();
         if (ac.c.done) break;
+        if (tid < 64) ab_coeffs_wave(ac.kcur, ac.c.dtcur, ac.hist, ac.g, ac.beta);
+        __syncthreads();
         const int k = ac.kcur;
         const double h = ac.c.dtcur;
         for (int e = tid; e < E; e += 256) {  // predictor
