@@ -445,39 +445,49 @@ struct PersistArgs {
 };
 
 // tau[r] = K row r . s for the owned rows, K rows in LDS at a pitch of
-// ldp = ld + 2 doubles, s in LDS.  Eight lanes share a row: lane q of the
-// group sums the column pairs 16k + 2q, 16k + 2q + 1 (one 16-byte LDS read of
-// K and one of s per pair; the eight lanes of a row read 128 contiguous
-// bytes and the row groups of a wavefront read the same s) into two
-// accumulators, then three DPP steps sum the group.  32 rows per pass of the
-// 256 threads; rows past Rv repeat row Rv - 1 and are not stored.
+// ldp = ld + 2 doubles, s in LDS.  Sixteen lanes share two rows (r and
+// r + 16 of a 32-row block): lane q sums the column pairs 32k + 2q,
+// 32k + 2q + 1 of both rows with one 16-byte LDS read of s and one of each
+// row per pair (the sixteen lanes of a row read 256 contiguous bytes), two
+// accumulators per row, and four DPP steps sum the sixteen lanes.  Rows
+// past Rv repeat row Rv - 1 and are not stored.
 __device__ __forceinline__ void lds_rows(const double *Ks, const double *sv, int64_t ld, int64_t ldp, int Rv,
                                          int tid, double *tau)
 {
-    const int q = tid & 7;
+    const int q = tid & 15, rp = tid >> 4;
     for (int r0 = 0; r0 < Rv; r0 += 32) {
-        const int r = r0 + (tid >> 3);
-        const double *kr = Ks + (size_t)(r < Rv ? r : Rv - 1) * ldp + 2 * q;
+        const int ra = r0 + rp, rb = ra + 16;
+        const double *ka = Ks + (size_t)(ra < Rv ? ra : Rv - 1) * ldp + 2 * q;
+        const double *kb = Ks + (size_t)(rb < Rv ? rb : Rv - 1) * ldp + 2 * q;
         const double *sp = sv + 2 * q;
-        double a0 = 0.0, a1 = 0.0;
-        for (int64_t c = 0; c < ld; c += 64) {  // ld is a multiple of 512: four pairs per pass, loads first
-            double2 k2[4], s2[4];
+        double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
+        for (int64_t c = 0; c < ld; c += 128) {  // ld is a multiple of 512: four pairs per pass, loads first
+            double2 s2[4], x2[4], y2[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                k2[u] = *reinterpret_cast<const double2 *>(kr + c + 16 * u);
-                s2[u] = *reinterpret_cast<const double2 *>(sp + c + 16 * u);
+                s2[u] = *reinterpret_cast<const double2 *>(sp + c + 32 * u);
+                x2[u] = *reinterpret_cast<const double2 *>(ka + c + 32 * u);
+                y2[u] = *reinterpret_cast<const double2 *>(kb + c + 32 * u);
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                a0 += k2[u].x * s2[u].x;
-                a1 += k2[u].y * s2[u].y;
+                a0 += x2[u].x * s2[u].x;
+                a1 += x2[u].y * s2[u].y;
+                b0 += y2[u].x * s2[u].x;
+                b1 += y2[u].y * s2[u].y;
             }
         }
-        double x = a0 + a1;
-        x += dpp_f64<0xB1>(x);   // quad_perm [1,0,3,2]
-        x += dpp_f64<0x4E>(x);   // quad_perm [2,3,0,1]
-        x += dpp_f64<0x141>(x);  // row_half_mirror: pairs the two quads of the 8 lanes
-        if (q == 0 && r < Rv) tau[r] = x;
+        double xa = a0 + a1, xb = b0 + b1;
+        xa += dpp_f64<0xB1>(xa);   // quad_perm [1,0,3,2]
+        xb += dpp_f64<0xB1>(xb);
+        xa += dpp_f64<0x4E>(xa);   // quad_perm [2,3,0,1]
+        xb += dpp_f64<0x4E>(xb);
+        xa += dpp_f64<0x141>(xa);  // row_half_mirror: the two quads of each 8 lanes
+        xb += dpp_f64<0x141>(xb);
+        xa += dpp_f64<0x140>(xa);  // row_mirror: the two 8-lane halves of the 16
+        xb += dpp_f64<0x140>(xb);
+        if (q == 0 && ra < Rv) tau[ra] = xa;
+        if (q == 0 && rb < Rv) tau[rb] = xb;
     }
 }
 
