@@ -519,6 +519,17 @@ __device__ __forceinline__ bool grid_barrier(const PersistArgs &a, unsigned &epo
     return *sh_bad == 0;
 }
 
+// Maximum over the 64 lanes of a wavefront (the steps of wave_sum with fmax).
+__device__ __forceinline__ double wave_max(double x)
+{
+    x = fmax(x, dpp_f64<0xB1>(x));
+    x = fmax(x, dpp_f64<0x4E>(x));
+    x = fmax(x, dpp_f64<0x141>(x));
+    x = fmax(x, dpp_f64<0x140>(x));
+    x = fmax(x, swizzle_xor16_f64(x));
+    return fmax(readlane_f64(x, 0), readlane_f64(x, 32));
+}
+
 __global__ void __launch_bounds__(256) k_persist(PersistArgs a)
 {
     extern __shared__ __attribute__((aligned(16))) double lds[];  // 16-byte LDS reads of K and s
@@ -632,19 +643,17 @@ __global__ void __launch_bounds__(256) k_persist(PersistArgs a)
             acc += w * w;
             if (e < R) vm = fmax(vm, fabs(yo[e]));
         }
-        red[tid] = acc;
-        red2[tid] = vm;
-        __syncthreads();
-        for (int o = 128; o > 0; o >>= 1) {
-            if (tid < o) {
-                red[tid] += red[tid + o];
-                red2[tid] = fmax(red2[tid], red2[tid + o]);
-            }
-            __syncthreads();
+        // wavefront sums and maxima, then the four wavefronts' in fixed order
+        acc = wave_sum(acc);
+        vm = wave_max(vm);
+        if ((tid & 63) == 0) {
+            red[tid >> 6] = acc;
+            red2[tid >> 6] = vm;
         }
+        __syncthreads();
         if (tid == 0) {
-            a.part[2 * blockIdx.x] = red[0];
-            a.part[2 * blockIdx.x + 1] = red2[0];
+            a.part[2 * blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+            a.part[2 * blockIdx.x + 1] = fmax(fmax(red2[0], red2[1]), fmax(red2[2], red2[3]));
         }
         // Stage 1 of the next attempt needs y = u + dt a21 k1 with (u, k1) =
         // (y_{n+1}, k7) after an acceptance or (u_n, k1) after a rejection,
@@ -666,7 +675,7 @@ __global__ void __launch_bounds__(256) k_persist(PersistArgs a)
         if (!grid_barrier(a, epoch, &bad)) return;
         for (int w = tid; w < 2 * a.nwg; w += 256) red[w] = a.part[w];  // all loads in flight at once
         __syncthreads();
-        if (tid == 0) {  // k_controller, on the same sums in every workgroup
+        if (tid < 64) {  // k_controller, on the same sums in every workgroup
             double sum = 0.0, vmax = 0.0;
             for (int w = 0; w < a.nwg; ++w) {
                 sum += red[2 * w];
@@ -675,38 +684,43 @@ __global__ void __launch_bounds__(256) k_persist(PersistArgs a)
             const double beta1 = 7.0 / 50.0, beta2 = 2.0 / 25.0, qmin = 0.2, qmax = 10.0,
                          gamma = 0.9, qoldinit = 1e-4;
             const double eest = sqrt(sum / (double)n);
-            double q11, q;
-            c.eest = eest;
-            if (eest == 0.0) {
-                q11 = 0.0;
-                q = 1.0 / qmax;
-            } else {
-                q11 = pow(eest, beta1);
-                q = q11 / pow(c.qold, beta2);
-                q = fmax(1.0 / qmax, fmin(1.0 / qmin, q / gamma));
-            }
-            take = 0;
-            if (eest <= 1.0) {
-                c.t = (c.t1 - (c.t + c.dtcur) <= 0.0) ? c.t1 : c.t + c.dtcur;
-                c.qold = fmax(eest, qoldinit);
-                c.dt = c.dtcur / q;
-                c.naccept += 1;
-                c.accept = 1;
-                if (a.recording) {  // k_record
-                    if (blockIdx.x == 0 && a.rec.step_cap > 0) {
-                        a.rec.step_t[nstep % a.rec.step_cap] = c.t;
-                        a.rec.step_vmax[nstep % a.rec.step_cap] = vmax;
-                    }
-                    nstep += 1;
-                    take = (a.rec.snap_cap > 0) && (c.naccept % a.rec.stride == 0);
-                }
-            } else {
-                c.nreject += 1;
-                if (!isfinite(eest)) {
-                    c.done = 1;
-                    c.retcode = OQ_RET_NONFINITE;
+            // eest^beta1 on lane 0 and qold^beta2 on lane 1, side by side
+            const double pw = pow(tid == 0 ? eest : c.qold, tid == 0 ? beta1 : beta2);
+            const double pw0 = readlane_f64(pw, 0), pw1 = readlane_f64(pw, 1);
+            if (tid == 0) {
+                double q11, q;
+                c.eest = eest;
+                if (eest == 0.0) {
+                    q11 = 0.0;
+                    q = 1.0 / qmax;
                 } else {
-                    c.dt = c.dtcur / fmin(1.0 / qmin, q11 / gamma);
+                    q11 = pw0;
+                    q = q11 / pw1;
+                    q = fmax(1.0 / qmax, fmin(1.0 / qmin, q / gamma));
+                }
+                take = 0;
+                if (eest <= 1.0) {
+                    c.t = (c.t1 - (c.t + c.dtcur) <= 0.0) ? c.t1 : c.t + c.dtcur;
+                    c.qold = fmax(eest, qoldinit);
+                    c.dt = c.dtcur / q;
+                    c.naccept += 1;
+                    c.accept = 1;
+                    if (a.recording) {  // k_record
+                        if (blockIdx.x == 0 && a.rec.step_cap > 0) {
+                            a.rec.step_t[nstep % a.rec.step_cap] = c.t;
+                            a.rec.step_vmax[nstep % a.rec.step_cap] = vmax;
+                        }
+                        nstep += 1;
+                        take = (a.rec.snap_cap > 0) && (c.naccept % a.rec.stride == 0);
+                    }
+                } else {
+                    c.nreject += 1;
+                    if (!isfinite(eest)) {
+                        c.done = 1;
+                        c.retcode = OQ_RET_NONFINITE;
+                    } else {
+                        c.dt = c.dtcur / fmin(1.0 / qmin, q11 / gamma);
+                    }
                 }
             }
         }
