@@ -7,14 +7,14 @@ The earthquake cycle of this fault is chaotic after its first two events
 summation order of the shear-stress sums follow different event sequences
 from ~80 yr on.  A single 300-year catalogue therefore cannot be matched
 event by event; its statistics can.  This script records, for Tsit5 and
-VCABM5 and for seeds 0..7 (seed 0 = the unperturbed initial state; seed s >
+VCABM5 and for seeds 0..23 (seed 0 = the unperturbed initial state; seed s >
 0 multiplies theta_0 by 1 + 1e-9 N(0, 1) drawn from default_rng(s)), the
 event count, the onsets, the peak slip rates and the step counts of the
 oracle's integration over the oracle's own operator.  The device runs the
 same seeded states and is compared with this spread.
 
 Events: max slip rate above 1e-3 m/s, crossings within 1 year merged.
-Run from the repository root (about 10 minutes on 8 cores):
+Run from the repository root (about 30 minutes on 8 cores):
 
     python tests/golden/make_c0_spread.py
 """
@@ -26,7 +26,7 @@ import time
 
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 HERE = os.path.dirname(os.path.abspath(__file__))
-SEEDS = range(8)
+SEEDS = range(24)
 ALGS = ("tsit5", "vcabm5")
 PERTURB = 1e-9
 MERGE_GAP = 365 * 86400.0
