@@ -7,7 +7,7 @@ device runs the same seeded initial states with the persistent steppers and
 must (i) reproduce, for each seed, the events that precede the divergence
 (the first three Tsit5 onsets, the first VCABM5 onset, within 1e-4
 relative), and (ii) fall inside the oracle's distribution: every event count
-within the oracle's range over its eight seeds widened by one, and the mean
+within the oracle's range over its 24 seeds widened by one, and the mean
 recurrence over the device seeds within three standard errors of the
 oracle's mean."""
 import json
