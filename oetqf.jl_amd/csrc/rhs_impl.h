@@ -96,17 +96,31 @@ __device__ __forceinline__ double wave_sum(double x)
 
 // Rate-and-state rates of a fault patch with parameters a, b, L, sigma
 // (update_fault! with the aging law, equation.jl:237-245 and :279): dv/dt and
-// dtheta/dt from v, theta and the traction rate dtau.
-__device__ __forceinline__ void fault_rates_p(const RhsDev &d, double a, double b, double L, double sigma,
-                                              double v, double th, double dtau, double &dv, double &dth)
+// dtheta/dt from v, theta and the traction rate dtau.  fault_pre holds every
+// term that does not involve dtau (the persistent steppers evaluate it while
+// the traction rate is still being gathered), fault_dv the rest.
+__device__ __forceinline__ void fault_pre(const RhsDev &d, double a, double b, double L, double sigma, double v,
+                                          double th, double &dmu_dv, double &dmu_dth, double &dth)
 {
     const double thp = th > 0.0 ? th : 0.0;
     const double psi1 = exp((d.f0 + b * log(d.v0 * thp / L)) / a) / (2.0 * d.v0);
     const double psi2 = sigma * psi1 / hypot(1.0, v * psi1);
-    const double dmu_dv = a * psi2;
-    const double dmu_dth = b / th * v * psi2;
+    dmu_dv = a * psi2;
+    dmu_dth = b / th * v * psi2;
     dth = 1.0 - v * th / L;
-    dv = (dtau - dmu_dth * dth) / (dmu_dv + d.eta);
+}
+
+__device__ __forceinline__ double fault_dv(const RhsDev &d, double dtau, double dmu_dv, double dmu_dth, double dth)
+{
+    return (dtau - dmu_dth * dth) / (dmu_dv + d.eta);
+}
+
+__device__ __forceinline__ void fault_rates_p(const RhsDev &d, double a, double b, double L, double sigma,
+                                              double v, double th, double dtau, double &dv, double &dth)
+{
+    double dmu_dv, dmu_dth;
+    fault_pre(d, a, b, L, sigma, v, th, dmu_dv, dmu_dth, dth);
+    dv = fault_dv(d, dtau, dmu_dv, dmu_dth, dth);
 }
 
 // The same for patch i, its parameters read from d (the persistent steppers

@@ -491,7 +491,8 @@ __device__ __forceinline__ void lds_rows(const double *Ks, const double *sv, int
     }
 }
 
-__device__ __forceinline__ bool grid_barrier(const PersistArgs &a, unsigned &epoch, int *sh_bad)
+template <class F>
+__device__ __forceinline__ bool grid_barrier(const PersistArgs &a, unsigned &epoch, int *sh_bad, F side)
 {
     __syncthreads();
     ++epoch;
@@ -514,9 +515,16 @@ __device__ __forceinline__ bool grid_barrier(const PersistArgs &a, unsigned &epo
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         *sh_bad = bad;
+    } else if (threadIdx.x >= 64) {
+        side();  // wavefronts 1..3, while thread 0 waits
     }
     __syncthreads();
     return *sh_bad == 0;
+}
+
+__device__ __forceinline__ bool grid_barrier(const PersistArgs &a, unsigned &epoch, int *sh_bad)
+{
+    return grid_barrier(a, epoch, sh_bad, [] {});
 }
 
 // Maximum over the 64 lanes of a wavefront (the steps of wave_sum with fmax).
@@ -534,7 +542,7 @@ __global__ void __launch_bounds__(256) k_persist(PersistArgs a)
 {
     extern __shared__ __attribute__((aligned(16))) double lds[];  // 16-byte LDS reads of K and s
     __shared__ Ctrl c;
-    __shared__ double red[256], red2[256], tau[64];
+    __shared__ double red[256], red2[256], tau[64], pdv[64], pdt[64], pth[64];
     __shared__ int bad, take;
     __shared__ long long nstep, nsnap;
     const int R = a.R, tid = threadIdx.x;
@@ -560,15 +568,21 @@ __global__ void __launch_bounds__(256) k_persist(PersistArgs a)
         nstep = a.recording ? *a.rec.nstep : 0;
         nsnap = a.recording ? *a.rec.nsnap : 0;
     }
-    // thread r < Rv (R <= 256) keeps patch p0 + r's friction parameters in
-    // registers: no global load in the rates of a stage
+    // thread 64 + r (r < Rv <= 64) keeps patch p0 + r's friction parameters
+    // in registers and evaluates its dtau-free friction terms (fault_pre)
+    // while thread 0 waits at the grid barrier; after the LDS sweep only
+    // fault_dv remains
     double fa = 1.0, fb = 0.0, fL = 1.0, fs = 0.0;
-    if (tid < Rv) {
-        fa = a.d.a[p0 + tid];
-        fb = a.d.b[p0 + tid];
-        fL = a.d.L[p0 + tid];
-        fs = a.d.sigma[p0 + tid];
+    const int pr = tid - 64;
+    if (pr >= 0 && pr < Rv) {
+        fa = a.d.a[p0 + pr];
+        fb = a.d.b[p0 + pr];
+        fL = a.d.L[p0 + pr];
+        fs = a.d.sigma[p0 + pr];
     }
+    auto pre = [&]() {
+        if (pr >= 0 && pr < Rv) fault_pre(a.d, fa, fb, fL, fs, yo[pr], yo[R + pr], pdv[pr], pdt[pr], pth[pr]);
+    };
     __syncthreads();
     unsigned epoch = 0;
     bool s1ready = false;  // stage 1's source vector crossed the last barrier
@@ -613,10 +627,11 @@ __global__ void __launch_bounds__(256) k_persist(PersistArgs a)
                 // with the same operations as yo above
                 const double *U = a.s2 + (s1acc ? 2 : 4) * N, *B = U + N;
                 for (int64_t j = tid; j < N; j += 256) sv[j] = (U[j] + dt * B[j]) - a.d.vpl;
+                pre();
             } else {
                 double *sg = a.s2 + (st & 1) * N;
                 for (int i = tid; i < Rv; i += 256) sg[p0 + i] = yo[i] - a.d.vpl;
-                if (!grid_barrier(a, epoch, &bad)) return;
+                if (!grid_barrier(a, epoch, &bad, pre)) return;
                 for (int64_t j = tid; j < N; j += 256) sv[j] = sg[j];
             }
             __syncthreads();
@@ -624,10 +639,8 @@ __global__ void __launch_bounds__(256) k_persist(PersistArgs a)
             __syncthreads();
             if (tid < Rv) {  // one patch per thread
                 const int r = tid;
-                double dv, dth;
-                fault_rates_p(a.d, fa, fb, fL, fs, yo[r], yo[R + r], tau[r], dv, dth);
-                ko[st * E + r] = dv;
-                ko[st * E + R + r] = dth;
+                ko[st * E + r] = fault_dv(a.d, tau[r], pdv[r], pdt[r], pth[r]);
+                ko[st * E + R + r] = pth[r];
                 ko[st * E + 2 * R + r] = yo[r];
             }
             __syncthreads();
@@ -772,7 +785,7 @@ __global__ void __launch_bounds__(256) k_persist_ab(PersistArgs a, AbCtrl *actrl
 {
     extern __shared__ __attribute__((aligned(16))) double lds[];  // 16-byte LDS reads of K and s
     __shared__ AbCtrl ac;
-    __shared__ double red[256], red2[256], tau[64];
+    __shared__ double red[256], red2[256], tau[64], pdv[64], pdt[64], pth[64];
     __shared__ int bad, take;
     __shared__ long long nstep, nsnap;
     const int R = a.R, tid = threadIdx.x;
@@ -799,16 +812,23 @@ __global__ void __launch_bounds__(256) k_persist_ab(PersistArgs a, AbCtrl *actrl
         nstep = a.recording ? *a.rec.nstep : 0;
         nsnap = a.recording ? *a.rec.nsnap : 0;
     }
-    double fa = 1.0, fb = 0.0, fL = 1.0, fs = 0.0;  // patch p0 + tid's parameters (as in k_persist)
-    if (tid < Rv) {
-        fa = a.d.a[p0 + tid];
-        fb = a.d.b[p0 + tid];
-        fL = a.d.L[p0 + tid];
-        fs = a.d.sigma[p0 + tid];
+    double fa = 1.0, fb = 0.0, fL = 1.0, fs = 0.0;  // patch p0 + tid - 64's parameters (as in k_persist)
+    const int pr = tid - 64;
+    if (pr >= 0 && pr < Rv) {
+        fa = a.d.a[p0 + pr];
+        fb = a.d.b[p0 + pr];
+        fL = a.d.L[p0 + pr];
+        fs = a.d.sigma[p0 + pr];
     }
     __syncthreads();
     unsigned epoch = 0;
     // s <- x[v] - vpl (owned), barrier, s -> LDS, owned rows of K s, rates -> out
+    // dtau-free friction terms of state x, on wavefronts 1..3 during a barrier
+    auto pre_of = [&](const double *x) {
+        return [=]() {
+            if (pr >= 0 && pr < Rv) fault_pre(a.d, fa, fb, fL, fs, x[pr], x[R + pr], pdv[pr], pdt[pr], pth[pr]);
+        };
+    };
     // s <- x[v] - vpl for the owned patches, into source buffer buf
     auto publish = [&](const double *x, int buf) {
         double *sg = a.s2 + buf * N;
@@ -822,12 +842,10 @@ __global__ void __launch_bounds__(256) k_persist_ab(PersistArgs a, AbCtrl *actrl
         __syncthreads();
         lds_rows(Ks, sv, ld, ldp, Rv, tid, tau);
         __syncthreads();
-        if (tid < Rv) {
+        if (tid < Rv) {  // the dtau-free terms of x came with the preceding barrier
             const int r = tid;
-            double dv, dth;
-            fault_rates_p(a.d, fa, fb, fL, fs, x[r], x[R + r], tau[r], dv, dth);
-            out[r] = dv;
-            out[R + r] = dth;
+            out[r] = fault_dv(a.d, tau[r], pdv[r], pdt[r], pth[r]);
+            out[R + r] = pth[r];
             out[2 * R + r] = x[r];
         }
         __syncthreads();
@@ -871,7 +889,7 @@ __global__ void __launch_bounds__(256) k_persist_ab(PersistArgs a, AbCtrl *actrl
         }
         __syncthreads();
         publish(po, 0);
-        if (!grid_barrier(a, epoch, &bad)) return;
+        if (!grid_barrier(a, epoch, &bad, pre_of(po))) return;
         consume(po, fpo, 0);
         double acc2 = 0.0, vm = 0.0;  // corrector, error and max |v| partials
         for (int e = tid; e < E; e += 256) {
@@ -903,7 +921,7 @@ __global__ void __launch_bounds__(256) k_persist_ab(PersistArgs a, AbCtrl *actrl
         // partials: an accepted step needs no third barrier (a rejected one
         // leaves buffer 1 unread)
         publish(yo, 1);
-        if (!grid_barrier(a, epoch, &bad)) return;
+        if (!grid_barrier(a, epoch, &bad, pre_of(yo))) return;
         for (int w = tid; w < 2 * a.nwg; w += 256) red[w] = a.part[w];
         __syncthreads();
         if (tid == 0) {  // k_ab_controller, on the same sums in every workgroup
