@@ -88,13 +88,27 @@ __device__ __forceinline__ int seg_find(const int64_t *__restrict__ off, int G, 
     return lo;
 }
 
+// The same for every thread of a block with x = this thread's index: one
+// binary search per block (for the block's first index), then a forward walk
+// (a block spans at most a few segments); called by all threads of the block
+// before any of them returns.
+__device__ __forceinline__ int seg_find_block(const int64_t *__restrict__ off, int G, int64_t x)
+{
+    __shared__ int g0;
+    if (threadIdx.x == 0) g0 = seg_find(off, G, (int64_t)blockIdx.x * blockDim.x);
+    __syncthreads();
+    int g = g0;
+    while (g + 1 < G && off[g + 1] <= x) ++g;
+    return g;
+}
+
 // ---------------- prologue: S_g = [v - vpl ; eps_dot - deps0], member-major ----------------
 __global__ void __launch_bounds__(256) k_grp_prologue(GrpDev d, const double *__restrict__ u,
                                                       double *__restrict__ du)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int g = seg_find_block(d.poffs, d.G, t);
     if (t >= d.nprolog) return;
-    const int g = seg_find(d.poffs, d.G, t);
     const GrpGeo &G = d.geo[g];
     const int64_t M = d.M, loc = t - d.poffs[g];
     const int64_t i = loc / M, m = loc % M, gm = (int64_t)g * M + m;
@@ -168,8 +182,8 @@ __global__ void __launch_bounds__(256) k_grp_finish(GrpDev d, const double *__re
                                                     double *__restrict__ du)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int g = seg_find_block(d.yoffs, d.G, t);
     if (t >= d.nrows) return;
-    const int g = seg_find(d.yoffs, d.G, t);
     const GrpGeo &G = d.geo[g];
     const int64_t M = d.M, loc = t - G.Yoff;
     const int64_t row = loc / M, m = loc % M, gm = (int64_t)g * M + m;
@@ -580,9 +594,10 @@ __global__ void k_grp_begin(GrpCtrl c)
     c.iter[q] += 1;
 }
 
+// member slot (g M + m) of state entry i; all threads of the block call it
 __device__ __forceinline__ int64_t grp_member(const GrpCtrl &c, int64_t i)
 {
-    const int g = seg_find(c.soffs, c.G, i);
+    const int g = seg_find_block(c.soffs, c.G, i);
     return (int64_t)g * c.M + (i - c.soffs[g]) % c.M;
 }
 
@@ -592,8 +607,9 @@ __global__ void __launch_bounds__(256) k_grp_stage(GrpCtrl c, int s, int64_t nto
                                                    double *__restrict__ y)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t q = grp_member(c, i);
     if (i >= ntot) return;
-    const double dt = c.dtcur[grp_member(c, i)];
+    const double dt = c.dtcur[q];
     double acc = 0.0;
     for (int j = 0; j < s; ++j) acc += kGA[s][j] * k[j * ntot + i];
     y[i] = u[i] + dt * acc;
@@ -667,7 +683,8 @@ __global__ void __launch_bounds__(256) k_grp_commit(GrpCtrl c, int64_t ntot,
                                                     double *__restrict__ k)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ntot || !c.accept[grp_member(c, i)]) return;
+    const int64_t q = grp_member(c, i);
+    if (i >= ntot || !c.accept[q]) return;
     u[i] = y[i];
     k[i] = k[6 * ntot + i];
 }
