@@ -903,19 +903,16 @@ __global__ void __launch_bounds__(256) k_persist_ab(PersistArgs a, AbCtrl *actrl
             acc2 += w * w;
             if (e < R) vm = fmax(vm, fabs(yn));
         }
-        red[tid] = acc2;
-        red2[tid] = vm;
-        __syncthreads();
-        for (int o = 128; o > 0; o >>= 1) {
-            if (tid < o) {
-                red[tid] += red[tid + o];
-                red2[tid] = fmax(red2[tid], red2[tid + o]);
-            }
-            __syncthreads();
+        acc2 = wave_sum(acc2);  // wavefront sums and maxima, as in k_persist
+        vm = wave_max(vm);
+        if ((tid & 63) == 0) {
+            red[tid >> 6] = acc2;
+            red2[tid >> 6] = vm;
         }
+        __syncthreads();
         if (tid == 0) {
-            a.part[2 * blockIdx.x] = red[0];
-            a.part[2 * blockIdx.x + 1] = red2[0];
+            a.part[2 * blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+            a.part[2 * blockIdx.x + 1] = fmax(fmax(red2[0], red2[1]), fmax(red2[2], red2[3]));
         }
         // the corrected source vector crosses the same barrier as the error
         // partials: an accepted step needs no third barrier (a rejected one
