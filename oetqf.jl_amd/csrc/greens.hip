@@ -274,11 +274,12 @@ __global__ void __launch_bounds__(128) k_gf_mm(DevHex8 h, const double *qc, cons
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ne * ne) return;
     const int64_t i = t % ne, j = t / ne;
-    double acc[6][6];
+    // The 36 sums live in LDS (a column per thread): in registers they would
+    // be live through the image quadrature and push it into scratch spills.
+    __shared__ double acc[36][128];
+    const int c = threadIdx.x;
 #pragma unroll
-    for (int p = 0; p < 6; ++p)
-#pragma unroll
-        for (int k = 0; k < 6; ++k) acc[p][k] = 0.0;
+    for (int m = 0; m < 36; ++m) acc[m][c] = 0.0;
     for (int q = 0; q < nq; ++q) {
         const double rx = h.cx[j] + qc[3 * q] * h.dx[j] / 2.0;
         const double ry = h.cy[j] + qc[3 * q + 1] * h.dy[j] / 2.0;
@@ -289,12 +290,12 @@ __global__ void __launch_bounds__(128) k_gf_mm(DevHex8 h, const double *qc, cons
             double s[6];
             hex8_stress_unit(P, p, mu, nu, s);
 #pragma unroll
-            for (int k = 0; k < 6; ++k) acc[p][k] += s[k] * qw[q];
+            for (int k = 0; k < 6; ++k) acc[p * 6 + k][c] += s[k] * qw[q];
         }
     }
     for (int p = 0; p < 6; ++p)
 #pragma unroll
-        for (int k = 0; k < 6; ++k) out[(k * ne + j) * rs + (p * ne + i) * cs] = acc[p][k];
+        for (int k = 0; k < 6; ++k) out[(k * ne + j) * rs + (p * ne + i) * cs] = acc[p * 6 + k][c];
 }
 
 // Dense fault-fault block: K[(i,j),(k,l)] = st[|i-k|, j, l] (the expansion

@@ -235,6 +235,11 @@ __device__ void hex8_pair(double x, double y, double z, double qx, double qy, do
     const double lo[3] = {qx - T / 2.0, qy, qz - W};
     const double hi[3] = {qx + T / 2.0, qy + L, qz};
     const double rv[3] = {x, y, z};
+    // The image integral first: its quadrature loop then runs without the
+    // Kelvin terms live, which keeps it free of register spills.
+#pragma unroll
+    for (int m = 0; m < 54; ++m) P.D[m] = 0.0;
+    h8_image_integral(x, y, z, lo, hi, nu, P.D);
 #pragma unroll
     for (int i = 0; i < 6; ++i) P.K.phi[i] = 0.0;
 #pragma unroll
@@ -253,9 +258,6 @@ __device__ void hex8_pair(double x, double y, double z, double qx, double qy, do
         }
         h8_kelvin_add(P.K, X, sg);
     }
-#pragma unroll
-    for (int m = 0; m < 54; ++m) P.D[m] = 0.0;
-    h8_image_integral(x, y, z, lo, hi, nu, P.D);
     P.inside = (x >= lo[0] && x < hi[0] && y >= lo[1] && y < hi[1] && z >= lo[2] && z < hi[2]);
 }
 
