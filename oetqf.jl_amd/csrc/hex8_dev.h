@@ -236,7 +236,7 @@ __device__ void hex8_pair(double x, double y, double z, double qx, double qy, do
     const double hi[3] = {qx + T / 2.0, qy + L, qz};
     const double rv[3] = {x, y, z};
     // The image integral first: its quadrature loop then runs without the
-    // Kelvin terms live, which keeps it free of register spills.
+    // 24 Kelvin sums live, which takes most of its register spills away.
 #pragma unroll
     for (int m = 0; m < 54; ++m) P.D[m] = 0.0;
     h8_image_integral(x, y, z, lo, hi, nu, P.D);
