@@ -43,6 +43,20 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo) and its logical CPU count."""
+    name = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    name = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return name, os.cpu_count()
+
+
 def cpu_baseline(w, u0, sample_seconds):
     """The reference algorithm (equation.jl:185-205: Toeplitz DFT K_ff and
     dense FP64 mantle products, OpenMP) timed on the host with the oracle."""
@@ -66,7 +80,9 @@ def cpu_baseline(w, u0, sample_seconds):
         if el >= sample_seconds:
             break
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    model, ncpu = cpu_model()
     return dict(value=n / el, unit="RHS evals/s", cores=cores, kind="port",
+                cpu_model=model, host_logical_cpus=ncpu,
                 sample="%d RHS evaluations of the same workload (%.1f s)" % (n, el)), out
 
 
@@ -475,6 +491,164 @@ def group_leg(oq, args):
     return out
 
 
+# ---------------- full simulated spans with event statistics ----------------
+
+def _catalogue(t, vmax):
+    from oetqf_amd.cycle import events, recurrence
+    from oetqf_amd.workloads import YEAR
+    ev = events(t, vmax, 1e-3, merge_gap=YEAR)
+    rec = recurrence(ev) / YEAR
+    return dict(events=len(ev), onsets_yr=[e["onset"] / YEAR for e in ev],
+                peak_v=[e["peak_v"] for e in ev], recurrence_yr=[float(x) for x in rec],
+                mean_recurrence_yr=float(rec.mean()) if len(rec) else None)
+
+
+def _compare(cat, ref):
+    """Event statistics of a run against the FP64 reference run: the same
+    event count, and the recurrence intervals' relative error (interval by
+    interval where the counts agree, and of the mean)."""
+    out = dict(same_event_count=cat["events"] == ref["events"])
+    if ref["mean_recurrence_yr"] and cat["mean_recurrence_yr"]:
+        out["mean_recurrence_rel_err"] = abs(cat["mean_recurrence_yr"] - ref["mean_recurrence_yr"]) / \
+            ref["mean_recurrence_yr"]
+    if out["same_event_count"] and cat["recurrence_yr"]:
+        a, b = np.array(cat["recurrence_yr"]), np.array(ref["recurrence_yr"])
+        out["max_interval_rel_err"] = float(np.max(np.abs(a - b) / b))
+        on_a, on_b = np.array(cat["onsets_yr"]), np.array(ref["onsets_yr"])
+        out["max_onset_rel_err"] = float(np.max(np.abs(on_a - on_b) / on_b))
+    return out
+
+
+def _progress(tag):
+    last = [time.perf_counter()]
+    from oetqf_amd.workloads import YEAR
+
+    def f(t):
+        if t is not None and time.perf_counter() - last[0] > 20:
+            last[0] = time.perf_counter()
+            log("%s: t = %.1f yr" % (tag, t / YEAR))
+    return f
+
+
+def spans_composite(oq):
+    """The composite workload (128x32 transform fault + 16x16x8 mantle) over
+    its whole 500-year span from the seeded interseismic start: FP64 with
+    Tsit5 and VCABM5, FP32 with VCABM5; event catalogues (max v > 1e-3 m/s,
+    crossings within 1 yr merged) against the FP64 run of the same method."""
+    from oetqf_amd.cycle import run_cycle
+    from oetqf_amd.workloads import YEAR, transform_fault_composite
+
+    w = transform_fault_composite()
+    out = dict(workload=w["name"], sim_years=(w["tspan"][1] - w["tspan"][0]) / YEAR,
+               reltol=w["reltol"], abstol=w["abstol"], runs={})
+    for kd, alg in (("fp64", "tsit5"), ("fp64", "vcabm5"), ("fp32", "vcabm5"), ("fp32", "tsit5")):
+        p = oq.assemble_from_mesh(w["mf"], w["ma"], w["lam"], w["mu"], w["pf"], w["pa"], w["u0"],
+                                  w["tspan"], nrept=w["nrept"], buffer_ratio=w["buffer_ratio"], kdtype=kd)
+        tag = "%s %s" % (kd, alg)
+        log("composite 500 yr: %s" % tag)
+        r = run_cycle(p, reltol=w["reltol"], abstol=w["abstol"], chunk=4096,
+                      alg=oq.VCABM5() if alg == "vcabm5" else oq.Tsit5(), progress=_progress(tag))
+        p.f.close()
+        cat = _catalogue(r["t"], r["vmax"])
+        res = dict(wall_s=r["wall"], sim_yr_per_hr=out["sim_years"] / (r["wall"] / 3600.0),
+                   accepted=int(r["naccept"]), rejected=int(r["nreject"]), retcode=r["retcode"], **cat)
+        if kd != "fp64" and ("fp64 " + alg) in out["runs"]:
+            res["vs_fp64"] = _compare(cat, out["runs"]["fp64 " + alg])
+            res["speedup_vs_fp64"] = out["runs"]["fp64 " + alg]["wall_s"] / r["wall"]
+        out["runs"][tag] = res
+        log("%s: %d events in %.0f s (%.0f yr/hr), mean recurrence %s yr%s" % (
+            tag, cat["events"], r["wall"], res["sim_yr_per_hr"], cat["mean_recurrence_yr"],
+            ", vs FP64 %s" % res["vs_fp64"] if "vs_fp64" in res else ""))
+    return out
+
+
+def spans_ensemble(oq, kdtypes, members_fp64):
+    """256 members on the composite geometry over 300 years with the lockstep
+    Tsit5 (per-member steps), per operator type; the sampled members are
+    then run alone in FP64 (Tsit5, same tolerances) and their catalogues
+    compared with the ensemble's."""
+    from oetqf_amd.cycle import run_cycle, run_ensemble_cycle
+    from oetqf_amd.workloads import YEAR, ensemble_members, transform_fault_composite
+
+    w = transform_fault_composite()
+    members, U0 = ensemble_members(w, 256)
+    span = (0.0, 300 * YEAR)
+    out = dict(workload=w["name"] + ", 256 frictional realizations", sim_years=300.0,
+               reltol=w["reltol"], abstol=w["abstol"], by_kdtype={}, fp64_members={})
+    cats = {}
+    for kd in kdtypes:
+        log("ensemble 300 yr: %s" % kd)
+        ens = oq.Ensemble(w["mf"], w["ma"], w["lam"], w["mu"], members, w["pa"], nrept=w["nrept"],
+                          buffer_ratio=w["buffer_ratio"], kdtype=kd)
+        r = run_ensemble_cycle(ens, U0, span, reltol=w["reltol"], abstol=w["abstol"], dt0=1.0)
+        ens.close()
+        cats[kd] = [_catalogue(r["t"][m], r["vmax"][m]) for m in range(256)]
+        na, nr = r["naccept"], r["nreject"]
+        out["by_kdtype"][kd] = dict(
+            wall_s=r["wall"], lockstep_iterations=r["iters"],
+            member_sim_yr_per_hr=256 * 300.0 / (r["wall"] / 3600.0),
+            all_success=all(x == "Success" for x in r["retcode"]),
+            rejection_rate=float(nr.sum() / (na.sum() + nr.sum())),
+            events_per_member=[c["events"] for c in cats[kd]],
+            mean_recurrence_yr=[c["mean_recurrence_yr"] for c in cats[kd]])
+        log("ensemble %s: 256 x 300 yr in %.0f s (%.0f member-yr/hr)" % (
+            kd, r["wall"], out["by_kdtype"][kd]["member_sim_yr_per_hr"]))
+    for m in members_fp64:
+        u0 = w["u0"].similar().unpack(U0[m])
+        p = oq.assemble_from_mesh(w["mf"], w["ma"], w["lam"], w["mu"], members[m], w["pa"], u0, span,
+                                  nrept=w["nrept"], buffer_ratio=w["buffer_ratio"])
+        log("member %d alone, FP64 Tsit5, 300 yr" % m)
+        r = run_cycle(p, reltol=w["reltol"], abstol=w["abstol"], chunk=4096, dt=1.0,
+                      progress=_progress("member %d fp64" % m))
+        p.f.close()
+        ref = _catalogue(r["t"], r["vmax"])
+        res = dict(wall_s=r["wall"], accepted=int(r["naccept"]), rejected=int(r["nreject"]), **ref)
+        for kd in kdtypes:
+            res["ensemble_%s_vs_fp64" % kd] = _compare(cats[kd][m], ref)
+        out["fp64_members"][str(m)] = res
+        log("member %d: FP64 %d events, mean recurrence %s yr; %s" % (
+            m, ref["events"], ref["mean_recurrence_yr"],
+            "; ".join("%s %s" % (kd, res["ensemble_%s_vs_fp64" % kd]) for kd in kdtypes)))
+    return out
+
+
+def spans_group(oq):
+    """The grouped batch (24 geometries x 16 members, FP32) over 300 years
+    with the lockstep grouped Tsit5: per-member step-rejection rates and
+    event counts."""
+    from oetqf_amd.workloads import YEAR, grouped_batch
+
+    ws = grouped_batch()
+    grp = oq.Group([(w["mf"], w["ma"], w["pa"]) for w in ws], ws[0]["lam"], ws[0]["mu"],
+                   [w["members"] for w in ws])
+    grp.autotune(3)
+    U0 = grp.pack([w["U0"] for w in ws])
+    grp.start(U0, (0.0, 300 * YEAR), dt0=1.0)
+    t0 = time.perf_counter()
+    iters, last = 0, t0
+    while True:
+        left = grp.run(256)
+        iters += 256
+        if time.perf_counter() - last > 20:
+            last = time.perf_counter()
+            t, _, _, _, _ = grp.state()
+            log("group 300 yr: %d iterations, %d members running, min t = %.1f yr" % (
+                iters, left, t.min() / YEAR))
+        if left == 0:
+            break
+    wall = time.perf_counter() - t0
+    t, U, na, nr, rc = grp.state()
+    grp.close()
+    rate = nr / np.maximum(na + nr, 1)
+    return dict(geometries=len(ws), members_per_geometry=16, sim_years=300.0, wall_s=wall,
+                lockstep_iterations=iters, member_sim_yr_per_hr=len(rc) * 300.0 / (wall / 3600.0),
+                all_success=all(x == "Success" for x in rc),
+                rejection_rate_mean=float(rate.mean()), rejection_rate_max=float(rate.max()),
+                rejection_rate_by_geometry=[float(rate[g * 16:(g + 1) * 16].mean()) for g in range(len(ws))],
+                accepted_by_geometry=[int(na[g * 16:(g + 1) * 16].mean()) for g in range(len(ws))],
+                geometry_names=[w["name"] for w in ws])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -496,6 +670,14 @@ def main():
                     help="trajectory length of the N = 65536 leg")
     ap.add_argument("--scale-kdtypes", default="fp64,fp32,bf16",
                     help="operator storage types of the N = 65536 leg (fp64 first: the reference)")
+    ap.add_argument("--spans", default="",
+                    help="instead of the bench line: whole simulated spans with event statistics, "
+                         "comma-separated from composite, ensemble, group (JSON to --spans-out)")
+    ap.add_argument("--spans-out", default="gpurun_out/spans.json")
+    ap.add_argument("--spans-members", default="0,37,64,101,128,170,203,255",
+                    help="ensemble members run alone in FP64 by --spans ensemble")
+    ap.add_argument("--spans-kdtypes", default="fp16x2,fp32",
+                    help="operator types of the ensemble spans")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (known traffic) for counter scaling")
     args = ap.parse_args()
@@ -507,6 +689,24 @@ def main():
 
     import oetqf_amd as oq
     from oetqf_amd.workloads import YEAR, transform_fault_composite
+
+    if args.spans:
+        res = {}
+        for leg in args.spans.split(","):
+            if leg == "composite":
+                res[leg] = spans_composite(oq)
+            elif leg == "ensemble":
+                res[leg] = spans_ensemble(oq, args.spans_kdtypes.split(","),
+                                          [int(m) for m in args.spans_members.split(",") if m])
+            elif leg == "group":
+                res[leg] = spans_group(oq)
+            else:
+                raise SystemExit("unknown span leg %r" % leg)
+            os.makedirs(os.path.dirname(os.path.abspath(args.spans_out)), exist_ok=True)
+            with open(args.spans_out, "w") as fh:
+                json.dump(res, fh, indent=1)
+        log("spans written to %s" % args.spans_out)
+        return
 
     if args.copy_calibration:
         # a copy of known size (reads 1 GiB, writes 1 GiB): the reference for
@@ -685,6 +885,10 @@ def main():
                    "wall_s_timed_loop": wall},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
+                     "traffic_source": ("not measured in this run: read from the committed counter "
+                                        "summary profiles/k_rhs_matvec_fp64_transform_fault_4096."
+                                        "traffic.txt (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE of "
+                                        "this bench, scaled by a 1 GiB copy)") if traffic else None,
                      "kernel": "k_rhs_matvec<FP64>", "ms_per_launch": ms_mv,
                      "algorithmic_bytes": nbytes},
         "variants": variants,
@@ -696,6 +900,9 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+    # release the device handles while the runtime (and any profiler attached
+    # to it) is still up, instead of from finalizers during interpreter exit
+    prob.f.close()
 
 
 if __name__ == "__main__":

@@ -146,10 +146,13 @@ int oq_gf_okada_ff(const oq_fault_mesh *mesh, double lambda, double mu,
 
 /* The fourier=true branch (src/BEM/GF.jl:60-68): the length-(2nx-1) real DFT
  * of [st[:,j,l]; reverse(st[2:end,j,l])]; st_dft: 2*nx*nxi*nxi doubles,
- * interleaved (re, im), Complex{Float64} layout. */
+ * interleaved (re, im), Complex{Float64} layout.  Both the kernel and its
+ * DFT are computed on the device (one workgroup per (j, l) column; the
+ * sequence is symmetric, so the imaginary parts are exactly 0).  nx <= 5120. */
 int oq_gf_okada_ff_dft(const oq_fault_mesh *mesh, double lambda, double mu,
                        int ftype, int64_t nrept, double buffer_ratio,
                        double *st_dft);
+
 
 /* Replaces stress_greens_function(mf::RectOkadaMesh, ma::BEMHex8Mesh, λ, μ;
  * ftype, qtype, nrept, buffer_ratio) (src/BEM/GF.jl:123-174), gf₁₂.
@@ -253,9 +256,25 @@ int oq_stepper_state(oq_stepper *s, double *t, double *dt, double *u,
 int oq_stepper_record(oq_stepper *s, int64_t stride, int64_t snap_capacity,
                       int64_t step_capacity);
 /* Copies out and clears the rings: *nsteps step records (t, max|v|) and
- * *nsnaps snapshots (t, packed u).  Buffers must hold the capacities. */
+ * *nsnaps snapshots (t, packed u).  Buffers must hold the capacities.  On a
+ * ring overrun the newest records are handed out and OQ_ERR_STATE names
+ * how many were lost (see oq_stepper_drain_du). */
 int oq_stepper_drain(oq_stepper *s, int64_t *nsteps, double *step_t, double *step_vmax,
                      int64_t *nsnaps, double *snap_t, double *snap_u);
+/* As oq_stepper_record; with du != 0 every snapshot also keeps the
+ * derivative f(u) at the saved state (Tsit5: the FSAL stage k7 of the
+ * accepted step; VCABM5: the corrector's f(y_{n+1})), which is what the
+ * reference's saving handlers read as integrator(integrator.t, Val{1})
+ * (src/io.jl:86, examples/otf-with-mantle.jl:157-159). */
+int oq_stepper_record_du(oq_stepper *s, int64_t stride, int64_t snap_capacity,
+                         int64_t step_capacity, int du);
+/* As oq_stepper_drain, plus the derivative snapshots (snap_du: snap_capacity
+ * packed states; may be NULL).  On a ring overrun (more records since the
+ * last drain than a ring holds) both drains hand out the newest records in
+ * order, restart the recorder and return OQ_ERR_STATE, with the number of
+ * lost records in oq_last_error(). */
+int oq_stepper_drain_du(oq_stepper *s, int64_t *nsteps, double *step_t, double *step_vmax,
+                        int64_t *nsnaps, double *snap_t, double *snap_u, double *snap_du);
 int oq_stepper_destroy(oq_stepper *s);
 /* As oq_stepper_create, with VCABM5 instead of Tsit5 (the solver of the
  * reference's example, examples/otf-with-mantle.jl:160,167-168): variable-
@@ -340,19 +359,21 @@ int oq_group_create(int device, int64_t ngeo, const oq_fault_mesh *mf,
 /* Total state entries, per-geometry state size nstate[ngeo] and the bytes
  * of all operators (any pointer may be NULL). */
 int oq_group_info(oq_group *e, int64_t *ntot, int64_t *nstate, int64_t *kbytes);
-/* Times every tile shape (10: register-streaming kernels with 16/32/64 rows
- * per wavefront, each with and without a software-pipelined K stream, and
+/* Times every tile shape (11: register-streaming kernels with 16/32/64 rows
+ * per wavefront, each with and without a software-pipelined K stream,
  * LDS-staged kernels with 16/32 rows per wavefront and one or two slices in
- * flight) on the geometries of each size bucket (bucket b: N < 2048 << b;
+ * flight, and the two-slice kernel on column-split units of ~2 MB balanced
+ * over the resident workgroups) on the geometries of each size bucket
+ * (bucket b: N < 2048 << b;
  * 8 buckets) and keeps the fastest; then times the whole grouped GEMM (one
  * launch per shape in use) with that per-bucket choice and with each single
  * shape for all buckets (one launch), and keeps the faster configuration.
  * shape_of_bucket[8] and ms_of_bucket[8] receive the choice (-1 for an empty
- * bucket), ms_table[8*10] (may be NULL) every per-bucket timing.  Call before
+ * bucket), ms_table[8*11] (may be NULL) every per-bucket timing.  Call before
  * oq_group_start (a running stepper must be started again). */
 int oq_group_autotune(oq_group *e, int iters, int *shape_of_bucket, double *ms_of_bucket,
                       double *ms_table);
-/* Pins the tile shape of each size bucket (shape_of_bucket[8]: 0..9, -1 =
+/* Pins the tile shape of each size bucket (shape_of_bucket[8]: 0..10, -1 =
  * keep), e.g. to reuse an earlier autotuning.  Like oq_group_autotune, call
  * before oq_group_start. */
 int oq_group_set_shapes(oq_group *e, const int *shape_of_bucket);

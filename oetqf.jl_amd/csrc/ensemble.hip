@@ -709,9 +709,10 @@ static int ens_step(oq_ensemble *e)
 static void ens_destroy(oq_ensemble *e)
 {
     if (!e) return;
-    if (e->h && e->h->stream) (void)hipStreamSynchronize(e->h->stream);
-    if (e->exec) (void)hipGraphExecDestroy(e->exec);
-    if (e->graph) (void)hipGraphDestroy(e->graph);
+    if (teardown_sync(e->h ? e->h->stream : nullptr)) {
+        if (e->exec) (void)hipGraphExecDestroy(e->exec);
+        if (e->graph) (void)hipGraphDestroy(e->graph);
+    }
     if (e->h) oq_rhs_destroy(e->h);
     delete e;
 }
@@ -956,6 +957,12 @@ int oq_ensemble_record(oq_ensemble *e, int64_t step_capacity)
     OQ_REQUIRE(step_capacity >= 0, "step_capacity must be >= 0");
     OQ_HIP(hipSetDevice(e->h->device));
     if (e->rec_cap == step_capacity) return OQ_OK;
+    // the captured step holds the recorder's buffers as kernel arguments:
+    // drop it before they are released (oq_ensemble_start captures anew)
+    OQ_HIP(hipStreamSynchronize(e->h->stream));
+    if (e->exec) { (void)hipGraphExecDestroy(e->exec); e->exec = nullptr; }
+    if (e->graph) { (void)hipGraphDestroy(e->graph); e->graph = nullptr; }
+    e->stepping = false;
     e->vpart.release();
     e->rec_t.release();
     e->rec_v.release();
@@ -1056,7 +1063,7 @@ int oq_ensemble_time(oq_ensemble *e, int iters, double *ms_eval)
 int oq_ensemble_destroy(oq_ensemble *e)
 {
     if (e) {
-        (void)hipSetDevice(e->h ? e->h->device : 0);
+        if (runtime_usable()) (void)hipSetDevice(e->h ? e->h->device : 0);
         ens_destroy(e);
     }
     return OQ_OK;

@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include <complex>
+#include <atomic>
 #include <vector>
 
 #include "hex8_dev.h"
@@ -23,6 +24,22 @@ namespace oq {
 // ---------------- small host utilities ----------------
 static thread_local std::string g_err;
 void set_error(const std::string &msg) { g_err = msg; }
+
+// Cleared once a teardown call finds the runtime unusable (deinitialised at
+// process exit, or a sticky device error): later releases then skip their
+// HIP calls and only drop the host objects.
+static std::atomic<bool> g_runtime_ok{true};
+bool runtime_usable() { return g_runtime_ok.load(std::memory_order_relaxed); }
+bool teardown_sync(hipStream_t s)
+{
+    if (!runtime_usable()) return false;
+    const hipError_t e = s ? hipStreamSynchronize(s) : hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        g_runtime_ok.store(false, std::memory_order_relaxed);
+        return false;
+    }
+    return true;
+}
 
 static double sin_deg_small(double r) { return sin(r * (M_PI / 180.0)); }
 static double cos_deg_small(double r) { return cos(r * (M_PI / 180.0)); }
@@ -77,7 +94,7 @@ int DevBuf::upload(const void *h, size_t n, hipStream_t s)
 
 void DevBuf::release()
 {
-    if (p) (void)hipFree(p);
+    if (p && runtime_usable()) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
 }
@@ -385,53 +402,98 @@ int launch_expand_toeplitz(const double *st, int64_t nx, int64_t nxi, double *k,
     return OQ_OK;
 }
 
-// Real DFT of length M = 2nx-1 of [st; reverse(st[2:end])] for every (j,l)
-// (GF.jl:60-67), on the host: O(nx^2 nxi^2), small next to assembly.
-void toeplitz_dft(const double *st, int64_t nx, int64_t nxi, double *out)
+// ---------------- DFT of the Toeplitz kernel (GF.jl:60-67) ----------------
+// The reference stores the real DFT, length M = 2nx-1, of the symmetric
+// sequence x = [st[:,j,l]; reverse(st[2:end,j,l])] (x_n = x_{M-n}) for every
+// (j, l).  Its transform is real:
+//   X_f = x_0 + 2 sum_{n=1}^{nx-1} x_n cos(2 pi f n / M),   f < nx,
+// and the imaginary parts are exactly zero here (FFTW returns rounding-level
+// values).  The inverse recovers st from nx complex coefficients c_f:
+//   st_n = (Re c_0 + 2 sum_{f>=1} (Re c_f cos - Im c_f sin)(2 pi f n / M)) / M.
+// One workgroup per (j, l) column; the column and the quarter-symmetric
+// twiddles cos/sin(2 pi k / M), k < nx, sit in LDS (16 or 32 bytes per
+// strike cell).  Phases k = f n mod M are advanced by addition.
+static constexpr int kDftThreads = 256;
+
+__device__ inline void dft_twiddles(double *cs, double *sn, int64_t nx, int64_t M)
 {
-    const int64_t M = 2 * nx - 1;
-    std::vector<double> cs(M), sn(M);
-    for (int64_t f = 0; f < M; ++f) {
-        cs[f] = cos(2.0 * M_PI * (double)f / (double)M);
-        sn[f] = sin(2.0 * M_PI * (double)f / (double)M);
-    }
-    for (int64_t jl = 0; jl < nxi * nxi; ++jl) {
-        const double *col = st + nx * jl;
-        for (int64_t f = 0; f < nx; ++f) {
-            double re = 0.0, im = 0.0;
-            for (int64_t n = 0; n < M; ++n) {
-                const double x = n < nx ? col[n] : col[M - n];
-                const int64_t k = (f * n) % M;
-                re += x * cs[k];
-                im -= x * sn[k];
-            }
-            out[2 * (f + nx * jl)] = re;
-            out[2 * (f + nx * jl) + 1] = im;
-        }
+    for (int64_t k = threadIdx.x; k < nx; k += blockDim.x) {
+        double sv, cv;
+        sincospi(2.0 * (double)k / (double)M, &sv, &cv);
+        cs[k] = cv;
+        if (sn) sn[k] = sv;
     }
 }
 
-// Inverse of toeplitz_dft: recovers st[0..nx-1] from the nx complex
-// coefficients of the symmetric length-(2nx-1) sequence.
-void toeplitz_idft(const double *dft, int64_t nx, int64_t nxi, double *st)
+__global__ void __launch_bounds__(kDftThreads) k_toeplitz_dft(const double *__restrict__ st, int64_t nx,
+                                                             double *__restrict__ out)
 {
-    const int64_t M = 2 * nx - 1;
-    std::vector<double> cs(M), sn(M);
-    for (int64_t f = 0; f < M; ++f) {
-        cs[f] = cos(2.0 * M_PI * (double)f / (double)M);
-        sn[f] = sin(2.0 * M_PI * (double)f / (double)M);
-    }
-    for (int64_t jl = 0; jl < nxi * nxi; ++jl) {
-        const double *c = dft + 2 * nx * jl;
-        for (int64_t n = 0; n < nx; ++n) {
-            double acc = c[0];
-            for (int64_t f = 1; f < nx; ++f) {
-                const int64_t k = (f * n) % M;
-                acc += 2.0 * (c[2 * f] * cs[k] - c[2 * f + 1] * sn[k]);
-            }
-            st[n + nx * jl] = acc / (double)M;
+    extern __shared__ double lds[];
+    double *cs = lds, *x = lds + nx;
+    const int64_t M = 2 * nx - 1, jl = blockIdx.x;
+    dft_twiddles(cs, nullptr, nx, M);
+    for (int64_t n = threadIdx.x; n < nx; n += blockDim.x) x[n] = st[jl * nx + n];
+    __syncthreads();
+    for (int64_t f = threadIdx.x; f < nx; f += blockDim.x) {
+        double acc = 0.0;
+        int64_t k = 0;
+        for (int64_t n = 1; n < nx; ++n) {
+            k += f;
+            if (k >= M) k -= M;
+            acc += x[n] * cs[k < nx ? k : M - k];
         }
+        out[2 * (jl * nx + f)] = x[0] + 2.0 * acc;
+        out[2 * (jl * nx + f) + 1] = 0.0;
     }
+}
+
+__global__ void __launch_bounds__(kDftThreads) k_toeplitz_idft(const double *__restrict__ dft, int64_t nx,
+                                                              double *__restrict__ st)
+{
+    extern __shared__ double lds[];
+    double *cs = lds, *sn = lds + nx, *re = lds + 2 * nx, *im = lds + 3 * nx;
+    const int64_t M = 2 * nx - 1, jl = blockIdx.x;
+    dft_twiddles(cs, sn, nx, M);
+    for (int64_t f = threadIdx.x; f < nx; f += blockDim.x) {
+        re[f] = dft[2 * (jl * nx + f)];
+        im[f] = dft[2 * (jl * nx + f) + 1];
+    }
+    __syncthreads();
+    for (int64_t n = threadIdx.x; n < nx; n += blockDim.x) {
+        double acc = 0.0;
+        int64_t k = 0;
+        for (int64_t f = 1; f < nx; ++f) {
+            k += n;
+            if (k >= M) k -= M;
+            const double c = cs[k < nx ? k : M - k], sv = k < nx ? sn[k] : -sn[M - k];
+            acc += re[f] * c - im[f] * sv;
+        }
+        st[jl * nx + n] = (re[0] + 2.0 * acc) / (double)M;
+    }
+}
+
+int launch_toeplitz_dft(const double *st, int64_t nx, int64_t nxi, double *out, hipStream_t s)
+{
+    OQ_REQUIRE(nx <= kMaxDftNx, "the device DFT of the Toeplitz kernel takes nx <= 5120");
+    const size_t lds = (size_t)(2 * nx) * 8;
+    if (lds > 64 * 1024)
+        OQ_HIP(hipFuncSetAttribute((const void *)k_toeplitz_dft,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_toeplitz_dft, dim3((unsigned)(nxi * nxi)), dim3(kDftThreads), lds, s, st, nx, out);
+    OQ_HIP(hipGetLastError());
+    return OQ_OK;
+}
+
+int launch_toeplitz_idft(const double *dft, int64_t nx, int64_t nxi, double *st, hipStream_t s)
+{
+    OQ_REQUIRE(nx <= kMaxDftNx, "the device DFT of the Toeplitz kernel takes nx <= 5120");
+    const size_t lds = (size_t)(4 * nx) * 8;
+    if (lds > 64 * 1024)
+        OQ_HIP(hipFuncSetAttribute((const void *)k_toeplitz_idft,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_toeplitz_idft, dim3((unsigned)(nxi * nxi)), dim3(kDftThreads), lds, s, dft, nx, st);
+    OQ_HIP(hipGetLastError());
+    return OQ_OK;
 }
 
 static int lrept_of(const oq_fault_mesh *m, double buffer_ratio, double *lrept)
@@ -482,12 +544,23 @@ int oq_gf_okada_ff(const oq_fault_mesh *mesh, double lambda, double mu, int ftyp
 int oq_gf_okada_ff_dft(const oq_fault_mesh *mesh, double lambda, double mu, int ftype,
                        int64_t nrept, double buffer_ratio, double *st_dft)
 {
+    int rc;
+    FaultUpload fu;
     OQ_REQUIRE(st_dft != nullptr, "output is NULL");
-    OQ_REQUIRE(mesh != nullptr, "fault mesh is NULL");
-    std::vector<double> st((size_t)(mesh->nx * mesh->nxi * mesh->nxi));
-    int rc = oq_gf_okada_ff(mesh, lambda, mu, ftype, nrept, buffer_ratio, st.data());
-    if (rc) return rc;
-    toeplitz_dft(st.data(), mesh->nx, mesh->nxi, st_dft);
+    OQ_REQUIRE(ftype == 0 || ftype == 1, "ftype must be StrikeSlip (0) or DipSlip (1)");
+    OQ_REQUIRE(nrept >= 0, "nrept must be >= 0");
+    if ((rc = check_fault(mesh))) return rc;
+    OQ_REQUIRE(mesh->nx <= kMaxDftNx, "the device DFT of the Toeplitz kernel takes nx <= 5120");
+    double lrept = 0.0;
+    if ((rc = lrept_of(mesh, buffer_ratio, &lrept))) return rc;
+    if ((rc = fu.load(mesh))) return rc;
+    fu.dev.lrept = lrept;
+    const size_t n = (size_t)(mesh->nx * mesh->nxi * mesh->nxi);
+    DevBuf st, out;  // st: the real kernel, scratch between the two launches
+    if ((rc = st.alloc(n * 8)) || (rc = out.alloc(2 * n * 8))) return rc;
+    if ((rc = launch_gf_ff(fu.dev, lambda, mu, ftype, (int)nrept, st.as<double>(), 0))) return rc;
+    if ((rc = launch_toeplitz_dft(st.as<double>(), mesh->nx, mesh->nxi, out.as<double>(), 0))) return rc;
+    OQ_HIP(hipMemcpy(st_dft, out.p, 2 * n * 8, hipMemcpyDeviceToHost));
     return OQ_OK;
 }
 

@@ -547,6 +547,163 @@ __global__ void __launch_bounds__(256) k_grp_gemm_lds2(GrpDev d, const int2 *__r
     }
 }
 
+// ---------------- column-split work units on a balanced static schedule ----------------
+// The row-block tiles above stream 0.6 MB (N = 1156) to 10.6 MB (N = 20692)
+// each, and with ~1.1 tiles per resident workgroup the largest ones set the
+// tail.  Here a tile's columns are split into chunks of ~2 MB ("units");
+// the units are assigned to the resident workgroups by longest-first
+// greedy balancing on the host (each workgroup gets a contiguous range of
+// the unit list), every unit writes its FP64 partial row sums to a slot of
+// P, and k_grp_sum_partials adds a row block's chunks in chunk order
+// (deterministic; no atomics).
+struct GrpUnit {
+    int g, row0, k0, k1, slot, pad;  // slices [k0, k1) of BK columns
+};
+struct GrpRowBlock {
+    int g, row0, slot0, nch;
+};
+
+template <int RB>
+__global__ void __launch_bounds__(256, 3) k_grp_gemm_units(GrpDev d, const GrpUnit *__restrict__ units,
+                                                        const int2 *__restrict__ wg_range,
+                                                        double *__restrict__ P)
+{
+    constexpr int BR = 64 * RB, BK = 128 / RB, PAD = BK + 4, C4 = BK / 4;
+    constexpr int KV = BR * C4 / 256, SV = 16 * C4 / 256, NC = 4 / RB, RS = 256 / C4;
+    __shared__ __attribute__((aligned(16))) float As[BR * PAD];
+    __shared__ __attribute__((aligned(16))) float Bs[16 * PAD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lr = lane & 15, lq = lane >> 4;
+    const int trow = tid / C4, tc4 = tid % C4;
+    const int2 range = wg_range[blockIdx.x];
+    for (int ui = range.x; ui < range.y; ++ui) {
+        const GrpUnit U = units[ui];
+        const GrpGeo &G = d.geo[U.g];
+        const int64_t N = G.N, ld = G.ld, r0 = U.row0;
+        const int64_t nk = U.k1 - U.k0;  // even: chunks are multiples of 512 columns
+        const int64_t pstart = ((int64_t)ui * 13) % nk;
+        const float *kp[KV];
+#pragma unroll
+        for (int j = 0; j < KV; ++j) {
+            const int64_t r = r0 + trow + j * RS;
+            kp[j] = G.K + (r < N ? r : N - 1) * ld + 4 * tc4;
+        }
+        const float *sp[SV];
+#pragma unroll
+        for (int j = 0; j < SV; ++j) sp[j] = d.S + G.Soff + (int64_t)(trow + j * RS) * ld + 4 * tc4;
+        f4x acc[RB][NC];
+        float hi[RB][4], lo[RB][4];
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) acc[rb][c] = (f4x){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int v = 0; v < 4; ++v) hi[rb][v] = lo[rb][v] = 0.f;
+        }
+        auto col = [&](int64_t k) {
+            const int64_t q = pstart + k;
+            return (U.k0 + (q >= nk ? q - nk : q)) * BK;
+        };
+        f4x ka[KV], sa[SV], kb[KV], sb[SV];
+        auto load = [&](f4x (&kr)[KV], f4x (&sr)[SV], int64_t k) {
+            const int64_t c0 = col(k);
+#pragma unroll
+            for (int j = 0; j < KV; ++j) kr[j] = gload4(kp[j] + c0);
+#pragma unroll
+            for (int j = 0; j < SV; ++j) sr[j] = gload4(sp[j] + c0);
+        };
+        auto store = [&](const f4x (&kr)[KV], const f4x (&sr)[SV]) {
+#pragma unroll
+            for (int j = 0; j < KV; ++j) *reinterpret_cast<f4x *>(&As[(trow + j * RS) * PAD + 4 * tc4]) = kr[j];
+#pragma unroll
+            for (int j = 0; j < SV; ++j) *reinterpret_cast<f4x *>(&Bs[(trow + j * RS) * PAD + 4 * tc4]) = sr[j];
+        };
+        auto reduce = [&](int64_t k) {
+#pragma unroll
+            for (int sub = 0; sub < BK / 64; ++sub) {
+                f4x a[RB][4], b[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    b[t] = *reinterpret_cast<const f4x *>(&Bs[lr * PAD + sub * 64 + 16 * t + 4 * lq]);
+#pragma unroll
+                    for (int rb = 0; rb < RB; ++rb)
+                        a[rb][t] = *reinterpret_cast<const f4x *>(
+                            &As[(wave * 16 * RB + rb * 16 + lr) * PAD + sub * 64 + 16 * t + 4 * lq]);
+                }
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+#pragma unroll
+                        for (int rb = 0; rb < RB; ++rb)
+                            acc[rb][t % NC] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                                a[rb][t][e], b[t][e], acc[rb][t % NC], 0, 0, 0);
+            }
+            if (((k + 1) * BK) % 256 == 0 || k + 1 == nk) {
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb) {
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) grp_two_sum(hi[rb][v], lo[rb][v], acc[rb][c][v]);
+                        acc[rb][c] = (f4x){0.f, 0.f, 0.f, 0.f};
+                    }
+                }
+            }
+        };
+        // as k_grp_gemm_lds2: two slices in flight in registers, loads and
+        // stores unconditional (indices clamp to the last slice)
+        auto kc = [&](int64_t k) { return k < nk ? k : nk - 1; };
+        load(ka, sa, 0);
+        load(kb, sb, kc(1));
+        __syncthreads();  // the previous unit's reads of As/Bs are done
+        store(ka, sa);
+        __syncthreads();
+        load(ka, sa, kc(2));
+        for (int64_t k = 0; k < nk; k += 2) {
+            reduce(k);
+            __syncthreads();
+            store(kb, sb);
+            __syncthreads();
+            load(kb, sb, kc(k + 3));
+            reduce(k + 1);
+            __syncthreads();
+            store(ka, sa);
+            __syncthreads();
+            load(ka, sa, kc(k + 4));
+        }
+        // FP64 partial row sums of this unit: P[(slot * BR + local row) * M + member]
+        if (lr < d.M) {
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int lrow = wave * 16 * RB + rb * 16 + 4 * lq + v;
+                    if (r0 + lrow < N)
+                        P[((int64_t)U.slot * BR + lrow) * d.M + lr] = (double)hi[rb][v] + (double)lo[rb][v];
+                }
+        }
+    }
+}
+
+// Y = the sum of a row block's chunk partials, in chunk order; thread (local
+// row, member) of block i of the row-block list.
+template <int BR>
+__global__ void __launch_bounds__(256) k_grp_sum_partials(GrpDev d, const GrpRowBlock *__restrict__ rbl,
+                                                          const double *__restrict__ P)
+{
+    const GrpRowBlock B = rbl[blockIdx.y];
+    const int M = d.M;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= BR * M) return;
+    const int lrow = t / M, m = t % M;
+    const GrpGeo &G = d.geo[B.g];
+    if (B.row0 + lrow >= G.N) return;
+    double y = 0.0;
+    for (int c = 0; c < B.nch; ++c) y += P[((int64_t)(B.slot0 + c) * BR + lrow) * M + m];
+    d.Y[G.Yoff + (int64_t)(B.row0 + lrow) * M + m] = y;
+}
+
 // ---------------- lockstep Tsit5 over all members ----------------
 __constant__ double kGA[7][6] = {
     {0, 0, 0, 0, 0, 0},
@@ -692,12 +849,23 @@ __global__ void __launch_bounds__(256) k_grp_commit(GrpCtrl c, int64_t ntot,
 // tile shapes: RB in {1, 2, 4} x {software-pipelined, plain} register
 // kernels, and the LDS-staged kernels with RB in {1, 2}, one or two slices
 // in flight (a third slice in flight measured the same as two: the kernel
-// is not limited by its prefetch depth)
-static constexpr int kGShapes = 10;
+// is not limited by its prefetch depth), and (shape 10) the two-slice LDS
+// kernel with RB = 2 on column-split units balanced over the workgroups
+static constexpr int kGShapes = 11;
+static constexpr int kGUnitShape = 10;
+static constexpr int kGUnitRB = 2, kGUnitBR = 64 * kGUnitRB, kGUnitBK = 128 / kGUnitRB;
+static constexpr int64_t kGUnitBytes = (int64_t)2 << 20;  // target bytes of K per unit
 static inline int shape_rb(int s)
 {
+    if (s == kGUnitShape) return kGUnitRB;
     return s < 6 ? (s / 2 == 0 ? 1 : (s / 2 == 1 ? 2 : 4)) : (s % 2 == 0 ? 1 : 2);
 }
+
+// Units of shape 10 over a set of geometries, with their schedule.
+struct GrpUnitTable {
+    oq::DevBuf units, wg, rbl;
+    int nunits = 0, nwg = 0, nrbl = 0, nslots = 0;
+};
 
 }  // namespace oq
 
@@ -717,6 +885,9 @@ struct oq_group {
     oq::DevBuf dtiles[oq::kGShapes];
     int ntiles[oq::kGShapes] = {0};
     int grid[oq::kGShapes] = {0};
+    oq::GrpUnitTable ut;  // shape 10 over the geometries that use it
+    oq::DevBuf P;         // partial row sums of the units (capacity pcap doubles)
+    size_t pcap = 0;
     oq::DevBuf u, y, k, part;
     oq::DevBuf ct, cdt, cdtcur, cqold, cna, cnr, citer, cdone, cacc, cret;
     oq::GrpDev dev{};
@@ -762,12 +933,97 @@ static std::vector<int2> build_tiles(const oq_group *e, int s, const std::vector
     return t;
 }
 
+// Units of shape 10 over the geometries in `sel`: every row block's slices
+// split into chunks of ~kGUnitBytes (multiples of 512 columns), then
+// longest-first greedy assignment to `nwg` workgroups.  Partials of row
+// block i go to slots [slot0, slot0 + nch).
+static int build_units(oq_group *e, const std::vector<int> &sel, GrpUnitTable &T)
+{
+    std::vector<GrpUnit> units;
+    std::vector<GrpRowBlock> rbl;
+    int slot = 0;
+    for (int g : sel) {
+        const GrpGeo &G = e->geo[g];
+        const int64_t nslices = G.ld / kGUnitBK;                 // multiple of 8
+        const int64_t per = std::max<int64_t>(8, kGUnitBytes / (kGUnitBR * kGUnitBK * 4) / 8 * 8);
+        const int64_t nch = (nslices + per - 1) / per;
+        const int64_t cs = (nslices / nch + 7) / 8 * 8;          // slices per chunk
+        for (int64_t r0 = 0; r0 < G.N; r0 += kGUnitBR) {
+            int c = 0;
+            for (int64_t k0 = 0; k0 < nslices; k0 += cs, ++c) {
+                const int64_t k1 = std::min(nslices, k0 + cs);
+                units.push_back(GrpUnit{g, (int)r0, (int)k0, (int)k1, slot + c, 0});
+            }
+            rbl.push_back(GrpRowBlock{g, (int)r0, slot, c});
+            slot += c;
+        }
+    }
+    // longest first onto the least loaded workgroup (cost = rows x columns)
+    const int W = std::max(1, std::min(e->grid[kGUnitShape], (int)units.size()));
+    auto cost = [&](const GrpUnit &u) {
+        const int64_t rows = std::min<int64_t>(kGUnitBR, e->geo[u.g].N - u.row0);
+        return rows * (int64_t)(u.k1 - u.k0);
+    };
+    std::vector<int> order(units.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return cost(units[a]) > cost(units[b]); });
+    std::vector<std::pair<int64_t, int>> heap;  // (load, workgroup), min-heap
+    for (int w = 0; w < W; ++w) heap.push_back({0, w});
+    std::vector<std::vector<int>> bins((size_t)W);
+    auto cmp = [](const std::pair<int64_t, int> &a, const std::pair<int64_t, int> &b) { return a > b; };
+    for (int i : order) {
+        std::pop_heap(heap.begin(), heap.end(), cmp);
+        heap.back().first += cost(units[i]);
+        bins[(size_t)heap.back().second].push_back(i);
+        std::push_heap(heap.begin(), heap.end(), cmp);
+    }
+    std::vector<GrpUnit> sched;
+    std::vector<int2> wg((size_t)W);
+    for (int w = 0; w < W; ++w) {
+        wg[(size_t)w].x = (int)sched.size();
+        for (int i : bins[(size_t)w]) sched.push_back(units[i]);
+        wg[(size_t)w].y = (int)sched.size();
+    }
+    T.units.release(); T.wg.release(); T.rbl.release();
+    T.nunits = (int)sched.size();
+    T.nwg = W;
+    T.nrbl = (int)rbl.size();
+    T.nslots = slot;
+    if (sched.empty()) return OQ_OK;
+    int rc = T.units.upload(sched.data(), sched.size() * sizeof(GrpUnit), e->stream);
+    if (!rc) rc = T.wg.upload(wg.data(), wg.size() * sizeof(int2), e->stream);
+    if (!rc) rc = T.rbl.upload(rbl.data(), rbl.size() * sizeof(GrpRowBlock), e->stream);
+    const size_t need = (size_t)slot * kGUnitBR * (size_t)e->M;
+    if (!rc && need > e->pcap) {
+        rc = e->P.alloc(need * 8);
+        e->pcap = rc ? 0 : need;
+    }
+    return rc;
+}
+
+static int launch_units(oq_group *e, const GrpUnitTable &T)
+{
+    if (T.nunits == 0) return OQ_OK;
+    hipLaunchKernelGGL((k_grp_gemm_units<kGUnitRB>), dim3((unsigned)T.nwg), dim3(256), 0, e->stream,
+                       e->dev, T.units.as<GrpUnit>(), T.wg.as<int2>(), e->P.as<double>());
+    hipLaunchKernelGGL((k_grp_sum_partials<kGUnitBR>), dim3(gbk(kGUnitBR * e->M, 256), (unsigned)T.nrbl),
+                       dim3(256), 0, e->stream, e->dev, T.rbl.as<GrpRowBlock>(), e->P.as<double>());
+    OQ_HIP(hipGetLastError());
+    return OQ_OK;
+}
+
 static int upload_tiles(oq_group *e)
 {
     for (int s = 0; s < kGShapes; ++s) {
         std::vector<int> sel;
         for (int g = 0; g < (int)e->G; ++g)
             if (e->bucket_shape[e->geo_bucket[g]] == s) sel.push_back(g);
+        if (s == kGUnitShape) {
+            int rc = build_units(e, sel, e->ut);
+            if (rc) return rc;
+            continue;
+        }
         const std::vector<int2> t = build_tiles(e, s, sel);
         e->ntiles[s] = (int)t.size();
         e->dtiles[s].release();
@@ -807,7 +1063,8 @@ static int grp_rhs(oq_group *e, const double *u, double *du)
                        u, du);
     OQ_HIP(hipGetLastError());
     for (int s = 0; s < kGShapes; ++s) {
-        int rc = launch_gemm(e, s, e->dtiles[s].as<int2>(), e->ntiles[s], u, du);
+        int rc = s == kGUnitShape ? launch_units(e, e->ut)
+                                  : launch_gemm(e, s, e->dtiles[s].as<int2>(), e->ntiles[s], u, du);
         if (rc) return rc;
     }
     hipLaunchKernelGGL(k_grp_finish, dim3(gbk(e->nrows, 256)), dim3(256), 0, e->stream, e->dev, u,
@@ -838,17 +1095,31 @@ static int grp_step(oq_group *e)
     return OQ_OK;
 }
 
+// Drops the captured step: its kernel arguments point at tile tables and
+// buffers that have just been replaced (oq_group_start captures anew).
+static void drop_step(oq_group *e)
+{
+    if (e->exec || e->graph) (void)hipStreamSynchronize(e->stream);
+    if (e->exec) { (void)hipGraphExecDestroy(e->exec); e->exec = nullptr; }
+    if (e->graph) { (void)hipGraphDestroy(e->graph); e->graph = nullptr; }
+    e->stepping = false;
+}
+
 static void grp_destroy(oq_group *e)
 {
     if (!e) return;
-    if (e->stream) (void)hipStreamSynchronize(e->stream);
-    if (e->exec) (void)hipGraphExecDestroy(e->exec);
-    if (e->graph) (void)hipGraphDestroy(e->graph);
+    const bool ok = teardown_sync(e->stream);
+    if (ok) {
+        if (e->exec) (void)hipGraphExecDestroy(e->exec);
+        if (e->graph) (void)hipGraphDestroy(e->graph);
+    }
     for (oq_rhs *h : e->rhs)
         if (h) oq_rhs_destroy(h);
-    if (e->ev0) (void)hipEventDestroy(e->ev0);
-    if (e->ev1) (void)hipEventDestroy(e->ev1);
-    if (e->stream) (void)hipStreamDestroy(e->stream);
+    if (ok) {
+        if (e->ev0) (void)hipEventDestroy(e->ev0);
+        if (e->ev1) (void)hipEventDestroy(e->ev1);
+        if (e->stream) (void)hipStreamDestroy(e->stream);
+    }
     delete e;
 }
 
@@ -1012,6 +1283,7 @@ int oq_group_create(int device, int64_t ngeo, const oq_fault_mesh *mf,
         if (!rc) rc = occupancy_grid(k_grp_gemm_lds<2>, &gr[7]);
         if (!rc) rc = occupancy_grid(k_grp_gemm_lds2<1>, &gr[8]);
         if (!rc) rc = occupancy_grid(k_grp_gemm_lds2<2>, &gr[9]);
+        if (!rc) rc = occupancy_grid(k_grp_gemm_units<kGUnitRB>, &gr[10]);
         for (int s = 0; s < kGShapes; ++s) e->grid[s] = gr[s];
     }
     if (!rc) rc = upload_tiles(e);
@@ -1065,7 +1337,7 @@ int oq_group_set_shapes(oq_group *e, const int *shape_of_bucket)
         if (shape_of_bucket[b] >= 0) e->bucket_shape[b] = shape_of_bucket[b];
     int rc = upload_tiles(e);
     if (rc) return rc;
-    e->stepping = false;  // a captured step holds the old tile tables
+    drop_step(e);  // a captured step holds the old tile tables
     return OQ_OK;
 }
 
@@ -1087,14 +1359,20 @@ int oq_group_autotune(oq_group *e, int iters, int *shape_of_bucket, double *ms_o
         double best = 1e300;
         int bs = e->bucket_shape[b];
         for (int s = 0; s < kGShapes; ++s) {
-            const std::vector<int2> t = build_tiles(e, s, sel);
+            const std::vector<int2> t = s == kGUnitShape ? std::vector<int2>() : build_tiles(e, s, sel);
             DevBuf dt;
-            int rc = dt.upload(t.data(), t.size() * sizeof(int2), e->stream);
+            GrpUnitTable ut;
+            int rc = s == kGUnitShape ? build_units(e, sel, ut)
+                                      : dt.upload(t.data(), t.size() * sizeof(int2), e->stream);
             if (rc) return rc;
-            if ((rc = launch_gemm(e, s, dt.as<int2>(), (int)t.size(), u, du))) return rc;
+            auto run = [&]() {
+                return s == kGUnitShape ? launch_units(e, ut)
+                                        : launch_gemm(e, s, dt.as<int2>(), (int)t.size(), u, du);
+            };
+            if ((rc = run())) return rc;
             OQ_HIP(hipEventRecord(e->ev0, e->stream));
             for (int i = 0; i < iters; ++i)
-                if ((rc = launch_gemm(e, s, dt.as<int2>(), (int)t.size(), u, du))) return rc;
+                if ((rc = run())) return rc;
             OQ_HIP(hipEventRecord(e->ev1, e->stream));
             OQ_HIP(hipEventSynchronize(e->ev1));
             float ms = 0.f;
@@ -1117,7 +1395,8 @@ int oq_group_autotune(oq_group *e, int iters, int *shape_of_bucket, double *ms_o
         for (int rep = 0; rep <= iters && !rc; ++rep) {  // rep 0 warms up
             if (rep == 1) OQ_HIP(hipEventRecord(e->ev0, e->stream));
             for (int s = 0; s < kGShapes && !rc; ++s)
-                rc = launch_gemm(e, s, e->dtiles[s].as<int2>(), e->ntiles[s], u, du);
+                rc = s == kGUnitShape ? launch_units(e, e->ut)
+                                      : launch_gemm(e, s, e->dtiles[s].as<int2>(), e->ntiles[s], u, du);
         }
         if (rc) return rc;
         OQ_HIP(hipEventRecord(e->ev1, e->stream));
@@ -1149,7 +1428,7 @@ int oq_group_autotune(oq_group *e, int iters, int *shape_of_bucket, double *ms_o
     if (ms_table) memcpy(ms_table, tab.data(), tab.size() * sizeof(double));
     rc = upload_tiles(e);
     if (rc) return rc;
-    e->stepping = false;  // a captured step holds the old tile tables
+    drop_step(e);  // a captured step holds the old tile tables
     return OQ_OK;
 }
 
@@ -1279,7 +1558,7 @@ int oq_group_state(oq_group *e, double *t, double *u, int64_t *naccept, int64_t 
 int oq_group_destroy(oq_group *e)
 {
     if (e) {
-        (void)hipSetDevice(e->device);
+        if (runtime_usable()) (void)hipSetDevice(e->device);
         grp_destroy(e);
     }
     return OQ_OK;

@@ -11,6 +11,13 @@ namespace oq {
 // Thread-local error message behind oq_last_error().
 void set_error(const std::string &msg);
 
+// Teardown guard: teardown_sync() waits for the stream (or the device) and
+// returns false -- and makes runtime_usable() false for good -- when the
+// runtime reports an error, so that destroy functions release nothing
+// through a runtime that has been shut down or has faulted.
+bool runtime_usable();
+bool teardown_sync(hipStream_t s);
+
 #define OQ_HIP(call)                                                          \
     do {                                                                      \
         hipError_t e_ = (call);                                               \
@@ -79,6 +86,13 @@ int launch_gf_mm(const DevHex8 &h, const double *qc, const double *qw,
 // a dense receiver-major operator (row pitch ld).
 int launch_expand_toeplitz(const double *st, int64_t nx, int64_t nxi,
                            double *k, int64_t ld, hipStream_t s);
+
+// Real DFT of length 2nx-1 of the symmetric extension of every (j, l) column
+// of the Toeplitz kernel (GF.jl:60-67) and its inverse, on the device:
+// st (nx*nxi*nxi) <-> interleaved complex (2*nx*nxi*nxi).
+static constexpr int64_t kMaxDftNx = 5120;  // column + twiddles in 160 KB of LDS
+int launch_toeplitz_dft(const double *st, int64_t nx, int64_t nxi, double *out, hipStream_t s);
+int launch_toeplitz_idft(const double *dft, int64_t nx, int64_t nxi, double *st, hipStream_t s);
 
 // Mesh upload helpers (owning buffers kept by the caller).
 struct FaultUpload {

@@ -31,7 +31,6 @@
 
 namespace oq {
 
-void toeplitz_idft(const double *dft, int64_t nx, int64_t nxi, double *st);
 
 static constexpr int64_t kPad = 512;  // row pitch multiple (elements)
 
@@ -914,13 +913,33 @@ static int finalize_k(oq_rhs *h, DevBuf &k64)
     return OQ_OK;
 }
 
+// gf11 from the host into `st` on the device: the real Toeplitz kernel as
+// given, or recovered from its DFT (the reference builder's default output)
+// by the device inverse DFT.
+static int upload_toeplitz(const double *gf11, int is_dft, int64_t nx, int64_t nxi, DevBuf &st,
+                           hipStream_t s)
+{
+    const size_t n = (size_t)(nx * nxi * nxi);
+    if (!is_dft) return st.upload(gf11, n * 8, s);
+    DevBuf dft;
+    int rc = dft.upload(gf11, 2 * n * 8, s);
+    if (!rc) rc = st.alloc(n * 8);
+    if (!rc) rc = launch_toeplitz_idft(dft.as<double>(), nx, nxi, st.as<double>(), s);
+    if (!rc && hipStreamSynchronize(s) != hipSuccess) {
+        set_error("hipStreamSynchronize failed after the inverse DFT");
+        rc = OQ_ERR_HIP;
+    }
+    return rc;
+}
+
 static void destroy(oq_rhs *h)
 {
     if (!h) return;
-    if (h->stream) (void)hipStreamSynchronize(h->stream);
-    if (h->ev0) (void)hipEventDestroy(h->ev0);
-    if (h->ev1) (void)hipEventDestroy(h->ev1);
-    if (h->stream) (void)hipStreamDestroy(h->stream);
+    if (teardown_sync(h->stream)) {
+        if (h->ev0) (void)hipEventDestroy(h->ev0);
+        if (h->ev1) (void)hipEventDestroy(h->ev1);
+        if (h->stream) (void)hipStreamDestroy(h->stream);
+    }
     delete h;
 }
 
@@ -956,14 +975,7 @@ int oq_rhs_create(int device, int64_t nx, int64_t nxi, int64_t ne, const oq_rs_p
     const int64_t nf = h->nf, m6 = 6 * ne, ld = h->ld;
     DevBuf k64, tmp;
     if (kdtype == OQ_K_TOEPLITZ || kdtype == OQ_K_FFT) {
-        std::vector<double> st;
-        const double *stp = gf11;
-        if (gf11_is_dft) {
-            st.resize((size_t)(nf * nxi));
-            toeplitz_idft(gf11, nx, nxi, st.data());
-            stp = st.data();
-        }
-        rc = tmp.upload(stp, (size_t)(nf * nxi) * 8, h->stream);
+        rc = upload_toeplitz(gf11, gf11_is_dft, nx, nxi, tmp, h->stream);
         if (!rc) rc = finalize_toeplitz(h, tmp);
         if (rc) {
             destroy(h);
@@ -978,14 +990,7 @@ int oq_rhs_create(int device, int64_t nx, int64_t nxi, int64_t ne, const oq_rs_p
         rc = OQ_ERR_HIP;
     }
     if (!rc) {
-        std::vector<double> st;
-        const double *stp = gf11;
-        if (gf11_is_dft) {
-            st.resize((size_t)(nf * nxi));
-            toeplitz_idft(gf11, nx, nxi, st.data());
-            stp = st.data();
-        }
-        rc = tmp.upload(stp, (size_t)(nf * nxi) * 8, h->stream);
+        rc = upload_toeplitz(gf11, gf11_is_dft, nx, nxi, tmp, h->stream);
         if (!rc) rc = launch_expand_toeplitz(tmp.as<double>(), nx, nxi, k64.as<double>(), ld, h->stream);
     }
     struct Blk { const double *src; int64_t m, n, srs, scs, r0, c0; };
@@ -1253,7 +1258,7 @@ int oq_rhs_time(oq_rhs *h, int iters, double *ms_eval, double *ms_matvec)
 int oq_rhs_destroy(oq_rhs *h)
 {
     if (h) {
-        (void)hipSetDevice(h->device);
+        if (runtime_usable()) (void)hipSetDevice(h->device);
         destroy(h);
     }
     return OQ_OK;

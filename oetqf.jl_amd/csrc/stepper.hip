@@ -177,6 +177,8 @@ struct Recorder {
     long long *nstep, *nsnap;  // totals since the last drain
     double *step_t, *step_vmax;
     double *snap_t, *snap_u;
+    double *snap_du;       // derivative ring (nullptr: not recorded)
+    const double *du;      // f(u) of the accepted state (graph mode)
     long long step_cap, snap_cap, stride, stride_count;
     int64_t n, nf;
 };
@@ -208,6 +210,8 @@ __global__ void __launch_bounds__(256) k_record(const Ctrl *__restrict__ c, Reco
     if (!take) return;
     const long long js = *r.nsnap % r.snap_cap;
     for (int64_t i = threadIdx.x; i < r.n; i += blockDim.x) r.snap_u[js * r.n + i] = u[i];
+    if (r.snap_du)
+        for (int64_t i = threadIdx.x; i < r.n; i += blockDim.x) r.snap_du[js * r.n + i] = r.du[i];
     __syncthreads();
     if (threadIdx.x == 0) {
         r.snap_t[js] = c->t;
@@ -748,7 +752,11 @@ __global__ void __launch_bounds__(256) k_persist(PersistArgs a)
             if (take) {
                 const long long js = nsnap % a.rec.snap_cap;
                 for (int e = tid; e < E; e += 256)
-                    if (e % R < Rv) a.rec.snap_u[js * n + comp_off[e / R] + p0 + e % R] = yo[e];
+                    if (e % R < Rv) {
+                        const int64_t at = js * n + comp_off[e / R] + p0 + e % R;
+                        a.rec.snap_u[at] = yo[e];
+                        if (a.rec.snap_du) a.rec.snap_du[at] = ko[e];  // k1 = f(u) after FSAL
+                    }
                 if (blockIdx.x == 0 && tid == 0) a.rec.snap_t[js] = c.t;
             }
             __syncthreads();
@@ -974,7 +982,11 @@ __global__ void __launch_bounds__(256) k_persist_ab(PersistArgs a, AbCtrl *actrl
             if (take) {
                 const long long js = nsnap % a.rec.snap_cap;
                 for (int e = tid; e < E; e += 256)
-                    if (e % R < Rv) a.rec.snap_u[js * n + comp_off[e / R] + p0 + e % R] = yo[e];
+                    if (e % R < Rv) {
+                        const int64_t at = js * n + comp_off[e / R] + p0 + e % R;
+                        a.rec.snap_u[at] = yo[e];
+                        if (a.rec.snap_du) a.rec.snap_du[at] = fo[e];  // f(y_{n+1})
+                    }
                 if (blockIdx.x == 0 && tid == 0) a.rec.snap_t[js] = ac.c.t;
             }
             __syncthreads();
@@ -1009,7 +1021,7 @@ struct oq_stepper {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     // recorder (disabled until oq_stepper_record)
-    oq::DevBuf rcnt, rstep_t, rstep_v, rsnap_t, rsnap_u;
+    oq::DevBuf rcnt, rstep_t, rstep_v, rsnap_t, rsnap_u, rsnap_du;
     oq::Recorder rec{};
     bool recording = false;
     // algorithm: 0 Tsit5, 1 VCABM5 (oq_stepper_create_vcabm5)
@@ -1121,10 +1133,11 @@ static double rms_scaled(const std::vector<double> &x, const std::vector<double>
 static void destroy_stepper(oq_stepper *s)
 {
     if (!s) return;
-    if (s->h && s->h->stream) (void)hipStreamSynchronize(s->h->stream);
-    if (s->exec) (void)hipGraphExecDestroy(s->exec);
-    if (s->graph) (void)hipGraphDestroy(s->graph);
-    if (s->ctrl_host) (void)hipHostFree(s->ctrl_host);
+    if (teardown_sync(s->h ? s->h->stream : nullptr)) {
+        if (s->exec) (void)hipGraphExecDestroy(s->exec);
+        if (s->graph) (void)hipGraphDestroy(s->graph);
+        if (s->ctrl_host) (void)hipHostFree(s->ctrl_host);
+    }
     delete s;
 }
 
@@ -1379,8 +1392,8 @@ int oq_stepper_state(oq_stepper *st, double *t, double *dt, double *u, int64_t *
     return OQ_OK;
 }
 
-int oq_stepper_record(oq_stepper *st, int64_t stride, int64_t snap_capacity,
-                      int64_t step_capacity)
+int oq_stepper_record_du(oq_stepper *st, int64_t stride, int64_t snap_capacity,
+                         int64_t step_capacity, int du)
 {
     OQ_REQUIRE(st != nullptr, "stepper is NULL");
     OQ_REQUIRE(stride >= 1 && snap_capacity >= 0 && step_capacity >= 0,
@@ -1393,6 +1406,11 @@ int oq_stepper_record(oq_stepper *st, int64_t stride, int64_t snap_capacity,
     if ((rc = st->rstep_v.alloc((size_t)step_capacity * 8))) return rc;
     if ((rc = st->rsnap_t.alloc((size_t)snap_capacity * 8))) return rc;
     if ((rc = st->rsnap_u.alloc((size_t)(snap_capacity * st->n) * 8))) return rc;
+    if (du) {
+        if ((rc = st->rsnap_du.alloc((size_t)(snap_capacity * st->n) * 8))) return rc;
+    } else {
+        st->rsnap_du.release();
+    }
     const long long zero[2] = {0, 0};
     OQ_HIP(hipMemcpyAsync(st->rcnt.p, zero, sizeof(zero), hipMemcpyHostToDevice, h->stream));
     Recorder &r = st->rec;
@@ -1402,6 +1420,10 @@ int oq_stepper_record(oq_stepper *st, int64_t stride, int64_t snap_capacity,
     r.step_vmax = st->rstep_v.as<double>();
     r.snap_t = st->rsnap_t.as<double>();
     r.snap_u = st->rsnap_u.as<double>();
+    r.snap_du = du ? st->rsnap_du.as<double>() : nullptr;
+    // f(u) of the accepted state: Tsit5 keeps it in k1 (FSAL), VCABM5 in the
+    // second half of its corrector buffer
+    r.du = st->alg == 1 ? st->ab.as<double>() + st->n : st->k.as<double>();
     r.step_cap = step_capacity;
     r.snap_cap = snap_capacity;
     r.stride = stride;
@@ -1411,8 +1433,30 @@ int oq_stepper_record(oq_stepper *st, int64_t stride, int64_t snap_capacity,
     return capture_step(st);
 }
 
-int oq_stepper_drain(oq_stepper *st, int64_t *nsteps, double *step_t, double *step_vmax,
-                     int64_t *nsnaps, double *snap_t, double *snap_u)
+int oq_stepper_record(oq_stepper *st, int64_t stride, int64_t snap_capacity,
+                      int64_t step_capacity)
+{
+    return oq_stepper_record_du(st, stride, snap_capacity, step_capacity, 0);
+}
+
+// Copies the `keep` newest of `cnt` ring entries (each `w` doubles, ring of
+// `cap` slots written at index count % cap) to `out` in the order written.
+static int drain_ring(const double *ring, long long cnt, long long cap, int64_t w, double *out,
+                      hipStream_t s)
+{
+    const long long keep = cnt < cap ? cnt : cap;
+    if (keep <= 0 || out == nullptr) return OQ_OK;
+    const long long first = (cnt - keep) % cap;  // slot of the oldest kept entry
+    const long long n1 = (first + keep <= cap) ? keep : cap - first;
+    OQ_HIP(hipMemcpyAsync(out, ring + first * w, (size_t)(n1 * w) * 8, hipMemcpyDeviceToHost, s));
+    if (keep > n1)
+        OQ_HIP(hipMemcpyAsync(out + n1 * w, ring, (size_t)((keep - n1) * w) * 8, hipMemcpyDeviceToHost,
+                              s));
+    return OQ_OK;
+}
+
+int oq_stepper_drain_du(oq_stepper *st, int64_t *nsteps, double *step_t, double *step_vmax,
+                        int64_t *nsnaps, double *snap_t, double *snap_u, double *snap_du)
 {
     OQ_REQUIRE(st != nullptr, "stepper is NULL");
     OQ_REQUIRE(st->recording, "recording is not enabled (oq_stepper_record)");
@@ -1422,29 +1466,45 @@ int oq_stepper_drain(oq_stepper *st, int64_t *nsteps, double *step_t, double *st
     OQ_HIP(hipMemcpyAsync(cnt, st->rcnt.p, sizeof(cnt), hipMemcpyDeviceToHost, h->stream));
     OQ_HIP(hipStreamSynchronize(h->stream));
     const Recorder &r = st->rec;
-    OQ_REQUIRE(cnt[0] <= r.step_cap || r.step_cap == 0, "step record overflow: drain more often");
-    OQ_REQUIRE(cnt[1] <= r.snap_cap || r.snap_cap == 0, "snapshot overflow: drain more often");
-    if (nsteps) *nsteps = cnt[0];
-    if (nsnaps) *nsnaps = cnt[1];
-    if (step_t && cnt[0] > 0 && r.step_cap > 0)
-        OQ_HIP(hipMemcpyAsync(step_t, r.step_t, (size_t)cnt[0] * 8, hipMemcpyDeviceToHost, h->stream));
-    if (step_vmax && cnt[0] > 0 && r.step_cap > 0)
-        OQ_HIP(hipMemcpyAsync(step_vmax, r.step_vmax, (size_t)cnt[0] * 8, hipMemcpyDeviceToHost, h->stream));
-    if (snap_t && cnt[1] > 0)
-        OQ_HIP(hipMemcpyAsync(snap_t, r.snap_t, (size_t)cnt[1] * 8, hipMemcpyDeviceToHost, h->stream));
-    if (snap_u && cnt[1] > 0)
-        OQ_HIP(hipMemcpyAsync(snap_u, r.snap_u, (size_t)(cnt[1] * st->n) * 8, hipMemcpyDeviceToHost,
-                              h->stream));
+    const long long ks = r.step_cap > 0 ? (cnt[0] < r.step_cap ? cnt[0] : r.step_cap) : 0;
+    const long long kn = r.snap_cap > 0 ? (cnt[1] < r.snap_cap ? cnt[1] : r.snap_cap) : 0;
+    const long long lost_s = r.step_cap > 0 ? cnt[0] - ks : 0, lost_n = r.snap_cap > 0 ? cnt[1] - kn : 0;
+    if (nsteps) *nsteps = r.step_cap > 0 ? ks : cnt[0];
+    if (nsnaps) *nsnaps = kn;
+    int rc;
+    if (r.step_cap > 0) {
+        if ((rc = drain_ring(r.step_t, cnt[0], r.step_cap, 1, step_t, h->stream))) return rc;
+        if ((rc = drain_ring(r.step_vmax, cnt[0], r.step_cap, 1, step_vmax, h->stream))) return rc;
+    }
+    if (r.snap_cap > 0) {
+        if ((rc = drain_ring(r.snap_t, cnt[1], r.snap_cap, 1, snap_t, h->stream))) return rc;
+        if ((rc = drain_ring(r.snap_u, cnt[1], r.snap_cap, st->n, snap_u, h->stream))) return rc;
+        if (r.snap_du && (rc = drain_ring(r.snap_du, cnt[1], r.snap_cap, st->n, snap_du, h->stream)))
+            return rc;
+    }
     const long long zero[2] = {0, 0};
     OQ_HIP(hipMemcpyAsync(st->rcnt.p, zero, sizeof(zero), hipMemcpyHostToDevice, h->stream));
     OQ_HIP(hipStreamSynchronize(h->stream));
+    if (lost_s > 0 || lost_n > 0) {
+        set_error("recorder overflow: " + std::to_string(lost_s) + " step records and " +
+                  std::to_string(lost_n) +
+                  " snapshots were overwritten before this drain; the newest ones were handed out "
+                  "and the recorder restarted (drain more often)");
+        return OQ_ERR_STATE;
+    }
     return OQ_OK;
+}
+
+int oq_stepper_drain(oq_stepper *st, int64_t *nsteps, double *step_t, double *step_vmax,
+                     int64_t *nsnaps, double *snap_t, double *snap_u)
+{
+    return oq_stepper_drain_du(st, nsteps, step_t, step_vmax, nsnaps, snap_t, snap_u, nullptr);
 }
 
 int oq_stepper_destroy(oq_stepper *st)
 {
     if (st) {
-        (void)hipSetDevice(st->h->device);
+        if (runtime_usable()) (void)hipSetDevice(st->h->device);
         destroy_stepper(st);
     }
     return OQ_OK;

@@ -9,7 +9,7 @@ from .ensemble import Ensemble  # noqa: F401
 from .group import Group  # noqa: F401
 from .equation import (ArrayPartition, Integrator, ODEProblem, ODESolution, Tsit5, VCABM5,  # noqa: F401,E501
                        assemble, assemble_from_mesh, similar, solve)
-from .io import wread, wsolve  # noqa: F401
+from .io import SavedState, Val, VΘdεΔ, VΘΔ, wread, wsolve  # noqa: F401
 from .greens import (DipSlip, StrikeSlip, get_quadrature, gmsh_quadrature,  # noqa: F401
                      stress_greens_function)
 from .mesh import (BEMHex8Mesh, RectOkadaMesh, cosd, gen_gmsh_mesh, gen_mesh,  # noqa: F401

@@ -4,8 +4,11 @@ The shared library is the product: every Green's function, RHS evaluation
 and time step below runs in its HIP kernels.  There is no CPU fallback --
 if the library is missing or no GPU is visible, calls raise.
 """
+import atexit
 import ctypes
 import os
+import warnings
+import weakref
 
 _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIBPATH = os.path.join(_PKG, "lib", "liboetqf_amd.so")
@@ -18,6 +21,7 @@ _vp = ctypes.c_void_p
 
 OQ_OK = 0
 OQ_ERR_ARG = 1
+OQ_ERR_STATE = 3
 OQ_K_FP64, OQ_K_FP32, OQ_K_BF16, OQ_K_TOEPLITZ, OQ_K_FP16X2, OQ_K_FFT = 0, 1, 2, 3, 4, 5
 RETCODES = {0: "Running", 1: "Success", 2: "MaxIters", 3: "DtLessThanMin", 4: "NonFinite"}
 
@@ -97,6 +101,9 @@ PROTOTYPES = {
     "oq_stepper_set_persistent": (_int, [_vp, _int]),
     "oq_stepper_record": (_int, [_vp, _i64, _i64, _i64]),
     "oq_stepper_drain": (_int, [_vp, ctypes.POINTER(_i64), _pd, _pd, ctypes.POINTER(_i64), _pd, _pd]),
+    "oq_stepper_record_du": (_int, [_vp, _i64, _i64, _i64, _int]),
+    "oq_stepper_drain_du": (_int, [_vp, ctypes.POINTER(_i64), _pd, _pd, ctypes.POINTER(_i64), _pd, _pd,
+                                   _pd]),
     "oq_ensemble_create": (_int, [_int, ctypes.POINTER(FaultMesh), ctypes.POINTER(Hex8Mesh), _d, _d,
                                   _int, _i64, _d, ctypes.POINTER(Quadrature), _i64,
                                   ctypes.POINTER(RSProps), ctypes.POINTER(ViscProps), _int,
@@ -140,6 +147,39 @@ def load():
             fn.argtypes = args
         _lib = lib
     return _lib
+
+
+def check_drain(rc):
+    """Status of a recorder drain: an overflow still hands out the newest
+    records (and restarts the recorder), so it becomes a RuntimeWarning
+    naming how many were lost; any other error raises."""
+    if rc == OQ_ERR_STATE:
+        msg = load().oq_last_error().decode("utf-8", "replace")
+        if "overflow" in msg:
+            warnings.warn(msg, RuntimeWarning, stacklevel=3)
+            return
+    check(rc)
+
+
+# Objects owning device handles.  They are closed by an exit hook, before the
+# interpreter and the HIP runtime shut down, so that no stream, graph or
+# buffer is released from a finalizer during teardown.
+_live = weakref.WeakSet()
+
+
+def register(obj):
+    _live.add(obj)
+    return obj
+
+
+@atexit.register
+def _close_live_handles():
+    # steppers borrow their RHS handle: close them first
+    for obj in sorted(list(_live), key=lambda o: getattr(o, "_close_rank", 1)):
+        try:
+            obj.close()
+        except Exception:
+            pass
 
 
 def check(rc):

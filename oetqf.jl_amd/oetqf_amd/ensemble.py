@@ -29,6 +29,7 @@ class Ensemble:
         keep.append(arr)
         L = _lib.load()
         self.h = ctypes.c_void_p()
+        _lib.register(self)
         a = fault_struct(mf, keep)
         if ma is not None:
             b = hex8_struct(ma, keep)
@@ -76,7 +77,7 @@ class Ensemble:
         cnt = np.zeros(self.M, dtype=np.int64)
         t = np.zeros((self.M, cap))
         v = np.zeros((self.M, cap))
-        _lib.check(_lib.load().oq_ensemble_drain(
+        _lib.check_drain(_lib.load().oq_ensemble_drain(
             self.h, cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), _lib.ptr(t), _lib.ptr(v)))
         return [(t[m, :cnt[m]].copy(), v[m, :cnt[m]].copy()) for m in range(self.M)]
 

@@ -8,6 +8,7 @@ RHS in the gfx950 kernels (equation.jl:156-166, :185-205); `solve(prob,
 Tsit5(); ...)` integrates on the device with the Tsit5 stepper.
 """
 import ctypes
+import weakref
 
 import numpy as np
 
@@ -76,6 +77,8 @@ class RHS:
         self.h = handle
         self.nx, self.nxi, self.ne = nx, nxi, ne
         self._keep = keep
+        self._steppers = weakref.WeakSet()  # they borrow this handle
+        _lib.register(self)
         self.dila = dila is not None
         if self.dila:
             ds = _dila_struct(dila, keep)
@@ -124,6 +127,8 @@ class RHS:
         return a.value, b.value
 
     def close(self):
+        for it in list(self._steppers):
+            it.close()
         if self.h:
             _lib.load().oq_rhs_destroy(self.h)
             self.h = None
@@ -245,6 +250,8 @@ class ODESolution:
 class Integrator:
     """Device-resident Tsit5 integration of an assembled problem."""
 
+    _close_rank = 0  # closed before the RHS handle it borrows
+
     def __init__(self, prob, reltol=1e-6, abstol=1e-8, dt=0.0, dtmax=0.0, dtmin=0.0,
                  maxiters=10**9, alg=None):
         self.prob = prob
@@ -256,8 +263,12 @@ class Integrator:
             raise NotImplementedError("the device steppers are Tsit5 and VCABM5")
         create = (_lib.load().oq_stepper_create_vcabm5 if isinstance(alg, VCABM5)
                   else _lib.load().oq_stepper_create)
+        if not prob.f.h:
+            raise ValueError("the problem's RHS handle is closed")
         _lib.check(create(prob.f.h, _lib.ptr(self._u0flat), prob.tspan[0], prob.tspan[1],
                           ctypes.byref(o), ctypes.byref(self.s)))
+        prob.f._steppers.add(self)
+        _lib.register(self)
 
     def step(self, nsteps):
         rc = ctypes.c_int()
@@ -278,25 +289,34 @@ class Integrator:
         problems whose operator rows fit in LDS); ValueError otherwise."""
         _lib.check(_lib.load().oq_stepper_set_persistent(self.s, 1 if on else 0))
 
-    def record(self, stride=1, snap_capacity=0, step_capacity=4096):
+    def record(self, stride=1, snap_capacity=0, step_capacity=4096, derivative=False):
         """Enables the device recorder: per accepted step (t, max|v|), and a
-        state snapshot every `stride` accepted steps."""
-        _lib.check(_lib.load().oq_stepper_record(self.s, int(stride), int(snap_capacity),
-                                                 int(step_capacity)))
+        state snapshot every `stride` accepted steps; with derivative=True
+        each snapshot also keeps f(u) at the saved state."""
+        _lib.check(_lib.load().oq_stepper_record_du(self.s, int(stride), int(snap_capacity),
+                                                    int(step_capacity), 1 if derivative else 0))
         self._caps = (int(snap_capacity), int(step_capacity))
+        self._du = bool(derivative)
 
     def drain(self):
         """Returns (step_t, step_vmax, snap_t, snaps (k, nstate)) recorded
-        since the last drain."""
+        since the last drain, plus the derivative snapshots (k, nstate) when
+        recorded with derivative=True.  A ring overrun hands out the newest
+        records with a RuntimeWarning naming how many were lost."""
         sc, kc = self._caps
         st_t, st_v = np.zeros(max(kc, 1)), np.zeros(max(kc, 1))
-        sn_t, sn_u = np.zeros(max(sc, 1)), np.zeros(max(sc, 1) * self._u0flat.size)
+        n = self._u0flat.size
+        sn_t, sn_u = np.zeros(max(sc, 1)), np.zeros(max(sc, 1) * n)
+        sn_du = np.zeros(max(sc, 1) * n) if self._du else None
         ns, nn = ctypes.c_int64(), ctypes.c_int64()
-        _lib.check(_lib.load().oq_stepper_drain(self.s, ctypes.byref(ns), _lib.ptr(st_t), _lib.ptr(st_v),
-                                                ctypes.byref(nn), _lib.ptr(sn_t), _lib.ptr(sn_u)))
+        _lib.check_drain(_lib.load().oq_stepper_drain_du(
+            self.s, ctypes.byref(ns), _lib.ptr(st_t), _lib.ptr(st_v), ctypes.byref(nn),
+            _lib.ptr(sn_t), _lib.ptr(sn_u), _lib.ptr(sn_du) if sn_du is not None else None))
         k, j = ns.value, nn.value
-        return st_t[:k].copy(), st_v[:k].copy(), sn_t[:j].copy(), \
-            sn_u[:j * self._u0flat.size].reshape(j, self._u0flat.size).copy()
+        out = (st_t[:k].copy(), st_v[:k].copy(), sn_t[:j].copy(), sn_u[:j * n].reshape(j, n).copy())
+        if self._du:
+            out += (sn_du[:j * n].reshape(j, n).copy(),)
+        return out
 
     def close(self):
         if self.s:

@@ -18,7 +18,7 @@ from .equation import _KDTYPES, _rs_struct, _visc_struct
 from .greens import StrikeSlip, _Keep, _ftype, fault_struct, get_quadrature, hex8_struct, quad_struct
 
 SHAPES = ["RB1-pipe", "RB1", "RB2-pipe", "RB2", "RB4-pipe", "RB4", "RB1-lds", "RB2-lds",
-          "RB1-lds-d2", "RB2-lds-d2"]
+          "RB1-lds-d2", "RB2-lds-d2", "RB2-lds-d2-split"]
 
 
 class Group:
@@ -50,6 +50,7 @@ class Group:
         arr = (_lib.RSProps * (G * M))(*[_rs_struct(p, keep) for mem in members for p in mem])
         keep.append(arr)
         self.h = ctypes.c_void_p()
+        _lib.register(self)
         _lib.check(_lib.load().oq_group_create(device, G, fa, hp, vp, lam, mu, _ftype(ftype), nrept,
                                                buffer_ratio, ctypes.byref(q), M, arr,
                                                _KDTYPES[kdtype], ctypes.byref(self.h)))

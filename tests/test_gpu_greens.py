@@ -35,6 +35,18 @@ def test_gf_ff_fourier_is_rfft_of_circulant(oq, oracle):
     assert relerr(g, ref) <= TOL
 
 
+def test_gf_ff_fourier_at_scale(oq, oracle):
+    """The device DFT (one workgroup per (j, l) column) on the 256 x 64 fault
+    of the N = 65536 workload: numpy's rfft of the circulant of the oracle's
+    kernel, to the Green's tolerance; its imaginary parts are exactly 0."""
+    mf = oq.gen_mesh("RectOkada", 80e3, 20e3, 312.5, 312.5, 90.0)
+    g = oq.stress_greens_function(mf, 3e10, 3e10, fourier=True)
+    st = oracle.gf_ff(mf, 3e10, 3e10)
+    ref = np.fft.rfft(np.concatenate([st, st[:0:-1]], axis=0), axis=0)
+    assert relerr(g, ref) <= TOL and np.all(g.imag == 0.0)
+    assert np.abs(ref.imag).max() <= 1e-12 * np.abs(ref.real).max()
+
+
 @pytest.mark.parametrize("qtype", ["Gauss1", "Gauss2"])
 @pytest.mark.parametrize("ftype", ["ss", "ds"])
 def test_gf_mf_matches_oracle(oq, oracle, qtype, ftype):
