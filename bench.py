@@ -709,15 +709,21 @@ def main():
         return
 
     if args.copy_calibration:
-        # a copy of known size (reads 1 GiB, writes 1 GiB): the reference for
+        # a device copy of known size (reads 1 GiB, writes 1 GiB) through the
+        # HIP runtime the library is linked against: the reference for
         # scaling the FETCH_SIZE / WRITE_SIZE counters of the other kernels
-        import torch
-        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
+        import ctypes
+        oq.load_library()
+        rt = ctypes.CDLL("libamdhip64.so.7")
+        n = ctypes.c_size_t(1 << 30)
+        a, b = ctypes.c_void_p(), ctypes.c_void_p()
+        assert rt.hipMalloc(ctypes.byref(a), n) == 0 and rt.hipMalloc(ctypes.byref(b), n) == 0
+        assert rt.hipMemset(a, 0, n) == 0
         for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+            assert rt.hipMemcpy(b, a, n, 3) == 0  # hipMemcpyDeviceToDevice
+        assert rt.hipDeviceSynchronize() == 0
+        rt.hipFree(a)
+        rt.hipFree(b)
 
     w = transform_fault_composite()
     mf, ma = w["mf"], w["ma"]

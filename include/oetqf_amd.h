@@ -51,13 +51,15 @@ extern "C" {
  * pair, FP32 chunk sums, FP64 totals).  Same bytes as FP32, FP32-class
  * error, eight times the FP32 matrix rate. */
 #define OQ_K_FP16X2 4
-/* Fault-only problems (ne = 0): K_ff applied as the reference applies it,
- * by DFT convolution along strike (equation.jl:44-61): FFT of the
+/* K_ff applied as the reference applies it, by DFT convolution along
+ * strike (equation.jl:44-61 fault only, :200 coupled): FFT of the
  * zero-padded source, per-frequency contraction with the DFT of the
  * symmetric circulant of st (GF.jl:60-67, real), inverse FFT.  The padded
  * length is the power of two L >= 2nx-1 (the reference pads to 2nx-1; both
  * give the same first nx rows of the circular convolution).  st stays
- * resident as well (operator expansion). */
+ * resident as well (operator expansion).  Coupled (ne > 0): K_fm
+ * (nf x 6ne), K_mf and K_mm (6ne x N) stay dense FP64; the fault rows add
+ * the DFT traction rates to K_fm s_m. */
 #define OQ_K_FFT 5
 
 /* RectOkadaMesh (src/BEM/mesh.jl:5-21).  Only x, ax, y, z, axi, nx, nxi, dx,
@@ -222,7 +224,9 @@ int oq_rhs_eval(oq_rhs *h, double t, const double *v, const double *theta,
  * ([v θ δ] or [v θ ϵ σ δ]), enqueued on the handle's stream. */
 int oq_rhs_eval_device(oq_rhs *h, double t, const double *u, double *du);
 
-/* Sizes: number of state entries, matrix order N = nf + 6ne, bytes of K. */
+/* Sizes: number of state entries, matrix order N = nf + 6ne, bytes of the
+ * resident operator (dense K; the Toeplitz kernel; or, coupled DFT, the
+ * kernel plus the dense K_fm, K_mf and K_mm blocks). */
 int oq_rhs_info(oq_rhs *h, int64_t *nstate, int64_t *n, int64_t *kbytes);
 
 /* Copies the dense operator back (N*N doubles, row r = receiver), for tests. */

@@ -38,6 +38,10 @@ struct EnsDev {
     const double *gamma, *nexp;        // nlaws*ne (shared)
     int nlaws;
     double deps0[6];
+    // 0 once every member of a lockstep run is done: the step's remaining
+    // kernels then return at once, so the host can queue many steps
+    // without watching for the end (1 outside lockstep runs)
+    const int *live;
 };
 
 // ---------------- prologue: S = [v - vpl ; eps_dot - deps0] ----------------
@@ -45,6 +49,7 @@ __global__ void __launch_bounds__(256) k_ens_prologue(EnsDev d, const double *__
                                                       double *__restrict__ du,
                                                       float *__restrict__ S)
 {
+    if (!*d.live) return;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t M = d.M;
     if (t < d.nf * M) {
@@ -116,6 +121,7 @@ __global__ void __launch_bounds__(256) k_ens_gemm(EnsDev d, const void *__restri
                                                   const double *__restrict__ u,
                                                   double *__restrict__ du)
 {
+    if (!*d.live) return;
     __shared__ float As[EBM][EBK + 1];
     __shared__ float Bs[EBK][EBN + 4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -297,6 +303,7 @@ __global__ void __launch_bounds__(256) k_h3_split_k(EnsDev d, EnsH3 h, const dou
 // 17.9 us here.)
 __global__ void __launch_bounds__(256) k_h3_prep_s(EnsDev d, EnsH3 h, const float *__restrict__ S32)
 {
+    if (!*d.live) return;
     __shared__ float tile[kHCh][17];
     __shared__ float red[16][17];
     const int tid = threadIdx.x, ml = tid & 15, cg = tid >> 4;
@@ -346,6 +353,7 @@ __global__ void __launch_bounds__(256) k_h3_prep_s(EnsDev d, EnsH3 h, const floa
 // XOR-swizzled rows ran 12 % slower.)
 __global__ void __launch_bounds__(512) k_ens_gemm_h3(EnsDev d, EnsH3 h)
 {
+    if (!*d.live) return;
     extern __shared__ __attribute__((aligned(16))) unsigned char hlds[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave & 1, wm = wave >> 1;
@@ -477,10 +485,143 @@ __global__ void __launch_bounds__(512) k_ens_gemm_h3(EnsDev d, EnsH3 h)
     }
 }
 
+// ---------------- FP32 GEMM streaming K once (OQ_K_FP32) ----------------
+// The same block shape as k_ens_gemm_h3: a 64-row x 256-member tile per
+// workgroup of 8 wavefronts, so each row of K is read from HBM once per RHS
+// (the 128 x 64 tiles of k_ens_gemm read it once per 64 members).  Wave w
+// owns rows [32 (w % 2), +32) x members [64 (w / 2), +64): two 32 x 32 tiles
+// of v_mfma_f32_32x32x2f32.  Stages of 32 columns (K: 64 rows x 128 B, S:
+// 256 members x 128 B, member-major) go through double-buffered LDS with the
+// global loads two stages ahead in registers.  Within a stage the 16 MFMA
+// k-steps take columns 16 (lane / 32) + j, so a lane's A and B fragments
+// for all 16 steps are 64 contiguous bytes each.  The FP32 accumulators are
+// added into FP64 every 256 columns.
+static constexpr int kFBM = 64, kFBN = 256, kFKS = 32, kFRow = 144, kFChunk = 256;
+static constexpr int kFStage = (kFBM + kFBN) * kFRow;
+static constexpr size_t kFLds = 2 * (size_t)kFStage;
+
+// S (entry-major [column][member], pitch Mp) -> member-major [member][column]
+// (pitch ld), 64 x 64 tiles through LDS.
+__global__ void __launch_bounds__(256) k_ens_s_member_major(EnsDev d, const float *__restrict__ S,
+                                                            float *__restrict__ Sm)
+{
+    if (!*d.live) return;
+    __shared__ float tile[64][65];
+    const int64_t c0 = (int64_t)blockIdx.x * 64, m0 = (int64_t)blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int r = ty; r < 64; r += 4) tile[r][tx] = S[(c0 + r) * d.Mp + m0 + tx];  // r: column
+    __syncthreads();
+    for (int r = ty; r < 64; r += 4) Sm[(m0 + r) * d.ld + c0 + tx] = tile[tx][r];  // r: member
+}
+
+__global__ void __launch_bounds__(512) k_ens_gemm_f32(EnsDev d, const float *__restrict__ K,
+                                                      const float *__restrict__ Sm,
+                                                      double *__restrict__ Y)
+{
+    if (!*d.live) return;
+    extern __shared__ __attribute__((aligned(16))) unsigned char flds[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave & 1, wm = wave >> 1;
+    const int64_t rb = (int64_t)blockIdx.x * kFBM, m0 = (int64_t)blockIdx.y * kFBN;
+    const int64_t rowb = 4 * d.ld;  // bytes per row of K and of Sm
+    const unsigned char *kg, *sg[4];
+    int kl, sl[4];
+    {
+        const int r = tid >> 3, pc = tid & 7;
+        const int64_t row = rb + r < d.N ? rb + r : d.N - 1;
+        kg = reinterpret_cast<const unsigned char *>(K) + row * rowb + 16 * pc;
+        kl = r * kFRow + 16 * pc;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int p = tid + 512 * i, mm = p >> 3, qc = p & 7;
+            sg[i] = reinterpret_cast<const unsigned char *>(Sm) + (m0 + mm) * rowb + 16 * qc;
+            sl[i] = (kFBM + mm) * kFRow + 16 * qc;
+        }
+    }
+    u4x pa[5], pb[5];
+    auto gload = [&](u4x *pr, int64_t k0) {  // k0: first column of the stage
+        pr[0] = __builtin_nontemporal_load(reinterpret_cast<const u4x *>(kg + 4 * k0));
+#pragma unroll
+        for (int i = 0; i < 4; ++i) pr[1 + i] = *reinterpret_cast<const u4x *>(sg[i] + 4 * k0);
+    };
+    auto lstore = [&](const u4x *pr, unsigned char *st) {
+        *reinterpret_cast<u4x *>(st + kl) = pr[0];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) *reinterpret_cast<u4x *>(st + sl[i]) = pr[1 + i];
+    };
+    f16v acc[2];
+    double y[2][16];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+        acc[b] = f16v{0};
+#pragma unroll
+        for (int i = 0; i < 16; ++i) y[b][i] = 0.0;
+    }
+    const int lr = lane & 31, lg = lane >> 5;
+    auto compute = [&](const unsigned char *cur) {
+        const unsigned char *pA = cur + (wr * 32 + lr) * kFRow + 64 * lg;
+        f4x a[4], bv[2][4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) a[q] = *reinterpret_cast<const f4x *>(pA + 16 * q);
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const unsigned char *pB = cur + (kFBM + wm * 64 + b * 32 + lr) * kFRow + 64 * lg;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) bv[b][q] = *reinterpret_cast<const f4x *>(pB + 16 * q);
+        }
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+                acc[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[j / 4][j % 4], bv[b][j / 4][j % 4], acc[b],
+                                                              0, 0, 0);
+    };
+    auto fold = [&](int64_t st) {
+        if (((st + 1) * kFKS) % kFChunk != 0) return;
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) y[b][i] += (double)acc[b][i];
+            acc[b] = f16v{0};
+        }
+    };
+    // pipeline as k_ens_gemm_h3 (nst even: ld is a multiple of 512)
+    const int64_t nst = d.ld / kFKS;
+    auto kof = [&](int64_t st) { return (st < nst ? st : nst - 1) * kFKS; };
+    gload(pa, 0);
+    gload(pb, kof(1));
+    lstore(pa, flds);
+    __syncthreads();
+    for (int64_t st = 0; st < nst; st += 2) {
+        gload(pa, kof(st + 2));
+        compute(flds);
+        lstore(pb, flds + kFStage);
+        fold(st);
+        __syncthreads();
+        gload(pb, kof(st + 3));
+        compute(flds + kFStage);
+        lstore(pa, flds);
+        fold(st + 1);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) y[b][i] += (double)acc[b][i];  // the last partial chunk
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int64_t row = rb + wr * 32 + (i / 4) * 8 + lg * 4 + (i % 4);
+        if (row >= d.N) continue;
+#pragma unroll
+        for (int b = 0; b < 2; ++b) Y[row * d.Mp + m0 + wm * 64 + b * 32 + lr] = y[b][i];
+    }
+}
+
 // epilogue of the split-FP16 GEMM: one thread per (row, member)
 __global__ void __launch_bounds__(256) k_ens_finish(EnsDev d, const double *__restrict__ Y,
                                                     const double *__restrict__ u, double *__restrict__ du)
 {
+    if (!*d.live) return;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= d.N * d.M) return;
     const int64_t row = t / d.M, m = t % d.M;
@@ -513,7 +654,21 @@ struct EnsCtrl {  // arrays of M
     // and per-member rings rec_t/rec_v [M x rec_cap], slot (naccept-1) % cap
     long long nf, rec_cap;
     double *vpart, *rec_t, *rec_v;
+    int *live;  // the flag EnsDev::live reads (see k_ens_live)
 };
+
+// After k_ens_begin: live = 1 while any member is still running.  One block.
+__global__ void __launch_bounds__(1024) k_ens_live(EnsCtrl c)
+{
+    __shared__ int any;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    int mine = 0;
+    for (int m = threadIdx.x; m < c.M; m += blockDim.x) mine |= !c.done[m];
+    if (mine) any = 1;  // benign race: every writer writes 1
+    __syncthreads();
+    if (threadIdx.x == 0) *c.live = any;
+}
 
 __global__ void k_ens_begin(EnsCtrl c)
 {
@@ -541,6 +696,7 @@ __global__ void __launch_bounds__(256) k_ens_stage(EnsCtrl c, int s, int64_t nM,
                                                    const double *__restrict__ k,
                                                    double *__restrict__ y)
 {
+    if (!*c.live) return;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nM) return;
     const double dt = c.dtcur[i % M];
@@ -557,6 +713,7 @@ __global__ void __launch_bounds__(256) k_ens_err(EnsCtrl c, int64_t n, int64_t M
                                                  const double *__restrict__ k,
                                                  double *__restrict__ part)
 {
+    if (!*c.live) return;
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int chunk = blockIdx.y;
     if (m >= M) return;
@@ -626,6 +783,7 @@ __global__ void __launch_bounds__(256) k_ens_commit(EnsCtrl c, int64_t nM, int64
                                                     const double *__restrict__ y,
                                                     double *__restrict__ k)
 {
+    if (!*c.live) return;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nM || !c.accept[i % M]) return;
     u[i] = y[i];
@@ -639,11 +797,13 @@ struct oq_ensemble {
     int kdtype = OQ_K_FP32;
     int64_t M = 0, Mp = 0, n = 0;
     oq::DevBuf hK, hrsc, hS, hssc, hY;  // OQ_K_FP16X2: split operator, split S, scales, Y
+    oq::DevBuf Sm, fY;                  // OQ_K_FP32: member-major S, Y
     oq::EnsH3 h3{};
     oq::DevBuf a, b, L, sigma, eta, vpl, f0, v0, S;
     oq::DevBuf u, y, k, part;
     oq::DevBuf ct, cdt, cdtcur, cqold, cna, cnr, citer, cdone, cacc, cret;
     oq::DevBuf vpart, rec_t, rec_v;  // recorder (oq_ensemble_record)
+    oq::DevBuf live;                 // EnsDev::live
     int64_t rec_cap = 0;
     std::vector<long long> rec_drained;  // records already handed out, per member
     oq::EnsDev dev{};
@@ -673,11 +833,17 @@ static int ens_rhs(oq_ensemble *e, const double *u, double *du)
         OQ_HIP(hipGetLastError());
         return OQ_OK;
     }
+    if (h->kdtype == OQ_K_FP32) {
+        hipLaunchKernelGGL(k_ens_s_member_major, dim3((unsigned)(h->ld / 64), (unsigned)(e->Mp / 64)),
+                           dim3(256), 0, h->stream, e->dev, e->S.as<float>(), e->Sm.as<float>());
+        hipLaunchKernelGGL(k_ens_gemm_f32, dim3(nbk(h->N, kFBM), (unsigned)(e->Mp / kFBN)), dim3(512), kFLds,
+                           h->stream, e->dev, h->K.as<float>(), e->Sm.as<float>(), e->fY.as<double>());
+        hipLaunchKernelGGL(k_ens_finish, dim3(nbk(h->N * e->M, 256)), dim3(256), 0, h->stream, e->dev,
+                           e->fY.as<double>(), u, du);
+        OQ_HIP(hipGetLastError());
+        return OQ_OK;
+    }
     const dim3 grid(nbk(h->N, EBM), (unsigned)(e->Mp / EBN));
-    if (h->kdtype == OQ_K_FP32)
-        hipLaunchKernelGGL(k_ens_gemm<OQ_K_FP32>, grid, dim3(256), 0, h->stream, e->dev, h->K.p,
-                           e->S.as<float>(), u, du);
-    else
         hipLaunchKernelGGL(k_ens_gemm<OQ_K_BF16>, grid, dim3(256), 0, h->stream, e->dev, h->K.p,
                            e->S.as<float>(), u, du);
     OQ_HIP(hipGetLastError());
@@ -691,6 +857,7 @@ static int ens_step(oq_ensemble *e)
     double *u = e->u.as<double>(), *y = e->y.as<double>(), *k = e->k.as<double>();
     int rc;
     hipLaunchKernelGGL(k_ens_begin, dim3(nbk(e->M, 256)), dim3(256), 0, h->stream, e->ctrl);
+    hipLaunchKernelGGL(k_ens_live, dim3(1), dim3(1024), 0, h->stream, e->ctrl);
     for (int s = 1; s < 7; ++s) {
         hipLaunchKernelGGL(k_ens_stage, dim3(nbk(nM, 256)), dim3(256), 0, h->stream, e->ctrl, s, nM,
                            e->M, u, k, y);
@@ -703,6 +870,15 @@ static int ens_step(oq_ensemble *e)
     hipLaunchKernelGGL(k_ens_commit, dim3(nbk(nM, 256)), dim3(256), 0, h->stream, e->ctrl, nM, e->M,
                        u, y, k);
     OQ_HIP(hipGetLastError());
+    return OQ_OK;
+}
+
+// Outside lockstep runs the RHS kernels must run: live = 1 (a copy on the
+// stream, ordered before the RHS that follows).
+static int set_live(oq_ensemble *e)
+{
+    static const int one = 1;
+    OQ_HIP(hipMemcpyAsync(e->live.p, &one, sizeof(int), hipMemcpyHostToDevice, e->h->stream));
     return OQ_OK;
 }
 
@@ -793,7 +969,7 @@ int oq_ensemble_create(int device, const oq_fault_mesh *mf, const oq_hex8_mesh *
     oq_rhs *h = e->h;
     const int64_t M = nmembers, nf = h->nf;
     e->M = M;
-    const int64_t mb = kdtype == OQ_K_FP16X2 ? kHBN : EBN;
+    const int64_t mb = kdtype == OQ_K_FP16X2 ? kHBN : (kdtype == OQ_K_FP32 ? kFBN : EBN);
     e->Mp = (M + mb - 1) / mb * mb;
     e->n = h->lay.n;
     std::vector<const double *> pa(M), pb(M), pL(M), ps(M);
@@ -818,6 +994,10 @@ int oq_ensemble_create(int device, const oq_fault_mesh *mf, const oq_hex8_mesh *
     if (!rc) rc = e->y.alloc((size_t)nM * 8);
     if (!rc) rc = e->k.alloc((size_t)(7 * nM) * 8);
     if (!rc) rc = e->part.alloc((size_t)(kEnsChunks * M) * 8);
+    {
+        const int one = 1;
+        if (!rc) rc = e->live.upload(&one, sizeof(int), h->stream);
+    }
     DevBuf *cb8[] = {&e->ct, &e->cdt, &e->cdtcur, &e->cqold, &e->cna, &e->cnr, &e->citer};
     for (DevBuf *bp : cb8)
         if (!rc) rc = bp->alloc(M8);
@@ -835,6 +1015,7 @@ int oq_ensemble_create(int device, const oq_fault_mesh *mf, const oq_hex8_mesh *
     d.a = e->a.as<double>(); d.b = e->b.as<double>(); d.L = e->L.as<double>(); d.sigma = e->sigma.as<double>();
     d.eta = e->eta.as<double>(); d.vpl = e->vpl.as<double>(); d.f0 = e->f0.as<double>(); d.v0 = e->v0.as<double>();
     d.gamma = h->dev.gamma; d.nexp = h->dev.nexp; d.nlaws = h->dev.nlaws;
+    d.live = e->live.as<int>();
     for (int k = 0; k < 6; ++k) d.deps0[k] = h->dev.deps0[k];
     EnsCtrl &c = e->ctrl;
     c.t = e->ct.as<double>(); c.dt = e->cdt.as<double>(); c.dtcur = e->cdtcur.as<double>();
@@ -842,9 +1023,25 @@ int oq_ensemble_create(int device, const oq_fault_mesh *mf, const oq_hex8_mesh *
     c.iter = e->citer.as<long long>(); c.done = e->cdone.as<int>(); c.accept = e->cacc.as<int>();
     c.retcode = e->cret.as<int>(); c.M = (int)M;
     c.nf = nf; c.rec_cap = 0; c.vpart = c.rec_t = c.rec_v = nullptr;
+    c.live = e->live.as<int>();
     if (kdtype == OQ_K_FP16X2 && (rc = build_h3(e))) {
         ens_destroy(e);
         return rc;
+    }
+    if (kdtype == OQ_K_FP32) {
+        const size_t sb = (size_t)(e->Mp * h->ld) * 4;
+        rc = e->Sm.alloc(sb);
+        if (!rc && hipMemsetAsync(e->Sm.p, 0, sb, h->stream) != hipSuccess) rc = OQ_ERR_HIP;
+        if (!rc) rc = e->fY.alloc((size_t)(h->N * e->Mp) * 8);
+        if (!rc && hipFuncSetAttribute((const void *)k_ens_gemm_f32,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFLds) != hipSuccess)
+            rc = OQ_ERR_HIP;
+        if (!rc && hipStreamSynchronize(h->stream) != hipSuccess) rc = OQ_ERR_HIP;
+        if (rc) {
+            set_error("FP32 ensemble buffers: allocation or kernel set-up failed");
+            ens_destroy(e);
+            return rc;
+        }
     }
     *out = e;
     return OQ_OK;
@@ -860,7 +1057,8 @@ int oq_ensemble_eval(oq_ensemble *e, const double *u, double *du)
     for (int64_t m = 0; m < e->M; ++m) src[m] = u + m * e->n;
     const std::vector<double> ui = interleave(src.data(), e->n, e->M);
     OQ_HIP(hipMemcpyAsync(e->y.p, ui.data(), (size_t)nM * 8, hipMemcpyHostToDevice, h->stream));
-    int rc = ens_rhs(e, e->y.as<double>(), e->k.as<double>());
+    int rc = set_live(e);
+    if (!rc) rc = ens_rhs(e, e->y.as<double>(), e->k.as<double>());
     if (rc) return rc;
     std::vector<double> o((size_t)nM);
     OQ_HIP(hipMemcpyAsync(o.data(), e->k.p, (size_t)nM * 8, hipMemcpyDeviceToHost, h->stream));
@@ -908,7 +1106,8 @@ int oq_ensemble_start(oq_ensemble *e, const double *u0, double t0, double t1,
     c.rec_v = e->rec_cap ? e->rec_v.as<double>() : nullptr;
     e->rec_drained.assign((size_t)M, 0);
     // FSAL seed: k1 = f(u0)
-    int rc = ens_rhs(e, e->u.as<double>(), e->k.as<double>());
+    int rc = set_live(e);
+    if (!rc) rc = ens_rhs(e, e->u.as<double>(), e->k.as<double>());
     if (rc) return rc;
     if (e->exec) { (void)hipGraphExecDestroy(e->exec); e->exec = nullptr; }
     if (e->graph) { (void)hipGraphDestroy(e->graph); e->graph = nullptr; }
@@ -936,10 +1135,14 @@ int oq_ensemble_run(oq_ensemble *e, int64_t max_steps, int64_t *nrunning)
                "with the recorder on, max_steps must be <= step_capacity (one drain per run)");
     oq_rhs *h = e->h;
     OQ_HIP(hipSetDevice(h->device));
+    // Steps are queued back to back: once every member is done, the
+    // remaining ones return at their first kernel (EnsDev::live), so the
+    // host looks at the done flags only every kRunSync steps.
+    constexpr int64_t kRunSync = 256;
     std::vector<int> done(e->M);
     int64_t left = e->M;
-    for (int64_t s = 0; s < max_steps; s += 8) {
-        const int64_t m = (max_steps - s) < 8 ? (max_steps - s) : 8;
+    for (int64_t s = 0; s < max_steps; s += kRunSync) {
+        const int64_t m = (max_steps - s) < kRunSync ? (max_steps - s) : kRunSync;
         for (int64_t i = 0; i < m; ++i) OQ_HIP(hipGraphLaunch(e->exec, h->stream));
         OQ_HIP(hipMemcpyAsync(done.data(), e->cdone.p, e->M * 4, hipMemcpyDeviceToHost, h->stream));
         OQ_HIP(hipStreamSynchronize(h->stream));
@@ -1049,6 +1252,7 @@ int oq_ensemble_time(oq_ensemble *e, int iters, double *ms_eval)
     oq_rhs *h = e->h;
     OQ_HIP(hipSetDevice(h->device));
     int rc;
+    if ((rc = set_live(e))) return rc;
     OQ_HIP(hipEventRecord(h->ev0, h->stream));
     for (int i = 0; i < iters; ++i)
         if ((rc = ens_rhs(e, e->y.as<double>(), e->k.as<double>()))) return rc;

@@ -74,6 +74,9 @@ struct GrpDev {
     float *S;
     double *Y;  // K S row sums, FP64, entry-major per geometry
     int64_t nprolog, ntot, nrows;
+    // 0 once every member of a lockstep run is done: the step's remaining
+    // kernels return at once (1 outside lockstep runs)
+    const int *live;
 };
 
 // segment g with off[g] <= x < off[g+1] (off ascending, off[0] = 0)
@@ -106,6 +109,7 @@ __device__ __forceinline__ int seg_find_block(const int64_t *__restrict__ off, i
 __global__ void __launch_bounds__(256) k_grp_prologue(GrpDev d, const double *__restrict__ u,
                                                       double *__restrict__ du)
 {
+    if (!*d.live) return;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int g = seg_find_block(d.poffs, d.G, t);
     if (t >= d.nprolog) return;
@@ -181,6 +185,7 @@ __device__ __forceinline__ void grp_store_rows(const GrpDev &d, const GrpGeo &G,
 __global__ void __launch_bounds__(256) k_grp_finish(GrpDev d, const double *__restrict__ u,
                                                     double *__restrict__ du)
 {
+    if (!*d.live) return;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int g = seg_find_block(d.yoffs, d.G, t);
     if (t >= d.nrows) return;
@@ -214,6 +219,7 @@ __global__ void __launch_bounds__(256) k_grp_gemm(GrpDev d, const int2 *__restri
                                                   int ntiles, const double *__restrict__ u,
                                                   double *__restrict__ du)
 {
+    if (!*d.live) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lr = lane & 15, lq = lane >> 4;
     for (int tix = blockIdx.x; tix < ntiles; tix += gridDim.x) {
@@ -329,6 +335,7 @@ __global__ void __launch_bounds__(256) k_grp_gemm_lds(GrpDev d, const int2 *__re
                                                       int ntiles, const double *__restrict__ u,
                                                       double *__restrict__ du)
 {
+    if (!*d.live) return;
     constexpr int BR = 64 * RB, BK = 128 / RB, PAD = BK + 4, C4 = BK / 4;
     constexpr int KV = BR * C4 / 256;  // float4 of K per thread per slice (8)
     constexpr int SV = 16 * C4 / 256;  // float4 of S per thread per slice (2 or 1)
@@ -438,6 +445,7 @@ __global__ void __launch_bounds__(256) k_grp_gemm_lds2(GrpDev d, const int2 *__r
                                                        int ntiles, const double *__restrict__ u,
                                                        double *__restrict__ du)
 {
+    if (!*d.live) return;
     constexpr int BR = 64 * RB, BK = 128 / RB, PAD = BK + 4, C4 = BK / 4;
     constexpr int KV = BR * C4 / 256, SV = 16 * C4 / 256, NC = 4 / RB, RS = 256 / C4;
     __shared__ __attribute__((aligned(16))) float As[BR * PAD];
@@ -568,6 +576,7 @@ __global__ void __launch_bounds__(256, 3) k_grp_gemm_units(GrpDev d, const GrpUn
                                                         const int2 *__restrict__ wg_range,
                                                         double *__restrict__ P)
 {
+    if (!*d.live) return;
     constexpr int BR = 64 * RB, BK = 128 / RB, PAD = BK + 4, C4 = BK / 4;
     constexpr int KV = BR * C4 / 256, SV = 16 * C4 / 256, NC = 4 / RB, RS = 256 / C4;
     __shared__ __attribute__((aligned(16))) float As[BR * PAD];
@@ -692,6 +701,7 @@ template <int BR>
 __global__ void __launch_bounds__(256) k_grp_sum_partials(GrpDev d, const GrpRowBlock *__restrict__ rbl,
                                                           const double *__restrict__ P)
 {
+    if (!*d.live) return;
     const GrpRowBlock B = rbl[blockIdx.y];
     const int M = d.M;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -728,7 +738,21 @@ struct GrpCtrl {  // arrays of G*M
     double t1, reltol, abstol, dtmax, dtmin;
     long long maxiters;
     int G, M;
+    int *live;  // the flag GrpDev::live reads (k_grp_live)
 };
+
+// After k_grp_begin: live = 1 while any member is still running.  One block.
+__global__ void __launch_bounds__(1024) k_grp_live(GrpCtrl c)
+{
+    __shared__ int any;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    int mine = 0;
+    for (int m = threadIdx.x; m < c.G * c.M; m += blockDim.x) mine |= !c.done[m];
+    if (mine) any = 1;  // benign race: every writer writes 1
+    __syncthreads();
+    if (threadIdx.x == 0) *c.live = any;
+}
 
 __global__ void k_grp_begin(GrpCtrl c)
 {
@@ -763,6 +787,7 @@ __global__ void __launch_bounds__(256) k_grp_stage(GrpCtrl c, int s, int64_t nto
                                                    const double *__restrict__ k,
                                                    double *__restrict__ y)
 {
+    if (!*c.live) return;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t q = grp_member(c, i);
     if (i >= ntot) return;
@@ -778,6 +803,7 @@ __global__ void __launch_bounds__(256) k_grp_err(GrpCtrl c, int64_t ntot,
                                                  const double *__restrict__ k,
                                                  double *__restrict__ part)
 {
+    if (!*c.live) return;
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int chunk = blockIdx.y;
     const int64_t GM = (int64_t)c.G * c.M;
@@ -839,6 +865,7 @@ __global__ void __launch_bounds__(256) k_grp_commit(GrpCtrl c, int64_t ntot,
                                                     const double *__restrict__ y,
                                                     double *__restrict__ k)
 {
+    if (!*c.live) return;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t q = grp_member(c, i);
     if (i >= ntot || !c.accept[q]) return;
@@ -890,6 +917,7 @@ struct oq_group {
     size_t pcap = 0;
     oq::DevBuf u, y, k, part;
     oq::DevBuf ct, cdt, cdtcur, cqold, cna, cnr, citer, cdone, cacc, cret;
+    oq::DevBuf live;  // GrpDev::live
     oq::GrpDev dev{};
     oq::GrpCtrl ctrl{};
     bool stepping = false;
@@ -1080,6 +1108,7 @@ static int grp_step(oq_group *e)
     double *u = e->u.as<double>(), *y = e->y.as<double>(), *k = e->k.as<double>();
     int rc;
     hipLaunchKernelGGL(k_grp_begin, dim3(gbk(GM, 256)), dim3(256), 0, e->stream, e->ctrl);
+    hipLaunchKernelGGL(k_grp_live, dim3(1), dim3(1024), 0, e->stream, e->ctrl);
     for (int s = 1; s < 7; ++s) {
         hipLaunchKernelGGL(k_grp_stage, dim3(gbk(nt, 256)), dim3(256), 0, e->stream, e->ctrl, s, nt,
                            u, k, y);
@@ -1092,6 +1121,15 @@ static int grp_step(oq_group *e)
     hipLaunchKernelGGL(k_grp_commit, dim3(gbk(nt, 256)), dim3(256), 0, e->stream, e->ctrl, nt, u, y,
                        k);
     OQ_HIP(hipGetLastError());
+    return OQ_OK;
+}
+
+// Outside lockstep runs the RHS kernels must run: live = 1 (a copy on the
+// stream, ordered before what follows).
+static int set_live(oq_group *e)
+{
+    static const int one = 1;
+    OQ_HIP(hipMemcpyAsync(e->live.p, &one, sizeof(int), hipMemcpyHostToDevice, e->stream));
     return OQ_OK;
 }
 
@@ -1265,6 +1303,10 @@ int oq_group_create(int device, int64_t ngeo, const oq_fault_mesh *mf,
     if (!rc) rc = e->y.alloc((size_t)soff * 8);
     if (!rc) rc = e->k.alloc((size_t)(7 * soff) * 8);
     if (!rc) rc = e->part.alloc((size_t)(kGChunks * GM) * 8);
+    {
+        const int one = 1;
+        if (!rc) rc = e->live.upload(&one, sizeof(int), e->stream);
+    }
     DevBuf *cb8[] = {&e->ct, &e->cdt, &e->cdtcur, &e->cqold, &e->cna, &e->cnr, &e->citer};
     for (DevBuf *bp : cb8)
         if (!rc) rc = bp->alloc((size_t)GM * 8);
@@ -1299,6 +1341,7 @@ int oq_group_create(int device, int64_t ngeo, const oq_fault_mesh *mf,
     d.eta = e->eta.as<double>(); d.vpl = e->vpl.as<double>(); d.f0 = e->f0.as<double>(); d.v0 = e->v0.as<double>();
     d.S = e->S.as<float>();
     d.Y = e->Y.as<double>();
+    d.live = e->live.as<int>();
     d.yoffs = e->dyoffs.as<int64_t>();
     d.nrows = e->nrows;
     d.nprolog = e->nprolog;
@@ -1310,6 +1353,7 @@ int oq_group_create(int device, int64_t ngeo, const oq_fault_mesh *mf,
     c.retcode = e->cret.as<int>();
     c.soffs = e->dsoffs.as<int64_t>(); c.nst = e->dnst.as<int64_t>();
     c.G = (int)ngeo; c.M = (int)M;
+    c.live = e->live.as<int>();
     *out = e;
     return OQ_OK;
 }
@@ -1346,6 +1390,7 @@ int oq_group_autotune(oq_group *e, int iters, int *shape_of_bucket, double *ms_o
 {
     OQ_REQUIRE(e != nullptr && iters > 0, "invalid argument");
     OQ_HIP(hipSetDevice(e->device));
+    if (int rc0 = set_live(e)) return rc0;
     // S = f(state 0) is irrelevant to the timing; the output goes to scratch
     double *u = e->y.as<double>(), *du = e->k.as<double>();
     std::vector<double> tab((size_t)(kGBuckets * kGShapes), 0.0);
@@ -1439,7 +1484,8 @@ int oq_group_eval(oq_group *e, const double *u, double *du)
     std::vector<double> ui;
     to_entry_major(e, u, ui);
     OQ_HIP(hipMemcpyAsync(e->y.p, ui.data(), ui.size() * 8, hipMemcpyHostToDevice, e->stream));
-    int rc = grp_rhs(e, e->y.as<double>(), e->k.as<double>());
+    int rc = set_live(e);
+    if (!rc) rc = grp_rhs(e, e->y.as<double>(), e->k.as<double>());
     if (rc) return rc;
     std::vector<double> o((size_t)e->ntot);
     OQ_HIP(hipMemcpyAsync(o.data(), e->k.p, o.size() * 8, hipMemcpyDeviceToHost, e->stream));
@@ -1453,6 +1499,7 @@ int oq_group_time(oq_group *e, int iters, double *ms_eval)
     OQ_REQUIRE(e && iters > 0, "invalid argument");
     OQ_HIP(hipSetDevice(e->device));
     int rc;
+    if ((rc = set_live(e))) return rc;
     OQ_HIP(hipEventRecord(e->ev0, e->stream));
     for (int i = 0; i < iters; ++i)
         if ((rc = grp_rhs(e, e->y.as<double>(), e->k.as<double>()))) return rc;
@@ -1494,7 +1541,8 @@ int oq_group_start(oq_group *e, const double *u0, double t0, double t1, const oq
     c.dtmin = o->dtmin;
     c.maxiters = o->maxiters > 0 ? o->maxiters : (long long)1e18;
     // FSAL seed: k1 = f(u0)
-    int rc = grp_rhs(e, e->u.as<double>(), e->k.as<double>());
+    int rc = set_live(e);
+    if (!rc) rc = grp_rhs(e, e->u.as<double>(), e->k.as<double>());
     if (rc) return rc;
     if (e->exec) { (void)hipGraphExecDestroy(e->exec); e->exec = nullptr; }
     if (e->graph) { (void)hipGraphDestroy(e->graph); e->graph = nullptr; }
@@ -1522,8 +1570,10 @@ int oq_group_run(oq_group *e, int64_t max_steps, int64_t *nrunning)
     const int64_t GM = e->G * e->M;
     std::vector<int> done(GM);
     int64_t left = GM;
-    for (int64_t s = 0; s < max_steps; s += 8) {
-        const int64_t m = (max_steps - s) < 8 ? (max_steps - s) : 8;
+    // steps queued back to back; past the end they return at once (GrpDev::live)
+    constexpr int64_t kRunSync = 256;
+    for (int64_t s = 0; s < max_steps; s += kRunSync) {
+        const int64_t m = (max_steps - s) < kRunSync ? (max_steps - s) : kRunSync;
         for (int64_t i = 0; i < m; ++i) OQ_HIP(hipGraphLaunch(e->exec, e->stream));
         OQ_HIP(hipMemcpyAsync(done.data(), e->cdone.p, GM * 4, hipMemcpyDeviceToHost, e->stream));
         OQ_HIP(hipStreamSynchronize(e->stream));

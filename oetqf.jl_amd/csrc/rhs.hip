@@ -38,7 +38,8 @@ static constexpr int64_t kPad = 512;  // row pitch multiple (elements)
 __global__ void __launch_bounds__(256) k_rhs_prologue(RhsDev d, const double *__restrict__ u,
                                                       double *__restrict__ du,
                                                       double *__restrict__ s,
-                                                      float *__restrict__ s32)
+                                                      float *__restrict__ s32,
+                                                      double *__restrict__ sm)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t nf = d.lay.nf, ne = d.lay.ne;
@@ -68,6 +69,7 @@ __global__ void __launch_bounds__(256) k_rhs_prologue(RhsDev d, const double *__
         de[i + k * ne] = e[k];
         s[nf + i + k * ne] = x;
         if (s32) s32[nf + i + k * ne] = (float)x;
+        if (sm) sm[i + k * ne] = x;
     }
 }
 
@@ -170,7 +172,8 @@ __device__ __forceinline__ double dot16(const u32x4 kv, const SVec<OQ_K_BF16> &s
 // concentrate on a few HBM channels (measured 4.9 vs 6.1 TB/s, FP64).
 template <int KT>
 __global__ void __launch_bounds__(KTraits<KT>::kWaves * 64)
-    k_rhs_matvec(RhsDev d, const void *__restrict__ Kp, const double *__restrict__ s,
+    k_rhs_matvec(RhsDev d, const void *__restrict__ Kp, int64_t ld, int64_t nrows, int64_t rowbase,
+                 const double *__restrict__ s, const double *__restrict__ tadd,
                  const double *__restrict__ u, double *__restrict__ du)
 {
     using T = typename KTraits<KT>::T;
@@ -180,20 +183,23 @@ __global__ void __launch_bounds__(KTraits<KT>::kWaves * 64)
     constexpr int64_t STEP = 64 * E;          // columns per wavefront pass
     const int lane = threadIdx.x & 63;
     const int64_t gw = (int64_t)blockIdx.x * W + (threadIdx.x >> 6);
+    // rows [0, nrows) of the operator at Kp (pitch ld) are receivers
+    // rowbase + r; tadd (fault receivers, may be NULL) adds a traction rate
+    // computed elsewhere (the DFT K_ff of the coupled model)
     const int64_t row0 = gw * R;
-    if (row0 >= d.N) return;
+    if (row0 >= nrows) return;
     using V = typename KVec<KT>::V;
     const T *K = static_cast<const T *>(Kp);
     const V *rp[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const int64_t row = row0 + r < d.N ? row0 + r : d.N - 1;
-        rp[r] = reinterpret_cast<const V *>(K + row * d.ld) + lane;
+        const int64_t row = row0 + r < nrows ? row0 + r : nrows - 1;
+        rp[r] = reinterpret_cast<const V *>(K + row * ld) + lane;
     }
     double acc[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) acc[r] = 0.0;
-    const int64_t npass = d.ld / STEP;
+    const int64_t npass = ld / STEP;
     // Software pipeline: the R loads of the next pass are issued before the
     // FMAs of the current one, so 2R 16-byte loads per lane are in flight;
     // the scheduling barriers keep the compiler from interleaving each load
@@ -228,9 +234,9 @@ __global__ void __launch_bounds__(KTraits<KT>::kWaves * 64)
 #pragma unroll
         for (int r = 1; r < R; ++r)
             if (lane == r) y = acc[r];
-        const int64_t row = row0 + lane;
-        if (row < d.N) {
-            if (row < d.lay.nf) fault_update(d, row, y, u, du);
+        if (row0 + lane < nrows) {
+            const int64_t row = rowbase + row0 + lane;
+            if (row < d.lay.nf) fault_update(d, row, tadd ? y + tadd[row] : y, u, du);
             else du[d.lay.sig + (row - d.lay.nf)] = y;
         }
     }
@@ -238,11 +244,13 @@ __global__ void __launch_bounds__(KTraits<KT>::kWaves * 64)
 
 static inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-template <int KT> static void launch_matvec(oq_rhs *h, const double *s, const double *u, double *du)
+template <int KT>
+static void launch_matvec(oq_rhs *h, const void *K, int64_t ld, int64_t nrows, int64_t rowbase,
+                          const double *s, const double *tadd, const double *u, double *du)
 {
     constexpr int rows = KTraits<KT>::kRows * KTraits<KT>::kWaves;
-    hipLaunchKernelGGL(k_rhs_matvec<KT>, dim3(nblk(h->N, rows)), dim3(KTraits<KT>::kWaves * 64), 0,
-                       h->stream, h->dev, h->K.p, s, u, du);
+    hipLaunchKernelGGL(k_rhs_matvec<KT>, dim3(nblk(nrows, rows)), dim3(KTraits<KT>::kWaves * 64), 0,
+                       h->stream, h->dev, K, ld, nrows, rowbase, s, tadd, u, du);
 }
 
 // ---------------- Toeplitz K_ff (fault-only problems, OQ_K_TOEPLITZ) ----------------
@@ -615,8 +623,12 @@ __global__ void __launch_bounds__(256) k_fft_inv(RhsDev d, const double *__restr
     }
     __syncthreads();
     lds_fft(re, im, L, cs, sn, 1.0);
-    for (int64_t n = threadIdx.x; n < nx; n += blockDim.x)
-        fault_update(d, n + nx * j, re[n] / (double)L, u, du);
+    // fault only: the rate-and-state update; coupled (ne > 0): the K_ff
+    // traction rates go to du = tauff for the fault rows' K_fm pass
+    for (int64_t n = threadIdx.x; n < nx; n += blockDim.x) {
+        if (d.lay.ne > 0) du[n + nx * j] = re[n] / (double)L;
+        else fault_update(d, n + nx * j, re[n] / (double)L, u, du);
+    }
 }
 
 static inline size_t fft_lds(int64_t nx) { return (size_t)(3 * fft_len(nx)) * 8; }
@@ -643,6 +655,24 @@ static void toep_shape(const oq_rhs *h, int *lchunk, int *nchunk, int *threads)
 
 int rhs_enqueue_matvec(oq_rhs *h, const double *u, double *du)
 {
+    if (h->kdtype == OQ_K_FFT && h->ne > 0) {
+        // coupled model (equation.jl:185-205 with dτ_dt! by DFT, :200): K_ff s_f
+        // by the DFT convolution into tauff, then the fault rows' K_fm s_m plus
+        // tauff with the rate-and-state update, and the mantle rows [K_mf K_mm] s
+        const int64_t L = fft_len(h->nx);
+        int bits = 0;
+        while (((int64_t)1 << bits) < L) ++bits;
+        hipLaunchKernelGGL(k_fft_fwd, dim3((unsigned)h->nxi), dim3(256), fft_lds(h->nx), h->stream, h->dev, u,
+                           h->nx, h->nxi, L, bits, h->ftw.as<double>(), h->fsh.as<double>());
+        hipLaunchKernelGGL(k_fft_mid, dim3((unsigned)(L / 2 + 1)), dim3(512), 0, h->stream, h->fg.as<double>(),
+                           h->fsh.as<double>(), L / 2 + 1, h->nxi, h->fth.as<double>());
+        hipLaunchKernelGGL(k_fft_inv, dim3((unsigned)h->nxi), dim3(256), fft_lds(h->nx), h->stream, h->dev,
+                           h->fth.as<double>(), h->nx, L, bits, h->ftw.as<double>(), u, h->tauff.as<double>());
+        launch_matvec<OQ_K_FP64>(h, h->Kf.p, h->ldm, h->nf, 0, h->sm.as<double>(), h->tauff.as<double>(), u, du);
+        launch_matvec<OQ_K_FP64>(h, h->K.p, h->ld, 6 * h->ne, h->nf, h->s64.as<double>(), nullptr, u, du);
+        OQ_HIP(hipGetLastError());
+        return OQ_OK;
+    }
     if (h->kdtype == OQ_K_FFT) {  // no separate prologue: k_fft_fwd forms s from u
         const int64_t L = fft_len(h->nx);
         int bits = 0;
@@ -668,9 +698,12 @@ int rhs_enqueue_matvec(oq_rhs *h, const double *u, double *du)
         return OQ_OK;
     }
     switch (h->kdtype) {
-    case OQ_K_FP64: launch_matvec<OQ_K_FP64>(h, h->s64.as<double>(), u, du); break;
-    case OQ_K_FP32: launch_matvec<OQ_K_FP32>(h, h->s64.as<double>(), u, du); break;
-    default: launch_matvec<OQ_K_BF16>(h, reinterpret_cast<const double *>(h->s32.p), u, du); break;
+    case OQ_K_FP64: launch_matvec<OQ_K_FP64>(h, h->K.p, h->ld, h->N, 0, h->s64.as<double>(), nullptr, u, du); break;
+    case OQ_K_FP32: launch_matvec<OQ_K_FP32>(h, h->K.p, h->ld, h->N, 0, h->s64.as<double>(), nullptr, u, du); break;
+    default:
+        launch_matvec<OQ_K_BF16>(h, h->K.p, h->ld, h->N, 0, reinterpret_cast<const double *>(h->s32.p), nullptr,
+                                 u, du);
+        break;
     }
     OQ_HIP(hipGetLastError());
     return OQ_OK;
@@ -695,10 +728,11 @@ int rhs_enqueue_fft_stage(oq_rhs *h, const StageCoef &sc, const double *dtp, con
 
 int rhs_enqueue(oq_rhs *h, const double *u, double *du)
 {
-    if (h->kdtype == OQ_K_FFT) return rhs_enqueue_matvec(h, u, du);  // s is formed there
+    if (h->kdtype == OQ_K_FFT && h->ne == 0) return rhs_enqueue_matvec(h, u, du);  // s is formed there
     const int64_t n = h->nf + h->ne;
     hipLaunchKernelGGL(k_rhs_prologue, dim3(nblk(n, 256)), dim3(256), 0, h->stream, h->dev, u, du,
-                       h->s64.as<double>(), h->kdtype == OQ_K_BF16 ? h->s32.as<float>() : nullptr);
+                       h->s64.as<double>(), h->kdtype == OQ_K_BF16 ? h->s32.as<float>() : nullptr,
+                       h->sm.as<double>());
     OQ_HIP(hipGetLastError());
     return rhs_enqueue_matvec(h, u, du);
 }
@@ -772,8 +806,7 @@ static int alloc_common(oq_rhs *h, int device, int64_t nx, int64_t nxi, int64_t 
     OQ_REQUIRE(kdtype == OQ_K_FP64 || kdtype == OQ_K_FP32 || kdtype == OQ_K_BF16 ||
                    kdtype == OQ_K_TOEPLITZ || kdtype == OQ_K_FFT,
                "kdtype must be OQ_K_FP64, OQ_K_FP32, OQ_K_BF16, OQ_K_TOEPLITZ or OQ_K_FFT");
-    OQ_REQUIRE((kdtype != OQ_K_TOEPLITZ && kdtype != OQ_K_FFT) || ne == 0,
-               "the Toeplitz and DFT operators are for fault-only problems");
+    OQ_REQUIRE(kdtype != OQ_K_TOEPLITZ || ne == 0, "the Toeplitz operator is for fault-only problems");
     OQ_REQUIRE((kdtype != OQ_K_TOEPLITZ && kdtype != OQ_K_FFT) || nx <= 2040,
                "the Toeplitz and DFT operators take nx <= 2040 (a dip row of the kernel or the twiddle "
                "table with one row of s in LDS)");
@@ -852,38 +885,69 @@ static int upload_props(oq_rhs *h, const oq_rs_props *f, const oq_visc_props *v)
 // nx j + nf l]) already on the device in `st`.  Toeplitz: allocate the chunk
 // partials.  DFT: the twiddle table, G (from st, on the device) and the
 // transform buffers.
+// DFT operator set-up from the Toeplitz kernel st (device): twiddles, the
+// kernel's DFT G and the spectra buffers.
+static int fft_setup(oq_rhs *h, const double *st)
+{
+    const int64_t nx = h->nx, nxi = h->nxi, L = fft_len(nx), hl = L / 2;
+    std::vector<double> tw((size_t)(2 * hl));
+    for (int64_t k = 0; k < hl; ++k) {  // cos, sin of 2 pi k / L
+        tw[(size_t)k] = cos(2.0 * M_PI * (double)k / (double)L);
+        tw[(size_t)(hl + k)] = sin(2.0 * M_PI * (double)k / (double)L);
+    }
+    int rc = h->ftw.upload(tw.data(), tw.size() * 8, h->stream);
+    if (!rc) rc = h->fg.alloc((size_t)((hl + 1) * nxi * nxi) * 8);
+    if (!rc) rc = h->fsh.alloc((size_t)(2 * (hl + 1) * nxi) * 8);
+    if (!rc) rc = h->fth.alloc((size_t)(2 * (hl + 1) * nxi) * 8);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_fft_kernel, dim3(grid_capped((hl + 1) * nxi * nxi, 256)), dim3(256), 0, h->stream,
+                       st, nx, nxi, L, h->ftw.as<double>(), h->fg.as<double>());
+    OQ_HIP(hipGetLastError());
+    OQ_HIP(hipFuncSetAttribute((const void *)k_fft_fwd, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)fft_lds(nx)));
+    OQ_HIP(hipFuncSetAttribute((const void *)k_fft_inv, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)fft_lds(nx)));
+    OQ_HIP(hipFuncSetAttribute((const void *)k_fft_fwd_stage, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)(fft_lds(nx) + (size_t)nx * 8)));
+    OQ_HIP(hipStreamSynchronize(h->stream));
+    return OQ_OK;
+}
+
 static int finalize_toeplitz(oq_rhs *h, DevBuf &st)
 {
     h->K.p = st.p;
     h->K.bytes = st.bytes;
     st.p = nullptr;
     st.bytes = 0;
-    if (h->kdtype == OQ_K_FFT) {
-        const int64_t nx = h->nx, nxi = h->nxi, L = fft_len(nx), hl = L / 2;
-        std::vector<double> tw((size_t)(2 * hl));
-        for (int64_t k = 0; k < hl; ++k) {  // cos, sin of 2 pi k / L
-            tw[(size_t)k] = cos(2.0 * M_PI * (double)k / (double)L);
-            tw[(size_t)(hl + k)] = sin(2.0 * M_PI * (double)k / (double)L);
-        }
-        int rc = h->ftw.upload(tw.data(), tw.size() * 8, h->stream);
-        if (!rc) rc = h->fg.alloc((size_t)((hl + 1) * nxi * nxi) * 8);
-        if (!rc) rc = h->fsh.alloc((size_t)(2 * (hl + 1) * nxi) * 8);
-        if (!rc) rc = h->fth.alloc((size_t)(2 * (hl + 1) * nxi) * 8);
-        if (rc) return rc;
-        hipLaunchKernelGGL(k_fft_kernel, dim3(grid_capped((hl + 1) * nxi * nxi, 256)), dim3(256), 0, h->stream,
-                           h->K.as<double>(), nx, nxi, L, h->ftw.as<double>(), h->fg.as<double>());
-        OQ_HIP(hipGetLastError());
-        OQ_HIP(hipFuncSetAttribute((const void *)k_fft_fwd, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)fft_lds(nx)));
-        OQ_HIP(hipFuncSetAttribute((const void *)k_fft_inv, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)fft_lds(nx)));
-        OQ_HIP(hipFuncSetAttribute((const void *)k_fft_fwd_stage, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)(fft_lds(nx) + (size_t)nx * 8)));
-        OQ_HIP(hipStreamSynchronize(h->stream));
-        return OQ_OK;
-    }    int lc, nc, nt;
+    if (h->kdtype == OQ_K_FFT) return fft_setup(h, h->K.as<double>());
+    int lc, nc, nt;
     toep_shape(h, &lc, &nc, &nt);
     return h->tpart.alloc((size_t)(nc * h->nf) * 8);
+}
+
+// Coupled model with the DFT K_ff (OQ_K_FFT, ne > 0): keeps st, sets up the
+// DFT, and allocates (zeroed) the fault rows' K_fm (nf x ldm), the mantle
+// rows [K_mf K_mm] (6ne x ld, in h->K), the mantle source copy and the K_ff
+// traction rates.  The blocks are filled by the caller.
+static int fft_coupled_alloc(oq_rhs *h, DevBuf &st)
+{
+    h->stk.p = st.p;
+    h->stk.bytes = st.bytes;
+    st.p = nullptr;
+    st.bytes = 0;
+    int rc = fft_setup(h, h->stk.as<double>());
+    const int64_t m6 = 6 * h->ne;
+    h->ldm = (m6 + kPad - 1) / kPad * kPad;
+    const size_t bf = (size_t)(h->nf * h->ldm) * 8, bm = (size_t)(m6 * h->ld) * 8;
+    if (!rc) rc = h->Kf.alloc(bf);
+    if (!rc) rc = h->K.alloc(bm);
+    if (!rc) rc = h->sm.alloc((size_t)h->ldm * 8);
+    if (!rc) rc = h->tauff.alloc((size_t)h->nf * 8);
+    if (rc) return rc;
+    OQ_HIP(hipMemsetAsync(h->Kf.p, 0, bf, h->stream));
+    OQ_HIP(hipMemsetAsync(h->K.p, 0, bm, h->stream));
+    OQ_HIP(hipMemsetAsync(h->sm.p, 0, (size_t)h->ldm * 8, h->stream));
+    return OQ_OK;
 }
 
 // Converts the FP64 operator in k64 (N x ld) to h->K in h->kdtype.
@@ -974,6 +1038,33 @@ int oq_rhs_create(int device, int64_t nx, int64_t nxi, int64_t ne, const oq_rs_p
     }
     const int64_t nf = h->nf, m6 = 6 * ne, ld = h->ld;
     DevBuf k64, tmp;
+    if (kdtype == OQ_K_FFT && ne > 0) {
+        rc = upload_toeplitz(gf11, gf11_is_dft, nx, nxi, tmp, h->stream);
+        if (!rc) rc = fft_coupled_alloc(h, tmp);
+        struct Blk { const double *src; int64_t m, n, srs, scs; double *dst; int64_t ldd; };
+        const Blk blks[3] = {{gf21, nf, m6, 1, nf, h->Kf.as<double>(), h->ldm},  // K_fm: nf x 6ne
+                             {gf12, m6, nf, 1, m6, h->K.as<double>(), ld},       // K_mf: 6ne x nf
+                             {gf22, m6, m6, 1, m6, h->K.as<double>() + nf, ld}}; // K_mm: 6ne x 6ne
+        for (int bi = 0; !rc && bi < 3; ++bi) {
+            const Blk &b = blks[bi];
+            if (hipStreamSynchronize(h->stream) != hipSuccess) { rc = OQ_ERR_HIP; break; }
+            rc = tmp.upload(b.src, (size_t)(b.m * b.n) * 8, h->stream);
+            if (rc) break;
+            hipLaunchKernelGGL(k_copy_block, dim3(grid_capped(b.m * b.n, 256)), dim3(256), 0, h->stream,
+                               tmp.as<double>(), b.m, b.n, b.srs, b.scs, b.dst, b.ldd);
+            if (hipGetLastError() != hipSuccess) {
+                set_error("k_copy_block launch failed");
+                rc = OQ_ERR_HIP;
+            }
+        }
+        if (!rc && hipStreamSynchronize(h->stream) != hipSuccess) rc = OQ_ERR_HIP;
+        if (rc) {
+            destroy(h);
+            return rc;
+        }
+        *out = h;
+        return OQ_OK;
+    }
     if (kdtype == OQ_K_TOEPLITZ || kdtype == OQ_K_FFT) {
         rc = upload_toeplitz(gf11, gf11_is_dft, nx, nxi, tmp, h->stream);
         if (!rc) rc = finalize_toeplitz(h, tmp);
@@ -1052,6 +1143,29 @@ int oq_rhs_create_from_mesh(int device, const oq_fault_mesh *mf, const oq_hex8_m
     if (!rc && ma) rc = hu.load(ma);
     if (!rc && ma) rc = qu.load(quad);
     const int64_t nf = h->nf, ld = h->ld;
+    if (!rc && kdtype == OQ_K_FFT && ne > 0) {
+        rc = st.alloc((size_t)(nf * mf->nxi) * 8);
+        if (!rc) rc = launch_gf_ff(fu.dev, lambda, mu, ftype, (int)nrept, st.as<double>(), h->stream);
+        if (!rc && hipStreamSynchronize(h->stream) != hipSuccess) rc = OQ_ERR_HIP;
+        if (!rc) rc = fft_coupled_alloc(h, st);
+        if (!rc) rc = launch_gf_fm(hu.dev, fu.dev, lambda, mu, ftype, h->Kf.as<double>(), h->ldm, 1, h->stream);
+        if (!rc)
+            rc = launch_gf_mf(fu.dev, hu.dev, qu.c.as<double>(), qu.w.as<double>(), qu.nq, lambda, mu, ftype,
+                              (int)nrept, h->K.as<double>(), ld, 1, h->stream);
+        if (!rc)
+            rc = launch_gf_mm(hu.dev, qu.c.as<double>(), qu.w.as<double>(), qu.nq, lambda, mu,
+                              h->K.as<double>() + nf, ld, 1, h->stream);
+        if (!rc && hipStreamSynchronize(h->stream) != hipSuccess) {
+            set_error("device assembly failed");
+            rc = OQ_ERR_HIP;
+        }
+        if (rc) {
+            destroy(h);
+            return rc;
+        }
+        *out = h;
+        return OQ_OK;
+    }
     if (!rc && (kdtype == OQ_K_TOEPLITZ || kdtype == OQ_K_FFT)) {
         rc = st.alloc((size_t)(nf * mf->nxi) * 8);
         if (!rc) rc = launch_gf_ff(fu.dev, lambda, mu, ftype, (int)nrept, st.as<double>(), h->stream);
@@ -1198,7 +1312,7 @@ int oq_rhs_info(oq_rhs *h, int64_t *nstate, int64_t *n, int64_t *kbytes)
     OQ_REQUIRE(h != nullptr, "handle is NULL");
     if (nstate) *nstate = h->lay.n;
     if (n) *n = h->N;
-    if (kbytes) *kbytes = (int64_t)h->K.bytes;
+    if (kbytes) *kbytes = (int64_t)(h->K.bytes + h->Kf.bytes + h->stk.bytes);
     return OQ_OK;
 }
 
@@ -1209,7 +1323,20 @@ int oq_rhs_get_k(oq_rhs *h, double *k)
     const int64_t n = h->N * h->ld;
     DevBuf tmp;
     const double *src = h->K.as<double>();
-    if (h->kdtype == OQ_K_TOEPLITZ || h->kdtype == OQ_K_FFT) {
+    if (h->kdtype == OQ_K_FFT && h->ne > 0) {  // [expanded K_ff  K_fm ; K_mf K_mm]
+        const int64_t nf = h->nf, m6 = 6 * h->ne;
+        int rc = tmp.alloc((size_t)n * 8);
+        if (rc) return rc;
+        OQ_HIP(hipMemsetAsync(tmp.p, 0, (size_t)n * 8, h->stream));
+        if ((rc = launch_expand_toeplitz(h->stk.as<double>(), h->nx, h->nxi, tmp.as<double>(), h->ld,
+                                         h->stream)))
+            return rc;
+        OQ_HIP(hipMemcpy2DAsync(tmp.as<double>() + nf, (size_t)h->ld * 8, h->Kf.p, (size_t)h->ldm * 8,
+                                (size_t)m6 * 8, (size_t)nf, hipMemcpyDeviceToDevice, h->stream));
+        OQ_HIP(hipMemcpy2DAsync(tmp.as<double>() + nf * h->ld, (size_t)h->ld * 8, h->K.p, (size_t)h->ld * 8,
+                                (size_t)h->N * 8, (size_t)m6, hipMemcpyDeviceToDevice, h->stream));
+        src = tmp.as<double>();
+    } else if (h->kdtype == OQ_K_TOEPLITZ || h->kdtype == OQ_K_FFT) {
         int rc = tmp.alloc((size_t)n * 8);
         if (rc) return rc;
         OQ_HIP(hipMemsetAsync(tmp.p, 0, (size_t)n * 8, h->stream));

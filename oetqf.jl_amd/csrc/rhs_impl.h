@@ -185,6 +185,12 @@ struct oq_rhs {
     oq::DevBuf u, du;                   // packed state scratch for the host API
     oq::DevBuf tpart;                   // Toeplitz operator: per-chunk partial sums
     oq::DevBuf ftw, fg, fsh, fth;       // DFT operator: twiddles, kernel DFT, S^, T^
+    // coupled model with the DFT K_ff (OQ_K_FFT, ne > 0): K holds the mantle
+    // rows [K_mf K_mm] (6ne x ld); Kf the fault rows' K_fm (nf x ldm), which
+    // act on sm, the mantle part of s (ldm doubles, zero padded); stk the
+    // Toeplitz kernel; tauff the K_ff traction rates
+    oq::DevBuf Kf, sm, stk, tauff;
+    int64_t ldm = 0;
     oq::RhsDev dev{};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };

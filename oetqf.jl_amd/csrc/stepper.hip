@@ -1075,7 +1075,7 @@ static int enqueue_step(oq_stepper *st)
     hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, h->stream, c);
     // the DFT operator (fault only, no dilatancy) forms each stage inside
     // its forward-FFT kernel: one launch less per stage
-    const bool fused = h->kdtype == OQ_K_FFT && !h->dev.dila;
+    const bool fused = h->kdtype == OQ_K_FFT && !h->dev.dila && h->ne == 0;
     for (int s = 1; s < 7; ++s) {
         if (fused) {
             StageCoef sc{};

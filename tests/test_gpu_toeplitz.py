@@ -72,12 +72,74 @@ def test_toeplitz_operator_and_trajectory(oracle, rng, kdtype):
     assert relerr(st_.u.pack(), sd.u.pack()) <= 1e-8
 
 
-@pytest.mark.parametrize("kdtype", ["toeplitz", "fft"])
-def test_toeplitz_rejects_coupled(oq, oracle, rng, kdtype):
+def test_toeplitz_rejects_coupled(oq, oracle, rng):
+    """The direct Toeplitz operator is fault-only; a mantle is refused."""
     from helpers import small_mantle, visc_props
     mf, ma = small_fault(90.0), small_mantle()
     pf, pa = rs_props(mf, rng), visc_props(ma, rng, 0.0)
     v = pf.vpl * np.ones((mf.nx, mf.nxi))
     u0 = oq.ArrayPartition(v, pf.L / v, np.zeros((ma.ne, 6)), np.zeros((ma.ne, 6)), np.zeros_like(v))
     with pytest.raises(ValueError):
-        oq.assemble_from_mesh(mf, ma, 3e10, 3e10, pf, pa, u0, (0.0, 1.0), kdtype=kdtype)
+        oq.assemble_from_mesh(mf, ma, 3e10, 3e10, pf, pa, u0, (0.0, 1.0), kdtype="toeplitz")
+
+
+def _coupled_state(mf, ma, pf, rng):
+    v, th, dl = (pf.vpl * np.exp(rng.standard_normal((mf.nx, mf.nxi))),
+                 pf.L / pf.vpl * np.exp(0.3 * rng.standard_normal((mf.nx, mf.nxi))),
+                 rng.random((mf.nx, mf.nxi)))
+    return oq.ArrayPartition(v, th, np.zeros((ma.ne, 6)), 1e6 * rng.standard_normal((ma.ne, 6)), dl)
+
+
+@pytest.mark.parametrize("fourier", [False, True])
+def test_fft_coupled_rhs_matches_oracle(oracle, rng, fourier):
+    """The coupled model with K_ff applied by DFT convolution, as the
+    reference's coupled ode applies it (equation.jl:185-205, dτ_dt! by FFT
+    at :200), the other three blocks dense: every component of du against
+    the oracle's coupled RHS, and against the same model with K dense."""
+    from helpers import small_mantle, visc_props
+    mf, ma = small_fault(60.0, x=40e3, xi=15e3, d=1.25e3), small_mantle(4, 2, 3)
+    lam = mu = 3e10
+    pf, pa = rs_props(mf, rng), visc_props(ma, rng, 3.0)
+    u = _coupled_state(mf, ma, pf, rng)
+    st = oracle.gf_ff(mf, lam, mu, buffer_ratio=1.0)
+    gf11 = oq.stress_greens_function(mf, lam, mu, buffer_ratio=1.0, fourier=fourier)
+    g12, g21, g22 = (oracle.gf_mf(mf, ma, lam, mu, buffer_ratio=1.0), oracle.gf_fm(ma, mf, lam, mu),
+                     oracle.gf_mm(ma, lam, mu))
+    pt = oq.assemble(gf11, g12, g21, g22, pf, pa, u, (0.0, 1.0), kdtype="fft")
+    pd = oq.assemble(st, g12, g21, g22, pf, pa, u, (0.0, 1.0))
+    dt_, dd = oq.similar(u), oq.similar(u)
+    pt.f(dt_, u, None, 0.0)
+    pd.f(dd, u, None, 0.0)
+    ref = oracle.rhs_coupled(pf, pa, st, g12, g21, g22, u.x[0], u.x[1], u.x[3])
+    for k in range(5):
+        assert relerr(dt_.x[k], ref[k]) <= 1e-12, k
+        assert relerr(dt_.x[k], dd.x[k]) <= 1e-12, k
+    assert np.array_equal(pt.f.dense_operator(), pd.f.dense_operator())
+    pt.f.close()
+    pd.f.close()
+
+
+def test_fft_coupled_from_mesh_and_trajectory(rng):
+    """Device assembly of the coupled DFT model equals the dense device
+    assembly (operator bit for bit, RHS to 1e-12), and a Tsit5 trajectory
+    on it follows the dense one."""
+    from helpers import small_mantle, visc_props
+    mf, ma = small_fault(90.0), small_mantle()
+    pf, pa = rs_props(mf, rng), visc_props(ma, rng, 0.0)
+    u = _coupled_state(mf, ma, pf, rng)
+    kw = dict(buffer_ratio=1.0, qtype="Gauss2")
+    pt = oq.assemble_from_mesh(mf, ma, 3e10, 3e10, pf, pa, u, (0.0, 0.05 * YEAR), kdtype="fft", **kw)
+    pd = oq.assemble_from_mesh(mf, ma, 3e10, 3e10, pf, pa, u, (0.0, 0.05 * YEAR), **kw)
+    assert np.array_equal(pt.f.dense_operator(), pd.f.dense_operator())
+    dt_, dd = oq.similar(u), oq.similar(u)
+    pt.f(dt_, u, None, 0.0)
+    pd.f(dd, u, None, 0.0)
+    for k in range(5):
+        assert relerr(dt_.x[k], dd.x[k]) <= 1e-12, k
+    st_ = oq.solve(pt, oq.Tsit5(), reltol=1e-6, abstol=1e-8)
+    sd = oq.solve(pd, oq.Tsit5(), reltol=1e-6, abstol=1e-8)
+    assert st_.retcode == sd.retcode == "Success"
+    assert abs(st_.naccept - sd.naccept) <= 1
+    assert relerr(st_.u.pack(), sd.u.pack()) <= 1e-8
+    pt.f.close()
+    pd.f.close()
