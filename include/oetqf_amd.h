@@ -155,6 +155,23 @@ int oq_gf_okada_ff_dft(const oq_fault_mesh *mesh, double lambda, double mu,
                        int ftype, int64_t nrept, double buffer_ratio,
                        double *st_dft);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): the
+ * fourier=true K_ff of oq_gf_okada_ff_dft with nothing allocated.  The mesh
+ * arrays (mesh->x, ax, y, z, axi) and st_dft (2*nx*nxi*nxi doubles, same
+ * layout) are DEVICE pointers; the work is enqueued on `stream` (a
+ * hipStream_t; NULL = the default stream) and the call returns without
+ * waiting.  The real kernel st between the two launches goes through a
+ * static device scratch array of 1048576 doubles (8 MiB, held by each GPU
+ * the library runs on until the process ends): nx*nxi*nxi above that is
+ * rejected with OQ_ERR_ARG before anything is enqueued.  Calls on one GPU
+ * share that array, so calls that overlap on one GPU (different streams or
+ * threads) race.  The first call on each device looks the array's address
+ * up (hipGetSymbolAddress, no allocation); make it outside graph capture. */
+int oq_gf_okada_ff_dft_noalloc(const oq_fault_mesh *mesh_dev, double lambda, double mu,
+                               int ftype, int64_t nrept, double buffer_ratio,
+                               double *st_dft_dev, void *stream);
+
 
 /* Replaces stress_greens_function(mf::RectOkadaMesh, ma::BEMHex8Mesh, λ, μ;
  * ftype, qtype, nrept, buffer_ratio) (src/BEM/GF.jl:123-174), gf₁₂.

@@ -71,6 +71,7 @@ PROTOTYPES = {
     "oq_device_count": (_int, [ctypes.POINTER(_int)]),
     "oq_gf_okada_ff": (_int, [ctypes.POINTER(FaultMesh), _d, _d, _int, _i64, _d, _pd]),
     "oq_gf_okada_ff_dft": (_int, [ctypes.POINTER(FaultMesh), _d, _d, _int, _i64, _d, _pd]),
+    "oq_gf_okada_ff_dft_noalloc": (_int, [ctypes.POINTER(FaultMesh), _d, _d, _int, _i64, _d, _vp, _vp]),
     "oq_gf_okada_mf": (_int, [ctypes.POINTER(FaultMesh), ctypes.POINTER(Hex8Mesh), _d, _d, _int,
                               ctypes.POINTER(Quadrature), _i64, _d, _pd]),
     "oq_gf_hex8_fm": (_int, [ctypes.POINTER(Hex8Mesh), ctypes.POINTER(FaultMesh), _d, _d, _int,
