@@ -530,18 +530,34 @@ def _progress(tag):
     return f
 
 
-def spans_composite(oq):
+def _load_refs(paths):
+    """Legs of earlier spans JSON files (comma-separated paths), merged."""
+    out = {}
+    for path in [x for x in (paths or "").split(",") if x]:
+        with open(path) as fh:
+            for leg, v in json.load(fh).items():
+                if leg == "composite" and leg in out:
+                    out[leg]["runs"].update(v["runs"])
+                else:
+                    out[leg] = v
+    return out
+
+
+def spans_composite(oq, runs, refs, save=None):
     """The composite workload (128x32 transform fault + 16x16x8 mantle) over
-    its whole 500-year span from the seeded interseismic start: FP64 with
-    Tsit5 and VCABM5, FP32 with VCABM5; event catalogues (max v > 1e-3 m/s,
-    crossings within 1 yr merged) against the FP64 run of the same method."""
+    its whole 500-year span from the seeded interseismic start, one run per
+    (operator type, method) in `runs` (FP64 Tsit5, FP64 VCABM5, FP32 VCABM5,
+    FP32 Tsit5 by default); event catalogues (max v > 1e-3 m/s, crossings
+    within 1 yr merged) against the FP64 run of the same method, from this
+    call or from an earlier spans JSON."""
     from oetqf_amd.cycle import run_cycle
     from oetqf_amd.workloads import YEAR, transform_fault_composite
 
     w = transform_fault_composite()
     out = dict(workload=w["name"], sim_years=(w["tspan"][1] - w["tspan"][0]) / YEAR,
                reltol=w["reltol"], abstol=w["abstol"], runs={})
-    for kd, alg in (("fp64", "tsit5"), ("fp64", "vcabm5"), ("fp32", "vcabm5"), ("fp32", "tsit5")):
+    prior = refs.get("composite", {}).get("runs", {})
+    for kd, alg in runs:
         p = oq.assemble_from_mesh(w["mf"], w["ma"], w["lam"], w["mu"], w["pf"], w["pa"], w["u0"],
                                   w["tspan"], nrept=w["nrept"], buffer_ratio=w["buffer_ratio"], kdtype=kd)
         tag = "%s %s" % (kd, alg)
@@ -551,48 +567,69 @@ def spans_composite(oq):
         p.f.close()
         cat = _catalogue(r["t"], r["vmax"])
         res = dict(wall_s=r["wall"], sim_yr_per_hr=out["sim_years"] / (r["wall"] / 3600.0),
-                   accepted=int(r["naccept"]), rejected=int(r["nreject"]), retcode=r["retcode"], **cat)
-        if kd != "fp64" and ("fp64 " + alg) in out["runs"]:
-            res["vs_fp64"] = _compare(cat, out["runs"]["fp64 " + alg])
-            res["speedup_vs_fp64"] = out["runs"]["fp64 " + alg]["wall_s"] / r["wall"]
+                   accepted=int(r["naccept"]), rejected=int(r["nreject"]),
+                   rejection_rate=float(r["nreject"] / max(r["naccept"] + r["nreject"], 1)),
+                   retcode=r["retcode"], **cat)
+        ref = out["runs"].get("fp64 " + alg) or prior.get("fp64 " + alg)
+        if kd != "fp64" and ref is not None:
+            res["vs_fp64"] = _compare(cat, ref)
+            res["speedup_vs_fp64"] = ref["wall_s"] / r["wall"]
         out["runs"][tag] = res
-        log("%s: %d events in %.0f s (%.0f yr/hr), mean recurrence %s yr%s" % (
-            tag, cat["events"], r["wall"], res["sim_yr_per_hr"], cat["mean_recurrence_yr"],
-            ", vs FP64 %s" % res["vs_fp64"] if "vs_fp64" in res else ""))
+        log("%s: %d events in %.0f s (%.0f yr/hr), %d accepted, %d rejected, mean recurrence %s yr%s" % (
+            tag, cat["events"], r["wall"], res["sim_yr_per_hr"], res["accepted"], res["rejected"],
+            cat["mean_recurrence_yr"], ", vs FP64 %s" % res["vs_fp64"] if "vs_fp64" in res else ""))
+        if save is not None:
+            save(out)
     return out
 
 
-def spans_ensemble(oq, kdtypes, members_fp64):
+def spans_ensemble(oq, kdtypes, sampled):
     """256 members on the composite geometry over 300 years with the lockstep
-    Tsit5 (per-member steps), per operator type; the sampled members are
-    then run alone in FP64 (Tsit5, same tolerances) and their catalogues
-    compared with the ensemble's."""
-    from oetqf_amd.cycle import run_cycle, run_ensemble_cycle
+    Tsit5 (per-member steps), per operator type: rejection rates, event
+    counts and mean recurrence of every member, and the whole catalogues of
+    the sampled members (for the members leg)."""
+    from oetqf_amd.cycle import run_ensemble_cycle
     from oetqf_amd.workloads import YEAR, ensemble_members, transform_fault_composite
 
     w = transform_fault_composite()
     members, U0 = ensemble_members(w, 256)
     span = (0.0, 300 * YEAR)
     out = dict(workload=w["name"] + ", 256 frictional realizations", sim_years=300.0,
-               reltol=w["reltol"], abstol=w["abstol"], by_kdtype={}, fp64_members={})
-    cats = {}
+               reltol=w["reltol"], abstol=w["abstol"], by_kdtype={})
     for kd in kdtypes:
         log("ensemble 300 yr: %s" % kd)
         ens = oq.Ensemble(w["mf"], w["ma"], w["lam"], w["mu"], members, w["pa"], nrept=w["nrept"],
                           buffer_ratio=w["buffer_ratio"], kdtype=kd)
         r = run_ensemble_cycle(ens, U0, span, reltol=w["reltol"], abstol=w["abstol"], dt0=1.0)
         ens.close()
-        cats[kd] = [_catalogue(r["t"][m], r["vmax"][m]) for m in range(256)]
+        cats = [_catalogue(r["t"][m], r["vmax"][m]) for m in range(256)]
         na, nr = r["naccept"], r["nreject"]
         out["by_kdtype"][kd] = dict(
             wall_s=r["wall"], lockstep_iterations=r["iters"],
             member_sim_yr_per_hr=256 * 300.0 / (r["wall"] / 3600.0),
             all_success=all(x == "Success" for x in r["retcode"]),
             rejection_rate=float(nr.sum() / (na.sum() + nr.sum())),
-            events_per_member=[c["events"] for c in cats[kd]],
-            mean_recurrence_yr=[c["mean_recurrence_yr"] for c in cats[kd]])
-        log("ensemble %s: 256 x 300 yr in %.0f s (%.0f member-yr/hr)" % (
-            kd, r["wall"], out["by_kdtype"][kd]["member_sim_yr_per_hr"]))
+            accepted_per_member=[int(x) for x in na], rejected_per_member=[int(x) for x in nr],
+            events_per_member=[c["events"] for c in cats],
+            mean_recurrence_yr=[c["mean_recurrence_yr"] for c in cats],
+            sampled_catalogues={str(m): cats[m] for m in sampled})
+        log("ensemble %s: 256 x 300 yr in %.0f s (%.0f member-yr/hr), rejection rate %.3f" % (
+            kd, r["wall"], out["by_kdtype"][kd]["member_sim_yr_per_hr"], out["by_kdtype"][kd]["rejection_rate"]))
+    return out
+
+
+def spans_members(oq, members_fp64, ens, ref_path):
+    """Sampled ensemble members run alone in FP64 (Tsit5, the same
+    tolerances and start) over the same 300 years; their catalogues are
+    compared with the ensemble's catalogues of those members, read from the
+    spans JSON of an ensemble leg."""
+    from oetqf_amd.cycle import run_cycle
+    from oetqf_amd.workloads import YEAR, ensemble_members, transform_fault_composite
+
+    w = transform_fault_composite()
+    members, U0 = ensemble_members(w, 256)
+    span = (0.0, 300 * YEAR)
+    out = dict(sim_years=300.0, ensemble_spans=ref_path or None, members={})
     for m in members_fp64:
         u0 = w["u0"].similar().unpack(U0[m])
         p = oq.assemble_from_mesh(w["mf"], w["ma"], w["lam"], w["mu"], members[m], w["pa"], u0, span,
@@ -603,12 +640,14 @@ def spans_ensemble(oq, kdtypes, members_fp64):
         p.f.close()
         ref = _catalogue(r["t"], r["vmax"])
         res = dict(wall_s=r["wall"], accepted=int(r["naccept"]), rejected=int(r["nreject"]), **ref)
-        for kd in kdtypes:
-            res["ensemble_%s_vs_fp64" % kd] = _compare(cats[kd][m], ref)
-        out["fp64_members"][str(m)] = res
+        for kd, e in (ens or {}).items():
+            cat = e["sampled_catalogues"].get(str(m))
+            if cat is not None:
+                res["ensemble_%s_vs_fp64" % kd] = _compare(cat, ref)
+        out["members"][str(m)] = res
         log("member %d: FP64 %d events, mean recurrence %s yr; %s" % (
             m, ref["events"], ref["mean_recurrence_yr"],
-            "; ".join("%s %s" % (kd, res["ensemble_%s_vs_fp64" % kd]) for kd in kdtypes)))
+            "; ".join("%s %s" % (k[9:-8], v) for k, v in res.items() if k.startswith("ensemble_"))))
     return out
 
 
@@ -672,10 +711,15 @@ def main():
                     help="operator storage types of the N = 65536 leg (fp64 first: the reference)")
     ap.add_argument("--spans", default="",
                     help="instead of the bench line: whole simulated spans with event statistics, "
-                         "comma-separated from composite, ensemble, group (JSON to --spans-out)")
+                         "comma-separated from composite, ensemble, members, group (JSON to --spans-out)")
     ap.add_argument("--spans-out", default="gpurun_out/spans.json")
     ap.add_argument("--spans-members", default="0,37,64,101,128,170,203,255",
-                    help="ensemble members run alone in FP64 by --spans ensemble")
+                    help="ensemble members sampled by --spans ensemble and run alone in FP64 by --spans members")
+    ap.add_argument("--spans-ref", default="",
+                    help="spans JSON files of earlier calls (comma-separated): the FP64 composite runs and "
+                         "the ensemble catalogues that this call's runs are compared with")
+    ap.add_argument("--spans-runs", default="fp64:tsit5,fp64:vcabm5,fp32:vcabm5,fp32:tsit5",
+                    help="(operator type:method) runs of the composite leg")
     ap.add_argument("--spans-kdtypes", default="fp16x2,fp32",
                     help="operator types of the ensemble spans")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -692,19 +736,30 @@ def main():
 
     if args.spans:
         res = {}
+        refs = _load_refs(args.spans_ref)
+        os.makedirs(os.path.dirname(os.path.abspath(args.spans_out)), exist_ok=True)
+
+        def save(partial=None, leg=None):
+            if leg is not None:
+                res[leg] = partial
+            with open(args.spans_out, "w") as fh:
+                json.dump(res, fh, indent=1)
+
+        sampled = [int(m) for m in args.spans_members.split(",") if m]
         for leg in args.spans.split(","):
             if leg == "composite":
-                res[leg] = spans_composite(oq)
+                runs = [tuple(x.split(":")) for x in args.spans_runs.split(",") if x]
+                res[leg] = spans_composite(oq, runs, refs, save=lambda o: save(o, "composite"))
             elif leg == "ensemble":
-                res[leg] = spans_ensemble(oq, args.spans_kdtypes.split(","),
-                                          [int(m) for m in args.spans_members.split(",") if m])
+                res[leg] = spans_ensemble(oq, args.spans_kdtypes.split(","), sampled)
+            elif leg == "members":
+                ref = refs.get("ensemble", {}).get("by_kdtype")
+                res[leg] = spans_members(oq, sampled, ref, args.spans_ref)
             elif leg == "group":
                 res[leg] = spans_group(oq)
             else:
                 raise SystemExit("unknown span leg %r" % leg)
-            os.makedirs(os.path.dirname(os.path.abspath(args.spans_out)), exist_ok=True)
-            with open(args.spans_out, "w") as fh:
-                json.dump(res, fh, indent=1)
+            save()
         log("spans written to %s" % args.spans_out)
         return
 

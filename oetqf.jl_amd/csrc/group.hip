@@ -149,6 +149,14 @@ __device__ __forceinline__ f4x gload4(const float *p)
     return *(gf4x *)p;
 }
 
+// s_waitcnt immediate that waits for at most n outstanding vector memory
+// operations (vmcnt bits [3:0] and [15:14]) and leaves the LDS/export counts
+// at their maxima
+__host__ __device__ constexpr int vmcnt_imm(int n)
+{
+    return (n & 0xf) | ((n >> 4) << 14) | (0x7 << 4) | (0xf << 8);
+}
+
 __device__ __forceinline__ void grp_two_sum(float &hi, float &lo, float x)
 {
     const float s = hi + x;
@@ -714,6 +722,145 @@ __global__ void __launch_bounds__(256) k_grp_sum_partials(GrpDev d, const GrpRow
     d.Y[G.Yoff + (int64_t)(B.row0 + lrow) * M + m] = y;
 }
 
+// ---------------- direct-to-LDS variant on column-split units ----------------
+// The same units and partial sums as k_grp_gemm_units (64-row blocks here),
+// with every slice (64 rows x 128 columns of K, 16 x 128 of S: 40 KB) moved
+// from HBM into LDS by global_load_lds_dwordx4: no register staging, so the
+// NS-stage ring keeps NS - 1 slices in flight per workgroup at 28 VGPRs of
+// MFMA state.  One wavefront instruction moves 1 KB (two 512-byte row
+// segments); a lane's 16-byte chunk j of row r lands in slot j but holds
+// column chunk j ^ (r mod 32), so that the MFMA-layout reads (16 rows at one
+// column chunk per quarter wavefront) fall in 16 distinct bank groups.
+// Wavefront w reduces columns [32w, 32w + 32) of every slice for all 64
+// rows; the four column partials are added through LDS at the end of a unit
+// in a fixed order.
+static constexpr int kDBR = 64, kDBK = 128, kDCh = kDBK / 4;
+static constexpr int kDKSl = kDBR * kDBK, kDSSl = kGM * kDBK, kDStage = kDKSl + kDSSl;  // floats
+typedef __attribute__((address_space(3))) void lds_void;
+
+__device__ __forceinline__ void dma16(const float *g, float *l)
+{
+    __builtin_amdgcn_global_load_lds(g, (lds_void *)l, 16, 0, 0);
+}
+
+// Workgroup barrier without the release fence of __syncthreads(): the
+// fence would wait for every copy in flight (LDS DMA counts as a pending
+// LDS write), draining the ring.  The copies a wavefront needs are waited
+// for explicitly before it; the empty asm statements keep the compiler from
+// moving memory accesses across it.
+__device__ __forceinline__ void ring_barrier()
+{
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+template <int NS>
+__global__ void __launch_bounds__(256, 1) k_grp_gemm_dma(GrpDev d, const GrpUnit *__restrict__ units,
+                                                        const int2 *__restrict__ wg_range,
+                                                        double *__restrict__ P)
+{
+    if (!*d.live) return;
+    extern __shared__ __attribute__((aligned(16))) float dsm[];
+    constexpr int KI = kDKSl / 4 / 256, SI = kDSSl / 4 / 256;  // DMA instructions per lane: 8 K, 2 S
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lr = lane & 15, lq = lane >> 4;
+    const int2 range = wg_range[blockIdx.x];
+    for (int ui = range.x; ui < range.y; ++ui) {
+        const GrpUnit U = units[ui];
+        const GrpGeo &G = d.geo[U.g];
+        const int64_t N = G.N, ld = G.ld, r0 = U.row0;
+        const int nk = U.k1 - U.k0;
+        const float *kp[KI], *sp[SI];
+#pragma unroll
+        for (int i = 0; i < KI; ++i) {
+            const int q = (wave * KI + i) * 64 + lane, r = q / kDCh, c = (q % kDCh) ^ (r % kDCh);
+            const int64_t row = r0 + r < N ? r0 + r : N - 1;
+            kp[i] = G.K + row * ld + (int64_t)U.k0 * kDBK + 4 * c;
+        }
+#pragma unroll
+        for (int i = 0; i < SI; ++i) {
+            const int q = (wave * SI + i) * 64 + lane, m = q / kDCh, c = (q % kDCh) ^ m;
+            sp[i] = d.S + G.Soff + (int64_t)m * ld + (int64_t)U.k0 * kDBK + 4 * c;
+        }
+        // slice k (clamped to the last one: the copies past the end land in
+        // stages nobody reads) into stage k mod NS; the sweep starts at a
+        // different slice per unit (row pitches are multiples of 2 KB, so
+        // aligned sweeps would load the same channels)
+        const int rot = (int)(((int64_t)ui * 13) % nk);
+        auto issue = [&](int k) {
+            int kr = (k < nk ? k : nk - 1) + rot;
+            kr = kr >= nk ? kr - nk : kr;
+            const int64_t col = (int64_t)kr * kDBK;
+            float *st = dsm + (k % NS) * kDStage;
+#pragma unroll
+            for (int i = 0; i < KI; ++i) dma16(kp[i] + col, st + (wave * KI + i) * 256);
+#pragma unroll
+            for (int i = 0; i < SI; ++i) dma16(sp[i] + col, st + kDKSl + (wave * SI + i) * 256);
+        };
+        f4x acc[4];
+        float hi[4][4], lo[4][4];
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) {
+            acc[rb] = (f4x){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int v = 0; v < 4; ++v) hi[rb][v] = lo[rb][v] = 0.f;
+        }
+        __syncthreads();  // the previous unit's reads of the LDS are done
+#pragma unroll
+        for (int k = 0; k < NS - 1; ++k) issue(k);
+        for (int k = 0; k < nk; ++k) {
+            // slice k has landed once at most the NS - 2 younger slices
+            // (KI + SI instructions each) are outstanding; then every
+            // wavefront's part of it has
+            __builtin_amdgcn_s_waitcnt(vmcnt_imm((NS - 2) * (KI + SI)));
+            ring_barrier();
+            issue(k + NS - 1);  // into the stage of slice k - 1, read by all
+            const float *A = dsm + (k % NS) * kDStage, *B = A + kDKSl;
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int c = 8 * wave + 4 * t + lq;
+                const f4x b = *reinterpret_cast<const f4x *>(&B[(lr * kDCh + (c ^ lr)) * 4]);
+                f4x a[4];
+#pragma unroll
+                for (int rb = 0; rb < 4; ++rb) {
+                    const int r = 16 * rb + lr;
+                    a[rb] = *reinterpret_cast<const f4x *>(&A[(r * kDCh + (c ^ (r % kDCh))) * 4]);
+                }
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+#pragma unroll
+                    for (int rb = 0; rb < 4; ++rb)
+                        acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rb][e], b[e], acc[rb], 0, 0, 0);
+            }
+            if ((k & 7) == 7 || k + 1 == nk) {  // every 256 columns of this wavefront
+#pragma unroll
+                for (int rb = 0; rb < 4; ++rb) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) grp_two_sum(hi[rb][v], lo[rb][v], acc[rb][v]);
+                    acc[rb] = (f4x){0.f, 0.f, 0.f, 0.f};
+                }
+            }
+        }
+        __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+        __syncthreads();  // every copy has landed and every read is done
+        double *red = reinterpret_cast<double *>(dsm);  // [wave][row][member]
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                red[(wave * kDBR + 16 * rb + 4 * lq + v) * kGM + lr] = (double)hi[rb][v] + (double)lo[rb][v];
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < kDBR * kGM / 256; ++e) {
+            const int x = tid + 256 * e, row = x / kGM, m = x % kGM;
+            const double y = ((red[x] + red[kDBR * kGM + x]) + red[2 * kDBR * kGM + x]) + red[3 * kDBR * kGM + x];
+            if (m < d.M && r0 + row < N) P[((int64_t)U.slot * kDBR + row) * d.M + m] = y;
+        }
+    }
+}
+
 // ---------------- lockstep Tsit5 over all members ----------------
 __constant__ double kGA[7][6] = {
     {0, 0, 0, 0, 0, 0},
@@ -877,18 +1024,25 @@ __global__ void __launch_bounds__(256) k_grp_commit(GrpCtrl c, int64_t ntot,
 // kernels, and the LDS-staged kernels with RB in {1, 2}, one or two slices
 // in flight (a third slice in flight measured the same as two: the kernel
 // is not limited by its prefetch depth), and (shape 10) the two-slice LDS
-// kernel with RB = 2 on column-split units balanced over the workgroups
-static constexpr int kGShapes = 11;
-static constexpr int kGUnitShape = 10;
+// kernel with RB = 2 on column-split units balanced over the workgroups;
+// (shapes 11, 12) the direct-to-LDS kernel on 64-row units with a 3- or
+// 4-stage ring
+static constexpr int kGShapes = 13;
+static constexpr int kGUnitShape = 10;  // first of the unit shapes 10..12
+static constexpr int kGUnitShapes = 3;
 static constexpr int kGUnitRB = 2, kGUnitBR = 64 * kGUnitRB, kGUnitBK = 128 / kGUnitRB;
 static constexpr int64_t kGUnitBytes = (int64_t)2 << 20;  // target bytes of K per unit
+static inline bool is_unit_shape(int s) { return s >= kGUnitShape; }
+static inline int unit_br(int s) { return s == kGUnitShape ? kGUnitBR : kDBR; }
+static inline int unit_bk(int s) { return s == kGUnitShape ? kGUnitBK : kDBK; }
+static inline size_t dma_lds(int s) { return (size_t)(s - kGUnitShape + 2) * kDStage * 4; }  // NS = 3, 4
 static inline int shape_rb(int s)
 {
-    if (s == kGUnitShape) return kGUnitRB;
+    if (is_unit_shape(s)) return s == kGUnitShape ? kGUnitRB : 1;
     return s < 6 ? (s / 2 == 0 ? 1 : (s / 2 == 1 ? 2 : 4)) : (s % 2 == 0 ? 1 : 2);
 }
 
-// Units of shape 10 over a set of geometries, with their schedule.
+// Units of a unit shape over a set of geometries, with their schedule.
 struct GrpUnitTable {
     oq::DevBuf units, wg, rbl;
     int nunits = 0, nwg = 0, nrbl = 0, nslots = 0;
@@ -912,7 +1066,7 @@ struct oq_group {
     oq::DevBuf dtiles[oq::kGShapes];
     int ntiles[oq::kGShapes] = {0};
     int grid[oq::kGShapes] = {0};
-    oq::GrpUnitTable ut;  // shape 10 over the geometries that use it
+    oq::GrpUnitTable ut[oq::kGUnitShapes];  // unit shapes over the geometries that use them
     oq::DevBuf P;         // partial row sums of the units (capacity pcap doubles)
     size_t pcap = 0;
     oq::DevBuf u, y, k, part;
@@ -937,13 +1091,13 @@ static int bucket_of(int64_t N)
 }
 
 template <class F>
-static int occupancy_grid(F kern, int *grid)
+static int occupancy_grid(F kern, int *grid, size_t dyn_lds = 0)
 {
     int nb = 0, dev = 0;
     OQ_HIP(hipGetDevice(&dev));
     hipDeviceProp_t prop;
     OQ_HIP(hipGetDeviceProperties(&prop, dev));
-    OQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, 256, 0));
+    OQ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, 256, dyn_lds));
     *grid = (nb > 0 ? nb : 1) * prop.multiProcessorCount;
     return OQ_OK;
 }
@@ -961,22 +1115,23 @@ static std::vector<int2> build_tiles(const oq_group *e, int s, const std::vector
     return t;
 }
 
-// Units of shape 10 over the geometries in `sel`: every row block's slices
-// split into chunks of ~kGUnitBytes (multiples of 512 columns), then
+// Units of unit shape s over the geometries in `sel`: every row block's
+// slices split into chunks of ~kGUnitBytes (multiples of 512 columns), then
 // longest-first greedy assignment to `nwg` workgroups.  Partials of row
 // block i go to slots [slot0, slot0 + nch).
-static int build_units(oq_group *e, const std::vector<int> &sel, GrpUnitTable &T)
+static int build_units(oq_group *e, int s, const std::vector<int> &sel, GrpUnitTable &T)
 {
+    const int BR = unit_br(s), BK = unit_bk(s);
     std::vector<GrpUnit> units;
     std::vector<GrpRowBlock> rbl;
     int slot = 0;
     for (int g : sel) {
         const GrpGeo &G = e->geo[g];
-        const int64_t nslices = G.ld / kGUnitBK;                 // multiple of 8
-        const int64_t per = std::max<int64_t>(8, kGUnitBytes / (kGUnitBR * kGUnitBK * 4) / 8 * 8);
+        const int64_t nslices = G.ld / BK;                       // multiple of 4
+        const int64_t per = std::max<int64_t>(8, kGUnitBytes / ((int64_t)BR * BK * 4) / 8 * 8);
         const int64_t nch = (nslices + per - 1) / per;
         const int64_t cs = (nslices / nch + 7) / 8 * 8;          // slices per chunk
-        for (int64_t r0 = 0; r0 < G.N; r0 += kGUnitBR) {
+        for (int64_t r0 = 0; r0 < G.N; r0 += BR) {
             int c = 0;
             for (int64_t k0 = 0; k0 < nslices; k0 += cs, ++c) {
                 const int64_t k1 = std::min(nslices, k0 + cs);
@@ -987,9 +1142,9 @@ static int build_units(oq_group *e, const std::vector<int> &sel, GrpUnitTable &T
         }
     }
     // longest first onto the least loaded workgroup (cost = rows x columns)
-    const int W = std::max(1, std::min(e->grid[kGUnitShape], (int)units.size()));
+    const int W = std::max(1, std::min(e->grid[s], (int)units.size()));
     auto cost = [&](const GrpUnit &u) {
-        const int64_t rows = std::min<int64_t>(kGUnitBR, e->geo[u.g].N - u.row0);
+        const int64_t rows = std::min<int64_t>(BR, e->geo[u.g].N - u.row0);
         return rows * (int64_t)(u.k1 - u.k0);
     };
     std::vector<int> order(units.size());
@@ -1022,7 +1177,7 @@ static int build_units(oq_group *e, const std::vector<int> &sel, GrpUnitTable &T
     int rc = T.units.upload(sched.data(), sched.size() * sizeof(GrpUnit), e->stream);
     if (!rc) rc = T.wg.upload(wg.data(), wg.size() * sizeof(int2), e->stream);
     if (!rc) rc = T.rbl.upload(rbl.data(), rbl.size() * sizeof(GrpRowBlock), e->stream);
-    const size_t need = (size_t)slot * kGUnitBR * (size_t)e->M;
+    const size_t need = (size_t)slot * BR * (size_t)e->M;
     if (!rc && need > e->pcap) {
         rc = e->P.alloc(need * 8);
         e->pcap = rc ? 0 : need;
@@ -1030,13 +1185,25 @@ static int build_units(oq_group *e, const std::vector<int> &sel, GrpUnitTable &T
     return rc;
 }
 
-static int launch_units(oq_group *e, const GrpUnitTable &T)
+static int launch_units(oq_group *e, int s, const GrpUnitTable &T)
 {
     if (T.nunits == 0) return OQ_OK;
-    hipLaunchKernelGGL((k_grp_gemm_units<kGUnitRB>), dim3((unsigned)T.nwg), dim3(256), 0, e->stream,
-                       e->dev, T.units.as<GrpUnit>(), T.wg.as<int2>(), e->P.as<double>());
-    hipLaunchKernelGGL((k_grp_sum_partials<kGUnitBR>), dim3(gbk(kGUnitBR * e->M, 256), (unsigned)T.nrbl),
-                       dim3(256), 0, e->stream, e->dev, T.rbl.as<GrpRowBlock>(), e->P.as<double>());
+    const dim3 grid((unsigned)T.nwg), blk(256);
+    if (s == kGUnitShape) {
+        hipLaunchKernelGGL((k_grp_gemm_units<kGUnitRB>), grid, blk, 0, e->stream, e->dev,
+                           T.units.as<GrpUnit>(), T.wg.as<int2>(), e->P.as<double>());
+        hipLaunchKernelGGL((k_grp_sum_partials<kGUnitBR>), dim3(gbk(kGUnitBR * e->M, 256), (unsigned)T.nrbl),
+                           blk, 0, e->stream, e->dev, T.rbl.as<GrpRowBlock>(), e->P.as<double>());
+    } else {
+        if (s == kGUnitShape + 1)
+            hipLaunchKernelGGL((k_grp_gemm_dma<3>), grid, blk, dma_lds(s), e->stream, e->dev,
+                               T.units.as<GrpUnit>(), T.wg.as<int2>(), e->P.as<double>());
+        else
+            hipLaunchKernelGGL((k_grp_gemm_dma<4>), grid, blk, dma_lds(s), e->stream, e->dev,
+                               T.units.as<GrpUnit>(), T.wg.as<int2>(), e->P.as<double>());
+        hipLaunchKernelGGL((k_grp_sum_partials<kDBR>), dim3(gbk(kDBR * e->M, 256), (unsigned)T.nrbl),
+                           blk, 0, e->stream, e->dev, T.rbl.as<GrpRowBlock>(), e->P.as<double>());
+    }
     OQ_HIP(hipGetLastError());
     return OQ_OK;
 }
@@ -1047,8 +1214,8 @@ static int upload_tiles(oq_group *e)
         std::vector<int> sel;
         for (int g = 0; g < (int)e->G; ++g)
             if (e->bucket_shape[e->geo_bucket[g]] == s) sel.push_back(g);
-        if (s == kGUnitShape) {
-            int rc = build_units(e, sel, e->ut);
+        if (is_unit_shape(s)) {
+            int rc = build_units(e, s, sel, e->ut[s - kGUnitShape]);
             if (rc) return rc;
             continue;
         }
@@ -1091,7 +1258,7 @@ static int grp_rhs(oq_group *e, const double *u, double *du)
                        u, du);
     OQ_HIP(hipGetLastError());
     for (int s = 0; s < kGShapes; ++s) {
-        int rc = s == kGUnitShape ? launch_units(e, e->ut)
+        int rc = is_unit_shape(s) ? launch_units(e, s, e->ut[s - kGUnitShape])
                                   : launch_gemm(e, s, e->dtiles[s].as<int2>(), e->ntiles[s], u, du);
         if (rc) return rc;
     }
@@ -1326,6 +1493,20 @@ int oq_group_create(int device, int64_t ngeo, const oq_fault_mesh *mf,
         if (!rc) rc = occupancy_grid(k_grp_gemm_lds2<1>, &gr[8]);
         if (!rc) rc = occupancy_grid(k_grp_gemm_lds2<2>, &gr[9]);
         if (!rc) rc = occupancy_grid(k_grp_gemm_units<kGUnitRB>, &gr[10]);
+        // the direct-to-LDS shapes need 120 / 160 KB of LDS per workgroup: a
+        // device that cannot give it leaves the shape out (grid 0)
+        const void *dk[2] = {(const void *)k_grp_gemm_dma<3>, (const void *)k_grp_gemm_dma<4>};
+        for (int i = 0; i < 2 && !rc; ++i) {
+            const int sh = kGUnitShape + 1 + i;
+            gr[sh] = 0;
+            if (hipFuncSetAttribute(dk[i], hipFuncAttributeMaxDynamicSharedMemorySize, (int)dma_lds(sh)) !=
+                hipSuccess) {
+                (void)hipGetLastError();
+                continue;
+            }
+            rc = i == 0 ? occupancy_grid(k_grp_gemm_dma<3>, &gr[sh], dma_lds(sh))
+                        : occupancy_grid(k_grp_gemm_dma<4>, &gr[sh], dma_lds(sh));
+        }
         for (int s = 0; s < kGShapes; ++s) e->grid[s] = gr[s];
     }
     if (!rc) rc = upload_tiles(e);
@@ -1375,7 +1556,9 @@ int oq_group_set_shapes(oq_group *e, const int *shape_of_bucket)
 {
     OQ_REQUIRE(e != nullptr && shape_of_bucket != nullptr, "NULL argument");
     for (int b = 0; b < kGBuckets; ++b)
-        OQ_REQUIRE(shape_of_bucket[b] >= -1 && shape_of_bucket[b] < kGShapes, "tile shape out of range");
+        OQ_REQUIRE(shape_of_bucket[b] >= -1 && shape_of_bucket[b] < kGShapes &&
+                       (shape_of_bucket[b] < 0 || e->grid[shape_of_bucket[b]] > 0),
+                   "tile shape out of range or not available on this device");
     OQ_HIP(hipSetDevice(e->device));
     for (int b = 0; b < kGBuckets; ++b)
         if (shape_of_bucket[b] >= 0) e->bucket_shape[b] = shape_of_bucket[b];
@@ -1404,14 +1587,15 @@ int oq_group_autotune(oq_group *e, int iters, int *shape_of_bucket, double *ms_o
         double best = 1e300;
         int bs = e->bucket_shape[b];
         for (int s = 0; s < kGShapes; ++s) {
-            const std::vector<int2> t = s == kGUnitShape ? std::vector<int2>() : build_tiles(e, s, sel);
+            if (e->grid[s] == 0) continue;  // not available on this device (timing left 0)
+            const std::vector<int2> t = is_unit_shape(s) ? std::vector<int2>() : build_tiles(e, s, sel);
             DevBuf dt;
             GrpUnitTable ut;
-            int rc = s == kGUnitShape ? build_units(e, sel, ut)
+            int rc = is_unit_shape(s) ? build_units(e, s, sel, ut)
                                       : dt.upload(t.data(), t.size() * sizeof(int2), e->stream);
             if (rc) return rc;
             auto run = [&]() {
-                return s == kGUnitShape ? launch_units(e, ut)
+                return is_unit_shape(s) ? launch_units(e, s, ut)
                                         : launch_gemm(e, s, dt.as<int2>(), (int)t.size(), u, du);
             };
             if ((rc = run())) return rc;
@@ -1440,7 +1624,7 @@ int oq_group_autotune(oq_group *e, int iters, int *shape_of_bucket, double *ms_o
         for (int rep = 0; rep <= iters && !rc; ++rep) {  // rep 0 warms up
             if (rep == 1) OQ_HIP(hipEventRecord(e->ev0, e->stream));
             for (int s = 0; s < kGShapes && !rc; ++s)
-                rc = s == kGUnitShape ? launch_units(e, e->ut)
+                rc = is_unit_shape(s) ? launch_units(e, s, e->ut[s - kGUnitShape])
                                       : launch_gemm(e, s, e->dtiles[s].as<int2>(), e->ntiles[s], u, du);
         }
         if (rc) return rc;
@@ -1456,6 +1640,7 @@ int oq_group_autotune(oq_group *e, int iters, int *shape_of_bucket, double *ms_o
     double best = 0.0;
     int rc = time_all(&best), pick = -1;
     for (int s = 0; s < kGShapes && !rc; ++s) {
+        if (e->grid[s] == 0) continue;
         for (int b = 0; b < kGBuckets; ++b) e->bucket_shape[b] = s;
         double ms = 0.0;
         rc = time_all(&ms);

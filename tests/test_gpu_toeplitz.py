@@ -106,7 +106,7 @@ def test_fft_coupled_rhs_matches_oracle(oracle, rng, fourier):
     g12, g21, g22 = (oracle.gf_mf(mf, ma, lam, mu, buffer_ratio=1.0), oracle.gf_fm(ma, mf, lam, mu),
                      oracle.gf_mm(ma, lam, mu))
     pt = oq.assemble(gf11, g12, g21, g22, pf, pa, u, (0.0, 1.0), kdtype="fft")
-    pd = oq.assemble(st, g12, g21, g22, pf, pa, u, (0.0, 1.0))
+    pd = oq.assemble(gf11, g12, g21, g22, pf, pa, u, (0.0, 1.0))  # the same K_ff input, dense
     dt_, dd = oq.similar(u), oq.similar(u)
     pt.f(dt_, u, None, 0.0)
     pd.f(dd, u, None, 0.0)
