@@ -436,7 +436,19 @@ def group_leg(oq, args):
     dU = grp.eval(U0)
     grp.time(3)
     ms_default = grp.time(20)
-    choice = grp.autotune(5)
+    if args.group_shape:  # pinned (profiling runs): no tuning launches
+        from oetqf_amd.group import SHAPES
+        grp.set_shapes({b: args.group_shape for b in range(8)})
+        def bucket(n):
+            b = 0
+            while b + 1 < 8 and n >= (2048 << b):
+                b += 1
+            return b
+        choice = {bucket(int(n)): (args.group_shape, 0.0) for n in grp.N}
+        grp.tune_table = {}
+        assert args.group_shape in SHAPES
+    else:
+        choice = grp.autotune(5)
     grp.eval(U0)
     grp.time(3)
     ms = grp.time(args.steps)
@@ -538,6 +550,11 @@ def _load_refs(paths):
             for leg, v in json.load(fh).items():
                 if leg == "composite" and leg in out:
                     out[leg]["runs"].update(v["runs"])
+                elif leg == "ensemble" and leg in out:
+                    for kd, e in v["by_kdtype"].items():
+                        o = out[leg]["by_kdtype"].setdefault(kd, dict(members={}, sampled_catalogues={}))
+                        o.setdefault("members", {}).update(e.get("members", {}))
+                        o.setdefault("sampled_catalogues", {}).update(e.get("sampled_catalogues", {}))
                 else:
                     out[leg] = v
     return out
@@ -583,38 +600,55 @@ def spans_composite(oq, runs, refs, save=None):
     return out
 
 
-def spans_ensemble(oq, kdtypes, sampled):
-    """256 members on the composite geometry over 300 years with the lockstep
-    Tsit5 (per-member steps), per operator type: rejection rates, event
-    counts and mean recurrence of every member, and the whole catalogues of
-    the sampled members (for the members leg)."""
+def _span_range(rng, n):
+    a, b = (int(x) for x in (rng or "0:%d" % n).split(":"))
+    assert 0 <= a < b <= n, "range %s outside 0:%d" % (rng, n)
+    return a, b
+
+
+def spans_ensemble(oq, kdtypes, sampled, member_range=""):
+    """Members a:b (default all 256) of the composite-geometry ensemble over
+    300 years with the lockstep Tsit5 (per-member steps), per operator type:
+    rejection rates, event counts and mean recurrence of every member, and
+    the whole catalogues of the sampled members (for the members leg).  A
+    member's trajectory does not depend on which other members share its
+    batch (each GEMM output column and each member's scales and controller
+    are its own), so a span can run as several member ranges."""
     from oetqf_amd.cycle import run_ensemble_cycle
     from oetqf_amd.workloads import YEAR, ensemble_members, transform_fault_composite
 
     w = transform_fault_composite()
     members, U0 = ensemble_members(w, 256)
+    lo, hi = _span_range(member_range, 256)
     span = (0.0, 300 * YEAR)
     out = dict(workload=w["name"] + ", 256 frictional realizations", sim_years=300.0,
                reltol=w["reltol"], abstol=w["abstol"], by_kdtype={})
     for kd in kdtypes:
-        log("ensemble 300 yr: %s" % kd)
-        ens = oq.Ensemble(w["mf"], w["ma"], w["lam"], w["mu"], members, w["pa"], nrept=w["nrept"],
+        log("ensemble 300 yr: %s, members %d:%d" % (kd, lo, hi))
+        ens = oq.Ensemble(w["mf"], w["ma"], w["lam"], w["mu"], members[lo:hi], w["pa"], nrept=w["nrept"],
                           buffer_ratio=w["buffer_ratio"], kdtype=kd)
-        r = run_ensemble_cycle(ens, U0, span, reltol=w["reltol"], abstol=w["abstol"], dt0=1.0)
+        last = [time.perf_counter()]
+
+        def progress(iters, left):
+            if time.perf_counter() - last[0] > 20:
+                last[0] = time.perf_counter()
+                log("ensemble %s: %d iterations, %d members running" % (kd, iters, left))
+        r = run_ensemble_cycle(ens, U0[lo:hi], span, reltol=w["reltol"], abstol=w["abstol"], dt0=1.0,
+                               progress=progress)
         ens.close()
-        cats = [_catalogue(r["t"][m], r["vmax"][m]) for m in range(256)]
+        M = hi - lo
+        cats = [_catalogue(r["t"][m], r["vmax"][m]) for m in range(M)]
         na, nr = r["naccept"], r["nreject"]
         out["by_kdtype"][kd] = dict(
-            wall_s=r["wall"], lockstep_iterations=r["iters"],
-            member_sim_yr_per_hr=256 * 300.0 / (r["wall"] / 3600.0),
+            member_range=[lo, hi], wall_s=r["wall"], lockstep_iterations=r["iters"],
+            member_sim_yr_per_hr=M * 300.0 / (r["wall"] / 3600.0),
             all_success=all(x == "Success" for x in r["retcode"]),
             rejection_rate=float(nr.sum() / (na.sum() + nr.sum())),
-            accepted_per_member=[int(x) for x in na], rejected_per_member=[int(x) for x in nr],
-            events_per_member=[c["events"] for c in cats],
-            mean_recurrence_yr=[c["mean_recurrence_yr"] for c in cats],
-            sampled_catalogues={str(m): cats[m] for m in sampled})
-        log("ensemble %s: 256 x 300 yr in %.0f s (%.0f member-yr/hr), rejection rate %.3f" % (
-            kd, r["wall"], out["by_kdtype"][kd]["member_sim_yr_per_hr"], out["by_kdtype"][kd]["rejection_rate"]))
+            members={str(lo + m): dict(accepted=int(na[m]), rejected=int(nr[m]), events=cats[m]["events"],
+                                       mean_recurrence_yr=cats[m]["mean_recurrence_yr"]) for m in range(M)},
+            sampled_catalogues={str(m): cats[m - lo] for m in sampled if lo <= m < hi})
+        log("ensemble %s: %d x 300 yr in %.0f s (%.0f member-yr/hr), rejection rate %.3f" % (
+            kd, M, r["wall"], out["by_kdtype"][kd]["member_sim_yr_per_hr"], out["by_kdtype"][kd]["rejection_rate"]))
     return out
 
 
@@ -641,7 +675,7 @@ def spans_members(oq, members_fp64, ens, ref_path):
         ref = _catalogue(r["t"], r["vmax"])
         res = dict(wall_s=r["wall"], accepted=int(r["naccept"]), rejected=int(r["nreject"]), **ref)
         for kd, e in (ens or {}).items():
-            cat = e["sampled_catalogues"].get(str(m))
+            cat = e.get("sampled_catalogues", {}).get(str(m))
             if cat is not None:
                 res["ensemble_%s_vs_fp64" % kd] = _compare(cat, ref)
         out["members"][str(m)] = res
@@ -651,18 +685,25 @@ def spans_members(oq, members_fp64, ens, ref_path):
     return out
 
 
-def spans_group(oq):
-    """The grouped batch (24 geometries x 16 members, FP32) over 300 years
-    with the lockstep grouped Tsit5: per-member step-rejection rates and
-    event counts."""
+def spans_group(oq, geo_range="", shape=""):
+    """Geometries a:b (default all 24) of the grouped batch (16 members each,
+    FP32) over 300 years with the lockstep grouped Tsit5: per-member
+    step-rejection rates and step counts.  With a pinned tile shape every
+    member's trajectory is independent of which geometries share the run, so
+    a span can run as several geometry ranges."""
     from oetqf_amd.workloads import YEAR, grouped_batch
 
     ws = grouped_batch()
+    lo, hi = _span_range(geo_range, len(ws))
+    ws = ws[lo:hi]
     grp = oq.Group([(w["mf"], w["ma"], w["pa"]) for w in ws], ws[0]["lam"], ws[0]["mu"],
                    [w["members"] for w in ws])
-    grp.autotune(3)
-    U0 = grp.pack([w["U0"] for w in ws])
-    grp.start(U0, (0.0, 300 * YEAR), dt0=1.0)
+    if shape:
+        grp.set_shapes({b: shape for b in range(8)})
+        shapes = shape
+    else:
+        shapes = {b: c[0] for b, c in grp.autotune(3).items()}
+    grp.start(grp.pack([w["U0"] for w in ws]), (0.0, 300 * YEAR), dt0=1.0)
     t0 = time.perf_counter()
     iters, last = 0, t0
     while True:
@@ -679,13 +720,15 @@ def spans_group(oq):
     t, U, na, nr, rc = grp.state()
     grp.close()
     rate = nr / np.maximum(na + nr, 1)
-    return dict(geometries=len(ws), members_per_geometry=16, sim_years=300.0, wall_s=wall,
+    G = hi - lo
+    return dict(geometry_range=[lo, hi], members_per_geometry=16, sim_years=300.0, wall_s=wall,
                 lockstep_iterations=iters, member_sim_yr_per_hr=len(rc) * 300.0 / (wall / 3600.0),
-                all_success=all(x == "Success" for x in rc),
+                all_success=all(x == "Success" for x in rc), tile_shapes=shapes,
                 rejection_rate_mean=float(rate.mean()), rejection_rate_max=float(rate.max()),
-                rejection_rate_by_geometry=[float(rate[g * 16:(g + 1) * 16].mean()) for g in range(len(ws))],
-                accepted_by_geometry=[int(na[g * 16:(g + 1) * 16].mean()) for g in range(len(ws))],
-                geometry_names=[w["name"] for w in ws])
+                by_geometry={ws[g]["name"]: dict(index=lo + g, rejection_rate=float(rate[g * 16:(g + 1) * 16].mean()),
+                                                 accepted_mean=float(na[g * 16:(g + 1) * 16].mean()),
+                                                 rejected_mean=float(nr[g * 16:(g + 1) * 16].mean()))
+                             for g in range(G)})
 
 
 def main():
@@ -705,6 +748,8 @@ def main():
                     help="skip the N = 65536 precision-variant leg")
     ap.add_argument("--no-group", dest="group", action="store_false",
                     help="skip the uneven-size grouped batch leg")
+    ap.add_argument("--group-shape", default="",
+                    help="pin this grouped GEMM tile shape for every size bucket instead of autotuning")
     ap.add_argument("--scale-years", type=float, default=1.0,
                     help="trajectory length of the N = 65536 leg")
     ap.add_argument("--scale-kdtypes", default="fp64,fp32,bf16",
@@ -718,6 +763,8 @@ def main():
     ap.add_argument("--spans-ref", default="",
                     help="spans JSON files of earlier calls (comma-separated): the FP64 composite runs and "
                          "the ensemble catalogues that this call's runs are compared with")
+    ap.add_argument("--spans-range", default="",
+                    help="a:b, the ensemble members (of 256) or grouped-batch geometries (of 24) of this call")
     ap.add_argument("--spans-runs", default="fp64:tsit5,fp64:vcabm5,fp32:vcabm5,fp32:tsit5",
                     help="(operator type:method) runs of the composite leg")
     ap.add_argument("--spans-kdtypes", default="fp16x2,fp32",
@@ -751,12 +798,12 @@ def main():
                 runs = [tuple(x.split(":")) for x in args.spans_runs.split(",") if x]
                 res[leg] = spans_composite(oq, runs, refs, save=lambda o: save(o, "composite"))
             elif leg == "ensemble":
-                res[leg] = spans_ensemble(oq, args.spans_kdtypes.split(","), sampled)
+                res[leg] = spans_ensemble(oq, args.spans_kdtypes.split(","), sampled, args.spans_range)
             elif leg == "members":
                 ref = refs.get("ensemble", {}).get("by_kdtype")
                 res[leg] = spans_members(oq, sampled, ref, args.spans_ref)
             elif leg == "group":
-                res[leg] = spans_group(oq)
+                res[leg] = spans_group(oq, args.spans_range, args.group_shape)
             else:
                 raise SystemExit("unknown span leg %r" % leg)
             save()

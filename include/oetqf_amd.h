@@ -342,6 +342,27 @@ int oq_ensemble_run(oq_ensemble *e, int64_t max_steps, int64_t *nrunning);
 /* Per-member t[M], u[M*nstate], step counters and OQ_RET_* codes (0 = running). */
 int oq_ensemble_state(oq_ensemble *e, double *t, double *u, int64_t *naccept,
                       int64_t *nreject, int *retcode);
+
+/* Checkpoint / resume of a lockstep run (no counterpart in the reference,
+ * whose solves are one per member; a restart there is a new ODEProblem from
+ * the saved u).  Per member: t, the next step dt, the PI controller's qold,
+ * the accepted / rejected / attempted step counts, done and the OQ_RET_*
+ * code.  Any pointer of oq_*_get_ctrl may be NULL (not copied). */
+typedef struct oq_lockstep_ctrl {
+    double *t, *dt, *qold;
+    int64_t *naccept, *nreject, *iter;
+    int *done, *retcode;
+} oq_lockstep_ctrl;
+/* Copies the controller state of the current (or last) run to host arrays
+ * of M entries. */
+int oq_ensemble_get_ctrl(oq_ensemble *e, oq_lockstep_ctrl *ctrl);
+/* Starts stepping from saved states u[M*nstate] and controller arrays (all
+ * required) towards t1 with the tolerances and limits of opts (opts->dt0 is
+ * not used).  With the same operator the continuation is bit for bit the
+ * uninterrupted run: k1 = f(u) is recomputed with the same kernels.  The
+ * recorder restarts empty. */
+int oq_ensemble_resume(oq_ensemble *e, const double *u, const oq_lockstep_ctrl *ctrl,
+                       double t1, const oq_solver_opts *opts);
 /* Per-member record of accepted steps, the ensemble form of the single-state
  * recorder (the saving callback of wsolve, io.jl:51-58, reduced to (t,
  * max |v|)): after each accepted step of member m, its new t and the
@@ -384,7 +405,8 @@ int oq_group_info(oq_group *e, int64_t *ntot, int64_t *nstate, int64_t *kbytes);
  * LDS-staged kernels with 16/32 rows per wavefront and one or two slices in
  * flight, the two-slice kernel on column-split units of ~2 MB balanced
  * over the resident workgroups, and a kernel on the same kind of units that
- * copies K and S from HBM straight into a 3- or 4-slice LDS ring) on the
+ * copies K and S from HBM straight into a 3-slice LDS ring, K with and
+ * without the non-temporal cache policy) on the
  * geometries of each size bucket
  * (bucket b: N < 2048 << b;
  * 8 buckets) and keeps the fastest; then times the whole grouped GEMM (one
@@ -412,6 +434,12 @@ int oq_group_run(oq_group *e, int64_t max_steps, int64_t *nrunning);
 /* Per-member t, u (ntot), step counters and OQ_RET_* codes (ngeo*nmembers). */
 int oq_group_state(oq_group *e, double *t, double *u, int64_t *naccept, int64_t *nreject,
                    int *retcode);
+/* Checkpoint / resume as oq_ensemble_get_ctrl / oq_ensemble_resume, over the
+ * ngeo*nmembers members (u in the layout of oq_group_start).  A bit-for-bit
+ * continuation also needs the same tile shapes (oq_group_set_shapes). */
+int oq_group_get_ctrl(oq_group *e, oq_lockstep_ctrl *ctrl);
+int oq_group_resume(oq_group *e, const double *u, const oq_lockstep_ctrl *ctrl, double t1,
+                    const oq_solver_opts *opts);
 int oq_group_destroy(oq_group *e);
 
 #ifdef __cplusplus

@@ -809,6 +809,7 @@ struct oq_ensemble {
     oq::EnsDev dev{};
     oq::EnsCtrl ctrl{};
     bool stepping = false;
+    bool started = false;  // the controller arrays hold a run (get_ctrl)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -1068,31 +1069,29 @@ int oq_ensemble_eval(oq_ensemble *e, const double *u, double *du)
     return OQ_OK;
 }
 
-int oq_ensemble_start(oq_ensemble *e, const double *u0, double t0, double t1,
-                      const oq_solver_opts *o)
+// Uploads the members' states (member-major host layout) and controller
+// arrays, seeds k1 = f(u) and captures the step graph.
+static int ens_begin(oq_ensemble *e, const double *u0, const double *tv, const double *dtv, const double *qv,
+                     const long long *na, const long long *nr, const long long *it, const int *done,
+                     const int *ret, double t1, const oq_solver_opts *o)
 {
-    OQ_REQUIRE(e && u0 && o, "NULL argument");
-    OQ_REQUIRE(t1 > t0, "tspan must satisfy t1 > t0");
-    OQ_REQUIRE(o->dt0 > 0.0, "the ensemble stepper needs an initial step dt0 > 0");
     oq_rhs *h = e->h;
     OQ_HIP(hipSetDevice(h->device));
     const int64_t M = e->M, nM = e->n * M;
     std::vector<const double *> src(M);
     for (int64_t m = 0; m < M; ++m) src[m] = u0 + m * e->n;
     const std::vector<double> ui = interleave(src.data(), e->n, M);
+    const std::vector<int> zi(M, 0);
     OQ_HIP(hipMemcpyAsync(e->u.p, ui.data(), (size_t)nM * 8, hipMemcpyHostToDevice, h->stream));
-    std::vector<double> tv(M, t0), dtv(M, o->dt0), qv(M, 1e-4);
-    std::vector<long long> zl(M, 0);
-    std::vector<int> zi(M, 0);
-    OQ_HIP(hipMemcpyAsync(e->ct.p, tv.data(), M * 8, hipMemcpyHostToDevice, h->stream));
-    OQ_HIP(hipMemcpyAsync(e->cdt.p, dtv.data(), M * 8, hipMemcpyHostToDevice, h->stream));
-    OQ_HIP(hipMemcpyAsync(e->cqold.p, qv.data(), M * 8, hipMemcpyHostToDevice, h->stream));
-    OQ_HIP(hipMemcpyAsync(e->cna.p, zl.data(), M * 8, hipMemcpyHostToDevice, h->stream));
-    OQ_HIP(hipMemcpyAsync(e->cnr.p, zl.data(), M * 8, hipMemcpyHostToDevice, h->stream));
-    OQ_HIP(hipMemcpyAsync(e->citer.p, zl.data(), M * 8, hipMemcpyHostToDevice, h->stream));
-    OQ_HIP(hipMemcpyAsync(e->cdone.p, zi.data(), M * 4, hipMemcpyHostToDevice, h->stream));
+    OQ_HIP(hipMemcpyAsync(e->ct.p, tv, M * 8, hipMemcpyHostToDevice, h->stream));
+    OQ_HIP(hipMemcpyAsync(e->cdt.p, dtv, M * 8, hipMemcpyHostToDevice, h->stream));
+    OQ_HIP(hipMemcpyAsync(e->cqold.p, qv, M * 8, hipMemcpyHostToDevice, h->stream));
+    OQ_HIP(hipMemcpyAsync(e->cna.p, na, M * 8, hipMemcpyHostToDevice, h->stream));
+    OQ_HIP(hipMemcpyAsync(e->cnr.p, nr, M * 8, hipMemcpyHostToDevice, h->stream));
+    OQ_HIP(hipMemcpyAsync(e->citer.p, it, M * 8, hipMemcpyHostToDevice, h->stream));
+    OQ_HIP(hipMemcpyAsync(e->cdone.p, done, M * 4, hipMemcpyHostToDevice, h->stream));
     OQ_HIP(hipMemcpyAsync(e->cacc.p, zi.data(), M * 4, hipMemcpyHostToDevice, h->stream));
-    OQ_HIP(hipMemcpyAsync(e->cret.p, zi.data(), M * 4, hipMemcpyHostToDevice, h->stream));
+    OQ_HIP(hipMemcpyAsync(e->cret.p, ret, M * 4, hipMemcpyHostToDevice, h->stream));
     EnsCtrl &c = e->ctrl;
     c.t1 = t1;
     c.reltol = o->reltol;
@@ -1104,8 +1103,9 @@ int oq_ensemble_start(oq_ensemble *e, const double *u0, double t0, double t1,
     c.vpart = e->rec_cap ? e->vpart.as<double>() : nullptr;
     c.rec_t = e->rec_cap ? e->rec_t.as<double>() : nullptr;
     c.rec_v = e->rec_cap ? e->rec_v.as<double>() : nullptr;
-    e->rec_drained.assign((size_t)M, 0);
-    // FSAL seed: k1 = f(u0)
+    e->rec_drained.assign(na, na + M);  // records from before a resume are not in the rings
+    // FSAL seed: k1 = f(u) (on a resume the same bits as the k7 of the last
+    // accepted step: the RHS kernels are deterministic)
     int rc = set_live(e);
     if (!rc) rc = ens_rhs(e, e->u.as<double>(), e->k.as<double>());
     if (rc) return rc;
@@ -1124,7 +1124,54 @@ int oq_ensemble_start(oq_ensemble *e, const double *u0, double t0, double t1,
     e->graph = g;
     OQ_HIP(hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0));
     e->stepping = true;
+    e->started = true;
     return OQ_OK;
+}
+
+int oq_ensemble_start(oq_ensemble *e, const double *u0, double t0, double t1,
+                      const oq_solver_opts *o)
+{
+    OQ_REQUIRE(e && u0 && o, "NULL argument");
+    OQ_REQUIRE(t1 > t0, "tspan must satisfy t1 > t0");
+    OQ_REQUIRE(o->dt0 > 0.0, "the ensemble stepper needs an initial step dt0 > 0");
+    const int64_t M = e->M;
+    const std::vector<double> tv(M, t0), dtv(M, o->dt0), qv(M, 1e-4);
+    const std::vector<long long> zl(M, 0);
+    const std::vector<int> zi(M, 0);
+    return ens_begin(e, u0, tv.data(), dtv.data(), qv.data(), zl.data(), zl.data(), zl.data(), zi.data(),
+                     zi.data(), t1, o);
+}
+
+int oq_ensemble_get_ctrl(oq_ensemble *e, oq_lockstep_ctrl *c)
+{
+    OQ_REQUIRE(e && c, "NULL argument");
+    OQ_REQUIRE(e->started, "the ensemble stepper has not been started");
+    oq_rhs *h = e->h;
+    OQ_HIP(hipSetDevice(h->device));
+    const int64_t M = e->M;
+    const std::pair<void *, const DevBuf *> d8[] = {{c->t, &e->ct}, {c->dt, &e->cdt}, {c->qold, &e->cqold},
+                                                     {c->naccept, &e->cna}, {c->nreject, &e->cnr},
+                                                     {c->iter, &e->citer}};
+    for (const auto &x : d8)
+        if (x.first) OQ_HIP(hipMemcpyAsync(x.first, x.second->p, M * 8, hipMemcpyDeviceToHost, h->stream));
+    if (c->done) OQ_HIP(hipMemcpyAsync(c->done, e->cdone.p, M * 4, hipMemcpyDeviceToHost, h->stream));
+    if (c->retcode) OQ_HIP(hipMemcpyAsync(c->retcode, e->cret.p, M * 4, hipMemcpyDeviceToHost, h->stream));
+    OQ_HIP(hipStreamSynchronize(h->stream));
+    return OQ_OK;
+}
+
+int oq_ensemble_resume(oq_ensemble *e, const double *u, const oq_lockstep_ctrl *c, double t1,
+                       const oq_solver_opts *o)
+{
+    OQ_REQUIRE(e && u && c && o, "NULL argument");
+    OQ_REQUIRE(c->t && c->dt && c->qold && c->naccept && c->nreject && c->iter && c->done && c->retcode,
+               "every controller array is needed to resume");
+    for (int64_t m = 0; m < e->M; ++m)
+        OQ_REQUIRE(c->done[m] || (c->t[m] < t1 && c->dt[m] > 0.0), "a running member needs t < t1 and dt > 0");
+    static_assert(sizeof(long long) == sizeof(int64_t), "int64_t counters");
+    return ens_begin(e, u, c->t, c->dt, c->qold, reinterpret_cast<const long long *>(c->naccept),
+                     reinterpret_cast<const long long *>(c->nreject), reinterpret_cast<const long long *>(c->iter),
+                     c->done, c->retcode, t1, o);
 }
 
 int oq_ensemble_run(oq_ensemble *e, int64_t max_steps, int64_t *nrunning)

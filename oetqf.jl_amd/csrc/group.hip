@@ -149,6 +149,14 @@ __device__ __forceinline__ f4x gload4(const float *p)
     return *(gf4x *)p;
 }
 
+// Streaming (non-temporal) load of K: each operator entry is read once per
+// RHS, while S is re-read by every row block of its geometry and should
+// stay in the caches.
+__device__ __forceinline__ f4x kload4(const float *p)
+{
+    return __builtin_nontemporal_load((gf4x *)p);
+}
+
 // s_waitcnt immediate that waits for at most n outstanding vector memory
 // operations (vmcnt bits [3:0] and [15:14]) and leaves the LDS/export counts
 // at their maxima
@@ -490,7 +498,7 @@ __global__ void __launch_bounds__(256) k_grp_gemm_lds2(GrpDev d, const int2 *__r
         auto load = [&](f4x (&kr)[KV], f4x (&sr)[SV], int64_t k) {
             const int64_t c0 = col(k);
 #pragma unroll
-            for (int j = 0; j < KV; ++j) kr[j] = gload4(kp[j] + c0);
+            for (int j = 0; j < KV; ++j) kr[j] = kload4(kp[j] + c0);
 #pragma unroll
             for (int j = 0; j < SV; ++j) sr[j] = gload4(sp[j] + c0);
         };
@@ -625,7 +633,7 @@ __global__ void __launch_bounds__(256, 3) k_grp_gemm_units(GrpDev d, const GrpUn
         auto load = [&](f4x (&kr)[KV], f4x (&sr)[SV], int64_t k) {
             const int64_t c0 = col(k);
 #pragma unroll
-            for (int j = 0; j < KV; ++j) kr[j] = gload4(kp[j] + c0);
+            for (int j = 0; j < KV; ++j) kr[j] = kload4(kp[j] + c0);
 #pragma unroll
             for (int j = 0; j < SV; ++j) sr[j] = gload4(sp[j] + c0);
         };
@@ -726,8 +734,8 @@ __global__ void __launch_bounds__(256) k_grp_sum_partials(GrpDev d, const GrpRow
 // The same units and partial sums as k_grp_gemm_units (64-row blocks here),
 // with every slice (64 rows x 128 columns of K, 16 x 128 of S: 40 KB) moved
 // from HBM into LDS by global_load_lds_dwordx4: no register staging, so the
-// NS-stage ring keeps NS - 1 slices in flight per workgroup at 28 VGPRs of
-// MFMA state.  One wavefront instruction moves 1 KB (two 512-byte row
+// depth of the NS-stage ring (NS - 1 slices in flight per workgroup) costs
+// LDS, not registers.  One wavefront instruction moves 1 KB (two 512-byte row
 // segments); a lane's 16-byte chunk j of row r lands in slot j but holds
 // column chunk j ^ (r mod 32), so that the MFMA-layout reads (16 rows at one
 // column chunk per quarter wavefront) fall in 16 distinct bank groups.
@@ -738,9 +746,10 @@ static constexpr int kDBR = 64, kDBK = 128, kDCh = kDBK / 4;
 static constexpr int kDKSl = kDBR * kDBK, kDSSl = kGM * kDBK, kDStage = kDKSl + kDSSl;  // floats
 typedef __attribute__((address_space(3))) void lds_void;
 
+template <bool NT>
 __device__ __forceinline__ void dma16(const float *g, float *l)
 {
-    __builtin_amdgcn_global_load_lds(g, (lds_void *)l, 16, 0, 0);
+    __builtin_amdgcn_global_load_lds(g, (lds_void *)l, 16, 0, NT ? 2 : 0);  // 2: the nt cache policy
 }
 
 // Workgroup barrier without the release fence of __syncthreads(): the
@@ -755,7 +764,7 @@ __device__ __forceinline__ void ring_barrier()
     asm volatile("" ::: "memory");
 }
 
-template <int NS>
+template <int NS, bool NT>
 __global__ void __launch_bounds__(256, 1) k_grp_gemm_dma(GrpDev d, const GrpUnit *__restrict__ units,
                                                         const int2 *__restrict__ wg_range,
                                                         double *__restrict__ P)
@@ -795,9 +804,9 @@ __global__ void __launch_bounds__(256, 1) k_grp_gemm_dma(GrpDev d, const GrpUnit
             const int64_t col = (int64_t)kr * kDBK;
             float *st = dsm + (k % NS) * kDStage;
 #pragma unroll
-            for (int i = 0; i < KI; ++i) dma16(kp[i] + col, st + (wave * KI + i) * 256);
+            for (int i = 0; i < KI; ++i) dma16<NT>(kp[i] + col, st + (wave * KI + i) * 256);
 #pragma unroll
-            for (int i = 0; i < SI; ++i) dma16(sp[i] + col, st + kDKSl + (wave * SI + i) * 256);
+            for (int i = 0; i < SI; ++i) dma16<false>(sp[i] + col, st + kDKSl + (wave * SI + i) * 256);
         };
         f4x acc[4];
         float hi[4][4], lo[4][4];
@@ -1025,8 +1034,9 @@ __global__ void __launch_bounds__(256) k_grp_commit(GrpCtrl c, int64_t ntot,
 // in flight (a third slice in flight measured the same as two: the kernel
 // is not limited by its prefetch depth), and (shape 10) the two-slice LDS
 // kernel with RB = 2 on column-split units balanced over the workgroups;
-// (shapes 11, 12) the direct-to-LDS kernel on 64-row units with a 3- or
-// 4-stage ring
+// (shapes 11, 12) the direct-to-LDS kernel on 64-row units with a 3-stage
+// ring (a 4-stage ring measured the same), K copied with and without the
+// non-temporal cache policy
 static constexpr int kGShapes = 13;
 static constexpr int kGUnitShape = 10;  // first of the unit shapes 10..12
 static constexpr int kGUnitShapes = 3;
@@ -1035,7 +1045,7 @@ static constexpr int64_t kGUnitBytes = (int64_t)2 << 20;  // target bytes of K p
 static inline bool is_unit_shape(int s) { return s >= kGUnitShape; }
 static inline int unit_br(int s) { return s == kGUnitShape ? kGUnitBR : kDBR; }
 static inline int unit_bk(int s) { return s == kGUnitShape ? kGUnitBK : kDBK; }
-static inline size_t dma_lds(int s) { return (size_t)(s - kGUnitShape + 2) * kDStage * 4; }  // NS = 3, 4
+static inline size_t dma_lds(int) { return (size_t)3 * kDStage * 4; }  // NS = 3
 static inline int shape_rb(int s)
 {
     if (is_unit_shape(s)) return s == kGUnitShape ? kGUnitRB : 1;
@@ -1075,6 +1085,7 @@ struct oq_group {
     oq::GrpDev dev{};
     oq::GrpCtrl ctrl{};
     bool stepping = false;
+    bool started = false;  // the controller arrays hold a run (get_ctrl)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -1196,10 +1207,10 @@ static int launch_units(oq_group *e, int s, const GrpUnitTable &T)
                            blk, 0, e->stream, e->dev, T.rbl.as<GrpRowBlock>(), e->P.as<double>());
     } else {
         if (s == kGUnitShape + 1)
-            hipLaunchKernelGGL((k_grp_gemm_dma<3>), grid, blk, dma_lds(s), e->stream, e->dev,
+            hipLaunchKernelGGL((k_grp_gemm_dma<3, true>), grid, blk, dma_lds(s), e->stream, e->dev,
                                T.units.as<GrpUnit>(), T.wg.as<int2>(), e->P.as<double>());
         else
-            hipLaunchKernelGGL((k_grp_gemm_dma<4>), grid, blk, dma_lds(s), e->stream, e->dev,
+            hipLaunchKernelGGL((k_grp_gemm_dma<3, false>), grid, blk, dma_lds(s), e->stream, e->dev,
                                T.units.as<GrpUnit>(), T.wg.as<int2>(), e->P.as<double>());
         hipLaunchKernelGGL((k_grp_sum_partials<kDBR>), dim3(gbk(kDBR * e->M, 256), (unsigned)T.nrbl),
                            blk, 0, e->stream, e->dev, T.rbl.as<GrpRowBlock>(), e->P.as<double>());
@@ -1493,9 +1504,9 @@ int oq_group_create(int device, int64_t ngeo, const oq_fault_mesh *mf,
         if (!rc) rc = occupancy_grid(k_grp_gemm_lds2<1>, &gr[8]);
         if (!rc) rc = occupancy_grid(k_grp_gemm_lds2<2>, &gr[9]);
         if (!rc) rc = occupancy_grid(k_grp_gemm_units<kGUnitRB>, &gr[10]);
-        // the direct-to-LDS shapes need 120 / 160 KB of LDS per workgroup: a
-        // device that cannot give it leaves the shape out (grid 0)
-        const void *dk[2] = {(const void *)k_grp_gemm_dma<3>, (const void *)k_grp_gemm_dma<4>};
+        // the direct-to-LDS shapes need 120 KB of LDS per workgroup: a device
+        // that cannot give it leaves the shape out (grid 0)
+        const void *dk[2] = {(const void *)k_grp_gemm_dma<3, true>, (const void *)k_grp_gemm_dma<3, false>};
         for (int i = 0; i < 2 && !rc; ++i) {
             const int sh = kGUnitShape + 1 + i;
             gr[sh] = 0;
@@ -1504,8 +1515,8 @@ int oq_group_create(int device, int64_t ngeo, const oq_fault_mesh *mf,
                 (void)hipGetLastError();
                 continue;
             }
-            rc = i == 0 ? occupancy_grid(k_grp_gemm_dma<3>, &gr[sh], dma_lds(sh))
-                        : occupancy_grid(k_grp_gemm_dma<4>, &gr[sh], dma_lds(sh));
+            rc = i == 0 ? occupancy_grid(k_grp_gemm_dma<3, true>, &gr[sh], dma_lds(sh))
+                        : occupancy_grid(k_grp_gemm_dma<3, false>, &gr[sh], dma_lds(sh));
         }
         for (int s = 0; s < kGShapes; ++s) e->grid[s] = gr[s];
     }
@@ -1696,28 +1707,27 @@ int oq_group_time(oq_group *e, int iters, double *ms_eval)
     return OQ_OK;
 }
 
-int oq_group_start(oq_group *e, const double *u0, double t0, double t1, const oq_solver_opts *o)
+// Uploads the members' states (host layout of oq_group_start) and
+// controller arrays, seeds k1 = f(u) and captures the step graph.
+static int grp_begin(oq_group *e, const double *u0, const double *tv, const double *dtv, const double *qv,
+                     const long long *na, const long long *nr, const long long *it, const int *done,
+                     const int *ret, double t1, const oq_solver_opts *o)
 {
-    OQ_REQUIRE(e && u0 && o, "NULL argument");
-    OQ_REQUIRE(t1 > t0, "tspan must satisfy t1 > t0");
-    OQ_REQUIRE(o->dt0 > 0.0, "the grouped stepper needs an initial step dt0 > 0");
     OQ_HIP(hipSetDevice(e->device));
     const int64_t GM = e->G * e->M;
     std::vector<double> ui;
     to_entry_major(e, u0, ui);
+    const std::vector<int> zi(GM, 0);
     OQ_HIP(hipMemcpyAsync(e->u.p, ui.data(), ui.size() * 8, hipMemcpyHostToDevice, e->stream));
-    std::vector<double> tv(GM, t0), dtv(GM, o->dt0), qv(GM, 1e-4);
-    std::vector<long long> zl(GM, 0);
-    std::vector<int> zi(GM, 0);
-    OQ_HIP(hipMemcpyAsync(e->ct.p, tv.data(), GM * 8, hipMemcpyHostToDevice, e->stream));
-    OQ_HIP(hipMemcpyAsync(e->cdt.p, dtv.data(), GM * 8, hipMemcpyHostToDevice, e->stream));
-    OQ_HIP(hipMemcpyAsync(e->cqold.p, qv.data(), GM * 8, hipMemcpyHostToDevice, e->stream));
-    OQ_HIP(hipMemcpyAsync(e->cna.p, zl.data(), GM * 8, hipMemcpyHostToDevice, e->stream));
-    OQ_HIP(hipMemcpyAsync(e->cnr.p, zl.data(), GM * 8, hipMemcpyHostToDevice, e->stream));
-    OQ_HIP(hipMemcpyAsync(e->citer.p, zl.data(), GM * 8, hipMemcpyHostToDevice, e->stream));
-    OQ_HIP(hipMemcpyAsync(e->cdone.p, zi.data(), GM * 4, hipMemcpyHostToDevice, e->stream));
+    OQ_HIP(hipMemcpyAsync(e->ct.p, tv, GM * 8, hipMemcpyHostToDevice, e->stream));
+    OQ_HIP(hipMemcpyAsync(e->cdt.p, dtv, GM * 8, hipMemcpyHostToDevice, e->stream));
+    OQ_HIP(hipMemcpyAsync(e->cqold.p, qv, GM * 8, hipMemcpyHostToDevice, e->stream));
+    OQ_HIP(hipMemcpyAsync(e->cna.p, na, GM * 8, hipMemcpyHostToDevice, e->stream));
+    OQ_HIP(hipMemcpyAsync(e->cnr.p, nr, GM * 8, hipMemcpyHostToDevice, e->stream));
+    OQ_HIP(hipMemcpyAsync(e->citer.p, it, GM * 8, hipMemcpyHostToDevice, e->stream));
+    OQ_HIP(hipMemcpyAsync(e->cdone.p, done, GM * 4, hipMemcpyHostToDevice, e->stream));
     OQ_HIP(hipMemcpyAsync(e->cacc.p, zi.data(), GM * 4, hipMemcpyHostToDevice, e->stream));
-    OQ_HIP(hipMemcpyAsync(e->cret.p, zi.data(), GM * 4, hipMemcpyHostToDevice, e->stream));
+    OQ_HIP(hipMemcpyAsync(e->cret.p, ret, GM * 4, hipMemcpyHostToDevice, e->stream));
     GrpCtrl &c = e->ctrl;
     c.t1 = t1;
     c.reltol = o->reltol;
@@ -1725,7 +1735,8 @@ int oq_group_start(oq_group *e, const double *u0, double t0, double t1, const oq
     c.dtmax = o->dtmax > 0.0 ? o->dtmax : INFINITY;
     c.dtmin = o->dtmin;
     c.maxiters = o->maxiters > 0 ? o->maxiters : (long long)1e18;
-    // FSAL seed: k1 = f(u0)
+    // FSAL seed: k1 = f(u) (on a resume with the same tile shapes, the same
+    // bits as the k7 of the last accepted step)
     int rc = set_live(e);
     if (!rc) rc = grp_rhs(e, e->u.as<double>(), e->k.as<double>());
     if (rc) return rc;
@@ -1744,7 +1755,51 @@ int oq_group_start(oq_group *e, const double *u0, double t0, double t1, const oq
     e->graph = g;
     OQ_HIP(hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0));
     e->stepping = true;
+    e->started = true;
     return OQ_OK;
+}
+
+int oq_group_start(oq_group *e, const double *u0, double t0, double t1, const oq_solver_opts *o)
+{
+    OQ_REQUIRE(e && u0 && o, "NULL argument");
+    OQ_REQUIRE(t1 > t0, "tspan must satisfy t1 > t0");
+    OQ_REQUIRE(o->dt0 > 0.0, "the grouped stepper needs an initial step dt0 > 0");
+    const int64_t GM = e->G * e->M;
+    const std::vector<double> tv(GM, t0), dtv(GM, o->dt0), qv(GM, 1e-4);
+    const std::vector<long long> zl(GM, 0);
+    const std::vector<int> zi(GM, 0);
+    return grp_begin(e, u0, tv.data(), dtv.data(), qv.data(), zl.data(), zl.data(), zl.data(), zi.data(),
+                     zi.data(), t1, o);
+}
+
+int oq_group_get_ctrl(oq_group *e, oq_lockstep_ctrl *c)
+{
+    OQ_REQUIRE(e && c, "NULL argument");
+    OQ_REQUIRE(e->started, "the grouped stepper has not been started");
+    OQ_HIP(hipSetDevice(e->device));
+    const int64_t GM = e->G * e->M;
+    const std::pair<void *, const DevBuf *> d8[] = {{c->t, &e->ct}, {c->dt, &e->cdt}, {c->qold, &e->cqold},
+                                                     {c->naccept, &e->cna}, {c->nreject, &e->cnr},
+                                                     {c->iter, &e->citer}};
+    for (const auto &x : d8)
+        if (x.first) OQ_HIP(hipMemcpyAsync(x.first, x.second->p, GM * 8, hipMemcpyDeviceToHost, e->stream));
+    if (c->done) OQ_HIP(hipMemcpyAsync(c->done, e->cdone.p, GM * 4, hipMemcpyDeviceToHost, e->stream));
+    if (c->retcode) OQ_HIP(hipMemcpyAsync(c->retcode, e->cret.p, GM * 4, hipMemcpyDeviceToHost, e->stream));
+    OQ_HIP(hipStreamSynchronize(e->stream));
+    return OQ_OK;
+}
+
+int oq_group_resume(oq_group *e, const double *u, const oq_lockstep_ctrl *c, double t1, const oq_solver_opts *o)
+{
+    OQ_REQUIRE(e && u && c && o, "NULL argument");
+    OQ_REQUIRE(c->t && c->dt && c->qold && c->naccept && c->nreject && c->iter && c->done && c->retcode,
+               "every controller array is needed to resume");
+    for (int64_t q = 0; q < e->G * e->M; ++q)
+        OQ_REQUIRE(c->done[q] || (c->t[q] < t1 && c->dt[q] > 0.0), "a running member needs t < t1 and dt > 0");
+    static_assert(sizeof(long long) == sizeof(int64_t), "int64_t counters");
+    return grp_begin(e, u, c->t, c->dt, c->qold, reinterpret_cast<const long long *>(c->naccept),
+                     reinterpret_cast<const long long *>(c->nreject), reinterpret_cast<const long long *>(c->iter),
+                     c->done, c->retcode, t1, o);
 }
 
 int oq_group_run(oq_group *e, int64_t max_steps, int64_t *nrunning)

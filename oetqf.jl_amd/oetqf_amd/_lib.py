@@ -10,6 +10,8 @@ import os
 import warnings
 import weakref
 
+import numpy as np
+
 _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIBPATH = os.path.join(_PKG, "lib", "liboetqf_amd.so")
 
@@ -65,6 +67,31 @@ class OetqfError(RuntimeError):
 _lib = None
 
 # name -> (restype, argtypes); the exported surface of include/oetqf_amd.h
+class LockstepCtrl(ctypes.Structure):
+    """oq_lockstep_ctrl: per-member controller arrays of a lockstep run."""
+    _fields_ = [("t", ctypes.POINTER(ctypes.c_double)), ("dt", ctypes.POINTER(ctypes.c_double)),
+                ("qold", ctypes.POINTER(ctypes.c_double)), ("naccept", ctypes.POINTER(ctypes.c_int64)),
+                ("nreject", ctypes.POINTER(ctypes.c_int64)), ("iter", ctypes.POINTER(ctypes.c_int64)),
+                ("done", ctypes.POINTER(ctypes.c_int)), ("retcode", ctypes.POINTER(ctypes.c_int))]
+
+    @classmethod
+    def of(cls, c):
+        """The structure over the numpy arrays of a controller dict (kept
+        alive by the caller)."""
+        pd, pl, pi = (ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
+                      ctypes.POINTER(ctypes.c_int))
+        return cls(*[c[k].ctypes.data_as(p) for k, p in (("t", pd), ("dt", pd), ("qold", pd), ("naccept", pl),
+                                                          ("nreject", pl), ("iter", pl), ("done", pi),
+                                                          ("retcode", pi))])
+
+
+def new_ctrl(n):
+    """Empty controller arrays of n members."""
+    return dict(t=np.zeros(n), dt=np.zeros(n), qold=np.zeros(n), naccept=np.zeros(n, np.int64),
+                nreject=np.zeros(n, np.int64), iter=np.zeros(n, np.int64), done=np.zeros(n, np.int32),
+                retcode=np.zeros(n, np.int32))
+
+
 PROTOTYPES = {
     "oq_last_error": (ctypes.c_char_p, []),
     "oq_version": (_int, []),
@@ -111,6 +138,8 @@ PROTOTYPES = {
                                   ctypes.POINTER(_vp)]),
     "oq_ensemble_eval": (_int, [_vp, _pd, _pd]),
     "oq_ensemble_start": (_int, [_vp, _pd, _d, _d, ctypes.POINTER(SolverOpts)]),
+    "oq_ensemble_get_ctrl": (_int, [_vp, ctypes.POINTER(LockstepCtrl)]),
+    "oq_ensemble_resume": (_int, [_vp, _pd, ctypes.POINTER(LockstepCtrl), _d, ctypes.POINTER(SolverOpts)]),
     "oq_ensemble_run": (_int, [_vp, _i64, ctypes.POINTER(_i64)]),
     "oq_ensemble_state": (_int, [_vp, _pd, _pd, ctypes.POINTER(_i64), ctypes.POINTER(_i64),
                                  ctypes.POINTER(_int)]),
@@ -127,6 +156,8 @@ PROTOTYPES = {
     "oq_group_eval": (_int, [_vp, _pd, _pd]),
     "oq_group_time": (_int, [_vp, _int, ctypes.POINTER(_d)]),
     "oq_group_start": (_int, [_vp, _pd, _d, _d, ctypes.POINTER(SolverOpts)]),
+    "oq_group_get_ctrl": (_int, [_vp, ctypes.POINTER(LockstepCtrl)]),
+    "oq_group_resume": (_int, [_vp, _pd, ctypes.POINTER(LockstepCtrl), _d, ctypes.POINTER(SolverOpts)]),
     "oq_group_run": (_int, [_vp, _i64, ctypes.POINTER(_i64)]),
     "oq_group_state": (_int, [_vp, _pd, _pd, ctypes.POINTER(_i64), ctypes.POINTER(_i64),
                               ctypes.POINTER(_int)]),

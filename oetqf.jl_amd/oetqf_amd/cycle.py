@@ -90,27 +90,43 @@ def recurrence(evs):
 
 
 def run_ensemble_cycle(ens, U0, tspan, reltol=1e-6, abstol=1e-8, dt0=1.0, dtmax=0.0, chunk=2048,
-                       wall_limit=None):
+                       wall_limit=None, resume=None, progress=None):
     """Integrates all members of an Ensemble over tspan with the per-member
     recorder on.  Returns per-member (t, vmax) records, final states and
-    counters."""
+    counters; `complete` is False when wall_limit stopped it, and then
+    `checkpoint` = (U, ctrl) continues it: pass the returned dict as
+    `resume` (its records are kept in front of the new ones)."""
     ens.record(chunk)
-    ens.start(U0, tspan, reltol=reltol, abstol=abstol, dt0=dt0, dtmax=dtmax)
     ts = [[] for _ in range(ens.M)]
     vs = [[] for _ in range(ens.M)]
+    iters0, wall0 = 0, 0.0
+    if resume is not None:
+        U, ctrl = resume["checkpoint"]
+        ens.resume(U, ctrl, tspan[1], reltol=reltol, abstol=abstol, dtmax=dtmax)
+        for m in range(ens.M):
+            ts[m].append(resume["t"][m])
+            vs[m].append(resume["vmax"][m])
+        iters0, wall0 = resume["iters"], resume["wall"]
+    else:
+        ens.start(U0, tspan, reltol=reltol, abstol=abstol, dt0=dt0, dtmax=dtmax)
     t0 = time.perf_counter()
-    iters = 0
+    iters = iters0
+    complete = False
     while True:
         left = ens.run(chunk)
         iters += chunk
         for m, (t, v) in enumerate(ens.drain()):
             ts[m].append(t)
             vs[m].append(v)
+        if progress is not None:
+            progress(iters, left)
         if left == 0:
+            complete = True
             break
         if wall_limit is not None and time.perf_counter() - t0 > wall_limit:
             break
-    wall = time.perf_counter() - t0
+    wall = wall0 + time.perf_counter() - t0
     t, U, na, nr, rc = ens.state()
     return dict(t=[np.concatenate(x) for x in ts], vmax=[np.concatenate(x) for x in vs],
-                t_end=t, U=U, naccept=na, nreject=nr, retcode=rc, wall=wall, iters=iters)
+                t_end=t, U=U, naccept=na, nreject=nr, retcode=rc, wall=wall, iters=iters,
+                complete=complete, checkpoint=None if complete else ens.checkpoint())

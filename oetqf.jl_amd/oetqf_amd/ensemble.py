@@ -81,6 +81,23 @@ class Ensemble:
             self.h, cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), _lib.ptr(t), _lib.ptr(v)))
         return [(t[m, :cnt[m]].copy(), v[m, :cnt[m]].copy()) for m in range(self.M)]
 
+    def checkpoint(self):
+        """(U, ctrl) of the current run: the members' states and controller
+        arrays (t, dt, qold, naccept, nreject, iter, done, retcode), for
+        resume() here or in another process."""
+        c = _lib.new_ctrl(self.M)
+        _lib.check(_lib.load().oq_ensemble_get_ctrl(self.h, ctypes.byref(_lib.LockstepCtrl.of(c))))
+        return self.state()[1], c
+
+    def resume(self, U, ctrl, t1, reltol=1e-6, abstol=1e-8, dtmax=0.0, dtmin=0.0, maxiters=10**9):
+        """Continues a checkpointed run towards t1 (bit for bit the
+        uninterrupted run on the same operator)."""
+        self._u0 = np.ascontiguousarray(U, dtype=np.float64)
+        self._ctrl = {k: np.ascontiguousarray(v, dtype=_lib.new_ctrl(1)[k].dtype) for k, v in ctrl.items()}
+        o = _lib.SolverOpts(reltol, abstol, 1.0, dtmax, dtmin, int(maxiters))
+        _lib.check(_lib.load().oq_ensemble_resume(self.h, _lib.ptr(self._u0), ctypes.byref(_lib.LockstepCtrl.of(self._ctrl)),
+                                  float(t1), ctypes.byref(o)))
+
     def run(self, max_steps):
         n = ctypes.c_int64()
         _lib.check(_lib.load().oq_ensemble_run(self.h, int(max_steps), ctypes.byref(n)))

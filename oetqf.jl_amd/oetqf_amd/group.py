@@ -18,7 +18,7 @@ from .equation import _KDTYPES, _rs_struct, _visc_struct
 from .greens import StrikeSlip, _Keep, _ftype, fault_struct, get_quadrature, hex8_struct, quad_struct
 
 SHAPES = ["RB1-pipe", "RB1", "RB2-pipe", "RB2", "RB4-pipe", "RB4", "RB1-lds", "RB2-lds",
-          "RB1-lds-d2", "RB2-lds-d2", "RB2-lds-d2-split", "RB1-dma-s3", "RB1-dma-s4"]
+          "RB1-lds-d2", "RB2-lds-d2", "RB2-lds-d2-split", "RB1-dma-nt", "RB1-dma"]
 
 
 class Group:
@@ -116,6 +116,23 @@ class Group:
         o = _lib.SolverOpts(reltol, abstol, dt0, dtmax, dtmin, int(maxiters))
         _lib.check(_lib.load().oq_group_start(self.h, _lib.ptr(self._u0), float(tspan[0]),
                                               float(tspan[1]), ctypes.byref(o)))
+
+    def checkpoint(self):
+        """(U, ctrl) of the current run: the members' states and controller
+        arrays (t, dt, qold, naccept, nreject, iter, done, retcode), for
+        resume() here or in another process."""
+        c = _lib.new_ctrl(self.G * self.M)
+        _lib.check(_lib.load().oq_group_get_ctrl(self.h, ctypes.byref(_lib.LockstepCtrl.of(c))))
+        return self.state()[1], c
+
+    def resume(self, U, ctrl, t1, reltol=1e-6, abstol=1e-8, dtmax=0.0, dtmin=0.0, maxiters=10**9):
+        """Continues a checkpointed run towards t1 (bit for bit the
+        uninterrupted run on the same operator)."""
+        self._u0 = np.ascontiguousarray(U, dtype=np.float64)
+        self._ctrl = {k: np.ascontiguousarray(v, dtype=_lib.new_ctrl(1)[k].dtype) for k, v in ctrl.items()}
+        o = _lib.SolverOpts(reltol, abstol, 1.0, dtmax, dtmin, int(maxiters))
+        _lib.check(_lib.load().oq_group_resume(self.h, _lib.ptr(self._u0), ctypes.byref(_lib.LockstepCtrl.of(self._ctrl)),
+                                  float(t1), ctypes.byref(o)))
 
     def run(self, max_steps):
         n = ctypes.c_int64()

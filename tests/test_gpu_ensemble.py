@@ -82,3 +82,38 @@ def test_ensemble_stepper_matches_single_member(oq, rng, kdtype):
         nf = mf.nx * mf.nxi
         assert relerr(U[m][:nf], ref[:nf]) <= 1e-4       # slip rate
         assert relerr(U[m][nf:2 * nf], ref[nf:2 * nf]) <= 1e-5  # state variable
+
+
+@pytest.mark.parametrize("kdtype", ["fp32", "fp16x2"])
+def test_ensemble_checkpoint_resume_is_the_same_run(oq, rng, kdtype):
+    """A lockstep run stopped after some steps, checkpointed, and resumed in a
+    second ensemble on the same operator continues bit for bit as the
+    uninterrupted run: states, times and every step counter."""
+    mf, ma = small_fault(90.0), small_mantle()
+    members = _members(mf, rng, 3)
+    pa = visc_props(ma, rng, 0.0)
+    U0 = np.stack([_state(mf, ma, p, rng).pack() for p in members])
+    T = 0.3 * 365 * 86400.0
+    kw = dict(buffer_ratio=1.0, kdtype=kdtype)
+    a = oq.Ensemble(mf, ma, 3e10, 3e10, members, pa, **kw)
+    a.start(U0, (0.0, T), dt0=1.0)
+    a.run(300)
+    U, ctrl = a.checkpoint()
+    assert np.all(ctrl["naccept"] > 0) and np.all(ctrl["t"] < T)
+    while a.run(256) > 0:
+        pass
+    t1, U1, na1, nr1, rc1 = a.state()
+    c1 = a.checkpoint()[1]
+    a.close()
+    b = oq.Ensemble(mf, ma, 3e10, 3e10, members, pa, **kw)
+    b.resume(U, ctrl, T)
+    while b.run(256) > 0:
+        pass
+    t2, U2, na2, nr2, rc2 = b.state()
+    c2 = b.checkpoint()[1]
+    b.close()
+    assert rc1 == rc2 and all(r == "Success" for r in rc1)
+    assert np.array_equal(t1, t2) and np.array_equal(U1, U2)
+    assert np.array_equal(na1, na2) and np.array_equal(nr1, nr2)
+    for k in ("dt", "qold", "iter"):
+        assert np.array_equal(c1[k], c2[k]), k

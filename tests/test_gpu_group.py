@@ -159,3 +159,32 @@ def test_group_stepper_matches_single_members(oq):
             ref = sol.u.pack()
             assert relerr(Us[g][m][:nf], ref[:nf]) <= 1e-4
             assert relerr(Us[g][m][nf:2 * nf], ref[nf:2 * nf]) <= 1e-5
+
+
+def test_group_checkpoint_resume_is_the_same_run(oq):
+    """Checkpoint after some lockstep steps, resume in a second group with the
+    same tile shapes: the continuation is bit for bit the uninterrupted run."""
+    rng = np.random.default_rng(47)
+    geoms = _geoms(rng)
+    members = [_members(mf, rng, 2) for mf, _, _ in geoms]
+    states = [np.stack([_state(mf, ma, p, rng).pack() for p in mem]) for (mf, ma, _), mem in zip(geoms, members)]
+    T = 0.3 * 365 * 86400.0
+    a = oq.Group(geoms, LAM, MU, members, buffer_ratio=1.0)
+    U0 = a.pack(states)
+    a.start(U0, (0.0, T), dt0=1.0)
+    a.run(300)
+    U, ctrl = a.checkpoint()
+    assert np.all(ctrl["naccept"] > 0) and np.all(ctrl["t"] < T)
+    while a.run(256) > 0:
+        pass
+    t1, U1, na1, nr1, rc1 = a.state()
+    a.close()
+    b = oq.Group(geoms, LAM, MU, members, buffer_ratio=1.0)
+    b.resume(U, ctrl, T)
+    while b.run(256) > 0:
+        pass
+    t2, U2, na2, nr2, rc2 = b.state()
+    b.close()
+    assert rc1 == rc2 and all(r == "Success" for r in rc1)
+    assert np.array_equal(t1, t2) and np.array_equal(U1, U2)
+    assert np.array_equal(na1, na2) and np.array_equal(nr1, nr2)
