@@ -606,7 +606,7 @@ def _span_range(rng, n):
     return a, b
 
 
-def spans_ensemble(oq, kdtypes, sampled, member_range=""):
+def spans_ensemble(oq, kdtypes, sampled, member_range="", wall=None):
     """Members a:b (default all 256) of the composite-geometry ensemble over
     300 years with the lockstep Tsit5 (per-member steps), per operator type:
     rejection rates, event counts and mean recurrence of every member, and
@@ -634,9 +634,19 @@ def spans_ensemble(oq, kdtypes, sampled, member_range=""):
                 last[0] = time.perf_counter()
                 log("ensemble %s: %d iterations, %d members running" % (kd, iters, left))
         r = run_ensemble_cycle(ens, U0[lo:hi], span, reltol=w["reltol"], abstol=w["abstol"], dt0=1.0,
-                               progress=progress)
+                               progress=progress, wall_limit=wall)
         ens.close()
         M = hi - lo
+        if not r["complete"]:  # stopped by the wall limit: where every member got to
+            tend = r["t_end"] / YEAR
+            out["by_kdtype"][kd] = dict(member_range=[lo, hi], complete=False, wall_s=r["wall"],
+                                        lockstep_iterations=r["iters"], t_reached_yr_min=float(tend.min()),
+                                        t_reached_yr_mean=float(tend.mean()),
+                                        members_done=int(np.sum(tend >= 300.0)),
+                                        accepted_max=int(r["naccept"].max()), rejected_max=int(r["nreject"].max()))
+            log("ensemble %s: stopped after %.0f s at %d iterations; t reached min %.1f / mean %.1f yr, %d done" % (
+                kd, r["wall"], r["iters"], tend.min(), tend.mean(), int(np.sum(tend >= 300.0))))
+            continue
         cats = [_catalogue(r["t"][m], r["vmax"][m]) for m in range(M)]
         na, nr = r["naccept"], r["nreject"]
         out["by_kdtype"][kd] = dict(
@@ -731,6 +741,80 @@ def spans_group(oq, geo_range="", shape=""):
                              for g in range(G)})
 
 
+def spans_summary(paths):
+    """Markdown tables of the span JSON files (comma-separated): composite
+    runs, ensemble member ranges, members run alone in FP64, grouped
+    geometry ranges."""
+    lines = []
+    runs, ens, alone, grp = {}, {}, {}, []
+    for path in [x for x in paths.split(",") if x]:
+        with open(path) as fh:
+            d = json.load(fh)
+        runs.update(d.get("composite", {}).get("runs", {}))
+        for kd, e in d.get("ensemble", {}).get("by_kdtype", {}).items():
+            ens.setdefault(kd, []).append(e)
+        alone.update(d.get("members", {}).get("members", {}))
+        if "group" in d:
+            grp.append(d["group"])
+    if runs:
+        lines += ["| composite 500 yr | wall s | sim yr/hr | accepted | rejected | rejection rate | events | "
+                  "mean recurrence yr | vs FP64 (same method) |", "|---|---:|---:|---:|---:|---:|---:|---:|---|"]
+        for tag, r in runs.items():
+            vs = r.get("vs_fp64")
+            vs_s = "" if vs is None else ("same event count" if vs["same_event_count"] else "event count differs") + \
+                "".join(", %s %.1e" % (k, v) for k, v in vs.items() if k != "same_event_count")
+            lines.append("| %s | %.0f | %.0f | %d | %d | %.3f | %d | %s | %s |" % (
+                tag, r["wall_s"], r["sim_yr_per_hr"], r["accepted"], r["rejected"],
+                r.get("rejection_rate", r["rejected"] / max(1, r["accepted"] + r["rejected"])), r["events"],
+                "%.2f" % r["mean_recurrence_yr"] if r["mean_recurrence_yr"] else "-", vs_s))
+        lines.append("")
+    for kd, parts in ens.items():
+        mem = {}
+        for e in parts:
+            mem.update(e.get("members", {}))
+        na = sum(m["accepted"] for m in mem.values())
+        nr = sum(m["rejected"] for m in mem.values())
+        ev = [m["events"] for m in mem.values()]
+        lines += ["| ensemble %s, members | wall s | lockstep iterations | member-yr/hr | rejection rate | all success |"
+                  % kd, "|---|---:|---:|---:|---:|---|"]
+        for e in parts:
+            lines.append("| %d:%d | %.0f | %d | %.0f | %.3f | %s |" % (
+                e["member_range"][0], e["member_range"][1], e["wall_s"], e["lockstep_iterations"],
+                e["member_sim_yr_per_hr"], e["rejection_rate"], e["all_success"]))
+        lines.append("")
+        lines.append("%d members: rejection rate %.3f; events per member min %d, median %d, max %d." % (
+            len(mem), nr / max(1, na + nr), min(ev), int(np.median(ev)), max(ev)))
+        lines.append("")
+    if alone:
+        kds = sorted({k[9:-8] for r in alone.values() for k in r if k.startswith("ensemble_")})
+        lines += ["| member | FP64 alone: wall s | accepted | rejected | events | mean recurrence yr | " +
+                  " | ".join("ensemble %s vs FP64" % k for k in kds) + " |",
+                  "|---:|---:|---:|---:|---:|---:|" + "---|" * len(kds)]
+        for m, r in sorted(alone.items(), key=lambda x: int(x[0])):
+            cells = []
+            for k in kds:
+                c = r.get("ensemble_%s_vs_fp64" % k)
+                cells.append("-" if c is None else ("same events" if c["same_event_count"] else "events differ") +
+                             "".join(", %s %.1e" % (a, b) for a, b in c.items() if a != "same_event_count"))
+            lines.append("| %s | %.0f | %d | %d | %d | %s | %s |" % (
+                m, r["wall_s"], r["accepted"], r["rejected"], r["events"],
+                "%.2f" % r["mean_recurrence_yr"] if r["mean_recurrence_yr"] else "-", " | ".join(cells)))
+        lines.append("")
+    if grp:
+        lines += ["| grouped geometries | wall s | lockstep iterations | member-yr/hr | rejection rate mean / max | all success |",
+                  "|---|---:|---:|---:|---|---|"]
+        for g in grp:
+            lines.append("| %d:%d | %.0f | %d | %.0f | %.3f / %.3f | %s |" % (
+                g["geometry_range"][0], g["geometry_range"][1], g["wall_s"], g["lockstep_iterations"],
+                g["member_sim_yr_per_hr"], g["rejection_rate_mean"], g["rejection_rate_max"], g["all_success"]))
+        lines += ["", "| geometry | rejection rate | accepted (mean) | rejected (mean) |", "|---|---:|---:|---:|"]
+        for g in grp:
+            for name, r in sorted(g["by_geometry"].items(), key=lambda x: x[1]["index"]):
+                lines.append("| %s | %.3f | %.0f | %.0f |" % (name, r["rejection_rate"], r["accepted_mean"],
+                                                              r["rejected_mean"]))
+    print("\n".join(lines))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -763,6 +847,10 @@ def main():
     ap.add_argument("--spans-ref", default="",
                     help="spans JSON files of earlier calls (comma-separated): the FP64 composite runs and "
                          "the ensemble catalogues that this call's runs are compared with")
+    ap.add_argument("--spans-summary", default="",
+                    help="print the markdown tables of these span JSON files (comma-separated) and exit")
+    ap.add_argument("--spans-wall", type=float, default=0.0,
+                    help="wall-clock seconds after which an ensemble span stops and reports how far it got")
     ap.add_argument("--spans-range", default="",
                     help="a:b, the ensemble members (of 256) or grouped-batch geometries (of 24) of this call")
     ap.add_argument("--spans-runs", default="fp64:tsit5,fp64:vcabm5,fp32:vcabm5,fp32:tsit5",
@@ -781,6 +869,9 @@ def main():
     import oetqf_amd as oq
     from oetqf_amd.workloads import YEAR, transform_fault_composite
 
+    if args.spans_summary:
+        spans_summary(args.spans_summary)
+        return
     if args.spans:
         res = {}
         refs = _load_refs(args.spans_ref)
@@ -798,7 +889,8 @@ def main():
                 runs = [tuple(x.split(":")) for x in args.spans_runs.split(",") if x]
                 res[leg] = spans_composite(oq, runs, refs, save=lambda o: save(o, "composite"))
             elif leg == "ensemble":
-                res[leg] = spans_ensemble(oq, args.spans_kdtypes.split(","), sampled, args.spans_range)
+                res[leg] = spans_ensemble(oq, args.spans_kdtypes.split(","), sampled, args.spans_range,
+                                          args.spans_wall or None)
             elif leg == "members":
                 ref = refs.get("ensemble", {}).get("by_kdtype")
                 res[leg] = spans_members(oq, sampled, ref, args.spans_ref)

@@ -31,7 +31,7 @@ typedef unsigned int u4x __attribute__((ext_vector_type(4)));
 namespace oq {
 
 struct EnsDev {
-    int64_t nf, ne, N, ld, M, Mp, n;  // Mp: padded member count (multiple of 64)
+    int64_t nf, ne, N, ld, M, Mp, n;  // Mp: padded member count (multiple of 64; 256 for FP32 and split FP16)
     StateLayout lay;                   // offsets in entries (rows)
     const double *a, *b, *L, *sigma;   // nf x M
     const double *eta, *vpl, *f0, *v0; // M

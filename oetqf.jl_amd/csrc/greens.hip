@@ -263,21 +263,36 @@ __global__ void __launch_bounds__(256) k_gf_mf(DevFault f, DevHex8 h, const doub
 }
 
 // K_fm = gf21 (GF.jl:206-224): thread (source cell j fastest, receiver
-// patch i); all 6 unit eigenstrains from one geometric evaluation.
-__global__ void __launch_bounds__(128) k_gf_fm(DevHex8 h, DevFault f, double mu, double nu,
-                                               int ftype, double *out, int64_t rs, int64_t cs)
+// patch i); all 6 unit eigenstrains from one geometric evaluation.  Only
+// the shear traction on the fault is needed, so the image quadrature
+// accumulates its 6-term contraction (h8_image_traction) instead of the 54
+// integrals; the full-space part and the eigenstress follow as before.
+template <int FT>
+__global__ void __launch_bounds__(128) k_gf_fm(DevHex8 h, DevFault f, double mu, double nu, double *out,
+                                               int64_t rs, int64_t cs)
 {
     const int64_t nf = f.nx * f.nxi;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nf * h.ne) return;
     const int64_t j = t % h.ne, i = t / h.ne;
     const int64_t a = i % f.nx, b = i / f.nx;
-    Hex8Pair P;
-    hex8_pair(f.x[a], f.y[b], f.z[b], h.qx[j], h.qy[j], h.qz[j], h.dx[j], h.dy[j], h.dz[j], nu, P);
+    const double x = f.x[a], y = f.y[b], z = f.z[b];
+    const double lo[3] = {h.qx[j] - h.dx[j] / 2.0, h.qy[j], h.qz[j] - h.dz[j]};
+    const double hi[3] = {h.qx[j] + h.dx[j] / 2.0, h.qy[j] + h.dy[j], h.qz[j]};
+    const double ca = FT == 0 ? -mu * f.sd : mu * f.s2d, cb = FT == 0 ? mu * f.cd : mu * f.c2d;
+    double E[6] = {0, 0, 0, 0, 0, 0};
+    h8_image_traction<FT>(x, y, z, lo, hi, nu, ca, cb, E);
+    Hex8Kelvin K;
+    const bool inside = hex8_kelvin(x, y, z, lo, hi, K);
+    const double ci = 1.0 / (16.0 * M_PI * mu * (1.0 - nu));
     for (int p = 0; p < 6; ++p) {
-        double s[6];
-        hex8_stress_unit(P, p, mu, nu, s);
-        out[i * rs + (p * h.ne + j) * cs] = traction_vol(ftype, s, f);
+        double s[6], mv[6];
+        hex8_stress_unit_kd(K, nullptr, inside, p, mu, nu, s);
+        hex8_moment(p, mu, nu, mv);
+        double ti = 0.0;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) ti += E[q] * mv[q];
+        out[i * rs + (p * h.ne + j) * cs] = traction_vol(FT, s, f) + ti * ci;
     }
 }
 
@@ -374,8 +389,10 @@ int launch_gf_fm(const DevHex8 &h, const DevFault &f, double lambda, double mu, 
     const double nu = lambda / 2.0 / (lambda + mu);  // GF.jl:203
     const int64_t n = f.nx * f.nxi * h.ne;
     OQ_PAIRS_FIT(n);
-    hipLaunchKernelGGL(k_gf_fm, dim3(nblocks(n, 128)), dim3(128), 0, s, h, f, mu, nu, ftype, out,
-                       rs, cs);
+    if (ftype == 0)
+        hipLaunchKernelGGL(k_gf_fm<0>, dim3(nblocks(n, 128)), dim3(128), 0, s, h, f, mu, nu, out, rs, cs);
+    else
+        hipLaunchKernelGGL(k_gf_fm<1>, dim3(nblocks(n, 128)), dim3(128), 0, s, h, f, mu, nu, out, rs, cs);
     OQ_HIP(hipGetLastError());
     return OQ_OK;
 }

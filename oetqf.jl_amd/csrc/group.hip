@@ -276,7 +276,7 @@ __global__ void __launch_bounds__(256) k_grp_gemm(GrpDev d, const int2 *__restri
 #pragma unroll
             for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-                for (int t = 0; t < 4; ++t) kv[rb][t] = gload4(kp[rb] + p * kGStep + 16 * t);
+                for (int t = 0; t < 4; ++t) kv[rb][t] = kload4(kp[rb] + p * kGStep + 16 * t);
         }
         for (int64_t it = 0; it < nstep; ++it) {
             const int64_t pn = (p + 1 == nstep) ? 0 : p + 1;
@@ -287,7 +287,7 @@ __global__ void __launch_bounds__(256) k_grp_gemm(GrpDev d, const int2 *__restri
 #pragma unroll
                 for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-                    for (int t = 0; t < 4; ++t) kv[rb][t] = gload4(kp[rb] + p * kGStep + 16 * t);
+                    for (int t = 0; t < 4; ++t) kv[rb][t] = kload4(kp[rb] + p * kGStep + 16 * t);
             }
             // the prefetch is unconditional (pn wraps to the first pass after the
             // last one): a conditional load would make the compiler's wait
@@ -299,7 +299,7 @@ __global__ void __launch_bounds__(256) k_grp_gemm(GrpDev d, const int2 *__restri
 #pragma unroll
                 for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-                    for (int t = 0; t < 4; ++t) kn[rb][t] = gload4(kp[rb] + pn * kGStep + 16 * t);
+                    for (int t = 0; t < 4; ++t) kn[rb][t] = kload4(kp[rb] + pn * kGStep + 16 * t);
             }
             __builtin_amdgcn_sched_barrier(0);
             // lane (row lr, group lq) holds columns c0 + 16t + 4lq + e; MFMA
@@ -388,7 +388,7 @@ __global__ void __launch_bounds__(256) k_grp_gemm_lds(GrpDev d, const int2 *__re
         int64_t p = ((int64_t)tix * 13) % nch;
         f4x kr[KV], sr[SV];
 #pragma unroll
-        for (int j = 0; j < KV; ++j) kr[j] = gload4(kp[j] + p * BK);
+        for (int j = 0; j < KV; ++j) kr[j] = kload4(kp[j] + p * BK);
 #pragma unroll
         for (int j = 0; j < SV; ++j) sr[j] = gload4(sp[j] + p * BK);
         __syncthreads();  // the previous tile's reads of As/Bs are done
@@ -406,7 +406,7 @@ __global__ void __launch_bounds__(256) k_grp_gemm_lds(GrpDev d, const int2 *__re
             // slice they are redundant and the next tile re-stages after a
             // barrier): conditional loads make the wait counts drain
 #pragma unroll
-            for (int j = 0; j < KV; ++j) kr[j] = gload4(kp[j] + pn * BK);
+            for (int j = 0; j < KV; ++j) kr[j] = kload4(kp[j] + pn * BK);
 #pragma unroll
             for (int j = 0; j < SV; ++j) sr[j] = gload4(sp[j] + pn * BK);
 #pragma unroll
@@ -731,20 +731,29 @@ __global__ void __launch_bounds__(256) k_grp_sum_partials(GrpDev d, const GrpRow
 }
 
 // ---------------- direct-to-LDS variant on column-split units ----------------
-// The same units and partial sums as k_grp_gemm_units (64-row blocks here),
-// with every slice (64 rows x 128 columns of K, 16 x 128 of S: 40 KB) moved
-// from HBM into LDS by global_load_lds_dwordx4: no register staging, so the
+// The same units and partial sums as k_grp_gemm_units, on BR-row blocks,
+// with every slice (BR rows x BK columns of K and 16 x BK of S) moved from
+// HBM into LDS by global_load_lds_dwordx4: no register staging, so the
 // depth of the NS-stage ring (NS - 1 slices in flight per workgroup) costs
-// LDS, not registers.  One wavefront instruction moves 1 KB (two 512-byte row
-// segments); a lane's 16-byte chunk j of row r lands in slot j but holds
-// column chunk j ^ (r mod 32), so that the MFMA-layout reads (16 rows at one
-// column chunk per quarter wavefront) fall in 16 distinct bank groups.
-// Wavefront w reduces columns [32w, 32w + 32) of every slice for all 64
-// rows; the four column partials are added through LDS at the end of a unit
-// in a fixed order.
-static constexpr int kDBR = 64, kDBK = 128, kDCh = kDBK / 4;
-static constexpr int kDKSl = kDBR * kDBK, kDSSl = kGM * kDBK, kDStage = kDKSl + kDSSl;  // floats
+// LDS, not registers.  One wavefront instruction moves 1 KB of whole row
+// segments (BK/4 16-byte chunks per row); a lane's chunk j of row r lands
+// in slot j but holds column chunk j ^ (r mod BK/4), so that the
+// MFMA-layout reads (16 rows at one column chunk per quarter wavefront) fall
+// in 16 distinct bank groups.  Wavefront w reduces columns [w BK/4, (w + 1)
+// BK/4) of every slice for all BR rows; the four column partials are added
+// through LDS at the end of a unit in a fixed order.
 typedef __attribute__((address_space(3))) void lds_void;
+
+template <int BR, int BK>
+struct DmaShape {
+    static constexpr int CH = BK / 4;                         // 16-byte chunks per row of a slice
+    static constexpr int KSL = BR * BK, SSL = kGM * BK;       // floats of K and S per slice
+    static constexpr int STAGE = KSL + SSL;
+    static constexpr int KI = KSL / 4 / 256, SI = SSL / 4 / 256;  // DMA instructions per lane
+    static constexpr int RB = BR / 16, T = CH / 16;           // row blocks; 4-chunk groups per wavefront
+    static constexpr int FOLD = 1024 / BK;                    // slices per 256 columns of a wavefront
+    static_assert(KI >= 1 && SI >= 1 && T >= 1 && RB >= 1, "slice shape");
+};
 
 template <bool NT>
 __device__ __forceinline__ void dma16(const float *g, float *l)
@@ -764,14 +773,15 @@ __device__ __forceinline__ void ring_barrier()
     asm volatile("" ::: "memory");
 }
 
-template <int NS, bool NT>
-__global__ void __launch_bounds__(256, 1) k_grp_gemm_dma(GrpDev d, const GrpUnit *__restrict__ units,
-                                                        const int2 *__restrict__ wg_range,
-                                                        double *__restrict__ P)
+template <int NS, int BR, int BK, int OCC>
+__global__ void __launch_bounds__(256, OCC) k_grp_gemm_dma(GrpDev d, const GrpUnit *__restrict__ units,
+                                                          const int2 *__restrict__ wg_range,
+                                                          double *__restrict__ P)
 {
     if (!*d.live) return;
+    using Sh = DmaShape<BR, BK>;
+    constexpr int CH = Sh::CH, KI = Sh::KI, SI = Sh::SI, RB = Sh::RB;
     extern __shared__ __attribute__((aligned(16))) float dsm[];
-    constexpr int KI = kDKSl / 4 / 256, SI = kDSSl / 4 / 256;  // DMA instructions per lane: 8 K, 2 S
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lr = lane & 15, lq = lane >> 4;
@@ -784,34 +794,35 @@ __global__ void __launch_bounds__(256, 1) k_grp_gemm_dma(GrpDev d, const GrpUnit
         const float *kp[KI], *sp[SI];
 #pragma unroll
         for (int i = 0; i < KI; ++i) {
-            const int q = (wave * KI + i) * 64 + lane, r = q / kDCh, c = (q % kDCh) ^ (r % kDCh);
+            const int q = (wave * KI + i) * 64 + lane, r = q / CH, c = (q % CH) ^ (r % CH);
             const int64_t row = r0 + r < N ? r0 + r : N - 1;
-            kp[i] = G.K + row * ld + (int64_t)U.k0 * kDBK + 4 * c;
+            kp[i] = G.K + row * ld + (int64_t)U.k0 * BK + 4 * c;
         }
 #pragma unroll
         for (int i = 0; i < SI; ++i) {
-            const int q = (wave * SI + i) * 64 + lane, m = q / kDCh, c = (q % kDCh) ^ m;
-            sp[i] = d.S + G.Soff + (int64_t)m * ld + (int64_t)U.k0 * kDBK + 4 * c;
+            const int q = (wave * SI + i) * 64 + lane, m = q / CH, c = (q % CH) ^ (m % CH);
+            sp[i] = d.S + G.Soff + (int64_t)m * ld + (int64_t)U.k0 * BK + 4 * c;
         }
         // slice k (clamped to the last one: the copies past the end land in
         // stages nobody reads) into stage k mod NS; the sweep starts at a
         // different slice per unit (row pitches are multiples of 2 KB, so
-        // aligned sweeps would load the same channels)
+        // aligned sweeps would load the same channels).  K is streamed with
+        // the non-temporal policy, S (re-read by every row block) is not.
         const int rot = (int)(((int64_t)ui * 13) % nk);
         auto issue = [&](int k) {
             int kr = (k < nk ? k : nk - 1) + rot;
             kr = kr >= nk ? kr - nk : kr;
-            const int64_t col = (int64_t)kr * kDBK;
-            float *st = dsm + (k % NS) * kDStage;
+            const int64_t col = (int64_t)kr * BK;
+            float *st = dsm + (k % NS) * Sh::STAGE;
 #pragma unroll
-            for (int i = 0; i < KI; ++i) dma16<NT>(kp[i] + col, st + (wave * KI + i) * 256);
+            for (int i = 0; i < KI; ++i) dma16<true>(kp[i] + col, st + (wave * KI + i) * 256);
 #pragma unroll
-            for (int i = 0; i < SI; ++i) dma16<false>(sp[i] + col, st + kDKSl + (wave * SI + i) * 256);
+            for (int i = 0; i < SI; ++i) dma16<false>(sp[i] + col, st + Sh::KSL + (wave * SI + i) * 256);
         };
-        f4x acc[4];
-        float hi[4][4], lo[4][4];
+        f4x acc[RB];
+        float hi[RB][4], lo[RB][4];
 #pragma unroll
-        for (int rb = 0; rb < 4; ++rb) {
+        for (int rb = 0; rb < RB; ++rb) {
             acc[rb] = (f4x){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int v = 0; v < 4; ++v) hi[rb][v] = lo[rb][v] = 0.f;
@@ -826,26 +837,26 @@ __global__ void __launch_bounds__(256, 1) k_grp_gemm_dma(GrpDev d, const GrpUnit
             __builtin_amdgcn_s_waitcnt(vmcnt_imm((NS - 2) * (KI + SI)));
             ring_barrier();
             issue(k + NS - 1);  // into the stage of slice k - 1, read by all
-            const float *A = dsm + (k % NS) * kDStage, *B = A + kDKSl;
+            const float *A = dsm + (k % NS) * Sh::STAGE, *B = A + Sh::KSL;
 #pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                const int c = 8 * wave + 4 * t + lq;
-                const f4x b = *reinterpret_cast<const f4x *>(&B[(lr * kDCh + (c ^ lr)) * 4]);
-                f4x a[4];
+            for (int t = 0; t < Sh::T; ++t) {
+                const int c = (CH / 4) * wave + 4 * t + lq;
+                const f4x b = *reinterpret_cast<const f4x *>(&B[(lr * CH + (c ^ (lr % CH))) * 4]);
+                f4x a[RB];
 #pragma unroll
-                for (int rb = 0; rb < 4; ++rb) {
+                for (int rb = 0; rb < RB; ++rb) {
                     const int r = 16 * rb + lr;
-                    a[rb] = *reinterpret_cast<const f4x *>(&A[(r * kDCh + (c ^ (r % kDCh))) * 4]);
+                    a[rb] = *reinterpret_cast<const f4x *>(&A[(r * CH + (c ^ (r % CH))) * 4]);
                 }
 #pragma unroll
                 for (int e = 0; e < 4; ++e)
 #pragma unroll
-                    for (int rb = 0; rb < 4; ++rb)
+                    for (int rb = 0; rb < RB; ++rb)
                         acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rb][e], b[e], acc[rb], 0, 0, 0);
             }
-            if ((k & 7) == 7 || k + 1 == nk) {  // every 256 columns of this wavefront
+            if ((k + 1) % Sh::FOLD == 0 || k + 1 == nk) {  // every 256 columns of this wavefront
 #pragma unroll
-                for (int rb = 0; rb < 4; ++rb) {
+                for (int rb = 0; rb < RB; ++rb) {
 #pragma unroll
                     for (int v = 0; v < 4; ++v) grp_two_sum(hi[rb][v], lo[rb][v], acc[rb][v]);
                     acc[rb] = (f4x){0.f, 0.f, 0.f, 0.f};
@@ -856,16 +867,16 @@ __global__ void __launch_bounds__(256, 1) k_grp_gemm_dma(GrpDev d, const GrpUnit
         __syncthreads();  // every copy has landed and every read is done
         double *red = reinterpret_cast<double *>(dsm);  // [wave][row][member]
 #pragma unroll
-        for (int rb = 0; rb < 4; ++rb)
+        for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
             for (int v = 0; v < 4; ++v)
-                red[(wave * kDBR + 16 * rb + 4 * lq + v) * kGM + lr] = (double)hi[rb][v] + (double)lo[rb][v];
+                red[(wave * BR + 16 * rb + 4 * lq + v) * kGM + lr] = (double)hi[rb][v] + (double)lo[rb][v];
         __syncthreads();
 #pragma unroll
-        for (int e = 0; e < kDBR * kGM / 256; ++e) {
+        for (int e = 0; e < BR * kGM / 256; ++e) {
             const int x = tid + 256 * e, row = x / kGM, m = x % kGM;
-            const double y = ((red[x] + red[kDBR * kGM + x]) + red[2 * kDBR * kGM + x]) + red[3 * kDBR * kGM + x];
-            if (m < d.M && r0 + row < N) P[((int64_t)U.slot * kDBR + row) * d.M + m] = y;
+            const double y = ((red[x] + red[BR * kGM + x]) + red[2 * BR * kGM + x]) + red[3 * BR * kGM + x];
+            if (m < d.M && r0 + row < N) P[((int64_t)U.slot * BR + row) * d.M + m] = y;
         }
     }
 }
@@ -1034,18 +1045,28 @@ __global__ void __launch_bounds__(256) k_grp_commit(GrpCtrl c, int64_t ntot,
 // in flight (a third slice in flight measured the same as two: the kernel
 // is not limited by its prefetch depth), and (shape 10) the two-slice LDS
 // kernel with RB = 2 on column-split units balanced over the workgroups;
-// (shapes 11, 12) the direct-to-LDS kernel on 64-row units with a 3-stage
-// ring (a 4-stage ring measured the same), K copied with and without the
-// non-temporal cache policy
-static constexpr int kGShapes = 13;
-static constexpr int kGUnitShape = 10;  // first of the unit shapes 10..12
-static constexpr int kGUnitShapes = 3;
+// (shapes 11-14) the direct-to-LDS kernel on units of BR rows with BK-column
+// slices in an NS-stage ring (NS, BR, BK, workgroups per CU in kDma; a
+// 4-stage ring of the 64 x 128 slices measured the same as 3 stages, and
+// the same kernel without the non-temporal policy on K 8 % slower)
+static constexpr int kGShapes = 15;
+static constexpr int kGUnitShape = 10;  // first of the unit shapes 10..14
+static constexpr int kGUnitShapes = 5;
 static constexpr int kGUnitRB = 2, kGUnitBR = 64 * kGUnitRB, kGUnitBK = 128 / kGUnitRB;
 static constexpr int64_t kGUnitBytes = (int64_t)2 << 20;  // target bytes of K per unit
+struct DmaCfg {
+    int ns, br, bk, occ;
+};
+static constexpr DmaCfg kDma[4] = {{3, 64, 128, 1}, {3, 32, 256, 1}, {3, 64, 64, 2}, {6, 64, 64, 1}};
 static inline bool is_unit_shape(int s) { return s >= kGUnitShape; }
-static inline int unit_br(int s) { return s == kGUnitShape ? kGUnitBR : kDBR; }
-static inline int unit_bk(int s) { return s == kGUnitShape ? kGUnitBK : kDBK; }
-static inline size_t dma_lds(int) { return (size_t)3 * kDStage * 4; }  // NS = 3
+static inline const DmaCfg &dma_cfg(int s) { return kDma[s - kGUnitShape - 1]; }
+static inline int unit_br(int s) { return s == kGUnitShape ? kGUnitBR : dma_cfg(s).br; }
+static inline int unit_bk(int s) { return s == kGUnitShape ? kGUnitBK : dma_cfg(s).bk; }
+static inline size_t dma_lds(int s)
+{
+    const DmaCfg &c = dma_cfg(s);
+    return (size_t)c.ns * (size_t)(c.br + kGM) * (size_t)c.bk * 4;
+}
 static inline int shape_rb(int s)
 {
     if (is_unit_shape(s)) return s == kGUnitShape ? kGUnitRB : 1;
@@ -1206,14 +1227,21 @@ static int launch_units(oq_group *e, int s, const GrpUnitTable &T)
         hipLaunchKernelGGL((k_grp_sum_partials<kGUnitBR>), dim3(gbk(kGUnitBR * e->M, 256), (unsigned)T.nrbl),
                            blk, 0, e->stream, e->dev, T.rbl.as<GrpRowBlock>(), e->P.as<double>());
     } else {
-        if (s == kGUnitShape + 1)
-            hipLaunchKernelGGL((k_grp_gemm_dma<3, true>), grid, blk, dma_lds(s), e->stream, e->dev,
-                               T.units.as<GrpUnit>(), T.wg.as<int2>(), e->P.as<double>());
+        const GrpUnit *un = T.units.as<GrpUnit>();
+        const int2 *wr = T.wg.as<int2>();
+        double *P = e->P.as<double>();
+        switch (s - kGUnitShape - 1) {
+        case 0: hipLaunchKernelGGL((k_grp_gemm_dma<3, 64, 128, 1>), grid, blk, dma_lds(s), e->stream, e->dev, un, wr, P); break;
+        case 1: hipLaunchKernelGGL((k_grp_gemm_dma<3, 32, 256, 1>), grid, blk, dma_lds(s), e->stream, e->dev, un, wr, P); break;
+        case 2: hipLaunchKernelGGL((k_grp_gemm_dma<3, 64, 64, 2>), grid, blk, dma_lds(s), e->stream, e->dev, un, wr, P); break;
+        default: hipLaunchKernelGGL((k_grp_gemm_dma<6, 64, 64, 1>), grid, blk, dma_lds(s), e->stream, e->dev, un, wr, P); break;
+        }
+        if (unit_br(s) == 32)
+            hipLaunchKernelGGL((k_grp_sum_partials<32>), dim3(gbk(32 * e->M, 256), (unsigned)T.nrbl), blk, 0,
+                               e->stream, e->dev, T.rbl.as<GrpRowBlock>(), P);
         else
-            hipLaunchKernelGGL((k_grp_gemm_dma<3, false>), grid, blk, dma_lds(s), e->stream, e->dev,
-                               T.units.as<GrpUnit>(), T.wg.as<int2>(), e->P.as<double>());
-        hipLaunchKernelGGL((k_grp_sum_partials<kDBR>), dim3(gbk(kDBR * e->M, 256), (unsigned)T.nrbl),
-                           blk, 0, e->stream, e->dev, T.rbl.as<GrpRowBlock>(), e->P.as<double>());
+            hipLaunchKernelGGL((k_grp_sum_partials<64>), dim3(gbk(64 * e->M, 256), (unsigned)T.nrbl), blk, 0,
+                               e->stream, e->dev, T.rbl.as<GrpRowBlock>(), P);
     }
     OQ_HIP(hipGetLastError());
     return OQ_OK;
@@ -1504,10 +1532,11 @@ int oq_group_create(int device, int64_t ngeo, const oq_fault_mesh *mf,
         if (!rc) rc = occupancy_grid(k_grp_gemm_lds2<1>, &gr[8]);
         if (!rc) rc = occupancy_grid(k_grp_gemm_lds2<2>, &gr[9]);
         if (!rc) rc = occupancy_grid(k_grp_gemm_units<kGUnitRB>, &gr[10]);
-        // the direct-to-LDS shapes need 120 KB of LDS per workgroup: a device
-        // that cannot give it leaves the shape out (grid 0)
-        const void *dk[2] = {(const void *)k_grp_gemm_dma<3, true>, (const void *)k_grp_gemm_dma<3, false>};
-        for (int i = 0; i < 2 && !rc; ++i) {
+        // the direct-to-LDS shapes need 60-144 KB of LDS per workgroup: a
+        // device that cannot give it leaves the shape out (grid 0)
+        const void *dk[4] = {(const void *)k_grp_gemm_dma<3, 64, 128, 1>, (const void *)k_grp_gemm_dma<3, 32, 256, 1>,
+                             (const void *)k_grp_gemm_dma<3, 64, 64, 2>, (const void *)k_grp_gemm_dma<6, 64, 64, 1>};
+        for (int i = 0; i < 4 && !rc; ++i) {
             const int sh = kGUnitShape + 1 + i;
             gr[sh] = 0;
             if (hipFuncSetAttribute(dk[i], hipFuncAttributeMaxDynamicSharedMemorySize, (int)dma_lds(sh)) !=
@@ -1515,8 +1544,12 @@ int oq_group_create(int device, int64_t ngeo, const oq_fault_mesh *mf,
                 (void)hipGetLastError();
                 continue;
             }
-            rc = i == 0 ? occupancy_grid(k_grp_gemm_dma<3, true>, &gr[sh], dma_lds(sh))
-                        : occupancy_grid(k_grp_gemm_dma<3, false>, &gr[sh], dma_lds(sh));
+            switch (i) {
+            case 0: rc = occupancy_grid(k_grp_gemm_dma<3, 64, 128, 1>, &gr[sh], dma_lds(sh)); break;
+            case 1: rc = occupancy_grid(k_grp_gemm_dma<3, 32, 256, 1>, &gr[sh], dma_lds(sh)); break;
+            case 2: rc = occupancy_grid(k_grp_gemm_dma<3, 64, 64, 2>, &gr[sh], dma_lds(sh)); break;
+            default: rc = occupancy_grid(k_grp_gemm_dma<6, 64, 64, 1>, &gr[sh], dma_lds(sh)); break;
+            }
         }
         for (int s = 0; s < kGShapes; ++s) e->grid[s] = gr[s];
     }

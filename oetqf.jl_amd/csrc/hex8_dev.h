@@ -222,6 +222,96 @@ __device__ void h8_image_integral(double x, double y, double z, const double lo0
     }
 }
 
+// Eshelby corner sums of the cuboid [lo, hi] at the receiver (x, y, z);
+// returns whether the receiver lies inside the cuboid.
+__device__ __forceinline__ bool hex8_kelvin(double x, double y, double z, const double lo[3], const double hi[3],
+                            Hex8Kelvin &K)
+{
+    const double rv[3] = {x, y, z};
+#pragma unroll
+    for (int i = 0; i < 6; ++i) K.phi[i] = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        K.P4[a] = K.P22[a] = K.P211[a] = 0.0;
+#pragma unroll
+        for (int b = 0; b < 3; ++b) K.P31[a][b] = 0.0;
+    }
+    for (int s = 0; s < 8; ++s) {
+        double X[3], sg = 1.0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const int bit = (s >> a) & 1;
+            X[a] = rv[a] - (bit ? hi[a] : lo[a]);
+            if (bit) sg = -sg;
+        }
+        h8_kelvin_add(K, X, sg);
+    }
+    return x >= lo[0] && x < hi[0] && y >= lo[1] && y < hi[1] && z >= lo[2] && z < hi[2];
+}
+
+// The image integral contracted with the shear-traction functional of a
+// fault receiver, for the 6 unit eigenstrains at once.  The traction is
+// linear in grad u, tau = sum_il c_il (grad u)_il, and only four c_il are
+// nonzero (FT 0, strike slip: tau = -s_xy sin d + s_xz cos d gives c_01 =
+// c_10 = -mu sin d, c_02 = c_20 = mu cos d; FT 1, dip slip: tau = (s_zz -
+// s_yy)/2 sin 2d + s_yz cos 2d gives c_22 = -c_11 = mu sin 2d, c_12 = c_21 =
+// mu cos 2d).  So the quadrature accumulates E[q] = sum_il c_il D[i][l][q]
+// (6 sums, from 24 of the 54 kernel terms; the other 30 and their common
+// subexpressions are dead code) instead of all 54 integrals: ca, cb are the
+// two coefficients above.  Same bisection and Gauss points as
+// h8_image_integral.
+template <int FT>
+__device__ void h8_image_traction(double x, double y, double z, const double lo0[3], const double hi0[3],
+                                  double nu, double ca, double cb, double E[6])
+{
+    const double xi[3] = {x, y, -z};
+    unsigned long long path = 0;
+    int depth = 0;
+    double Dk[54];
+    for (;;) {
+        double lo[3], hi[3], h[3], ctr[3], dist2 = 0.0, hd2 = 0.0;
+        h8_subbox(lo0, hi0, path, depth, lo, hi);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            h[a] = 0.5 * (hi[a] - lo[a]);
+            ctr[a] = 0.5 * (hi[a] + lo[a]);
+            const double dd = fabs(xi[a] - ctr[a]) - h[a];
+            if (dd > 0.0) dist2 += dd * dd;
+            hd2 += h[a] * h[a];
+        }
+        if (depth < OQ_H8_MAXDEPTH && dist2 < OQ_H8_KAPPA * OQ_H8_KAPPA * hd2) {
+            ++depth;
+            path <<= 1;
+            continue;
+        }
+        const double vol = h[0] * h[1] * h[2];
+        for (int i = 0; i < OQ_H8_NQ; ++i)
+            for (int j = 0; j < OQ_H8_NQ; ++j)
+                for (int k = 0; k < OQ_H8_NQ; ++k) {
+                    const double ys1 = ctr[0] + h[0] * kGLX[OQ_H8_NQ][i];
+                    const double ys2 = ctr[1] + h[1] * kGLX[OQ_H8_NQ][j];
+                    const double ys3 = ctr[2] + h[2] * kGLX[OQ_H8_NQ][k];
+                    const double w = kGLW[OQ_H8_NQ][i] * kGLW[OQ_H8_NQ][j] * kGLW[OQ_H8_NQ][k] * vol;
+                    hex8_image_kernel(x - ys1, y - ys2, z, ys3, nu, Dk);
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) {
+                        const double t = FT == 0
+                                             ? ca * (Dk[0 * 18 + 1 * 6 + q] + Dk[1 * 18 + 0 * 6 + q]) +
+                                                   cb * (Dk[0 * 18 + 2 * 6 + q] + Dk[2 * 18 + 0 * 6 + q])
+                                             : ca * (Dk[2 * 18 + 2 * 6 + q] - Dk[1 * 18 + 1 * 6 + q]) +
+                                                   cb * (Dk[1 * 18 + 2 * 6 + q] + Dk[2 * 18 + 1 * 6 + q]);
+                        E[q] += w * t;
+                    }
+                }
+        while (depth > 0 && (path & 1ull)) {
+            path >>= 1;
+            --depth;
+        }
+        if (depth == 0) break;
+        path += 1;
+    }
+}
+
 // Per receiver-cell pair: everything independent of the eigenstrain.
 struct Hex8Pair {
     Hex8Kelvin K;
@@ -240,29 +330,38 @@ __device__ void hex8_pair(double x, double y, double z, double qx, double qy, do
 #pragma unroll
     for (int m = 0; m < 54; ++m) P.D[m] = 0.0;
     h8_image_integral(x, y, z, lo, hi, nu, P.D);
-#pragma unroll
-    for (int i = 0; i < 6; ++i) P.K.phi[i] = 0.0;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        P.K.P4[a] = P.K.P22[a] = P.K.P211[a] = 0.0;
-#pragma unroll
-        for (int b = 0; b < 3; ++b) P.K.P31[a][b] = 0.0;
-    }
-    for (int s = 0; s < 8; ++s) {
-        double X[3], sg = 1.0;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const int bit = (s >> a) & 1;
-            X[a] = rv[a] - (bit ? hi[a] : lo[a]);
-            if (bit) sg = -sg;
-        }
-        h8_kelvin_add(P.K, X, sg);
-    }
-    P.inside = (x >= lo[0] && x < hi[0] && y >= lo[1] && y < hi[1] && z >= lo[2] && z < hi[2]);
+    (void)rv;
+    P.inside = hex8_kelvin(x, y, z, lo, hi, P.K);
 }
 
 // Stress s[6] for the unit eigenstrain component p (0..5 = xx xy xz yy yz zz).
+// Stress s[6] for the unit eigenstrain component p from the Eshelby sums K
+// and, when D is not NULL, the 54 image integrals D (without them: the
+// full-space part plus the eigenstress inside the cuboid).
+__device__ void hex8_stress_unit_kd(const Hex8Kelvin &K, const double *D, bool inside, int p, double mu,
+                                    double nu, double s[6]);
+
 __device__ void hex8_stress_unit(const Hex8Pair &P, int p, double mu, double nu, double s[6])
+{
+    hex8_stress_unit_kd(P.K, P.D, P.inside, p, mu, nu, s);
+}
+
+// mv = the eigenstress C : e* of the unit eigenstrain p, as [xx xy xz yy yz zz]
+__device__ __forceinline__ void hex8_moment(int p, double mu, double nu, double mv[6])
+{
+    const double lambda = 2.0 * mu * nu / (1.0 - 2.0 * nu);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) mv[q] = 0.0;
+    if (p == 0 || p == 3 || p == 5) {
+        mv[0] = mv[3] = mv[5] = lambda;
+        mv[p] += 2.0 * mu;
+    } else {
+        mv[p] = 2.0 * mu;
+    }
+}
+
+__device__ void hex8_stress_unit_kd(const Hex8Kelvin &K, const double *D, bool inside, int p, double mu,
+                                    double nu, double s[6])
 {
     const int pj[6] = {0, 0, 0, 1, 1, 2}, pk[6] = {0, 1, 2, 1, 2, 2};
     const double lambda = 2.0 * mu * nu / (1.0 - 2.0 * nu);
@@ -289,18 +388,20 @@ __device__ void hex8_stress_unit(const Hex8Pair &P, int p, double mu, double nu,
             for (int j = 0; j < 3; ++j)
 #pragma unroll
                 for (int k = 0; k < 3; ++k)
-                    if (e[j][k] != 0.0) acc += e[j][k] * h8_psi(P.K, i, j, k, l);
+                    if (e[j][k] != 0.0) acc += e[j][k] * h8_psi(K, i, j, k, l);
 #pragma unroll
             for (int k = 0; k < 3; ++k)
-                if (e[i][k] != 0.0) acc -= 4.0 * (1.0 - nu) * e[i][k] * h8_phi(P.K, k, l);
-            acc -= 2.0 * nu * tr * h8_phi(P.K, i, l);
+                if (e[i][k] != 0.0) acc -= 4.0 * (1.0 - nu) * e[i][k] * h8_phi(K, k, l);
+            acc -= 2.0 * nu * tr * h8_phi(K, i, l);
             double g = 0.0;
+            if (D != nullptr) {
 #pragma unroll
-            for (int q = 0; q < 6; ++q) g += P.D[i * 18 + l * 6 + q] * mv[q];
+                for (int q = 0; q < 6; ++q) g += D[i * 18 + l * 6 + q] * mv[q];
+            }
             gu[i][l] = acc * ck + g * ci;
         }
     const double div = gu[0][0] + gu[1][1] + gu[2][2];
-    const double in = P.inside ? 1.0 : 0.0;
+    const double in = inside ? 1.0 : 0.0;
     s[0] = 2.0 * mu * gu[0][0] + lambda * div - in * m[0][0];
     s[1] = mu * (gu[0][1] + gu[1][0]) - in * m[0][1];
     s[2] = mu * (gu[0][2] + gu[2][0]) - in * m[0][2];
