@@ -276,7 +276,7 @@ __global__ void __launch_bounds__(256) k_grp_gemm(GrpDev d, const int2 *__restri
 #pragma unroll
             for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-                for (int t = 0; t < 4; ++t) kv[rb][t] = kload4(kp[rb] + p * kGStep + 16 * t);
+                for (int t = 0; t < 4; ++t) kv[rb][t] = gload4(kp[rb] + p * kGStep + 16 * t);
         }
         for (int64_t it = 0; it < nstep; ++it) {
             const int64_t pn = (p + 1 == nstep) ? 0 : p + 1;
@@ -287,7 +287,7 @@ __global__ void __launch_bounds__(256) k_grp_gemm(GrpDev d, const int2 *__restri
 #pragma unroll
                 for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-                    for (int t = 0; t < 4; ++t) kv[rb][t] = kload4(kp[rb] + p * kGStep + 16 * t);
+                    for (int t = 0; t < 4; ++t) kv[rb][t] = gload4(kp[rb] + p * kGStep + 16 * t);
             }
             // the prefetch is unconditional (pn wraps to the first pass after the
             // last one): a conditional load would make the compiler's wait
@@ -299,7 +299,7 @@ __global__ void __launch_bounds__(256) k_grp_gemm(GrpDev d, const int2 *__restri
 #pragma unroll
                 for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-                    for (int t = 0; t < 4; ++t) kn[rb][t] = kload4(kp[rb] + pn * kGStep + 16 * t);
+                    for (int t = 0; t < 4; ++t) kn[rb][t] = gload4(kp[rb] + pn * kGStep + 16 * t);
             }
             __builtin_amdgcn_sched_barrier(0);
             // lane (row lr, group lq) holds columns c0 + 16t + 4lq + e; MFMA
@@ -1047,8 +1047,11 @@ __global__ void __launch_bounds__(256) k_grp_commit(GrpCtrl c, int64_t ntot,
 // kernel with RB = 2 on column-split units balanced over the workgroups;
 // (shapes 11-14) the direct-to-LDS kernel on units of BR rows with BK-column
 // slices in an NS-stage ring (NS, BR, BK, workgroups per CU in kDma; a
-// 4-stage ring of the 64 x 128 slices measured the same as 3 stages, and
-// the same kernel without the non-temporal policy on K 8 % slower)
+// 4-stage ring of the 64 x 128 slices measured the same as 3 stages, 32 x
+// 256 slices and a 6-stage ring of 64 x 64 slices (one workgroup per CU)
+// 2-7 % slower than two workgroups of 3 stages, and the same kernel without
+// the non-temporal policy on K 8 % slower; the register-streaming kernels,
+// whose lanes load 64-byte row pieces, were slower with it)
 static constexpr int kGShapes = 15;
 static constexpr int kGUnitShape = 10;  // first of the unit shapes 10..14
 static constexpr int kGUnitShapes = 5;
@@ -1057,7 +1060,7 @@ static constexpr int64_t kGUnitBytes = (int64_t)2 << 20;  // target bytes of K p
 struct DmaCfg {
     int ns, br, bk, occ;
 };
-static constexpr DmaCfg kDma[4] = {{3, 64, 128, 1}, {3, 32, 256, 1}, {3, 64, 64, 2}, {6, 64, 64, 1}};
+static constexpr DmaCfg kDma[4] = {{3, 64, 128, 1}, {3, 32, 128, 2}, {3, 64, 64, 2}, {2, 64, 64, 4}};
 static inline bool is_unit_shape(int s) { return s >= kGUnitShape; }
 static inline const DmaCfg &dma_cfg(int s) { return kDma[s - kGUnitShape - 1]; }
 static inline int unit_br(int s) { return s == kGUnitShape ? kGUnitBR : dma_cfg(s).br; }
@@ -1232,9 +1235,9 @@ static int launch_units(oq_group *e, int s, const GrpUnitTable &T)
         double *P = e->P.as<double>();
         switch (s - kGUnitShape - 1) {
         case 0: hipLaunchKernelGGL((k_grp_gemm_dma<3, 64, 128, 1>), grid, blk, dma_lds(s), e->stream, e->dev, un, wr, P); break;
-        case 1: hipLaunchKernelGGL((k_grp_gemm_dma<3, 32, 256, 1>), grid, blk, dma_lds(s), e->stream, e->dev, un, wr, P); break;
+        case 1: hipLaunchKernelGGL((k_grp_gemm_dma<3, 32, 128, 2>), grid, blk, dma_lds(s), e->stream, e->dev, un, wr, P); break;
         case 2: hipLaunchKernelGGL((k_grp_gemm_dma<3, 64, 64, 2>), grid, blk, dma_lds(s), e->stream, e->dev, un, wr, P); break;
-        default: hipLaunchKernelGGL((k_grp_gemm_dma<6, 64, 64, 1>), grid, blk, dma_lds(s), e->stream, e->dev, un, wr, P); break;
+        default: hipLaunchKernelGGL((k_grp_gemm_dma<2, 64, 64, 4>), grid, blk, dma_lds(s), e->stream, e->dev, un, wr, P); break;
         }
         if (unit_br(s) == 32)
             hipLaunchKernelGGL((k_grp_sum_partials<32>), dim3(gbk(32 * e->M, 256), (unsigned)T.nrbl), blk, 0,
@@ -1534,8 +1537,8 @@ int oq_group_create(int device, int64_t ngeo, const oq_fault_mesh *mf,
         if (!rc) rc = occupancy_grid(k_grp_gemm_units<kGUnitRB>, &gr[10]);
         // the direct-to-LDS shapes need 60-144 KB of LDS per workgroup: a
         // device that cannot give it leaves the shape out (grid 0)
-        const void *dk[4] = {(const void *)k_grp_gemm_dma<3, 64, 128, 1>, (const void *)k_grp_gemm_dma<3, 32, 256, 1>,
-                             (const void *)k_grp_gemm_dma<3, 64, 64, 2>, (const void *)k_grp_gemm_dma<6, 64, 64, 1>};
+        const void *dk[4] = {(const void *)k_grp_gemm_dma<3, 64, 128, 1>, (const void *)k_grp_gemm_dma<3, 32, 128, 2>,
+                             (const void *)k_grp_gemm_dma<3, 64, 64, 2>, (const void *)k_grp_gemm_dma<2, 64, 64, 4>};
         for (int i = 0; i < 4 && !rc; ++i) {
             const int sh = kGUnitShape + 1 + i;
             gr[sh] = 0;
@@ -1546,9 +1549,9 @@ int oq_group_create(int device, int64_t ngeo, const oq_fault_mesh *mf,
             }
             switch (i) {
             case 0: rc = occupancy_grid(k_grp_gemm_dma<3, 64, 128, 1>, &gr[sh], dma_lds(sh)); break;
-            case 1: rc = occupancy_grid(k_grp_gemm_dma<3, 32, 256, 1>, &gr[sh], dma_lds(sh)); break;
+            case 1: rc = occupancy_grid(k_grp_gemm_dma<3, 32, 128, 2>, &gr[sh], dma_lds(sh)); break;
             case 2: rc = occupancy_grid(k_grp_gemm_dma<3, 64, 64, 2>, &gr[sh], dma_lds(sh)); break;
-            default: rc = occupancy_grid(k_grp_gemm_dma<6, 64, 64, 1>, &gr[sh], dma_lds(sh)); break;
+            default: rc = occupancy_grid(k_grp_gemm_dma<2, 64, 64, 4>, &gr[sh], dma_lds(sh)); break;
             }
         }
         for (int s = 0; s < kGShapes; ++s) e->grid[s] = gr[s];

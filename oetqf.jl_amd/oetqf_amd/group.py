@@ -18,7 +18,7 @@ from .equation import _KDTYPES, _rs_struct, _visc_struct
 from .greens import StrikeSlip, _Keep, _ftype, fault_struct, get_quadrature, hex8_struct, quad_struct
 
 SHAPES = ["RB1-pipe", "RB1", "RB2-pipe", "RB2", "RB4-pipe", "RB4", "RB1-lds", "RB2-lds",
-          "RB1-lds-d2", "RB2-lds-d2", "RB2-lds-d2-split", "dma-r64-k128", "dma-r32-k256", "dma-r64-k64-2wg", "dma-r64-k64-s6"]
+          "RB1-lds-d2", "RB2-lds-d2", "RB2-lds-d2-split", "dma-r64-k128", "dma-r32-k128-2wg", "dma-r64-k64-2wg", "dma-r64-k64-s2-4wg"]
 
 
 class Group:
