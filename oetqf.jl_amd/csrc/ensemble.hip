@@ -810,11 +810,23 @@ struct oq_ensemble {
     oq::EnsCtrl ctrl{};
     bool stepping = false;
     bool started = false;  // the controller arrays hold a run (get_ctrl)
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    // the captured lockstep step: [0] one step, [1] kGraphSteps steps back
+    // to back (fewer launches; steps past the end of the run are no-ops)
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
 };
 
 namespace oq {
+
+static constexpr int kGraphSteps = 8;  // steps per launch of the multi-step graph
+
+static void drop_graphs(oq_ensemble *e)
+{
+    for (int q = 0; q < 2; ++q) {
+        if (e->exec[q]) { (void)hipGraphExecDestroy(e->exec[q]); e->exec[q] = nullptr; }
+        if (e->graph[q]) { (void)hipGraphDestroy(e->graph[q]); e->graph[q] = nullptr; }
+    }
+}
 
 static inline unsigned nbk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
@@ -887,8 +899,10 @@ static void ens_destroy(oq_ensemble *e)
 {
     if (!e) return;
     if (teardown_sync(e->h ? e->h->stream : nullptr)) {
-        if (e->exec) (void)hipGraphExecDestroy(e->exec);
-        if (e->graph) (void)hipGraphDestroy(e->graph);
+        for (int q = 0; q < 2; ++q) {
+            if (e->exec[q]) (void)hipGraphExecDestroy(e->exec[q]);
+            if (e->graph[q]) (void)hipGraphDestroy(e->graph[q]);
+        }
     }
     if (e->h) oq_rhs_destroy(e->h);
     delete e;
@@ -1109,20 +1123,21 @@ static int ens_begin(oq_ensemble *e, const double *u0, const double *tv, const d
     int rc = set_live(e);
     if (!rc) rc = ens_rhs(e, e->u.as<double>(), e->k.as<double>());
     if (rc) return rc;
-    if (e->exec) { (void)hipGraphExecDestroy(e->exec); e->exec = nullptr; }
-    if (e->graph) { (void)hipGraphDestroy(e->graph); e->graph = nullptr; }
+    drop_graphs(e);
     OQ_HIP(hipStreamSynchronize(h->stream));
-    OQ_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    rc = ens_step(e);
-    hipGraph_t g = nullptr;
-    const hipError_t ce = hipStreamEndCapture(h->stream, &g);
-    if (rc) {
-        if (g) (void)hipGraphDestroy(g);
-        return rc;
+    for (int q = 0; q < 2 && !rc; ++q) {
+        OQ_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < (q ? kGraphSteps : 1) && !rc; ++i) rc = ens_step(e);
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(h->stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        OQ_HIP(ce);
+        e->graph[q] = g;
+        OQ_HIP(hipGraphInstantiate(&e->exec[q], g, nullptr, nullptr, 0));
     }
-    OQ_HIP(ce);
-    e->graph = g;
-    OQ_HIP(hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0));
     e->stepping = true;
     e->started = true;
     return OQ_OK;
@@ -1190,7 +1205,8 @@ int oq_ensemble_run(oq_ensemble *e, int64_t max_steps, int64_t *nrunning)
     int64_t left = e->M;
     for (int64_t s = 0; s < max_steps; s += kRunSync) {
         const int64_t m = (max_steps - s) < kRunSync ? (max_steps - s) : kRunSync;
-        for (int64_t i = 0; i < m; ++i) OQ_HIP(hipGraphLaunch(e->exec, h->stream));
+        for (int64_t i = 0; i + kGraphSteps <= m; i += kGraphSteps) OQ_HIP(hipGraphLaunch(e->exec[1], h->stream));
+        for (int64_t i = m / kGraphSteps * kGraphSteps; i < m; ++i) OQ_HIP(hipGraphLaunch(e->exec[0], h->stream));
         OQ_HIP(hipMemcpyAsync(done.data(), e->cdone.p, e->M * 4, hipMemcpyDeviceToHost, h->stream));
         OQ_HIP(hipStreamSynchronize(h->stream));
         left = 0;
@@ -1210,8 +1226,7 @@ int oq_ensemble_record(oq_ensemble *e, int64_t step_capacity)
     // the captured step holds the recorder's buffers as kernel arguments:
     // drop it before they are released (oq_ensemble_start captures anew)
     OQ_HIP(hipStreamSynchronize(e->h->stream));
-    if (e->exec) { (void)hipGraphExecDestroy(e->exec); e->exec = nullptr; }
-    if (e->graph) { (void)hipGraphDestroy(e->graph); e->graph = nullptr; }
+    drop_graphs(e);
     e->stepping = false;
     e->vpart.release();
     e->rec_t.release();

@@ -1110,11 +1110,23 @@ struct oq_group {
     oq::GrpCtrl ctrl{};
     bool stepping = false;
     bool started = false;  // the controller arrays hold a run (get_ctrl)
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    // the captured lockstep step: [0] one step, [1] kGraphSteps steps back
+    // to back (fewer launches; steps past the end of the run are no-ops)
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
 };
 
 namespace oq {
+
+static constexpr int kGraphSteps = 8;  // steps per launch of the multi-step graph
+
+static void drop_graphs(oq_group *e)
+{
+    for (int q = 0; q < 2; ++q) {
+        if (e->exec[q]) { (void)hipGraphExecDestroy(e->exec[q]); e->exec[q] = nullptr; }
+        if (e->graph[q]) { (void)hipGraphDestroy(e->graph[q]); e->graph[q] = nullptr; }
+    }
+}
 
 static inline unsigned gbk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
@@ -1346,9 +1358,8 @@ static int set_live(oq_group *e)
 // buffers that have just been replaced (oq_group_start captures anew).
 static void drop_step(oq_group *e)
 {
-    if (e->exec || e->graph) (void)hipStreamSynchronize(e->stream);
-    if (e->exec) { (void)hipGraphExecDestroy(e->exec); e->exec = nullptr; }
-    if (e->graph) { (void)hipGraphDestroy(e->graph); e->graph = nullptr; }
+    if (e->exec[0] || e->graph[0]) (void)hipStreamSynchronize(e->stream);
+    drop_graphs(e);
     e->stepping = false;
 }
 
@@ -1357,8 +1368,10 @@ static void grp_destroy(oq_group *e)
     if (!e) return;
     const bool ok = teardown_sync(e->stream);
     if (ok) {
-        if (e->exec) (void)hipGraphExecDestroy(e->exec);
-        if (e->graph) (void)hipGraphDestroy(e->graph);
+        for (int q = 0; q < 2; ++q) {
+            if (e->exec[q]) (void)hipGraphExecDestroy(e->exec[q]);
+            if (e->graph[q]) (void)hipGraphDestroy(e->graph[q]);
+        }
     }
     for (oq_rhs *h : e->rhs)
         if (h) oq_rhs_destroy(h);
@@ -1776,20 +1789,21 @@ static int grp_begin(oq_group *e, const double *u0, const double *tv, const doub
     int rc = set_live(e);
     if (!rc) rc = grp_rhs(e, e->u.as<double>(), e->k.as<double>());
     if (rc) return rc;
-    if (e->exec) { (void)hipGraphExecDestroy(e->exec); e->exec = nullptr; }
-    if (e->graph) { (void)hipGraphDestroy(e->graph); e->graph = nullptr; }
+    drop_graphs(e);
     OQ_HIP(hipStreamSynchronize(e->stream));
-    OQ_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-    rc = grp_step(e);
-    hipGraph_t g = nullptr;
-    const hipError_t ce = hipStreamEndCapture(e->stream, &g);
-    if (rc) {
-        if (g) (void)hipGraphDestroy(g);
-        return rc;
+    for (int q = 0; q < 2 && !rc; ++q) {
+        OQ_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < (q ? kGraphSteps : 1) && !rc; ++i) rc = grp_step(e);
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(e->stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        OQ_HIP(ce);
+        e->graph[q] = g;
+        OQ_HIP(hipGraphInstantiate(&e->exec[q], g, nullptr, nullptr, 0));
     }
-    OQ_HIP(ce);
-    e->graph = g;
-    OQ_HIP(hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0));
     e->stepping = true;
     e->started = true;
     return OQ_OK;
@@ -1850,7 +1864,8 @@ int oq_group_run(oq_group *e, int64_t max_steps, int64_t *nrunning)
     constexpr int64_t kRunSync = 256;
     for (int64_t s = 0; s < max_steps; s += kRunSync) {
         const int64_t m = (max_steps - s) < kRunSync ? (max_steps - s) : kRunSync;
-        for (int64_t i = 0; i < m; ++i) OQ_HIP(hipGraphLaunch(e->exec, e->stream));
+        for (int64_t i = 0; i + kGraphSteps <= m; i += kGraphSteps) OQ_HIP(hipGraphLaunch(e->exec[1], e->stream));
+        for (int64_t i = m / kGraphSteps * kGraphSteps; i < m; ++i) OQ_HIP(hipGraphLaunch(e->exec[0], e->stream));
         OQ_HIP(hipMemcpyAsync(done.data(), e->cdone.p, GM * 4, hipMemcpyDeviceToHost, e->stream));
         OQ_HIP(hipStreamSynchronize(e->stream));
         left = 0;
