@@ -518,8 +518,19 @@ def _catalogue(t, vmax):
 def _compare(cat, ref):
     """Event statistics of a run against the FP64 reference run: the same
     event count, and the recurrence intervals' relative error (interval by
-    interval where the counts agree, and of the mean)."""
+    interval where the counts agree, and of the mean); and how many leading
+    events agree (onsets within 0.1 % of each other, i.e. before the
+    trajectories part: the cycle is chaotic, and two FP64 methods part as
+    well), with the time the first disagreeing event starts."""
     out = dict(same_event_count=cat["events"] == ref["events"])
+    on_a, on_b = cat["onsets_yr"], ref["onsets_yr"]
+    k = 0
+    while k < min(len(on_a), len(on_b)) and abs(on_a[k] - on_b[k]) <= 1e-3 * on_b[k]:
+        k += 1
+    out["leading_events_agree"] = k
+    out["max_leading_onset_rel_err"] = max([abs(on_a[i] - on_b[i]) / on_b[i] for i in range(k)], default=0.0)
+    if k < min(len(on_a), len(on_b)):
+        out["first_difference_yr"] = min(on_a[k], on_b[k])
     if ref["mean_recurrence_yr"] and cat["mean_recurrence_yr"]:
         out["mean_recurrence_rel_err"] = abs(cat["mean_recurrence_yr"] - ref["mean_recurrence_yr"]) / \
             ref["mean_recurrence_yr"]
@@ -768,6 +779,19 @@ def spans_summary(paths):
                 r.get("rejection_rate", r["rejected"] / max(1, r["accepted"] + r["rejected"])), r["events"],
                 "%.2f" % r["mean_recurrence_yr"] if r["mean_recurrence_yr"] else "-", vs_s))
         lines.append("")
+        pairs = [(a, b) for a in runs for b in runs if a < b]
+        if pairs:
+            lines += ["| composite pair | events | leading events that agree (onsets within 0.1 %) | max onset error "
+                      "over them | first difference at yr | mean recurrence, relative difference |",
+                      "|---|---|---:|---:|---:|---:|"]
+            for a, b in pairs:
+                c = _compare(runs[a], runs[b])
+                ma, mb = runs[a]["mean_recurrence_yr"], runs[b]["mean_recurrence_yr"]
+                lines.append("| %s vs %s | %d / %d | %d | %.1e | %s | %.3f |" % (
+                    a, b, runs[a]["events"], runs[b]["events"], c["leading_events_agree"],
+                    c["max_leading_onset_rel_err"], "%.1f" % c["first_difference_yr"]
+                    if "first_difference_yr" in c else "-", abs(ma - mb) / mb if ma and mb else float("nan")))
+            lines.append("")
     for kd, parts in ens.items():
         mem = {}
         for e in parts:
