@@ -287,7 +287,7 @@ __global__ void __launch_bounds__(128) k_gf_fm(DevHex8 h, DevFault f, double mu,
     const double ci = 1.0 / (16.0 * M_PI * mu * (1.0 - nu));
     for (int p = 0; p < 6; ++p) {
         double s[6], mv[6];
-        hex8_stress_unit_kd(K, nullptr, inside, p, mu, nu, s);
+        hex8_stress_unit_kelvin(K, inside, p, mu, nu, s);
         hex8_moment(p, mu, nu, mv);
         double ti = 0.0;
 #pragma unroll

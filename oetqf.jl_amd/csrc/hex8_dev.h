@@ -324,26 +324,12 @@ __device__ void hex8_pair(double x, double y, double z, double qx, double qy, do
 {
     const double lo[3] = {qx - T / 2.0, qy, qz - W};
     const double hi[3] = {qx + T / 2.0, qy + L, qz};
-    const double rv[3] = {x, y, z};
     // The image integral first: its quadrature loop then runs without the
     // 24 Kelvin sums live, which takes most of its register spills away.
 #pragma unroll
     for (int m = 0; m < 54; ++m) P.D[m] = 0.0;
     h8_image_integral(x, y, z, lo, hi, nu, P.D);
-    (void)rv;
     P.inside = hex8_kelvin(x, y, z, lo, hi, P.K);
-}
-
-// Stress s[6] for the unit eigenstrain component p (0..5 = xx xy xz yy yz zz).
-// Stress s[6] for the unit eigenstrain component p from the Eshelby sums K
-// and, when D is not NULL, the 54 image integrals D (without them: the
-// full-space part plus the eigenstress inside the cuboid).
-__device__ void hex8_stress_unit_kd(const Hex8Kelvin &K, const double *D, bool inside, int p, double mu,
-                                    double nu, double s[6]);
-
-__device__ void hex8_stress_unit(const Hex8Pair &P, int p, double mu, double nu, double s[6])
-{
-    hex8_stress_unit_kd(P.K, P.D, P.inside, p, mu, nu, s);
 }
 
 // mv = the eigenstress C : e* of the unit eigenstrain p, as [xx xy xz yy yz zz]
@@ -360,8 +346,12 @@ __device__ __forceinline__ void hex8_moment(int p, double mu, double nu, double 
     }
 }
 
-__device__ void hex8_stress_unit_kd(const Hex8Kelvin &K, const double *D, bool inside, int p, double mu,
-                                    double nu, double s[6])
+// Stress s[6] for the unit eigenstrain component p from the Eshelby sums K
+// and (IMG) the 54 image integrals D; without them, the full-space part
+// plus the eigenstress inside the cuboid.
+template <bool IMG>
+__device__ __forceinline__ void hex8_stress_unit_body(const Hex8Kelvin &K, const double *D, bool inside, int p,
+                                                      double mu, double nu, double s[6])
 {
     const int pj[6] = {0, 0, 0, 1, 1, 2}, pk[6] = {0, 1, 2, 1, 2, 2};
     const double lambda = 2.0 * mu * nu / (1.0 - 2.0 * nu);
@@ -394,7 +384,7 @@ __device__ void hex8_stress_unit_kd(const Hex8Kelvin &K, const double *D, bool i
                 if (e[i][k] != 0.0) acc -= 4.0 * (1.0 - nu) * e[i][k] * h8_phi(K, k, l);
             acc -= 2.0 * nu * tr * h8_phi(K, i, l);
             double g = 0.0;
-            if (D != nullptr) {
+            if (IMG) {
 #pragma unroll
                 for (int q = 0; q < 6; ++q) g += D[i * 18 + l * 6 + q] * mv[q];
             }
@@ -408,6 +398,18 @@ __device__ void hex8_stress_unit_kd(const Hex8Kelvin &K, const double *D, bool i
     s[3] = 2.0 * mu * gu[1][1] + lambda * div - in * m[1][1];
     s[4] = mu * (gu[1][2] + gu[2][1]) - in * m[1][2];
     s[5] = 2.0 * mu * gu[2][2] + lambda * div - in * m[2][2];
+}
+
+// Stress s[6] for the unit eigenstrain component p (0..5 = xx xy xz yy yz zz).
+__device__ void hex8_stress_unit(const Hex8Pair &P, int p, double mu, double nu, double s[6])
+{
+    hex8_stress_unit_body<true>(P.K, P.D, P.inside, p, mu, nu, s);
+}
+
+// The same without the image part (the caller adds it).
+__device__ void hex8_stress_unit_kelvin(const Hex8Kelvin &K, bool inside, int p, double mu, double nu, double s[6])
+{
+    hex8_stress_unit_body<false>(K, nullptr, inside, p, mu, nu, s);
 }
 
 }  // namespace oq
