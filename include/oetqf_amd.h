@@ -337,6 +337,12 @@ int oq_ensemble_eval(oq_ensemble *e, const double *u, double *du);
  * and captures one lockstep step into a HIP graph. */
 int oq_ensemble_start(oq_ensemble *e, const double *u0, double t0, double t1,
                       const oq_solver_opts *opts);
+/* The same with VCABM5 (the solver of examples/otf-with-mantle.jl:160, as
+ * oq_stepper_create_vcabm5: order 1 rising to 5, two RHS per attempted
+ * step) for every member, each with its own order, step history and
+ * coefficients. */
+int oq_ensemble_start_vcabm5(oq_ensemble *e, const double *u0, double t0, double t1,
+                             const oq_solver_opts *opts);
 /* Advances up to max_steps lockstep attempts; *nrunning = members not done. */
 int oq_ensemble_run(oq_ensemble *e, int64_t max_steps, int64_t *nrunning);
 /* Per-member t[M], u[M*nstate], step counters and OQ_RET_* codes (0 = running). */
@@ -360,7 +366,8 @@ int oq_ensemble_get_ctrl(oq_ensemble *e, oq_lockstep_ctrl *ctrl);
  * required) towards t1 with the tolerances and limits of opts (opts->dt0 is
  * not used).  With the same operator the continuation is bit for bit the
  * uninterrupted run: k1 = f(u) is recomputed with the same kernels.  The
- * recorder restarts empty. */
+ * recorder restarts empty.  Tsit5 runs (a VCABM5 run's divided-difference
+ * history is not part of the checkpoint). */
 int oq_ensemble_resume(oq_ensemble *e, const double *u, const oq_lockstep_ctrl *ctrl,
                        double t1, const oq_solver_opts *opts);
 /* Per-member record of accepted steps, the ensemble form of the single-state

@@ -22,6 +22,7 @@
 
 #include <vector>
 
+#include "adams.h"
 #include "rhs_impl.h"
 
 typedef float f16v __attribute__((ext_vector_type(16)));
@@ -790,6 +791,144 @@ __global__ void __launch_bounds__(256) k_ens_commit(EnsCtrl c, int64_t nM, int64
     k[i] = k[6 * nM + i];
 }
 
+// ---------------- lockstep VCABM5 ----------------
+// The single-state VCABM5 (PECE in divided-difference form, order 1 rising
+// to 5, two RHS per attempted step) with per-member order, step history,
+// coefficients, error norm and acceptance, in the ensemble's entry-major
+// layout.  Buffers: phi = k[0, 6 nM) (divided differences of f), f(p) =
+// k[6 nM, 7 nM), p = y, and yn, fn (the corrector and its f) in ab.
+struct EnsAb {
+    int *k, *kcur;             // M: order of the next step / of the current one
+    double *hist, *g, *beta;   // M x (kAbK + 1), member-major
+    double *yn, *fn;           // nM each
+};
+
+__global__ void k_eab_coeffs(EnsCtrl c, EnsAb a)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= c.M || c.done[m] || !(c.dtcur[m] > 0.0)) return;
+    const int k = a.k[m];
+    a.kcur[m] = k;
+    ab_coeffs(k, c.dtcur[m], a.hist + m * (kAbK + 1), a.g + m * (kAbK + 1), a.beta + m * (kAbK + 1));
+}
+
+// p = u + h sum_{j<k} g_j beta_j Phi_j (members not stepping: p = u)
+__global__ void __launch_bounds__(256) k_eab_predict(EnsCtrl c, EnsAb a, int64_t nM, int64_t M,
+                                                     const double *__restrict__ u,
+                                                     const double *__restrict__ phi, double *__restrict__ p)
+{
+    if (!*c.live) return;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nM) return;
+    const int64_t m = i % M;
+    const double h = c.dtcur[m];
+    if (!(h > 0.0)) {
+        p[i] = u[i];
+        return;
+    }
+    const int k = a.kcur[m];
+    const double *g = a.g + m * (kAbK + 1), *be = a.beta + m * (kAbK + 1);
+    double acc = 0.0;
+    for (int j = 0; j < k; ++j) acc += g[j] * be[j] * phi[j * nM + i];
+    p[i] = u[i] + h * acc;
+}
+
+// corrector y_{n+1} = p + h g_k Phi^p_k, Phi^p_k = f(p) - sum_{j<k} beta_j
+// Phi_j, and per-(row chunk, member) sums of the squared scaled error
+// h (g_k - g_{k-1}) Phi^p_k (and the max |v| of y_{n+1} for the recorder)
+__global__ void __launch_bounds__(256) k_eab_correct(EnsCtrl c, EnsAb a, int64_t n, int64_t M,
+                                                     const double *__restrict__ u,
+                                                     const double *__restrict__ p,
+                                                     const double *__restrict__ fp,
+                                                     const double *__restrict__ phi,
+                                                     double *__restrict__ part)
+{
+    if (!*c.live) return;
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int chunk = blockIdx.y;
+    if (m >= M) return;
+    const int64_t nM = n * M;
+    const int64_t r0 = n * chunk / kEnsChunks, r1 = n * (chunk + 1) / kEnsChunks;
+    const double h = c.dtcur[m];
+    const int k = a.kcur[m];
+    const double *g = a.g + m * (kAbK + 1), *be = a.beta + m * (kAbK + 1);
+    const double gk = h > 0.0 ? g[k] : 0.0, dg = h > 0.0 ? g[k] - g[k - 1] : 0.0;
+    double acc = 0.0, vm = 0.0;
+    for (int64_t r = r0; r < r1; ++r) {
+        const int64_t i = r * M + m;
+        double ph = fp[i];
+        if (h > 0.0)
+            for (int j = 0; j < k; ++j) ph -= be[j] * phi[j * nM + i];
+        const double yn = p[i] + h * gk * ph;
+        a.yn[i] = yn;
+        const double w = h * dg * ph / (c.abstol + fmax(fabs(u[i]), fabs(yn)) * c.reltol);
+        acc += w * w;
+        if (r < c.nf) vm = fmax(vm, fabs(yn));  // rows [0, nf) of the state are v
+    }
+    part[chunk * M + m] = acc;
+    if (c.rec_cap > 0) c.vpart[chunk * M + m] = vm;
+}
+
+// per member: step factor 1/clamp(err^(1/(k+1))/0.9, 1/2, 5), acceptance,
+// step history and order
+__global__ void k_eab_controller(EnsCtrl c, EnsAb a, int64_t n, const double *__restrict__ part)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= c.M || c.done[m] || !(c.dtcur[m] > 0.0)) return;
+    double acc = 0.0;
+    for (int ch = 0; ch < kEnsChunks; ++ch) acc += part[ch * c.M + m];
+    const double eest = sqrt(acc / (double)n);
+    const int k = a.kcur[m];
+    const double q = (eest == 0.0) ? 0.5 : fmax(0.5, fmin(5.0, pow(eest, 1.0 / (k + 1)) / 0.9));
+    const double dtc = c.dtcur[m];
+    if (eest <= 1.0) {
+        c.t[m] = (c.t1 - (c.t[m] + dtc) <= 0.0) ? c.t1 : c.t[m] + dtc;
+        double *hist = a.hist + m * (kAbK + 1);
+        for (int j = kAbK; j > 0; --j) hist[j] = hist[j - 1];
+        hist[0] = dtc;
+        if (a.k[m] < kAbK) a.k[m] += 1;
+        c.dt[m] = dtc / q;
+        c.naccept[m] += 1;
+        c.accept[m] = 1;
+        if (c.rec_cap > 0) {
+            double vm = 0.0;
+            for (int ch = 0; ch < kEnsChunks; ++ch) vm = fmax(vm, c.vpart[ch * c.M + m]);
+            const long long slot = (long long)m * c.rec_cap + (c.naccept[m] - 1) % c.rec_cap;
+            c.rec_t[slot] = c.t[m];
+            c.rec_v[slot] = vm;
+        }
+    } else {
+        c.nreject[m] += 1;
+        if (!isfinite(eest)) {
+            c.done[m] = 1;
+            c.retcode[m] = OQ_RET_NONFINITE;
+            return;
+        }
+        c.dt[m] = dtc / q;
+    }
+}
+
+// accepted members: u <- y_{n+1}; divided differences of the new f
+__global__ void __launch_bounds__(256) k_eab_commit(EnsCtrl c, EnsAb a, int64_t nM, int64_t M,
+                                                    double *__restrict__ u, double *__restrict__ phi)
+{
+    if (!*c.live) return;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nM) return;
+    const int64_t m = i % M;
+    if (!c.accept[m]) return;
+    const int k = a.kcur[m];
+    const double *be = a.beta + m * (kAbK + 1);
+    double tmp = a.fn[i];
+    for (int j = 0; j < k; ++j) {
+        const double old = phi[j * nM + i];
+        phi[j * nM + i] = tmp;
+        tmp -= be[j] * old;
+    }
+    phi[k * nM + i] = tmp;
+    u[i] = a.yn[i];
+}
+
 }  // namespace oq
 
 struct oq_ensemble {
@@ -810,6 +949,9 @@ struct oq_ensemble {
     oq::EnsCtrl ctrl{};
     bool stepping = false;
     bool started = false;  // the controller arrays hold a run (get_ctrl)
+    int alg = 0;           // 0 Tsit5, 1 VCABM5 (oq_ensemble_start_vcabm5)
+    oq::DevBuf abk, abkcur, abhist, abg, abbeta, ab;  // VCABM5 state (ab: yn, fn)
+    oq::EnsAb eab{};
     // the captured lockstep step: [0] one step, [1] kGraphSteps steps back
     // to back (fewer launches; steps past the end of the run are no-ops)
     hipGraph_t graph[2] = {nullptr, nullptr};
@@ -863,8 +1005,11 @@ static int ens_rhs(oq_ensemble *e, const double *u, double *du)
     return OQ_OK;
 }
 
+static int ens_step_ab(oq_ensemble *e);
+
 static int ens_step(oq_ensemble *e)
 {
+    if (e->alg == 1) return ens_step_ab(e);
     oq_rhs *h = e->h;
     const int64_t nM = e->n * e->M;
     double *u = e->u.as<double>(), *y = e->y.as<double>(), *k = e->k.as<double>();
@@ -882,6 +1027,29 @@ static int ens_step(oq_ensemble *e)
                        e->part.as<double>());
     hipLaunchKernelGGL(k_ens_commit, dim3(nbk(nM, 256)), dim3(256), 0, h->stream, e->ctrl, nM, e->M,
                        u, y, k);
+    OQ_HIP(hipGetLastError());
+    return OQ_OK;
+}
+
+// One attempted lockstep VCABM5 step: two ensemble RHS (the second also runs
+// for members that rejected; its result is not used for them).
+static int ens_step_ab(oq_ensemble *e)
+{
+    oq_rhs *h = e->h;
+    const int64_t nM = e->n * e->M;
+    double *u = e->u.as<double>(), *p = e->y.as<double>(), *phi = e->k.as<double>(), *fp = phi + 6 * nM;
+    const dim3 gm(nbk(e->M, 256)), gn(nbk(nM, 256)), blk(256);
+    int rc;
+    hipLaunchKernelGGL(k_ens_begin, gm, blk, 0, h->stream, e->ctrl);
+    hipLaunchKernelGGL(k_eab_coeffs, gm, blk, 0, h->stream, e->ctrl, e->eab);
+    hipLaunchKernelGGL(k_ens_live, dim3(1), dim3(1024), 0, h->stream, e->ctrl);
+    hipLaunchKernelGGL(k_eab_predict, gn, blk, 0, h->stream, e->ctrl, e->eab, nM, e->M, u, phi, p);
+    if ((rc = ens_rhs(e, p, fp))) return rc;
+    hipLaunchKernelGGL(k_eab_correct, dim3(nbk(e->M, 256), kEnsChunks), blk, 0, h->stream, e->ctrl, e->eab,
+                       e->n, e->M, u, p, fp, phi, e->part.as<double>());
+    hipLaunchKernelGGL(k_eab_controller, gm, blk, 0, h->stream, e->ctrl, e->eab, e->n, e->part.as<double>());
+    if ((rc = ens_rhs(e, e->eab.yn, e->eab.fn))) return rc;
+    hipLaunchKernelGGL(k_eab_commit, gn, blk, 0, h->stream, e->ctrl, e->eab, nM, e->M, u, phi);
     OQ_HIP(hipGetLastError());
     return OQ_OK;
 }
@@ -1087,11 +1255,36 @@ int oq_ensemble_eval(oq_ensemble *e, const double *u, double *du)
 // arrays, seeds k1 = f(u) and captures the step graph.
 static int ens_begin(oq_ensemble *e, const double *u0, const double *tv, const double *dtv, const double *qv,
                      const long long *na, const long long *nr, const long long *it, const int *done,
-                     const int *ret, double t1, const oq_solver_opts *o)
+                     const int *ret, double t1, const oq_solver_opts *o, int alg)
 {
     oq_rhs *h = e->h;
     OQ_HIP(hipSetDevice(h->device));
     const int64_t M = e->M, nM = e->n * M;
+    if (alg == 1) {  // VCABM5: order 1, no step history; phi_0 = f(u) is the seed below
+        int rc = OQ_OK;
+        if (!e->ab.p) {
+            const size_t h6 = (size_t)(M * (kAbK + 1)) * 8;
+            rc = e->abk.alloc((size_t)M * 4);
+            if (!rc) rc = e->abkcur.alloc((size_t)M * 4);
+            if (!rc) rc = e->abhist.alloc(h6);
+            if (!rc) rc = e->abg.alloc(h6);
+            if (!rc) rc = e->abbeta.alloc(h6);
+            if (!rc) rc = e->ab.alloc((size_t)(2 * nM) * 8);
+            if (rc) return rc;
+        }
+        const std::vector<int> one(M, 1);
+        const std::vector<double> zh((size_t)(M * (kAbK + 1)), 0.0);
+        OQ_HIP(hipMemcpyAsync(e->abk.p, one.data(), M * 4, hipMemcpyHostToDevice, h->stream));
+        OQ_HIP(hipMemcpyAsync(e->abkcur.p, one.data(), M * 4, hipMemcpyHostToDevice, h->stream));
+        OQ_HIP(hipMemcpyAsync(e->abhist.p, zh.data(), zh.size() * 8, hipMemcpyHostToDevice, h->stream));
+        OQ_HIP(hipMemcpyAsync(e->abg.p, zh.data(), zh.size() * 8, hipMemcpyHostToDevice, h->stream));
+        OQ_HIP(hipMemcpyAsync(e->abbeta.p, zh.data(), zh.size() * 8, hipMemcpyHostToDevice, h->stream));
+        e->eab = EnsAb{e->abk.as<int>(), e->abkcur.as<int>(), e->abhist.as<double>(), e->abg.as<double>(),
+                       e->abbeta.as<double>(), e->ab.as<double>(), e->ab.as<double>() + nM};
+        // the host vectors above are read by the copies before they go out of scope
+        OQ_HIP(hipStreamSynchronize(h->stream));
+    }
+    e->alg = alg;
     std::vector<const double *> src(M);
     for (int64_t m = 0; m < M; ++m) src[m] = u0 + m * e->n;
     const std::vector<double> ui = interleave(src.data(), e->n, M);
@@ -1119,7 +1312,7 @@ static int ens_begin(oq_ensemble *e, const double *u0, const double *tv, const d
     c.rec_v = e->rec_cap ? e->rec_v.as<double>() : nullptr;
     e->rec_drained.assign(na, na + M);  // records from before a resume are not in the rings
     // FSAL seed: k1 = f(u) (on a resume the same bits as the k7 of the last
-    // accepted step: the RHS kernels are deterministic)
+    // accepted step: the RHS kernels are deterministic); for VCABM5 phi_0 = f(u)
     int rc = set_live(e);
     if (!rc) rc = ens_rhs(e, e->u.as<double>(), e->k.as<double>());
     if (rc) return rc;
@@ -1154,7 +1347,20 @@ int oq_ensemble_start(oq_ensemble *e, const double *u0, double t0, double t1,
     const std::vector<long long> zl(M, 0);
     const std::vector<int> zi(M, 0);
     return ens_begin(e, u0, tv.data(), dtv.data(), qv.data(), zl.data(), zl.data(), zl.data(), zi.data(),
-                     zi.data(), t1, o);
+                     zi.data(), t1, o, 0);
+}
+
+int oq_ensemble_start_vcabm5(oq_ensemble *e, const double *u0, double t0, double t1, const oq_solver_opts *o)
+{
+    OQ_REQUIRE(e && u0 && o, "NULL argument");
+    OQ_REQUIRE(t1 > t0, "tspan must satisfy t1 > t0");
+    OQ_REQUIRE(o->dt0 > 0.0, "the ensemble stepper needs an initial step dt0 > 0");
+    const int64_t M = e->M;
+    const std::vector<double> tv(M, t0), dtv(M, o->dt0), qv(M, 1e-4);
+    const std::vector<long long> zl(M, 0);
+    const std::vector<int> zi(M, 0);
+    return ens_begin(e, u0, tv.data(), dtv.data(), qv.data(), zl.data(), zl.data(), zl.data(), zi.data(),
+                     zi.data(), t1, o, 1);
 }
 
 int oq_ensemble_get_ctrl(oq_ensemble *e, oq_lockstep_ctrl *c)
@@ -1186,7 +1392,7 @@ int oq_ensemble_resume(oq_ensemble *e, const double *u, const oq_lockstep_ctrl *
     static_assert(sizeof(long long) == sizeof(int64_t), "int64_t counters");
     return ens_begin(e, u, c->t, c->dt, c->qold, reinterpret_cast<const long long *>(c->naccept),
                      reinterpret_cast<const long long *>(c->nreject), reinterpret_cast<const long long *>(c->iter),
-                     c->done, c->retcode, t1, o);
+                     c->done, c->retcode, t1, o, 0);
 }
 
 int oq_ensemble_run(oq_ensemble *e, int64_t max_steps, int64_t *nrunning)

@@ -15,6 +15,7 @@
 
 #include <vector>
 
+#include "adams.h"
 #include "rhs_impl.h"
 
 namespace oq {
@@ -229,33 +230,12 @@ __global__ void __launch_bounds__(256) k_record(const Ctrl *__restrict__ c, Reco
 //   Phi_0 = f_{n+1}, Phi_{j+1} = Phi_j(n+1) - beta_j Phi_j(n).
 // Two RHS per step; k = 1 at the start, + 1 per accepted step up to 5; step
 // factor h_new/h = 1/clamp(err^(1/(k+1))/0.9, 1/2, 5).
-static constexpr int kAbK = 5;
-
 struct AbCtrl {
     Ctrl c;  // t, dt (next step), dtcur, eest, tolerances, counters, flags
     int k, kcur;
     double hist[kAbK + 1];  // past steps, most recent first
     double g[kAbK + 1], beta[kAbK + 1];
 };
-
-__device__ void ab_coeffs(int k, double h, const double *hist, double *g, double *beta)
-{
-    double pn[kAbK + 1], po[kAbK + 1], c[kAbK + 2];
-    pn[0] = h;
-    po[0] = hist[0];
-    for (int i = 1; i <= k; ++i) {
-        pn[i] = pn[i - 1] + hist[i - 1];
-        po[i] = po[i - 1] + hist[i];
-    }
-    beta[0] = 1.0;
-    for (int j = 1; j < k; ++j) beta[j] = beta[j - 1] * pn[j - 1] / po[j - 1];
-    for (int q = 1; q <= k + 1; ++q) c[q] = 1.0 / q;
-    g[0] = c[1];
-    for (int j = 1; j <= k; ++j) {
-        for (int q = 1; q <= k + 1 - j; ++q) c[q] = c[q] - c[q + 1] * h / pn[j - 1];
-        g[j] = c[1];
-    }
-}
 
 // ab_coeffs on the lanes of one wavefront: lane q (1 <= q <= k + 1) holds
 // c_q through the j recursion (each update reads c_{q+1} before its own

@@ -138,6 +138,7 @@ PROTOTYPES = {
                                   ctypes.POINTER(_vp)]),
     "oq_ensemble_eval": (_int, [_vp, _pd, _pd]),
     "oq_ensemble_start": (_int, [_vp, _pd, _d, _d, ctypes.POINTER(SolverOpts)]),
+    "oq_ensemble_start_vcabm5": (_int, [_vp, _pd, _d, _d, ctypes.POINTER(SolverOpts)]),
     "oq_ensemble_get_ctrl": (_int, [_vp, ctypes.POINTER(LockstepCtrl)]),
     "oq_ensemble_resume": (_int, [_vp, _pd, ctypes.POINTER(LockstepCtrl), _d, ctypes.POINTER(SolverOpts)]),
     "oq_ensemble_run": (_int, [_vp, _i64, ctypes.POINTER(_i64)]),

@@ -90,7 +90,7 @@ def recurrence(evs):
 
 
 def run_ensemble_cycle(ens, U0, tspan, reltol=1e-6, abstol=1e-8, dt0=1.0, dtmax=0.0, chunk=2048,
-                       wall_limit=None, resume=None, progress=None):
+                       wall_limit=None, resume=None, progress=None, alg=None):
     """Integrates all members of an Ensemble over tspan with the per-member
     recorder on.  Returns per-member (t, vmax) records, final states and
     counters; `complete` is False when wall_limit stopped it, and then
@@ -108,7 +108,7 @@ def run_ensemble_cycle(ens, U0, tspan, reltol=1e-6, abstol=1e-8, dt0=1.0, dtmax=
             vs[m].append(resume["vmax"][m])
         iters0, wall0 = resume["iters"], resume["wall"]
     else:
-        ens.start(U0, tspan, reltol=reltol, abstol=abstol, dt0=dt0, dtmax=dtmax)
+        ens.start(U0, tspan, reltol=reltol, abstol=abstol, dt0=dt0, dtmax=dtmax, alg=alg)
     t0 = time.perf_counter()
     iters = iters0
     complete = False
