@@ -436,14 +436,18 @@ int oq_group_time(oq_group *e, int iters, double *ms_eval);
  * step into a HIP graph. */
 int oq_group_start(oq_group *e, const double *u0, double t0, double t1,
                    const oq_solver_opts *opts);
+/* The same with VCABM5 for every member (as oq_ensemble_start_vcabm5). */
+int oq_group_start_vcabm5(oq_group *e, const double *u0, double t0, double t1,
+                          const oq_solver_opts *opts);
 /* Advances up to max_steps lockstep attempts; *nrunning = members not done. */
 int oq_group_run(oq_group *e, int64_t max_steps, int64_t *nrunning);
 /* Per-member t, u (ntot), step counters and OQ_RET_* codes (ngeo*nmembers). */
 int oq_group_state(oq_group *e, double *t, double *u, int64_t *naccept, int64_t *nreject,
                    int *retcode);
-/* Checkpoint / resume as oq_ensemble_get_ctrl / oq_ensemble_resume, over the
- * ngeo*nmembers members (u in the layout of oq_group_start).  A bit-for-bit
- * continuation also needs the same tile shapes (oq_group_set_shapes). */
+/* Checkpoint / resume as oq_ensemble_get_ctrl / oq_ensemble_resume (Tsit5
+ * runs), over the ngeo*nmembers members (u in the layout of
+ * oq_group_start).  A bit-for-bit continuation also needs the same tile
+ * shapes (oq_group_set_shapes). */
 int oq_group_get_ctrl(oq_group *e, oq_lockstep_ctrl *ctrl);
 int oq_group_resume(oq_group *e, const double *u, const oq_lockstep_ctrl *ctrl, double t1,
                     const oq_solver_opts *opts);

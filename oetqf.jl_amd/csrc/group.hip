@@ -38,6 +38,7 @@
 #include <string>
 #include <vector>
 
+#include "adams.h"
 #include "rhs_impl.h"
 
 typedef float f4x __attribute__((ext_vector_type(4)));
@@ -1082,6 +1083,126 @@ struct GrpUnitTable {
     int nunits = 0, nwg = 0, nrbl = 0, nslots = 0;
 };
 
+// ---------------- lockstep VCABM5 over all members ----------------
+// As the ensemble's lockstep VCABM5, over the members of every geometry:
+// phi = k[0, 6 ntot), f(p) = k[6 ntot, 7 ntot), p = y; yn, fn in ab.
+struct GrpAb {
+    int *k, *kcur;            // G*M
+    double *hist, *g, *beta;  // G*M x (kAbK + 1)
+    double *yn, *fn;          // ntot each
+};
+
+__global__ void k_gab_coeffs(GrpCtrl c, GrpAb a)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= c.G * c.M || c.done[q] || !(c.dtcur[q] > 0.0)) return;
+    const int k = a.k[q];
+    a.kcur[q] = k;
+    ab_coeffs(k, c.dtcur[q], a.hist + q * (kAbK + 1), a.g + q * (kAbK + 1), a.beta + q * (kAbK + 1));
+}
+
+__global__ void __launch_bounds__(256) k_gab_predict(GrpCtrl c, GrpAb a, int64_t ntot,
+                                                     const double *__restrict__ u,
+                                                     const double *__restrict__ phi, double *__restrict__ p)
+{
+    if (!*c.live) return;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t q = grp_member(c, i);
+    if (i >= ntot) return;
+    const double h = c.dtcur[q];
+    if (!(h > 0.0)) {
+        p[i] = u[i];
+        return;
+    }
+    const int k = a.kcur[q];
+    const double *g = a.g + q * (kAbK + 1), *be = a.beta + q * (kAbK + 1);
+    double acc = 0.0;
+    for (int j = 0; j < k; ++j) acc += g[j] * be[j] * phi[j * ntot + i];
+    p[i] = u[i] + h * acc;
+}
+
+__global__ void __launch_bounds__(256) k_gab_correct(GrpCtrl c, GrpAb a, int64_t ntot,
+                                                     const double *__restrict__ u,
+                                                     const double *__restrict__ p,
+                                                     const double *__restrict__ fp,
+                                                     const double *__restrict__ phi,
+                                                     double *__restrict__ part)
+{
+    if (!*c.live) return;
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int chunk = blockIdx.y;
+    const int64_t GM = (int64_t)c.G * c.M;
+    if (q >= GM) return;
+    const int64_t g = q / c.M, m = q % c.M, M = c.M, n = c.nst[g], base = c.soffs[g];
+    const int64_t r0 = n * chunk / kGChunks, r1 = n * (chunk + 1) / kGChunks;
+    const double h = c.dtcur[q];
+    const int k = a.kcur[q];
+    const double *gg = a.g + q * (kAbK + 1), *be = a.beta + q * (kAbK + 1);
+    const double gk = h > 0.0 ? gg[k] : 0.0, dg = h > 0.0 ? gg[k] - gg[k - 1] : 0.0;
+    double acc = 0.0;
+    for (int64_t r = r0; r < r1; ++r) {
+        const int64_t i = base + r * M + m;
+        double ph = fp[i];
+        if (h > 0.0)
+            for (int j = 0; j < k; ++j) ph -= be[j] * phi[j * ntot + i];
+        const double yn = p[i] + h * gk * ph;
+        a.yn[i] = yn;
+        const double w = h * dg * ph / (c.abstol + fmax(fabs(u[i]), fabs(yn)) * c.reltol);
+        acc += w * w;
+    }
+    part[chunk * GM + q] = acc;
+}
+
+__global__ void k_gab_controller(GrpCtrl c, GrpAb a, const double *__restrict__ part)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int GM = c.G * c.M;
+    if (q >= GM || c.done[q] || !(c.dtcur[q] > 0.0)) return;
+    double acc = 0.0;
+    for (int ch = 0; ch < kGChunks; ++ch) acc += part[ch * GM + q];
+    const double eest = sqrt(acc / (double)c.nst[q / c.M]);
+    const int k = a.kcur[q];
+    const double f = (eest == 0.0) ? 0.5 : fmax(0.5, fmin(5.0, pow(eest, 1.0 / (k + 1)) / 0.9));
+    const double dtc = c.dtcur[q];
+    if (eest <= 1.0) {
+        c.t[q] = (c.t1 - (c.t[q] + dtc) <= 0.0) ? c.t1 : c.t[q] + dtc;
+        double *hist = a.hist + q * (kAbK + 1);
+        for (int j = kAbK; j > 0; --j) hist[j] = hist[j - 1];
+        hist[0] = dtc;
+        if (a.k[q] < kAbK) a.k[q] += 1;
+        c.dt[q] = dtc / f;
+        c.naccept[q] += 1;
+        c.accept[q] = 1;
+    } else {
+        c.nreject[q] += 1;
+        if (!isfinite(eest)) {
+            c.done[q] = 1;
+            c.retcode[q] = OQ_RET_NONFINITE;
+            return;
+        }
+        c.dt[q] = dtc / f;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_gab_commit(GrpCtrl c, GrpAb a, int64_t ntot, double *__restrict__ u,
+                                                    double *__restrict__ phi)
+{
+    if (!*c.live) return;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t q = grp_member(c, i);
+    if (i >= ntot || !c.accept[q]) return;
+    const int k = a.kcur[q];
+    const double *be = a.beta + q * (kAbK + 1);
+    double tmp = a.fn[i];
+    for (int j = 0; j < k; ++j) {
+        const double old = phi[j * ntot + i];
+        phi[j * ntot + i] = tmp;
+        tmp -= be[j] * old;
+    }
+    phi[k * ntot + i] = tmp;
+    u[i] = a.yn[i];
+}
+
 }  // namespace oq
 
 struct oq_group {
@@ -1110,6 +1231,9 @@ struct oq_group {
     oq::GrpCtrl ctrl{};
     bool stepping = false;
     bool started = false;  // the controller arrays hold a run (get_ctrl)
+    int alg = 0;           // 0 Tsit5, 1 VCABM5 (oq_group_start_vcabm5)
+    oq::DevBuf abk, abkcur, abhist, abg, abbeta, ab;  // VCABM5 state (ab: yn, fn)
+    oq::GrpAb gab{};
     // the captured lockstep step: [0] one step, [1] kGraphSteps steps back
     // to back (fewer launches; steps past the end of the run are no-ops)
     hipGraph_t graph[2] = {nullptr, nullptr};
@@ -1322,8 +1446,11 @@ static int grp_rhs(oq_group *e, const double *u, double *du)
     return OQ_OK;
 }
 
+static int grp_step_ab(oq_group *e);
+
 static int grp_step(oq_group *e)
 {
+    if (e->alg == 1) return grp_step_ab(e);
     const int64_t nt = e->ntot;
     const int GM = (int)(e->G * e->M);
     double *u = e->u.as<double>(), *y = e->y.as<double>(), *k = e->k.as<double>();
@@ -1341,6 +1468,28 @@ static int grp_step(oq_group *e)
                        e->part.as<double>());
     hipLaunchKernelGGL(k_grp_commit, dim3(gbk(nt, 256)), dim3(256), 0, e->stream, e->ctrl, nt, u, y,
                        k);
+    OQ_HIP(hipGetLastError());
+    return OQ_OK;
+}
+
+// One attempted lockstep VCABM5 step: two grouped RHS.
+static int grp_step_ab(oq_group *e)
+{
+    const int64_t nt = e->ntot;
+    const int GM = (int)(e->G * e->M);
+    double *u = e->u.as<double>(), *p = e->y.as<double>(), *phi = e->k.as<double>(), *fp = phi + 6 * nt;
+    const dim3 gq(gbk(GM, 256)), gn(gbk(nt, 256)), blk(256);
+    int rc;
+    hipLaunchKernelGGL(k_grp_begin, gq, blk, 0, e->stream, e->ctrl);
+    hipLaunchKernelGGL(k_gab_coeffs, gq, blk, 0, e->stream, e->ctrl, e->gab);
+    hipLaunchKernelGGL(k_grp_live, dim3(1), dim3(1024), 0, e->stream, e->ctrl);
+    hipLaunchKernelGGL(k_gab_predict, gn, blk, 0, e->stream, e->ctrl, e->gab, nt, u, phi, p);
+    if ((rc = grp_rhs(e, p, fp))) return rc;
+    hipLaunchKernelGGL(k_gab_correct, dim3(gbk(GM, 256), kGChunks), blk, 0, e->stream, e->ctrl, e->gab, nt, u, p,
+                       fp, phi, e->part.as<double>());
+    hipLaunchKernelGGL(k_gab_controller, gq, blk, 0, e->stream, e->ctrl, e->gab, e->part.as<double>());
+    if ((rc = grp_rhs(e, e->gab.yn, e->gab.fn))) return rc;
+    hipLaunchKernelGGL(k_gab_commit, gn, blk, 0, e->stream, e->ctrl, e->gab, nt, u, phi);
     OQ_HIP(hipGetLastError());
     return OQ_OK;
 }
@@ -1760,10 +1909,34 @@ int oq_group_time(oq_group *e, int iters, double *ms_eval)
 // controller arrays, seeds k1 = f(u) and captures the step graph.
 static int grp_begin(oq_group *e, const double *u0, const double *tv, const double *dtv, const double *qv,
                      const long long *na, const long long *nr, const long long *it, const int *done,
-                     const int *ret, double t1, const oq_solver_opts *o)
+                     const int *ret, double t1, const oq_solver_opts *o, int alg)
 {
     OQ_HIP(hipSetDevice(e->device));
     const int64_t GM = e->G * e->M;
+    if (alg == 1) {  // VCABM5: order 1, no step history; phi_0 = f(u) is the seed below
+        int rc = OQ_OK;
+        if (!e->ab.p) {
+            const size_t h6 = (size_t)(GM * (kAbK + 1)) * 8;
+            rc = e->abk.alloc((size_t)GM * 4);
+            if (!rc) rc = e->abkcur.alloc((size_t)GM * 4);
+            if (!rc) rc = e->abhist.alloc(h6);
+            if (!rc) rc = e->abg.alloc(h6);
+            if (!rc) rc = e->abbeta.alloc(h6);
+            if (!rc) rc = e->ab.alloc((size_t)(2 * e->ntot) * 8);
+            if (rc) return rc;
+        }
+        const std::vector<int> one(GM, 1);
+        const std::vector<double> zh((size_t)(GM * (kAbK + 1)), 0.0);
+        OQ_HIP(hipMemcpyAsync(e->abk.p, one.data(), GM * 4, hipMemcpyHostToDevice, e->stream));
+        OQ_HIP(hipMemcpyAsync(e->abkcur.p, one.data(), GM * 4, hipMemcpyHostToDevice, e->stream));
+        OQ_HIP(hipMemcpyAsync(e->abhist.p, zh.data(), zh.size() * 8, hipMemcpyHostToDevice, e->stream));
+        OQ_HIP(hipMemcpyAsync(e->abg.p, zh.data(), zh.size() * 8, hipMemcpyHostToDevice, e->stream));
+        OQ_HIP(hipMemcpyAsync(e->abbeta.p, zh.data(), zh.size() * 8, hipMemcpyHostToDevice, e->stream));
+        e->gab = GrpAb{e->abk.as<int>(), e->abkcur.as<int>(), e->abhist.as<double>(), e->abg.as<double>(),
+                       e->abbeta.as<double>(), e->ab.as<double>(), e->ab.as<double>() + e->ntot};
+        OQ_HIP(hipStreamSynchronize(e->stream));  // the host vectors are read before they go
+    }
+    e->alg = alg;
     std::vector<double> ui;
     to_entry_major(e, u0, ui);
     const std::vector<int> zi(GM, 0);
@@ -1819,7 +1992,20 @@ int oq_group_start(oq_group *e, const double *u0, double t0, double t1, const oq
     const std::vector<long long> zl(GM, 0);
     const std::vector<int> zi(GM, 0);
     return grp_begin(e, u0, tv.data(), dtv.data(), qv.data(), zl.data(), zl.data(), zl.data(), zi.data(),
-                     zi.data(), t1, o);
+                     zi.data(), t1, o, 0);
+}
+
+int oq_group_start_vcabm5(oq_group *e, const double *u0, double t0, double t1, const oq_solver_opts *o)
+{
+    OQ_REQUIRE(e && u0 && o, "NULL argument");
+    OQ_REQUIRE(t1 > t0, "tspan must satisfy t1 > t0");
+    OQ_REQUIRE(o->dt0 > 0.0, "the grouped stepper needs an initial step dt0 > 0");
+    const int64_t GM = e->G * e->M;
+    const std::vector<double> tv(GM, t0), dtv(GM, o->dt0), qv(GM, 1e-4);
+    const std::vector<long long> zl(GM, 0);
+    const std::vector<int> zi(GM, 0);
+    return grp_begin(e, u0, tv.data(), dtv.data(), qv.data(), zl.data(), zl.data(), zl.data(), zi.data(),
+                     zi.data(), t1, o, 1);
 }
 
 int oq_group_get_ctrl(oq_group *e, oq_lockstep_ctrl *c)
@@ -1849,7 +2035,7 @@ int oq_group_resume(oq_group *e, const double *u, const oq_lockstep_ctrl *c, dou
     static_assert(sizeof(long long) == sizeof(int64_t), "int64_t counters");
     return grp_begin(e, u, c->t, c->dt, c->qold, reinterpret_cast<const long long *>(c->naccept),
                      reinterpret_cast<const long long *>(c->nreject), reinterpret_cast<const long long *>(c->iter),
-                     c->done, c->retcode, t1, o);
+                     c->done, c->retcode, t1, o, 0);
 }
 
 int oq_group_run(oq_group *e, int64_t max_steps, int64_t *nrunning)

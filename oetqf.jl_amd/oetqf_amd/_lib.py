@@ -157,6 +157,7 @@ PROTOTYPES = {
     "oq_group_eval": (_int, [_vp, _pd, _pd]),
     "oq_group_time": (_int, [_vp, _int, ctypes.POINTER(_d)]),
     "oq_group_start": (_int, [_vp, _pd, _d, _d, ctypes.POINTER(SolverOpts)]),
+    "oq_group_start_vcabm5": (_int, [_vp, _pd, _d, _d, ctypes.POINTER(SolverOpts)]),
     "oq_group_get_ctrl": (_int, [_vp, ctypes.POINTER(LockstepCtrl)]),
     "oq_group_resume": (_int, [_vp, _pd, ctypes.POINTER(LockstepCtrl), _d, ctypes.POINTER(SolverOpts)]),
     "oq_group_run": (_int, [_vp, _i64, ctypes.POINTER(_i64)]),

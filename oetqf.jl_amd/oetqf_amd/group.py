@@ -111,11 +111,15 @@ class Group:
         return ms.value
 
     def start(self, U0, tspan, reltol=1e-6, abstol=1e-8, dt0=1.0, dtmax=0.0, dtmin=0.0,
-              maxiters=10**9):
+              maxiters=10**9, alg=None):
+        """Lockstep run of every member: Tsit5 (default) or alg=VCABM5()."""
+        from .equation import Tsit5, VCABM5
+        if alg is not None and not isinstance(alg, (Tsit5, VCABM5)):
+            raise NotImplementedError("the grouped batch steps with Tsit5() or VCABM5()")
         self._u0 = np.ascontiguousarray(U0, dtype=np.float64)
         o = _lib.SolverOpts(reltol, abstol, dt0, dtmax, dtmin, int(maxiters))
-        _lib.check(_lib.load().oq_group_start(self.h, _lib.ptr(self._u0), float(tspan[0]),
-                                              float(tspan[1]), ctypes.byref(o)))
+        fn = _lib.load().oq_group_start_vcabm5 if isinstance(alg, VCABM5) else _lib.load().oq_group_start
+        _lib.check(fn(self.h, _lib.ptr(self._u0), float(tspan[0]), float(tspan[1]), ctypes.byref(o)))
 
     def checkpoint(self):
         """(U, ctrl) of the current run: the members' states and controller

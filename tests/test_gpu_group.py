@@ -125,7 +125,8 @@ def test_group_every_tile_shape_gives_the_same_rhs(batch):
         assert e <= 1e-6, s
 
 
-def test_group_stepper_matches_single_members(oq):
+@pytest.mark.parametrize("alg", ["tsit5", "vcabm5"])
+def test_group_stepper_matches_single_members(oq, alg):
     rng = np.random.default_rng(43)
     geoms = _geoms(rng)
     M = 2
@@ -143,7 +144,8 @@ def test_group_stepper_matches_single_members(oq):
         U0.append(rows)
     T = 0.2 * 365 * 86400.0
     grp = oq.Group(geoms, LAM, MU, members, buffer_ratio=1.0)
-    grp.start(grp.pack([np.stack([u.pack() for u in r]) for r in U0]), (0.0, T), dt0=1.0)
+    method = oq.VCABM5() if alg == "vcabm5" else oq.Tsit5()
+    grp.start(grp.pack([np.stack([u.pack() for u in r]) for r in U0]), (0.0, T), dt0=1.0, alg=method)
     while grp.run(256) > 0:
         pass
     t, U, na, nr, rc = grp.state()
@@ -155,7 +157,7 @@ def test_group_stepper_matches_single_members(oq):
         for m, p in enumerate(members[g]):
             prob = oq.assemble_from_mesh(mf, ma, LAM, MU, p, pa, U0[g][m], (0.0, T),
                                          buffer_ratio=1.0, kdtype="fp32")
-            sol = oq.solve(prob, oq.Tsit5(), reltol=1e-6, abstol=1e-8, dt=1.0)
+            sol = oq.solve(prob, method, reltol=1e-6, abstol=1e-8, dt=1.0)
             ref = sol.u.pack()
             assert relerr(Us[g][m][:nf], ref[:nf]) <= 1e-4
             assert relerr(Us[g][m][nf:2 * nf], ref[nf:2 * nf]) <= 1e-5
