@@ -617,7 +617,7 @@ def _span_range(rng, n):
     return a, b
 
 
-def spans_ensemble(oq, kdtypes, sampled, member_range="", wall=None):
+def spans_ensemble(oq, kdtypes, sampled, member_range="", wall=None, method="tsit5"):
     """Members a:b (default all 256) of the composite-geometry ensemble over
     300 years with the lockstep Tsit5 (per-member steps), per operator type:
     rejection rates, event counts and mean recurrence of every member, and
@@ -632,10 +632,10 @@ def spans_ensemble(oq, kdtypes, sampled, member_range="", wall=None):
     members, U0 = ensemble_members(w, 256)
     lo, hi = _span_range(member_range, 256)
     span = (0.0, 300 * YEAR)
-    out = dict(workload=w["name"] + ", 256 frictional realizations", sim_years=300.0,
+    out = dict(workload=w["name"] + ", 256 frictional realizations", sim_years=300.0, method=method,
                reltol=w["reltol"], abstol=w["abstol"], by_kdtype={})
     for kd in kdtypes:
-        log("ensemble 300 yr: %s, members %d:%d" % (kd, lo, hi))
+        log("ensemble 300 yr: %s, %s, members %d:%d" % (kd, method, lo, hi))
         ens = oq.Ensemble(w["mf"], w["ma"], w["lam"], w["mu"], members[lo:hi], w["pa"], nrept=w["nrept"],
                           buffer_ratio=w["buffer_ratio"], kdtype=kd)
         last = [time.perf_counter()]
@@ -645,7 +645,8 @@ def spans_ensemble(oq, kdtypes, sampled, member_range="", wall=None):
                 last[0] = time.perf_counter()
                 log("ensemble %s: %d iterations, %d members running" % (kd, iters, left))
         r = run_ensemble_cycle(ens, U0[lo:hi], span, reltol=w["reltol"], abstol=w["abstol"], dt0=1.0,
-                               progress=progress, wall_limit=wall)
+                               progress=progress, wall_limit=wall,
+                               alg=oq.VCABM5() if method == "vcabm5" else oq.Tsit5())
         ens.close()
         M = hi - lo
         if not r["complete"]:  # stopped by the wall limit: where every member got to
@@ -653,27 +654,36 @@ def spans_ensemble(oq, kdtypes, sampled, member_range="", wall=None):
             out["by_kdtype"][kd] = dict(member_range=[lo, hi], complete=False, wall_s=r["wall"],
                                         lockstep_iterations=r["iters"], t_reached_yr_min=float(tend.min()),
                                         t_reached_yr_mean=float(tend.mean()),
-                                        members_done=int(np.sum(tend >= 300.0)),
-                                        accepted_max=int(r["naccept"].max()), rejected_max=int(r["nreject"].max()))
+                                        members_done=int(np.sum(tend >= 300.0)))
             log("ensemble %s: stopped after %.0f s at %d iterations; t reached min %.1f / mean %.1f yr, %d done" % (
                 kd, r["wall"], r["iters"], tend.min(), tend.mean(), int(np.sum(tend >= 300.0))))
-            continue
         cats = [_catalogue(r["t"][m], r["vmax"][m]) for m in range(M)]
         na, nr = r["naccept"], r["nreject"]
+        def params(p):  # the member's scalars: L, sigma and the velocity-weakening a - b
+            vw = p.b > p.a
+            return dict(L_mm=float(p.L.flat[0]) * 1e3, sigma_MPa=float(p.σ.flat[0]) / 1e6,
+                        a_minus_b=float((p.a - p.b)[vw].flat[0]) if vw.any() else 0.0)
+        members_out = {str(lo + m): dict(accepted=int(na[m]), rejected=int(nr[m]), events=cats[m]["events"],
+                                         mean_recurrence_yr=cats[m]["mean_recurrence_yr"],
+                                         t_reached_yr=float(r["t_end"][m] / YEAR),
+                                         complete=bool(r["t_end"][m] >= span[1]), **params(members[lo + m]))
+                       for m in range(M)}
+        if not r["complete"]:
+            done_cats = {str(m): cats[m - lo] for m in sampled if lo <= m < hi and members_out[str(m)]["complete"]}
+            out["by_kdtype"][kd].update(members=members_out, sampled_catalogues=done_cats)
+            continue
         out["by_kdtype"][kd] = dict(
-            member_range=[lo, hi], wall_s=r["wall"], lockstep_iterations=r["iters"],
+            member_range=[lo, hi], complete=True, wall_s=r["wall"], lockstep_iterations=r["iters"],
             member_sim_yr_per_hr=M * 300.0 / (r["wall"] / 3600.0),
             all_success=all(x == "Success" for x in r["retcode"]),
             rejection_rate=float(nr.sum() / (na.sum() + nr.sum())),
-            members={str(lo + m): dict(accepted=int(na[m]), rejected=int(nr[m]), events=cats[m]["events"],
-                                       mean_recurrence_yr=cats[m]["mean_recurrence_yr"]) for m in range(M)},
-            sampled_catalogues={str(m): cats[m - lo] for m in sampled if lo <= m < hi})
+            members=members_out, sampled_catalogues={str(m): cats[m - lo] for m in sampled if lo <= m < hi})
         log("ensemble %s: %d x 300 yr in %.0f s (%.0f member-yr/hr), rejection rate %.3f" % (
             kd, M, r["wall"], out["by_kdtype"][kd]["member_sim_yr_per_hr"], out["by_kdtype"][kd]["rejection_rate"]))
     return out
 
 
-def spans_members(oq, members_fp64, ens, ref_path):
+def spans_members(oq, members_fp64, ens, ref_path, method="tsit5"):
     """Sampled ensemble members run alone in FP64 (Tsit5, the same
     tolerances and start) over the same 300 years; their catalogues are
     compared with the ensemble's catalogues of those members, read from the
@@ -689,9 +699,10 @@ def spans_members(oq, members_fp64, ens, ref_path):
         u0 = w["u0"].similar().unpack(U0[m])
         p = oq.assemble_from_mesh(w["mf"], w["ma"], w["lam"], w["mu"], members[m], w["pa"], u0, span,
                                   nrept=w["nrept"], buffer_ratio=w["buffer_ratio"])
-        log("member %d alone, FP64 Tsit5, 300 yr" % m)
+        log("member %d alone, FP64 %s, 300 yr" % (m, method))
         r = run_cycle(p, reltol=w["reltol"], abstol=w["abstol"], chunk=4096, dt=1.0,
-                      progress=_progress("member %d fp64" % m))
+                      progress=_progress("member %d fp64" % m),
+                      alg=oq.VCABM5() if method == "vcabm5" else oq.Tsit5())
         p.f.close()
         ref = _catalogue(r["t"], r["vmax"])
         res = dict(wall_s=r["wall"], accepted=int(r["naccept"]), rejected=int(r["nreject"]), **ref)
@@ -706,7 +717,7 @@ def spans_members(oq, members_fp64, ens, ref_path):
     return out
 
 
-def spans_group(oq, geo_range="", shape=""):
+def spans_group(oq, geo_range="", shape="", wall=None, method="tsit5"):
     """Geometries a:b (default all 24) of the grouped batch (16 members each,
     FP32) over 300 years with the lockstep grouped Tsit5: per-member
     step-rejection rates and step counts.  With a pinned tile shape every
@@ -724,9 +735,11 @@ def spans_group(oq, geo_range="", shape=""):
         shapes = shape
     else:
         shapes = {b: c[0] for b, c in grp.autotune(3).items()}
-    grp.start(grp.pack([w["U0"] for w in ws]), (0.0, 300 * YEAR), dt0=1.0)
+    grp.start(grp.pack([w["U0"] for w in ws]), (0.0, 300 * YEAR), dt0=1.0,
+              alg=oq.VCABM5() if method == "vcabm5" else oq.Tsit5())
     t0 = time.perf_counter()
     iters, last = 0, t0
+    complete = False
     while True:
         left = grp.run(256)
         iters += 256
@@ -736,19 +749,27 @@ def spans_group(oq, geo_range="", shape=""):
             log("group 300 yr: %d iterations, %d members running, min t = %.1f yr" % (
                 iters, left, t.min() / YEAR))
         if left == 0:
+            complete = True
             break
-    wall = time.perf_counter() - t0
+        if wall is not None and time.perf_counter() - t0 > wall:
+            break
+    wall_s = time.perf_counter() - t0
     t, U, na, nr, rc = grp.state()
     grp.close()
     rate = nr / np.maximum(na + nr, 1)
     G = hi - lo
-    return dict(geometry_range=[lo, hi], members_per_geometry=16, sim_years=300.0, wall_s=wall,
-                lockstep_iterations=iters, member_sim_yr_per_hr=len(rc) * 300.0 / (wall / 3600.0),
+    tyr = t / YEAR
+    return dict(geometry_range=[lo, hi], members_per_geometry=16, sim_years=300.0, wall_s=wall_s, method=method,
+                complete=complete, members_done=int(np.sum(tyr >= 300.0)), t_reached_yr_min=float(tyr.min()),
+                t_reached_yr_mean=float(tyr.mean()),
+                lockstep_iterations=iters, member_sim_yr_per_hr=float(tyr.sum()) / (wall_s / 3600.0),
                 all_success=all(x == "Success" for x in rc), tile_shapes=shapes,
                 rejection_rate_mean=float(rate.mean()), rejection_rate_max=float(rate.max()),
                 by_geometry={ws[g]["name"]: dict(index=lo + g, rejection_rate=float(rate[g * 16:(g + 1) * 16].mean()),
                                                  accepted_mean=float(na[g * 16:(g + 1) * 16].mean()),
-                                                 rejected_mean=float(nr[g * 16:(g + 1) * 16].mean()))
+                                                 rejected_mean=float(nr[g * 16:(g + 1) * 16].mean()),
+                                                 t_reached_yr_min=float(tyr[g * 16:(g + 1) * 16].min()),
+                                                 members_done=int(np.sum(tyr[g * 16:(g + 1) * 16] >= 300.0)))
                              for g in range(G)})
 
 
@@ -796,19 +817,41 @@ def spans_summary(paths):
         mem = {}
         for e in parts:
             mem.update(e.get("members", {}))
-        na = sum(m["accepted"] for m in mem.values())
-        nr = sum(m["rejected"] for m in mem.values())
-        ev = [m["events"] for m in mem.values()]
-        lines += ["| ensemble %s, members | wall s | lockstep iterations | member-yr/hr | rejection rate | all success |"
-                  % kd, "|---|---:|---:|---:|---:|---|"]
+        lines += ["| ensemble %s, members | wall s | lockstep iterations | members at 300 yr | t reached (yr) min / mean | "
+                  "member-yr/hr |" % kd, "|---|---:|---:|---:|---|---:|"]
         for e in parts:
-            lines.append("| %d:%d | %.0f | %d | %.0f | %.3f | %s |" % (
+            ms = [v for k, v in e.get("members", {}).items()]
+            tr = [v.get("t_reached_yr", 300.0) for v in ms] or [300.0]
+            lines.append("| %d:%d | %.0f | %d | %d of %d | %.1f / %.1f | %.0f |" % (
                 e["member_range"][0], e["member_range"][1], e["wall_s"], e["lockstep_iterations"],
-                e["member_sim_yr_per_hr"], e["rejection_rate"], e["all_success"]))
+                sum(1 for v in ms if v.get("complete", True)), len(ms), min(tr), float(np.mean(tr)),
+                sum(tr) / (e["wall_s"] / 3600.0)))
         lines.append("")
-        lines.append("%d members: rejection rate %.3f; events per member min %d, median %d, max %d." % (
-            len(mem), nr / max(1, na + nr), min(ev), int(np.median(ev)), max(ev)))
-        lines.append("")
+        done = [v for v in mem.values() if v.get("complete", True)]
+        if done:
+            na = sum(v["accepted"] for v in done)
+            nr = sum(v["rejected"] for v in done)
+            ev = [v["events"] for v in done]
+            rec = [v["mean_recurrence_yr"] for v in done if v["mean_recurrence_yr"]]
+            lines.append("%d members over the whole 300 yr: rejection rate %.3f; events per member min %d, median %d, "
+                         "max %d; mean recurrence median %.1f yr." % (len(done), nr / max(1, na + nr), min(ev),
+                                                                       int(np.median(ev)), max(ev),
+                                                                       float(np.median(rec)) if rec else float("nan")))
+            slow = sorted(((v.get("t_reached_yr", 300.0), k, v) for k, v in mem.items() if not v.get("complete", True)))
+            if slow:
+                lines.append("Members short of 300 yr when the wall limit stopped the run (t reached, accepted + rejected "
+                             "steps, L, sigma, a - b): " + "; ".join(
+                                 "%s: %.1f yr, %d + %d, %.1f mm, %.0f MPa, %.4f" % (
+                                     k, t, v["accepted"], v["rejected"], v.get("L_mm", float("nan")),
+                                     v.get("sigma_MPa", float("nan")), v.get("a_minus_b", float("nan")))
+                                 for t, k, v in slow[:12]) + ("; ..." if len(slow) > 12 else "") + ".")
+                lo_l = [v["L_mm"] for v in mem.values() if "L_mm" in v and not v.get("complete", True)]
+                hi_l = [v["L_mm"] for v in done if "L_mm" in v]
+                if lo_l and hi_l:
+                    lines.append("L of the members short of 300 yr: median %.1f mm (max %.1f); of the finished ones: "
+                                 "median %.1f mm (min %.1f)." % (float(np.median(lo_l)), max(lo_l),
+                                                                  float(np.median(hi_l)), min(hi_l)))
+            lines.append("")
     if alone:
         kds = sorted({k[9:-8] for r in alone.values() for k in r if k.startswith("ensemble_")})
         lines += ["| member | FP64 alone: wall s | accepted | rejected | events | mean recurrence yr | " +
@@ -825,12 +868,15 @@ def spans_summary(paths):
                 "%.2f" % r["mean_recurrence_yr"] if r["mean_recurrence_yr"] else "-", " | ".join(cells)))
         lines.append("")
     if grp:
-        lines += ["| grouped geometries | wall s | lockstep iterations | member-yr/hr | rejection rate mean / max | all success |",
+        lines += ["| grouped geometries | wall s | lockstep iterations | member-yr/hr | rejection rate mean / max | whole span |",
                   "|---|---:|---:|---:|---|---|"]
         for g in grp:
             lines.append("| %d:%d | %.0f | %d | %.0f | %.3f / %.3f | %s |" % (
                 g["geometry_range"][0], g["geometry_range"][1], g["wall_s"], g["lockstep_iterations"],
-                g["member_sim_yr_per_hr"], g["rejection_rate_mean"], g["rejection_rate_max"], g["all_success"]))
+                g["member_sim_yr_per_hr"], g["rejection_rate_mean"], g["rejection_rate_max"],
+                "yes" if g.get("complete", True) and g["all_success"] else
+                "%d of %d members at 300 yr (min t %.1f yr)" % (g.get("members_done", 0), 16 * (
+                    g["geometry_range"][1] - g["geometry_range"][0]), g.get("t_reached_yr_min", 0.0))))
         lines += ["", "| geometry | rejection rate | accepted (mean) | rejected (mean) |", "|---|---:|---:|---:|"]
         for g in grp:
             for name, r in sorted(g["by_geometry"].items(), key=lambda x: x[1]["index"]):
@@ -875,6 +921,8 @@ def main():
                     help="print the markdown tables of these span JSON files (comma-separated) and exit")
     ap.add_argument("--spans-wall", type=float, default=0.0,
                     help="wall-clock seconds after which an ensemble span stops and reports how far it got")
+    ap.add_argument("--spans-method", default="tsit5", choices=["tsit5", "vcabm5"],
+                    help="method of the ensemble, members and group spans")
     ap.add_argument("--spans-range", default="",
                     help="a:b, the ensemble members (of 256) or grouped-batch geometries (of 24) of this call")
     ap.add_argument("--spans-runs", default="fp64:tsit5,fp64:vcabm5,fp32:vcabm5,fp32:tsit5",
@@ -914,12 +962,13 @@ def main():
                 res[leg] = spans_composite(oq, runs, refs, save=lambda o: save(o, "composite"))
             elif leg == "ensemble":
                 res[leg] = spans_ensemble(oq, args.spans_kdtypes.split(","), sampled, args.spans_range,
-                                          args.spans_wall or None)
+                                          args.spans_wall or None, args.spans_method)
             elif leg == "members":
                 ref = refs.get("ensemble", {}).get("by_kdtype")
-                res[leg] = spans_members(oq, sampled, ref, args.spans_ref)
+                res[leg] = spans_members(oq, sampled, ref, args.spans_ref, args.spans_method)
             elif leg == "group":
-                res[leg] = spans_group(oq, args.spans_range, args.group_shape)
+                res[leg] = spans_group(oq, args.spans_range, args.group_shape, args.spans_wall or None,
+                                       args.spans_method)
             else:
                 raise SystemExit("unknown span leg %r" % leg)
             save()

@@ -58,29 +58,28 @@ def events(step_t, step_vmax, vthresh=1e-3, merge_gap=0.0):
     t = np.asarray(step_t)
     above = v >= vthresh
     out = []
-    i = 1
     n = len(v)
-    while i < n:
-        if above[i] and not above[i - 1]:
-            la, lb = np.log(v[i - 1]), np.log(v[i])
-            f = (np.log(vthresh) - la) / (lb - la) if lb != la else 0.0
-            onset = t[i - 1] + f * (t[i] - t[i - 1])
-            j = i
-            while j < n and above[j]:
-                j += 1
-            k = i + int(np.argmax(v[i:j]))
-            end = t[j - 1]
-            if out and onset - out[-1]["end"] < merge_gap:
-                ev = out[-1]
-                if v[k] > ev["peak_v"]:
-                    ev["peak_v"], ev["t_peak"] = float(v[k]), float(t[k])
-                ev["end"] = float(end)
-            else:
-                out.append(dict(onset=float(onset), peak_v=float(v[k]), t_peak=float(t[k]),
-                                end=float(end)))
-            i = j
+    if n < 2:
+        return out
+    # upward crossings (above[i], not above[i - 1]) and the first index below
+    # the threshold after each of them
+    starts = np.flatnonzero(above[1:] & ~above[:-1]) + 1
+    falls = np.flatnonzero(~above[1:] & above[:-1]) + 1
+    for i in starts:
+        la, lb = np.log(v[i - 1]), np.log(v[i])
+        f = (np.log(vthresh) - la) / (lb - la) if lb != la else 0.0
+        onset = t[i - 1] + f * (t[i] - t[i - 1])
+        q = np.searchsorted(falls, i)
+        j = int(falls[q]) if q < len(falls) else n
+        k = i + int(np.argmax(v[i:j]))
+        end = t[j - 1]
+        if out and onset - out[-1]["end"] < merge_gap:
+            ev = out[-1]
+            if v[k] > ev["peak_v"]:
+                ev["peak_v"], ev["t_peak"] = float(v[k]), float(t[k])
+            ev["end"] = float(end)
         else:
-            i += 1
+            out.append(dict(onset=float(onset), peak_v=float(v[k]), t_peak=float(t[k]), end=float(end)))
     return out
 
 
