@@ -278,8 +278,25 @@ end
 state is recorded on the GPU, drained every `nstep` snapshots and appended
 through `getu(u, t, nothing)` to HDF5 datasets `ustrs` and `tstr` (time is the
 trailing, extendible dimension)."""
-function wsolve_gpu(prob, file, nstep, getu, ustrs, tstr; stride=1, append=false, force=false,
+# The `integrator` argument wsolve_gpu hands to getu at each saved state:
+# `it(it.t, Val{1})` is f(u) there (what the reference's handlers read,
+# src/io.jl:86, examples/otf-with-mantle.jl:157-159), recorded on the device.
+mutable struct GPUSavedState{T}
+    t::Float64
+    u::T
+    du::Union{Nothing,T}
+    used::Bool
+end
+function (s::GPUSavedState)(t, ::Type{Val{1}})
+    s.used = true
+    t == s.t || throw(ArgumentError("only the derivative at the saved time t = $(s.t) is available"))
+    s.du === nothing && throw(ArgumentError("the derivative was not recorded (getu did not ask for it at t0)"))
+    return s.du
+end
+
+function wsolve_gpu(prob, file, nstep, getu, ustrs, tstr; alg=:Tsit5, stride=1, append=false, force=false,
                     reltol=1e-6, abstol=1e-8, dt=0.0, dtmax=0.0, dtmin=0.0, maxiters=10^9)
+    alg in (:Tsit5, :VCABM5) || throw(ArgumentError("alg must be :Tsit5 or :VCABM5"))
     if isfile(file) && !force && !append
         @info "Overwrite existing file $(file) must set `force = true`."
         return nothing
@@ -287,14 +304,21 @@ function wsolve_gpu(prob, file, nstep, getu, ustrs, tstr; stride=1, append=false
     @assert tstr ∉ ustrs "Duplicate name of $tstr in $ustrs."
     f = prob.p[end]::GPURHS
     u0 = collect(Float64, prob.u0)
+    # getu's first call (io.jl:126) also tells whether it reads the derivative
+    du0 = similar(prob.u0)
+    rhs!(f, du0, prob.u0, prob.tspan[1])
+    first = GPUSavedState(prob.tspan[1], prob.u0, du0, false)
+    xs0 = getu(prob.u0, prob.tspan[1], first)
+    want_du = first.used
     s = Ref{Ptr{Cvoid}}(C_NULL)
-    check(ccall((:oq_stepper_create, libpath), Cint,
+    check(ccall((alg == :VCABM5 ? :oq_stepper_create_vcabm5 : :oq_stepper_create, libpath), Cint,
         (Ptr{Cvoid}, Ptr{Float64}, Float64, Float64, Ref{CSolverOpts}, Ref{Ptr{Cvoid}}),
         f.h, u0, prob.tspan[1], prob.tspan[2], Ref(CSolverOpts(reltol, abstol, dt, dtmax, dtmin, maxiters)), s))
-    check(ccall((:oq_stepper_record, libpath), Cint, (Ptr{Cvoid}, Int64, Int64, Int64), s[], stride, nstep, 0))
+    check(ccall((:oq_stepper_record_du, libpath), Cint, (Ptr{Cvoid}, Int64, Int64, Int64, Cint),
+        s[], stride, nstep, 0, want_du ? 1 : 0))
     h5 = HDF5.h5open(file, append ? "r+" : "w")
-    function save(u, t)
-        xs = getu(u, t, nothing)
+    function save(xs, t)
+        @assert length(xs) == length(ustrs) "Unmatched length between solution components and names."
         for (name, x) in zip(ustrs, xs)
             if !haskey(h5, name)
                 HDF5.create_dataset(h5, name, Float64, ((size(x)..., 0), (size(x)..., -1)); chunk=(size(x)..., nstep))
@@ -306,19 +330,21 @@ function wsolve_gpu(prob, file, nstep, getu, ustrs, tstr; stride=1, append=false
         dt_ = h5[tstr]; n = length(dt_) + 1
         HDF5.set_extent_dims(dt_, (n,)); dt_[n] = t
     end
-    save(prob.u0, prob.tspan[1])
+    save(xs0, prob.tspan[1])
     rc = Ref{Cint}(0)
     nsn = Ref{Int64}(0); nst = Ref{Int64}(0)
     snap_t = Vector{Float64}(undef, nstep); snap_u = Vector{Float64}(undef, nstep * length(u0))
-    ubuf = similar(prob.u0)
+    snap_du = want_du ? Vector{Float64}(undef, nstep * length(u0)) : Float64[]
     while rc[] == 0
         check(ccall((:oq_stepper_run, libpath), Cint, (Ptr{Cvoid}, Int64, Ref{Cint}), s[], nstep, rc))
-        check(ccall((:oq_stepper_drain, libpath), Cint,
-            (Ptr{Cvoid}, Ref{Int64}, Ptr{Float64}, Ptr{Float64}, Ref{Int64}, Ptr{Float64}, Ptr{Float64}),
-            s[], nst, C_NULL, C_NULL, nsn, snap_t, snap_u))
+        check(ccall((:oq_stepper_drain_du, libpath), Cint,
+            (Ptr{Cvoid}, Ref{Int64}, Ptr{Float64}, Ptr{Float64}, Ref{Int64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+            s[], nst, C_NULL, C_NULL, nsn, snap_t, snap_u, want_du ? snap_du : C_NULL))
         for k in 1:nsn[]
-            copyto!(ubuf, view(snap_u, (k - 1) * length(u0) + 1:k * length(u0)))
-            save(ubuf, snap_t[k])
+            r = (k - 1) * length(u0) + 1:k * length(u0)
+            ubuf = similar(prob.u0); copyto!(ubuf, view(snap_u, r))
+            dbuf = want_du ? copyto!(similar(prob.u0), view(snap_du, r)) : nothing
+            save(getu(ubuf, snap_t[k], GPUSavedState(snap_t[k], ubuf, dbuf, false)), snap_t[k])
         end
         flush(h5)
     end
