@@ -416,11 +416,12 @@ end
 """ensemble_solve(e, U0, tspan; reltol, abstol, dt0): lockstep Tsit5 of all
 members, each with its own step control.  Returns (t, U, naccept, nreject,
 retcodes) per member."""
-function ensemble_solve(e::GPUEnsemble, U0::Matrix{Float64}, tspan::NTuple{2};
+function ensemble_solve(e::GPUEnsemble, U0::Matrix{Float64}, tspan::NTuple{2}; alg=:Tsit5,
         reltol=1e-6, abstol=1e-8, dt0=1.0, dtmax=0.0, dtmin=0.0, maxiters=10^9, chunk::Integer=4096)
     size(U0) == (e.n, e.M) || throw(DimensionMismatch("U0 must be n x M"))
+    alg in (:Tsit5, :VCABM5) || throw(ArgumentError("alg must be :Tsit5 or :VCABM5"))
     opts = CSolverOpts(reltol, abstol, dt0, dtmax, dtmin, maxiters)
-    check(ccall((:oq_ensemble_start, libpath), Cint,
+    check(ccall((alg == :VCABM5 ? :oq_ensemble_start_vcabm5 : :oq_ensemble_start, libpath), Cint,
         (Ptr{Cvoid}, Ptr{Float64}, Float64, Float64, Ref{CSolverOpts}), e.h, U0, tspan[1], tspan[2], Ref(opts)))
     left = Ref{Int64}(e.M)
     while left[] > 0
