@@ -621,7 +621,8 @@ def spans_ensemble(oq, kdtypes, sampled, member_range="", wall=None, method="tsi
     """Members a:b (default all 256) of the composite-geometry ensemble over
     300 years with the lockstep Tsit5 (per-member steps), per operator type:
     rejection rates, event counts and mean recurrence of every member, and
-    the whole catalogues of the sampled members (for the members leg).  A
+    the whole catalogues of every member that finished (for the members
+    leg).  A
     member's trajectory does not depend on which other members share its
     batch (each GEMM output column and each member's scales and controller
     are its own), so a span can run as several member ranges."""
@@ -669,7 +670,7 @@ def spans_ensemble(oq, kdtypes, sampled, member_range="", wall=None, method="tsi
                                          complete=bool(r["t_end"][m] >= span[1]), **params(members[lo + m]))
                        for m in range(M)}
         if not r["complete"]:
-            done_cats = {str(m): cats[m - lo] for m in sampled if lo <= m < hi and members_out[str(m)]["complete"]}
+            done_cats = {str(lo + m): cats[m] for m in range(M) if members_out[str(lo + m)]["complete"]}
             out["by_kdtype"][kd].update(members=members_out, sampled_catalogues=done_cats)
             continue
         out["by_kdtype"][kd] = dict(
@@ -677,7 +678,7 @@ def spans_ensemble(oq, kdtypes, sampled, member_range="", wall=None, method="tsi
             member_sim_yr_per_hr=M * 300.0 / (r["wall"] / 3600.0),
             all_success=all(x == "Success" for x in r["retcode"]),
             rejection_rate=float(nr.sum() / (na.sum() + nr.sum())),
-            members=members_out, sampled_catalogues={str(m): cats[m - lo] for m in sampled if lo <= m < hi})
+            members=members_out, sampled_catalogues={str(lo + m): cats[m] for m in range(M)})
         log("ensemble %s: %d x 300 yr in %.0f s (%.0f member-yr/hr), rejection rate %.3f" % (
             kd, M, r["wall"], out["by_kdtype"][kd]["member_sim_yr_per_hr"], out["by_kdtype"][kd]["rejection_rate"]))
     return out
