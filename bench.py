@@ -619,13 +619,14 @@ def _span_range(rng, n):
 
 def spans_ensemble(oq, kdtypes, sampled, member_range="", wall=None, method="tsit5"):
     """Members a:b (default all 256) of the composite-geometry ensemble over
-    300 years with the lockstep Tsit5 (per-member steps), per operator type:
-    rejection rates, event counts and mean recurrence of every member, and
-    the whole catalogues of every member that finished (for the members
-    leg).  A
-    member's trajectory does not depend on which other members share its
-    batch (each GEMM output column and each member's scales and controller
-    are its own), so a span can run as several member ranges."""
+    300 years with the lockstep Tsit5 or VCABM5 (per-member steps), per
+    operator type: rejection rates, event counts and mean recurrence of
+    every member, and the whole catalogues of every member that finished
+    (for the members leg).  A member's trajectory does not depend on which
+    other members share its batch (each GEMM output column and each
+    member's scales and controller are its own), so a span can run as
+    several member ranges; with a wall limit the members still running
+    report how far they got."""
     from oetqf_amd.cycle import run_ensemble_cycle
     from oetqf_amd.workloads import YEAR, ensemble_members, transform_fault_composite
 
