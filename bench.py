@@ -712,6 +712,14 @@ def spans_members(oq, members_fp64, ens, ref_path, method="tsit5"):
             cat = e.get("sampled_catalogues", {}).get(str(m))
             if cat is not None:
                 res["ensemble_%s_vs_fp64" % kd] = _compare(cat, ref)
+            elif str(m) in e.get("members", {}):  # the ensemble kept only this member's summary
+                s_ = e["members"][str(m)]
+                c = dict(same_event_count=s_["events"] == ref["events"])
+                if s_["mean_recurrence_yr"] and ref["mean_recurrence_yr"]:
+                    c["mean_recurrence_rel_err"] = abs(s_["mean_recurrence_yr"] - ref["mean_recurrence_yr"]) / \
+                        ref["mean_recurrence_yr"]
+                c["events"] = [s_["events"], ref["events"]]
+                res["ensemble_%s_vs_fp64" % kd] = c
         out["members"][str(m)] = res
         log("member %d: FP64 %d events, mean recurrence %s yr; %s" % (
             m, ref["events"], ref["mean_recurrence_yr"],
