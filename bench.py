@@ -685,7 +685,7 @@ def spans_ensemble(oq, kdtypes, sampled, member_range="", wall=None, method="tsi
     return out
 
 
-def spans_members(oq, members_fp64, ens, ref_path, method="tsit5"):
+def spans_members(oq, members_fp64, ens, ref_path, method="tsit5", save=None):
     """Sampled ensemble members run alone in FP64 (Tsit5, the same
     tolerances and start) over the same 300 years; their catalogues are
     compared with the ensemble's catalogues of those members, read from the
@@ -721,6 +721,8 @@ def spans_members(oq, members_fp64, ens, ref_path, method="tsit5"):
                 c["events"] = [s_["events"], ref["events"]]
                 res["ensemble_%s_vs_fp64" % kd] = c
         out["members"][str(m)] = res
+        if save is not None:
+            save(out)
         log("member %d: FP64 %d events, mean recurrence %s yr; %s" % (
             m, ref["events"], ref["mean_recurrence_yr"],
             "; ".join("%s %s" % (k[9:-8], v) for k, v in res.items() if k.startswith("ensemble_"))))
@@ -975,7 +977,8 @@ def main():
                                           args.spans_wall or None, args.spans_method)
             elif leg == "members":
                 ref = refs.get("ensemble", {}).get("by_kdtype")
-                res[leg] = spans_members(oq, sampled, ref, args.spans_ref, args.spans_method)
+                res[leg] = spans_members(oq, sampled, ref, args.spans_ref, args.spans_method,
+                                         save=lambda o: save(o, "members"))
             elif leg == "group":
                 res[leg] = spans_group(oq, args.spans_range, args.group_shape, args.spans_wall or None,
                                        args.spans_method)
