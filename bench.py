@@ -873,8 +873,20 @@ def spans_summary(paths):
             cells = []
             for k in kds:
                 c = r.get("ensemble_%s_vs_fp64" % k)
-                cells.append("-" if c is None else ("same events" if c["same_event_count"] else "events differ") +
-                             "".join(", %s %.1e" % (a, b) for a, b in c.items() if a != "same_event_count"))
+                if c is None:
+                    cells.append("-")
+                    continue
+                txt = "same events" if c["same_event_count"] else "events differ"
+                if "events" in c:
+                    txt += " (%d vs %d)" % tuple(c["events"])
+                if "leading_events_agree" in c:
+                    txt += ", %d leading agree (onsets within %.1e)" % (c["leading_events_agree"],
+                                                                       c["max_leading_onset_rel_err"])
+                if "first_difference_yr" in c:
+                    txt += ", part at %.1f yr" % c["first_difference_yr"]
+                if "mean_recurrence_rel_err" in c:
+                    txt += ", mean recurrence %.1e" % c["mean_recurrence_rel_err"]
+                cells.append(txt)
             lines.append("| %s | %.0f | %d | %d | %d | %s | %s |" % (
                 m, r["wall_s"], r["accepted"], r["rejected"], r["events"],
                 "%.2f" % r["mean_recurrence_yr"] if r["mean_recurrence_yr"] else "-", " | ".join(cells)))
