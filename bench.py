@@ -777,7 +777,7 @@ def spans_group(oq, geo_range="", shape="", wall=None, method="tsit5"):
                 lockstep_iterations=iters, member_sim_yr_per_hr=float(tyr.sum()) / (wall_s / 3600.0),
                 all_success=all(x == "Success" for x in rc), tile_shapes=shapes,
                 rejection_rate_mean=float(rate.mean()), rejection_rate_max=float(rate.max()),
-                by_geometry={ws[g]["name"]: dict(index=lo + g, rejection_rate=float(rate[g * 16:(g + 1) * 16].mean()),
+                by_geometry={"%d: %s" % (lo + g, ws[g]["name"]): dict(index=lo + g, rejection_rate=float(rate[g * 16:(g + 1) * 16].mean()),
                                                  accepted_mean=float(na[g * 16:(g + 1) * 16].mean()),
                                                  rejected_mean=float(nr[g * 16:(g + 1) * 16].mean()),
                                                  t_reached_yr_min=float(tyr[g * 16:(g + 1) * 16].min()),
@@ -901,11 +901,13 @@ def spans_summary(paths):
                 "yes" if g.get("complete", True) and g["all_success"] else
                 "%d of %d members at 300 yr (min t %.1f yr)" % (g.get("members_done", 0), 16 * (
                     g["geometry_range"][1] - g["geometry_range"][0]), g.get("t_reached_yr_min", 0.0))))
-        lines += ["", "| geometry | rejection rate | accepted (mean) | rejected (mean) |", "|---|---:|---:|---:|"]
+        lines += ["", "| geometry | rejection rate | accepted (mean) | rejected (mean) | members at 300 yr | "
+                  "min t reached (yr) |", "|---|---:|---:|---:|---:|---:|"]
         for g in grp:
             for name, r in sorted(g["by_geometry"].items(), key=lambda x: x[1]["index"]):
-                lines.append("| %s | %.3f | %.0f | %.0f |" % (name, r["rejection_rate"], r["accepted_mean"],
-                                                              r["rejected_mean"]))
+                lines.append("| %s | %.3f | %.0f | %.0f | %s | %s |" % (
+                    name, r["rejection_rate"], r["accepted_mean"], r["rejected_mean"],
+                    r.get("members_done", "-"), "%.1f" % r["t_reached_yr_min"] if "t_reached_yr_min" in r else "-"))
     print("\n".join(lines))
 
 
