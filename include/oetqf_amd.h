@@ -407,24 +407,25 @@ int oq_group_create(int device, int64_t ngeo, const oq_fault_mesh *mf,
 /* Total state entries, per-geometry state size nstate[ngeo] and the bytes
  * of all operators (any pointer may be NULL). */
 int oq_group_info(oq_group *e, int64_t *ntot, int64_t *nstate, int64_t *kbytes);
-/* Times every tile shape (13: register-streaming kernels with 16/32/64 rows
+/* Times every tile shape (15: register-streaming kernels with 16/32/64 rows
  * per wavefront, each with and without a software-pipelined K stream,
  * LDS-staged kernels with 16/32 rows per wavefront and one or two slices in
  * flight, the two-slice kernel on column-split units of ~2 MB balanced
- * over the resident workgroups, and a kernel on the same kind of units that
- * copies K and S from HBM straight into a 3-slice LDS ring, K with and
- * without the non-temporal cache policy) on the
+ * over the resident workgroups, and four configurations of a kernel on the
+ * same kind of units that copies K and S from HBM straight into an LDS ring
+ * (3 slices of 64 rows x 128 columns, 1 workgroup per CU; 3 x 32 x 128,
+ * 2 per CU; 3 x 64 x 64, 2 per CU; 2 x 64 x 64, 4 per CU)) on the
  * geometries of each size bucket
  * (bucket b: N < 2048 << b;
  * 8 buckets) and keeps the fastest; then times the whole grouped GEMM (one
  * launch per shape in use) with that per-bucket choice and with each single
  * shape for all buckets (one launch), and keeps the faster configuration.
  * shape_of_bucket[8] and ms_of_bucket[8] receive the choice (-1 for an empty
- * bucket), ms_table[8*13] (may be NULL) every per-bucket timing.  Call before
+ * bucket), ms_table[8*15] (may be NULL) every per-bucket timing.  Call before
  * oq_group_start (a running stepper must be started again). */
 int oq_group_autotune(oq_group *e, int iters, int *shape_of_bucket, double *ms_of_bucket,
                       double *ms_table);
-/* Pins the tile shape of each size bucket (shape_of_bucket[8]: 0..12, -1 =
+/* Pins the tile shape of each size bucket (shape_of_bucket[8]: 0..14, -1 =
  * keep), e.g. to reuse an earlier autotuning.  Like oq_group_autotune, call
  * before oq_group_start. */
 int oq_group_set_shapes(oq_group *e, const int *shape_of_bucket);
