@@ -731,8 +731,8 @@ def spans_members(oq, members_fp64, ens, ref_path, method="tsit5", save=None):
 
 def spans_group(oq, geo_range="", shape="", wall=None, method="tsit5"):
     """Geometries a:b (default all 24) of the grouped batch (16 members each,
-    FP32) over 300 years with the lockstep grouped Tsit5: per-member
-    step-rejection rates and step counts.  With a pinned tile shape every
+    FP32) over 300 years with the lockstep grouped Tsit5 or VCABM5 (method):
+    per-member step-rejection rates and step counts.  With a pinned tile shape every
     member's trajectory is independent of which geometries share the run, so
     a span can run as several geometry ranges."""
     from oetqf_amd.workloads import YEAR, grouped_batch
