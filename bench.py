@@ -1066,21 +1066,23 @@ def main():
             p.f.close()
 
     if not args.no_variants:
-        # the coupled model with K_ff by DFT convolution beside the dense FP64 mantle blocks, as the
-        # reference's coupled ode applies it (equation.jl:200)
-        p = oq.assemble_from_mesh(mf, ma, w["lam"], w["mu"], w["pf"], w["pa"], u0, w["tspan"],
-                                  nrept=w["nrept"], buffer_ratio=w["buffer_ratio"], kdtype="fft")
-        du = oq.similar(u0)
-        p.f(du, u0, None, 0.0)
-        p.f.time(args.warmup)
-        me, _ = p.f.time(args.steps)
-        errs = [float(np.linalg.norm(du.x[k] - du64.x[k]) / np.linalg.norm(du64.x[k])) for k in range(5)]
-        _, _, kb = p.f.info()
-        variants["fft_coupled"] = dict(rhs_per_s=1000.0 / me, ms_per_rhs=me, operator_bytes=int(kb),
-                                       rel_err_vs_dense_fp64=max(errs), speedup_vs_fp64=ms_eval / me)
-        log("coupled, K_ff by DFT: %.4f ms/eval (%.2fx dense FP64), operator %.2f GB, max rel diff %.1e" % (
-            me, ms_eval / me, kb / 1e9, max(errs)))
-        p.f.close()
+        # fft_coupled: the coupled model with K_ff by DFT convolution beside the dense FP64 mantle
+        # blocks, as the reference's coupled ode applies it (equation.jl:200); fft_grid: the same
+        # with K_mm as 2D convolutions over the mantle grid (all FP64)
+        for name, kd in (("fft_coupled", "fft"), ("fft_grid", "fft_grid")):
+            p = oq.assemble_from_mesh(mf, ma, w["lam"], w["mu"], w["pf"], w["pa"], u0, w["tspan"],
+                                      nrept=w["nrept"], buffer_ratio=w["buffer_ratio"], kdtype=kd)
+            du = oq.similar(u0)
+            p.f(du, u0, None, 0.0)
+            p.f.time(args.warmup)
+            me, _ = p.f.time(args.steps)
+            errs = [float(np.linalg.norm(du.x[k] - du64.x[k]) / np.linalg.norm(du64.x[k])) for k in range(5)]
+            _, _, kb = p.f.info()
+            variants[name] = dict(rhs_per_s=1000.0 / me, ms_per_rhs=me, operator_bytes=int(kb),
+                                  rel_err_vs_dense_fp64=max(errs), speedup_vs_fp64=ms_eval / me)
+            log("coupled, %s: %.4f ms/eval (%.2fx dense FP64), operator %.3f GB, max rel diff %.1e" % (
+                kd, me, ms_eval / me, kb / 1e9, max(errs)))
+            p.f.close()
         variants["toeplitz_fault_only"] = toeplitz_leg(oq, args)
 
     ensemble = None
@@ -1103,29 +1105,32 @@ def main():
                                onsets_yr=[e["onset"] / YEAR for e in ev[name]])
             log("%s: %.3f yr in %.1f s (%d+%d steps), %d events" % (name, t / YEAR, el, r["naccept"],
                                                                    r["nreject"], len(ev[name])))
-        # VCABM5 on the FP32 operator over the span the FP64 run reached; its
-        # event catalogue against the FP64 one ("at stated error")
+        # VCABM5 on the FP32 operator, and on the FP64 grid operator, over the span the FP64 run
+        # reached; their event catalogues against the FP64 one ("at stated error")
         t64 = cycle["vcabm5"]["sim_years"] * YEAR
-        p32 = oq.assemble_from_mesh(mf, ma, w["lam"], w["mu"], w["pf"], w["pa"], u0, (0.0, t64),
-                                    nrept=w["nrept"], buffer_ratio=w["buffer_ratio"], kdtype="fp32")
-        r = run_cycle(p32, reltol=w["reltol"], abstol=w["abstol"], alg=oq.VCABM5(), chunk=1024)
-        p32.f.close()
-        t, el = float(r["t"][-1]), r["wall"]
-        e32, e64 = events(r["t"], r["vmax"], 1e-3, merge_gap=YEAR), ev["vcabm5"]
-        cycle["vcabm5_fp32"] = dict(
-            sim_years=t / YEAR, wall_s=el, sim_yr_per_hr=(t / YEAR) / (el / 3600.0),
-            accepted=int(r["naccept"]), rejected=int(r["nreject"]),
-            speedup_vs_fp64=(t / YEAR) / (el / 3600.0) / cycle["vcabm5"]["sim_yr_per_hr"],
-            events=len(e32), onsets_yr=[e["onset"] / YEAR for e in e32],
-            onset_rel_err_vs_fp64=[abs(a["onset"] - b["onset"]) / b["onset"] for a, b in zip(e32, e64)],
-            peak_v_rel_err_vs_fp64=[abs(a["peak_v"] - b["peak_v"]) / b["peak_v"] for a, b in zip(e32, e64)])
-        log("vcabm5 (FP32 K): %.3f yr in %.1f s (%.2fx FP64), %d events (FP64 %d), onset err %s" % (
-            t / YEAR, el, cycle["vcabm5_fp32"]["speedup_vs_fp64"], len(e32), len(e64),
-            ["%.1e" % x for x in cycle["vcabm5_fp32"]["onset_rel_err_vs_fp64"]]))
+        for name, kd, label in (("vcabm5_fp32", "fp32", "FP32 K"), ("vcabm5_fft_grid", "fft_grid", "grid K_mm")):
+            pv = oq.assemble_from_mesh(mf, ma, w["lam"], w["mu"], w["pf"], w["pa"], u0, (0.0, t64),
+                                       nrept=w["nrept"], buffer_ratio=w["buffer_ratio"], kdtype=kd)
+            r = run_cycle(pv, reltol=w["reltol"], abstol=w["abstol"], alg=oq.VCABM5(), chunk=1024)
+            pv.f.close()
+            t, el = float(r["t"][-1]), r["wall"]
+            ev_, e64 = events(r["t"], r["vmax"], 1e-3, merge_gap=YEAR), ev["vcabm5"]
+            cycle[name] = dict(
+                sim_years=t / YEAR, wall_s=el, sim_yr_per_hr=(t / YEAR) / (el / 3600.0),
+                accepted=int(r["naccept"]), rejected=int(r["nreject"]),
+                speedup_vs_fp64=(t / YEAR) / (el / 3600.0) / cycle["vcabm5"]["sim_yr_per_hr"],
+                events=len(ev_), onsets_yr=[e["onset"] / YEAR for e in ev_],
+                onset_rel_err_vs_fp64=[abs(a["onset"] - b["onset"]) / b["onset"] for a, b in zip(ev_, e64)],
+                peak_v_rel_err_vs_fp64=[abs(a["peak_v"] - b["peak_v"]) / b["peak_v"] for a, b in zip(ev_, e64)])
+            log("vcabm5 (%s): %.3f yr in %.1f s (%.2fx FP64), %d events (FP64 %d), onset err %s" % (
+                label, t / YEAR, el, cycle[name]["speedup_vs_fp64"], len(ev_), len(e64),
+                ["%.1e" % x for x in cycle[name]["onset_rel_err_vs_fp64"]]))
         cycle.update(reltol=w["reltol"], abstol=w["abstol"],
                      note="interseismic start of the cycle, FP64 K; each method over the same "
                           "wall-clock window (events: max v > 1e-3 m/s, crossings within 1 yr "
-                          "merged); vcabm5_fp32 = the FP32 operator over the FP64 VCABM5 span")
+                          "merged); vcabm5_fp32 = the FP32 operator, vcabm5_fft_grid = the FP64 "
+                          "operator with K_ff by DFT and K_mm on the mantle grid, each over the FP64 "
+                          "VCABM5 span")
         cycle["sim_yr_per_hr"] = cycle["tsit5"]["sim_yr_per_hr"]
 
     cycle512 = None

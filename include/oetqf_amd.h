@@ -61,6 +61,18 @@ extern "C" {
  * (nf x 6ne), K_mf and K_mm (6ne x N) stay dense FP64; the fault rows add
  * the DFT traction rates to K_fm s_m. */
 #define OQ_K_FFT 5
+/* oq_rhs_create_from_mesh only, coupled model: OQ_K_FFT, and K_mm applied
+ * as 2D convolutions over a mantle whose cells form a uniform horizontal
+ * grid (cell ix + nx (iy + ny iz), equal spacing and extents along x and y,
+ * the same layers under every column; OQ_ERR_ARG otherwise, and for more
+ * than about 60 x 60 cells per layer).  The half-space stress between two
+ * cells then depends on their horizontal offset and their two layers only,
+ * so K_mm is held as 36 nz^2 tiles of (2ny-1) x (2nx-1) FP64 entries
+ * (17.7 MB for 16 x 16 x 8 cells instead of 1.21 GB), assembled by the same
+ * pair kernel for the shifted receivers; K_fm and K_mf stay dense FP64.
+ * No counterpart in the reference (which multiplies the dense K_mm,
+ * equation.jl:203); entries equal the dense block's to rounding. */
+#define OQ_K_FFT_GRID 6
 
 /* RectOkadaMesh (src/BEM/mesh.jl:5-21).  Only x, ax, y, z, axi, nx, nxi, dx,
  * dep and dip enter the Green's functions (GF.jl:38,49-53,150-160). */

@@ -297,15 +297,17 @@ __global__ void __launch_bounds__(128) k_gf_fm(DevHex8 h, DevFault f, double mu,
 }
 
 // K_mm = gf22 (GF.jl:262-289): thread (source cell i fastest, receiver
-// cell j); entry [(k)ne + j, (p)ne + i] += w_q sigma_k.
-__global__ void __launch_bounds__(128) k_gf_mm(DevHex8 h, const double *qc, const double *qw,
-                                               int nq, double mu, double nu, double *out,
-                                               int64_t rs, int64_t cs)
+// cell j); entry [(k)nrec + j, (p)nsrc + i] += w_q sigma_k.  Receivers come
+// from rm, sources from sm: the same mesh for the dense block, a set of
+// shifted receiver cells and one source column for the grid kernel table.
+__global__ void __launch_bounds__(128) k_gf_mm(DevHex8 rm, DevHex8 sm, const double *qc,
+                                               const double *qw, int nq, double mu, double nu,
+                                               double *out, int64_t rs, int64_t cs)
 {
-    const int64_t ne = h.ne;
+    const int64_t nrec = rm.ne, nsrc = sm.ne;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ne * ne) return;
-    const int64_t i = t % ne, j = t / ne;
+    if (t >= nrec * nsrc) return;
+    const int64_t i = t % nsrc, j = t / nsrc;
     // The 36 sums live in LDS (a column per thread): in registers they would
     // be live through the image quadrature and push it into scratch spills.
     __shared__ double acc[36][128];
@@ -313,11 +315,11 @@ __global__ void __launch_bounds__(128) k_gf_mm(DevHex8 h, const double *qc, cons
 #pragma unroll
     for (int m = 0; m < 36; ++m) acc[m][c] = 0.0;
     for (int q = 0; q < nq; ++q) {
-        const double rx = h.cx[j] + qc[3 * q] * h.dx[j] / 2.0;
-        const double ry = h.cy[j] + qc[3 * q + 1] * h.dy[j] / 2.0;
-        const double rz = h.cz[j] + qc[3 * q + 2] * h.dz[j] / 2.0;
+        const double rx = rm.cx[j] + qc[3 * q] * rm.dx[j] / 2.0;
+        const double ry = rm.cy[j] + qc[3 * q + 1] * rm.dy[j] / 2.0;
+        const double rz = rm.cz[j] + qc[3 * q + 2] * rm.dz[j] / 2.0;
         Hex8Pair P;
-        hex8_pair(rx, ry, rz, h.qx[i], h.qy[i], h.qz[i], h.dx[i], h.dy[i], h.dz[i], nu, P);
+        hex8_pair(rx, ry, rz, sm.qx[i], sm.qy[i], sm.qz[i], sm.dx[i], sm.dy[i], sm.dz[i], nu, P);
         for (int p = 0; p < 6; ++p) {
             double s[6];
             hex8_stress_unit(P, p, mu, nu, s);
@@ -327,7 +329,7 @@ __global__ void __launch_bounds__(128) k_gf_mm(DevHex8 h, const double *qc, cons
     }
     for (int p = 0; p < 6; ++p)
 #pragma unroll
-        for (int k = 0; k < 6; ++k) out[(k * ne + j) * rs + (p * ne + i) * cs] = acc[p * 6 + k][c];
+        for (int k = 0; k < 6; ++k) out[(k * nrec + j) * rs + (p * nsrc + i) * cs] = acc[p * 6 + k][c];
 }
 
 // Dense fault-fault block: K[(i,j),(k,l)] = st[|i-k|, j, l] (the expansion
@@ -400,11 +402,18 @@ int launch_gf_fm(const DevHex8 &h, const DevFault &f, double lambda, double mu, 
 int launch_gf_mm(const DevHex8 &h, const double *qc, const double *qw, int nq, double lambda,
                  double mu, double *out, int64_t rs, int64_t cs, hipStream_t s)
 {
+    return launch_gf_mm_pairs(h, h, qc, qw, nq, lambda, mu, out, rs, cs, s);
+}
+
+int launch_gf_mm_pairs(const DevHex8 &rm, const DevHex8 &sm, const double *qc, const double *qw,
+                       int nq, double lambda, double mu, double *out, int64_t rs, int64_t cs,
+                       hipStream_t s)
+{
     const double nu = lambda / 2.0 / (lambda + mu);  // GF.jl:259
-    const int64_t n = h.ne * h.ne;
+    const int64_t n = rm.ne * sm.ne;
     OQ_PAIRS_FIT(n);
-    hipLaunchKernelGGL(k_gf_mm, dim3(nblocks(n, 128)), dim3(128), 0, s, h, qc, qw, nq, mu, nu, out,
-                       rs, cs);
+    hipLaunchKernelGGL(k_gf_mm, dim3(nblocks(n, 128)), dim3(128), 0, s, rm, sm, qc, qw, nq, mu, nu,
+                       out, rs, cs);
     OQ_HIP(hipGetLastError());
     return OQ_OK;
 }

@@ -82,6 +82,11 @@ int launch_gf_fm(const DevHex8 &h, const DevFault &f, double lambda,
 int launch_gf_mm(const DevHex8 &h, const double *qc, const double *qw,
                  int nq, double lambda, double mu, double *out, int64_t rs,
                  int64_t cs, hipStream_t s);
+// The same pair kernel with receivers from rm and sources from sm: element
+// [(k)rm.ne + j, (p)sm.ne + i] at out[row*rs + col*cs].
+int launch_gf_mm_pairs(const DevHex8 &rm, const DevHex8 &sm, const double *qc,
+                       const double *qw, int nq, double lambda, double mu,
+                       double *out, int64_t rs, int64_t cs, hipStream_t s);
 // Expands the Toeplitz kernel st (nx,nxi,nxi) into the fault-fault block of
 // a dense receiver-major operator (row pitch ld).
 int launch_expand_toeplitz(const double *st, int64_t nx, int64_t nxi,

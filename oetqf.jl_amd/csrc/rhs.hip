@@ -165,46 +165,24 @@ __device__ __forceinline__ double dot16(const u32x4 kv, const SVec<OQ_K_BF16> &s
     return (double)r + (double)q;
 }
 
-// One wavefront streams KTraits::kRows consecutive rows.  Each wavefront
-// starts its sweep over the columns at a different pass (rotation by a
-// multiple of its index): with all wavefronts reading the same column
-// offset of rows whose pitch is a large power of two, the loads would
-// concentrate on a few HBM channels (measured 4.9 vs 6.1 TB/s, FP64).
-template <int KT>
-__global__ void __launch_bounds__(KTraits<KT>::kWaves * 64)
-    k_rhs_matvec(RhsDev d, const void *__restrict__ Kp, int64_t ld, int64_t nrows, int64_t rowbase,
-                 const double *__restrict__ s, const double *__restrict__ tadd,
-                 const double *__restrict__ u, double *__restrict__ du)
+// Row sums of R rows over npass passes of 64 16-byte chunks (rp[r] points at
+// row r's chunk of this lane in pass 0; pass q covers columns q * STEP ..),
+// starting at pass p0 and wrapping around, reduced over the wavefront.
+// Software pipeline: the R loads of the next pass are issued before the
+// FMAs of the current one, so 2R 16-byte loads per lane are in flight; the
+// scheduling barriers keep the compiler from interleaving each load with
+// its use (which it does by default to save registers).
+template <int KT, int R>
+__device__ __forceinline__ void stream_rows(const typename KVec<KT>::V *const (&rp)[R],
+                                            const double *__restrict__ s, int64_t npass, int64_t p0,
+                                            int lane, double (&acc)[R])
 {
-    using T = typename KTraits<KT>::T;
-    constexpr int E = KTraits<KT>::kPerLane;  // elements per 16-byte load
-    constexpr int R = KTraits<KT>::kRows;
-    constexpr int W = KTraits<KT>::kWaves;
-    constexpr int64_t STEP = 64 * E;          // columns per wavefront pass
-    const int lane = threadIdx.x & 63;
-    const int64_t gw = (int64_t)blockIdx.x * W + (threadIdx.x >> 6);
-    // rows [0, nrows) of the operator at Kp (pitch ld) are receivers
-    // rowbase + r; tadd (fault receivers, may be NULL) adds a traction rate
-    // computed elsewhere (the DFT K_ff of the coupled model)
-    const int64_t row0 = gw * R;
-    if (row0 >= nrows) return;
     using V = typename KVec<KT>::V;
-    const T *K = static_cast<const T *>(Kp);
-    const V *rp[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int64_t row = row0 + r < nrows ? row0 + r : nrows - 1;
-        rp[r] = reinterpret_cast<const V *>(K + row * ld) + lane;
-    }
-    double acc[R];
+    constexpr int E = KTraits<KT>::kPerLane;
+    constexpr int64_t STEP = 64 * E;
 #pragma unroll
     for (int r = 0; r < R; ++r) acc[r] = 0.0;
-    const int64_t npass = ld / STEP;
-    // Software pipeline: the R loads of the next pass are issued before the
-    // FMAs of the current one, so 2R 16-byte loads per lane are in flight;
-    // the scheduling barriers keep the compiler from interleaving each load
-    // with its use (which it does by default to save registers).
-    int64_t p = (gw * 13) % npass;
+    int64_t p = p0;
     V cur[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) cur[r] = ld_nt(rp[r] + p * 64);
@@ -229,14 +207,51 @@ __global__ void __launch_bounds__(KTraits<KT>::kWaves * 64)
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) acc[r] = wave_sum(acc[r]);
+}
+
+// One wavefront streams KTraits::kRows consecutive rows.  Each wavefront
+// starts its sweep over the columns at a different pass (rotation by a
+// multiple of its index): with all wavefronts reading the same column
+// offset of rows whose pitch is a large power of two, the loads would
+// concentrate on a few HBM channels (measured 4.9 vs 6.1 TB/s, FP64).
+template <int KT>
+__global__ void __launch_bounds__(KTraits<KT>::kWaves * 64)
+    k_rhs_matvec(RhsDev d, const void *__restrict__ Kp, int64_t ld, int64_t nrows, int64_t rowbase,
+                 const double *__restrict__ s, const double *__restrict__ tadd, int tcount,
+                 int64_t tstride, const double *__restrict__ u, double *__restrict__ du)
+{
+    using T = typename KTraits<KT>::T;
+    constexpr int E = KTraits<KT>::kPerLane;  // elements per 16-byte load
+    constexpr int R = KTraits<KT>::kRows;
+    constexpr int W = KTraits<KT>::kWaves;
+    constexpr int64_t STEP = 64 * E;          // columns per wavefront pass
+    const int lane = threadIdx.x & 63;
+    const int64_t gw = (int64_t)blockIdx.x * W + (threadIdx.x >> 6);
+    // rows [0, nrows) of the operator at Kp (pitch ld) are receivers
+    // rowbase + r; row r adds tadd[c * tstride + r] for c < tcount, parts of
+    // the operator applied elsewhere (the DFT K_ff of the coupled model, the
+    // grid convolution partials of K_mm), in that order
+    const int64_t row0 = gw * R;
+    if (row0 >= nrows) return;
+    using V = typename KVec<KT>::V;
+    const T *K = static_cast<const T *>(Kp);
+    const V *rp[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int64_t row = row0 + r < nrows ? row0 + r : nrows - 1;
+        rp[r] = reinterpret_cast<const V *>(K + row * ld) + lane;
+    }
+    double acc[R];
+    stream_rows<KT, R>(rp, s, ld / STEP, (gw * 13) % (ld / STEP), lane, acc);
     if (lane < R) {
         double y = acc[0];
 #pragma unroll
         for (int r = 1; r < R; ++r)
             if (lane == r) y = acc[r];
         if (row0 + lane < nrows) {
-            const int64_t row = rowbase + row0 + lane;
-            if (row < d.lay.nf) fault_update(d, row, tadd ? y + tadd[row] : y, u, du);
+            const int64_t loc = row0 + lane, row = rowbase + loc;
+            for (int c = 0; c < tcount; ++c) y += tadd[c * tstride + loc];
+            if (row < d.lay.nf) fault_update(d, row, y, u, du);
             else du[d.lay.sig + (row - d.lay.nf)] = y;
         }
     }
@@ -246,11 +261,19 @@ static inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) 
 
 template <int KT>
 static void launch_matvec(oq_rhs *h, const void *K, int64_t ld, int64_t nrows, int64_t rowbase,
-                          const double *s, const double *tadd, const double *u, double *du)
+                          const double *s, const double *tadd, int tcount, int64_t tstride,
+                          const double *u, double *du)
 {
     constexpr int rows = KTraits<KT>::kRows * KTraits<KT>::kWaves;
     hipLaunchKernelGGL(k_rhs_matvec<KT>, dim3(nblk(nrows, rows)), dim3(KTraits<KT>::kWaves * 64), 0,
-                       h->stream, h->dev, K, ld, nrows, rowbase, s, tadd, u, du);
+                       h->stream, h->dev, K, ld, nrows, rowbase, s, tadd, tcount, tstride, u, du);
+}
+
+template <int KT>
+static void launch_matvec(oq_rhs *h, const void *K, int64_t ld, int64_t nrows, int64_t rowbase,
+                          const double *s, const double *tadd, const double *u, double *du)
+{
+    launch_matvec<KT>(h, K, ld, nrows, rowbase, s, tadd, tadd ? 1 : 0, 0, u, du);
 }
 
 // ---------------- Toeplitz K_ff (fault-only problems, OQ_K_TOEPLITZ) ----------------
@@ -631,6 +654,228 @@ __global__ void __launch_bounds__(256) k_fft_inv(RhsDev d, const double *__restr
     }
 }
 
+// ---------------- grid K_mm (coupled DFT model, OQ_K_FFT_GRID) ----------------
+// Over a mantle whose cells form a uniform horizontal grid (nx x ny cells per
+// layer, nz layers, cell ix + nx (iy + ny iz)) the half-space stress at
+// receiver cell (jx, jy, jz) from source cell (ix, iy, iz) depends on
+// (jx - ix, jy - iy, jz, iz) only (the free surface is horizontal).  K_mm is
+// kept as tiles T[PO][PI] over output planes PO = k nz + jz and source
+// planes PI = p nz + iz (stress component k, eigenstrain component p), row
+// yy = jy - iy + ny - 1 in [0, 2ny - 1), and applied as 2D convolutions:
+//   out[PO][jy][jx] = sum_PI sum_{iy, ix} T[PO][PI][jy - iy][jx - ix] s[PI][iy][ix].
+// A tile row holds the offsets m = jx - ix at m + nxp for m in [-nxp, nxp)
+// (Wp = 2 nxp entries, nxp = nx rounded up to R), zero where |m| >= nx, so
+// that a block stages it with straight copies and the sliding window below
+// runs over nxp source columns (s zero-padded the same way) without bounds
+// tests.
+__host__ __device__ inline int grid_nxp(int64_t nx, int r) { return (int)((nx + r - 1) / r * r); }
+
+// Block (PO, chunk c) stages the tiles of its tp source planes and those
+// planes of s in LDS; thread (plane q, row jy, R consecutive jx) forms its
+// outputs with a register window over the tile row that slides by one entry
+// per source column (one LDS load and one broadcast load per R
+// multiply-adds, as in k_toep_partial); the tp plane sums are added in a
+// fixed order into part[c] (c-th partial of every mantle row).
+template <int R>
+__device__ __forceinline__ void mm_grid_conv_block(const double *__restrict__ T, const double *__restrict__ s,
+                                                   int nx, int ny, int nz, int tp, double *__restrict__ part,
+                                                   int PO, int c, double *gsh)
+{
+    const int npl = 6 * nz, nxy = nx * ny;
+    const int pi0 = c * tp, np = (pi0 + tp < npl ? pi0 + tp : npl) - pi0;
+    const int nxp = grid_nxp(nx, R), Wp = 2 * nxp, Wy = 2 * ny - 1, tile = Wy * Wp;
+    double *Ts = gsh;                 // tp x Wy x Wp
+    double *Ss = Ts + tp * tile;      // tp x ny x nxp
+    double *Os = Ss + tp * ny * nxp;  // tp x ny x nxp
+    {
+        const double2 *src = reinterpret_cast<const double2 *>(T + (size_t)(PO * npl + pi0) * tile);
+        double2 *dst = reinterpret_cast<double2 *>(Ts);
+        for (int x = threadIdx.x; x < np * tile / 2; x += blockDim.x) dst[x] = src[x];
+    }
+    for (int x = threadIdx.x; x < np * ny * nxp; x += blockDim.x) {
+        const int row = x / nxp, ix = x - row * nxp;  // row = q ny + iy
+        Ss[x] = ix < nx ? s[(size_t)pi0 * nxy + row * nx + ix] : 0.0;
+    }
+    __syncthreads();
+    const int nxg = nxp / R, U = ny * nxg;
+    for (int t = threadIdx.x; t < np * U; t += blockDim.x) {
+        const int q = t / U, g = t - q * U, jy = g / nxg, jx0 = (g - jy * nxg) * R;
+        double acc[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] = 0.0;
+        for (int iy = 0; iy < ny; ++iy) {
+            const double *Tr = Ts + (q * Wy + (jy - iy + ny - 1)) * Wp + nxp;  // Tr[m], m in [-nxp, nxp)
+            const double *Sr = Ss + (q * ny + iy) * nxp;
+            double w[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) w[r] = Tr[jx0 + r];  // column 0: slot r holds Tr[jx0 + r]
+            for (int ix = 0; ix < nxp; ix += R) {
+#pragma unroll
+                for (int kk = 0; kk < R; ++kk) {
+                    // column ix + kk: slot (r - kk) mod R holds Tr[jx0 + r - ix - kk]
+                    const double sv = Sr[ix + kk];
+#pragma unroll
+                    for (int r = 0; r < R; ++r) acc[r] += w[(r - kk + R) % R] * sv;
+                    w[(R - 1 - kk) % R] = Tr[jx0 - (ix + kk) - 1];
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) Os[(q * ny + jy) * nxp + jx0 + r] = acc[r];
+    }
+    __syncthreads();
+    for (int o = threadIdx.x; o < nxy; o += blockDim.x) {
+        const int jy = o / nx, jx = o - jy * nx;
+        double a = 0.0;
+        for (int q = 0; q < np; ++q) a += Os[(q * ny + jy) * nxp + jx];
+        part[((size_t)c * npl + PO) * nxy + o] = a;
+    }
+}
+
+// The padded tiles from the pair kernel's block over the shifted receivers
+// (receiver xx + Wx (yy + Wy jz), Wx = 2nx - 1, offset m = xx - (nx - 1);
+// row k nrec + receiver) and the source column (cells (0, 0, iz), column
+// p nz + iz, row pitch 6 nz).
+__global__ void __launch_bounds__(256) k_mm_grid_table(const double *__restrict__ raw, int64_t nx,
+                                                       int64_t ny, int64_t nz, int64_t nxp,
+                                                       double *__restrict__ T)
+{
+    const int64_t Wx = 2 * nx - 1, Wy = 2 * ny - 1, Wp = 2 * nxp, npl = 6 * nz, nrec = Wx * Wy * nz;
+    const int64_t n = npl * npl * Wy * Wp;
+    const int64_t step = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += step) {
+        const int64_t xp = t % Wp, r1 = t / Wp, yy = r1 % Wy, r2 = r1 / Wy;
+        const int64_t PI = r2 % npl, PO = r2 / npl, k = PO / nz, jz = PO % nz;
+        const int64_t m = xp - nxp;
+        T[t] = (m > -nx && m < nx) ? raw[(k * nrec + (m + nx - 1) + Wx * (yy + Wy * jz)) * npl + PI] : 0.0;
+    }
+}
+
+// The dense K_mm block (row pitch ld) from the tiles (operator expansion).
+__global__ void __launch_bounds__(256) k_expand_mm_grid(const double *__restrict__ T, int64_t nx,
+                                                        int64_t ny, int64_t nz, int64_t nxp,
+                                                        double *__restrict__ k, int64_t ld)
+{
+    const int64_t nxy = nx * ny, m6 = 6 * nz * nxy, npl = 6 * nz, Wp = 2 * nxp, Wy = 2 * ny - 1;
+    const int64_t step = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < m6 * m6; t += step) {
+        const int64_t row = t / m6, col = t % m6;
+        const int64_t PO = row / nxy, o = row % nxy, PI = col / nxy, o2 = col % nxy;
+        const int64_t jy = o / nx, jx = o % nx, iy = o2 / nx, ix = o2 % nx;
+        k[row * ld + col] = T[((PO * npl + PI) * Wy + (jy - iy + ny - 1)) * Wp + (jx - ix + nxp)];
+    }
+}
+
+// The grid model's dense blocks: wavefronts [0, nfw) stream the fault rows'
+// K_fm (pitch ldm) over s_m in S column chunks of cw columns into partial
+// sums fpart[(1 + c) nf + row] (fpart[row] holds the K_ff traction rates),
+// the rest the mantle rows' K_mf (pitch ldf) over s_f into mvout.  cw is
+// chosen near ldf, so every wavefront streams about the same bytes.
+struct GridMv {
+    const double *Kf, *K, *sf, *sm;
+    int64_t ldm, ldf, cw;
+    int S;
+    double *fpart, *mvout;
+};
+
+__device__ __forceinline__ void grid_matvec_wave(const RhsDev &d, const GridMv &a, int64_t gw, int lane)
+{
+    constexpr int KT = OQ_K_FP64;
+    constexpr int R = KTraits<KT>::kRows;
+    constexpr int64_t STEP = 64 * KTraits<KT>::kPerLane;
+    using V = typename KVec<KT>::V;
+    const int64_t nf = d.lay.nf, m6 = 6 * d.lay.ne, nfw = (nf + R - 1) / R * a.S;
+    const V *rp[R];
+    const double *s;
+    int64_t row0, nrows, npass, c = -1;
+    if (gw < nfw) {
+        c = gw % a.S;
+        row0 = (gw / a.S) * R;
+        nrows = nf;
+        const int64_t c0 = c * a.cw, ncol = (c0 + a.cw < a.ldm ? c0 + a.cw : a.ldm) - c0;
+        npass = ncol / STEP;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int64_t row = row0 + r < nf ? row0 + r : nf - 1;
+            rp[r] = reinterpret_cast<const V *>(a.Kf + row * a.ldm + c0) + lane;
+        }
+        s = a.sm + c0;
+    } else {
+        row0 = (gw - nfw) * R;
+        if (row0 >= m6) return;
+        nrows = m6;
+        npass = a.ldf / STEP;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int64_t row = row0 + r < m6 ? row0 + r : m6 - 1;
+            rp[r] = reinterpret_cast<const V *>(a.K + row * a.ldf) + lane;
+        }
+        s = a.sf;
+    }
+    double acc[R];
+    stream_rows<KT, R>(rp, s, npass, (gw * 13) % npass, lane, acc);
+    if (lane < R) {
+        double y = acc[0];
+#pragma unroll
+        for (int r = 1; r < R; ++r)
+            if (lane == r) y = acc[r];
+        const int64_t loc = row0 + lane;
+        if (loc < nrows) {
+            if (c >= 0) a.fpart[(1 + c) * nf + loc] = y;
+            else a.mvout[loc] = y;
+        }
+    }
+}
+
+// One launch for the grid model's O(N^2) work: nconv convolution blocks
+// (PO = b mod 6nz, chunk b / 6nz) and the matvec blocks of four wavefronts,
+// interleaved one convolution block to two matvec blocks while both last,
+// so the LDS/FP64 work of the convolution runs beside the HBM stream.
+template <int R>
+__global__ void __launch_bounds__(256) k_grid_fused(RhsDev d, GridMv a, const double *__restrict__ T,
+                                                    int nx, int ny, int nz, int tp, double *__restrict__ mpart,
+                                                    int nconv, int nmv)
+{
+    extern __shared__ double gsh[];
+    const int b = blockIdx.x, nmix = nconv < nmv / 2 ? nconv : nmv / 2;
+    int conv = -1, mv = -1;
+    if (b < 3 * nmix) {
+        if (b % 3 == 0) conv = b / 3;
+        else mv = 2 * (b / 3) + b % 3 - 1;
+    } else if (b - 3 * nmix < nconv - nmix) {
+        conv = nmix + (b - 3 * nmix);
+    } else {
+        mv = 2 * nmix + (b - 3 * nmix - (nconv - nmix));
+    }
+    if (conv >= 0) {
+        const int npl = 6 * nz;
+        mm_grid_conv_block<R>(T, a.sm, nx, ny, nz, tp, mpart, conv % npl, conv / npl, gsh);
+        return;
+    }
+    grid_matvec_wave(d, a, (int64_t)mv * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
+}
+
+// Fault rows: the K_ff rates and K_fm chunk partials, then the friction
+// update; mantle rows: du.sigma = K_mf s_f + the convolution partials.
+__global__ void __launch_bounds__(256) k_grid_finish(RhsDev d, const double *__restrict__ fpart, int nfp,
+                                                     const double *__restrict__ mvout,
+                                                     const double *__restrict__ mpart, int nchunk,
+                                                     const double *__restrict__ u, double *__restrict__ du)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nf = d.lay.nf, m6 = 6 * d.lay.ne;
+    if (q < nf) {
+        double tau = 0.0;
+        for (int c = 0; c < nfp; ++c) tau += fpart[c * nf + q];
+        fault_update(d, q, tau, u, du);
+    } else if (q < nf + m6) {
+        const int64_t r = q - nf;
+        double y = mvout[r];
+        for (int c = 0; c < nchunk; ++c) y += mpart[c * m6 + r];
+        du[d.lay.sig + r] = y;
+    }
+}
+
 static inline size_t fft_lds(int64_t nx) { return (size_t)(3 * fft_len(nx)) * 8; }
 
 // LDS bytes of a block with lchunk dip rows
@@ -666,6 +911,32 @@ int rhs_enqueue_matvec(oq_rhs *h, const double *u, double *du)
                            h->nx, h->nxi, L, bits, h->ftw.as<double>(), h->fsh.as<double>());
         hipLaunchKernelGGL(k_fft_mid, dim3((unsigned)(L / 2 + 1)), dim3(512), 0, h->stream, h->fg.as<double>(),
                            h->fsh.as<double>(), L / 2 + 1, h->nxi, h->fth.as<double>());
+        if (h->mmgrid) {
+            // K_ff rates into fpart[0, nf); the grid convolution of K_mm, K_fm in column chunks
+            // and K_mf in one launch; then the friction update and du.sigma
+            const MmGrid &g = h->grid;
+            hipLaunchKernelGGL(k_fft_inv, dim3((unsigned)h->nxi), dim3(256), fft_lds(h->nx), h->stream, h->dev,
+                               h->fth.as<double>(), h->nx, L, bits, h->ftw.as<double>(), u, h->fpart.as<double>());
+            constexpr int R = KTraits<OQ_K_FP64>::kRows, W = KTraits<OQ_K_FP64>::kWaves;
+            static_assert(W == 4, "k_grid_fused runs four matvec wavefronts per block");
+            const int64_t waves = (h->nf + R - 1) / R * g.fs + (6 * h->ne + R - 1) / R;
+            const int nconv = (int)(6 * g.nz) * g.nchunk, nmv = (int)nblk(waves, W);
+            const GridMv a = {h->Kf.as<double>(), h->K.as<double>(), h->s64.as<double>(), h->sm.as<double>(),
+                              h->ldm, h->ldf, g.fcw, g.fs, h->fpart.as<double>(), h->mvout.as<double>()};
+            if (g.r == 8)
+                hipLaunchKernelGGL(k_grid_fused<8>, dim3((unsigned)(nconv + nmv)), dim3(256), g.lds, h->stream,
+                                   h->dev, a, h->Tmm.as<double>(), (int)g.nx, (int)g.ny, (int)g.nz, g.tp,
+                                   h->mpart.as<double>(), nconv, nmv);
+            else
+                hipLaunchKernelGGL(k_grid_fused<4>, dim3((unsigned)(nconv + nmv)), dim3(256), g.lds, h->stream,
+                                   h->dev, a, h->Tmm.as<double>(), (int)g.nx, (int)g.ny, (int)g.nz, g.tp,
+                                   h->mpart.as<double>(), nconv, nmv);
+            hipLaunchKernelGGL(k_grid_finish, dim3(nblk(h->nf + 6 * h->ne, 256)), dim3(256), 0, h->stream, h->dev,
+                               h->fpart.as<double>(), g.fs + 1, h->mvout.as<double>(), h->mpart.as<double>(),
+                               g.nchunk, u, du);
+            OQ_HIP(hipGetLastError());
+            return OQ_OK;
+        }
         hipLaunchKernelGGL(k_fft_inv, dim3((unsigned)h->nxi), dim3(256), fft_lds(h->nx), h->stream, h->dev,
                            h->fth.as<double>(), h->nx, L, bits, h->ftw.as<double>(), u, h->tauff.as<double>());
         launch_matvec<OQ_K_FP64>(h, h->Kf.p, h->ldm, h->nf, 0, h->sm.as<double>(), h->tauff.as<double>(), u, du);
@@ -803,6 +1074,7 @@ __global__ void __launch_bounds__(256) k_from_lowp(const void *src, int kt, doub
 static int alloc_common(oq_rhs *h, int device, int64_t nx, int64_t nxi, int64_t ne, int kdtype)
 {
     OQ_REQUIRE(nx > 0 && nxi > 0 && ne >= 0, "sizes must be positive");
+    OQ_REQUIRE(kdtype != OQ_K_FFT_GRID, "OQ_K_FFT_GRID needs the mantle mesh: use oq_rhs_create_from_mesh");
     OQ_REQUIRE(kdtype == OQ_K_FP64 || kdtype == OQ_K_FP32 || kdtype == OQ_K_BF16 ||
                    kdtype == OQ_K_TOEPLITZ || kdtype == OQ_K_FFT,
                "kdtype must be OQ_K_FP64, OQ_K_FP32, OQ_K_BF16, OQ_K_TOEPLITZ or OQ_K_FFT");
@@ -925,6 +1197,135 @@ static int finalize_toeplitz(oq_rhs *h, DevBuf &st)
     return h->tpart.alloc((size_t)(nc * h->nf) * 8);
 }
 
+// Grid mantle (OQ_K_FFT_GRID): the cells must be ix + nx (iy + ny iz) with x
+// and y centres on uniform spacings hx, hy, the same extents within a layer
+// and the same reference-point offsets (to 1e-11 of the largest cell
+// extent; a generated box is exact to rounding).
+static int mm_grid_detect(const oq_hex8_mesh *m, MmGrid &g, double &hx, double &hy)
+{
+    const int64_t ne = m->ne;
+    const double ext = fmax(fabs(m->dx[0]), fmax(fabs(m->dy[0]), fabs(m->dz[0])));
+    const double tol = 1e-11 * ext;
+    auto near = [tol](double a, double b) { return fabs(a - b) <= tol; };
+    int64_t nx = 1;
+    while (nx < ne && near(m->cy[nx], m->cy[0]) && near(m->cz[nx], m->cz[0])) ++nx;
+    int64_t nxy = nx;
+    while (nxy < ne && near(m->cz[nxy], m->cz[0])) ++nxy;
+    OQ_REQUIRE(nxy % nx == 0 && ne % nxy == 0,
+               "OQ_K_FFT_GRID: the mantle cells do not form a grid (x fastest, then y, then layer)");
+    const int64_t ny = nxy / nx, nz = ne / nxy;
+    hx = nx > 1 ? (m->cx[nx - 1] - m->cx[0]) / (double)(nx - 1) : 0.0;
+    hy = ny > 1 ? (m->cy[(ny - 1) * nx] - m->cy[0]) / (double)(ny - 1) : 0.0;
+    for (int64_t c = 0; c < ne; ++c) {
+        const int64_t ix = c % nx, iy = (c / nx) % ny, b = (c / nxy) * nxy;
+        const bool ok = near(m->cx[c], m->cx[0] + (double)ix * hx) && near(m->cy[c], m->cy[0] + (double)iy * hy) &&
+                        near(m->cz[c], m->cz[b]) && near(m->dx[c], m->dx[0]) && near(m->dy[c], m->dy[0]) &&
+                        near(m->dz[c], m->dz[b]) && near(m->qx[c] - m->cx[c], m->qx[0] - m->cx[0]) &&
+                        near(m->qy[c] - m->cy[c], m->qy[0] - m->cy[0]) &&
+                        near(m->qz[c] - m->cz[c], m->qz[b] - m->cz[b]);
+        OQ_REQUIRE(ok, "OQ_K_FFT_GRID: the mantle cells are not a uniform horizontal grid (equal spacing "
+                       "and extents along x and y, the same layers under every column)");
+    }
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
+    return OQ_OK;
+}
+
+// Shape of the convolution blocks: 256 threads over (source planes x outputs
+// / R), at most 64 KB of LDS unless one plane needs more.
+static int mm_grid_shape(MmGrid &g)
+{
+    g.r = g.nx >= 24 ? 8 : 4;
+    g.nxp = grid_nxp(g.nx, g.r);
+    const int64_t Wy = 2 * g.ny - 1, Wp = 2 * g.nxp, nxg = g.nxp / g.r;
+    const int64_t U = g.ny * nxg, per = Wy * Wp + 2 * g.ny * g.nxp;
+    OQ_REQUIRE(per * 8 <= 160 * 1024,
+               "OQ_K_FFT_GRID: one source plane of the grid kernel exceeds the LDS of a workgroup "
+               "(at most about 60 x 60 cells per layer)");
+    int64_t tp = 256 / U;
+    if (tp < 1) tp = 1;
+    if (tp > 6 * g.nz) tp = 6 * g.nz;
+    while (tp > 1 && tp * per * 8 > 64 * 1024) --tp;
+    g.tp = (int)tp;
+    g.nchunk = (int)((6 * g.nz + tp - 1) / tp);
+    g.threads = 256;
+    g.lds = (size_t)(tp * per * 8);
+    if (g.r == 8)
+        OQ_HIP(hipFuncSetAttribute((const void *)k_grid_fused<8>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)g.lds));
+    else
+        OQ_HIP(hipFuncSetAttribute((const void *)k_grid_fused<4>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)g.lds));
+    return OQ_OK;
+}
+
+// Column chunks of the fault rows' K_fm pass (pitch ldm) near the mantle
+// rows' length ldf, in whole passes of 128 columns.
+static void mm_grid_chunks(MmGrid &g, int64_t ldm, int64_t ldf)
+{
+    int64_t S = (ldm + ldf / 2) / ldf;
+    if (S < 1) S = 1;
+    const int64_t cw = ((ldm + S - 1) / S + 127) / 128 * 128;
+    g.fcw = cw;
+    g.fs = (int)((ldm + cw - 1) / cw);
+}
+
+// h->Tmm: the pair kernel (GF.jl:262-289) over receivers shifted by every
+// horizontal offset from the source column (cells (0, 0, iz)), rearranged
+// into the convolution tiles; h->mpart for the partial sums.
+static int mm_grid_build(oq_hex8_mesh const *m, double hx, double hy, const QuadUpload &qu, double lambda,
+                         double mu, oq_rhs *h)
+{
+    const MmGrid &g = h->grid;
+    const int64_t Wx = 2 * g.nx - 1, Wy = 2 * g.ny - 1, nxy = g.nx * g.ny, nrec = Wx * Wy * g.nz;
+    std::vector<double> r[9];
+    for (auto &v : r) v.resize((size_t)nrec);
+    for (int64_t jz = 0; jz < g.nz; ++jz)
+        for (int64_t yy = 0; yy < Wy; ++yy)
+            for (int64_t xx = 0; xx < Wx; ++xx) {
+                const size_t i = (size_t)(xx + Wx * (yy + Wy * jz)), b = (size_t)(jz * nxy);
+                const double cx = m->cx[0] + (double)(xx - (g.nx - 1)) * hx;
+                const double cy = m->cy[0] + (double)(yy - (g.ny - 1)) * hy;
+                r[0][i] = cx;
+                r[1][i] = cy;
+                r[2][i] = m->cz[b];
+                r[3][i] = cx + (m->qx[0] - m->cx[0]);
+                r[4][i] = cy + (m->qy[0] - m->cy[0]);
+                r[5][i] = m->qz[b];
+                r[6][i] = m->dx[0];
+                r[7][i] = m->dy[0];
+                r[8][i] = m->dz[b];
+            }
+    std::vector<double> c[9];
+    const double *src[9] = {m->cx, m->cy, m->cz, m->qx, m->qy, m->qz, m->dx, m->dy, m->dz};
+    for (int a = 0; a < 9; ++a)
+        for (int64_t iz = 0; iz < g.nz; ++iz) c[a].push_back(src[a][iz * nxy]);
+    const oq_hex8_mesh rm = {r[0].data(), r[1].data(), r[2].data(), r[3].data(), r[4].data(),
+                             r[5].data(), r[6].data(), r[7].data(), r[8].data(), nrec};
+    const oq_hex8_mesh sm = {c[0].data(), c[1].data(), c[2].data(), c[3].data(), c[4].data(),
+                             c[5].data(), c[6].data(), c[7].data(), c[8].data(), g.nz};
+    Hex8Upload ru, su;
+    DevBuf raw;
+    const int64_t npl = 6 * g.nz, nt = npl * npl * Wy * 2 * g.nxp;
+    int rc = ru.load(&rm);
+    if (!rc) rc = su.load(&sm);
+    if (!rc) rc = raw.alloc((size_t)(6 * nrec * npl) * 8);
+    if (!rc) rc = h->Tmm.alloc((size_t)nt * 8);
+    if (!rc) rc = h->mpart.alloc((size_t)(h->grid.nchunk * 6 * h->ne) * 8);
+    if (!rc) rc = h->fpart.alloc((size_t)((h->grid.fs + 1) * h->nf) * 8);
+    if (!rc) rc = h->mvout.alloc((size_t)(6 * h->ne) * 8);
+    if (!rc)
+        rc = launch_gf_mm_pairs(ru.dev, su.dev, qu.c.as<double>(), qu.w.as<double>(), qu.nq, lambda, mu,
+                                raw.as<double>(), npl, 1, h->stream);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_mm_grid_table, dim3(grid_capped(nt, 256)), dim3(256), 0, h->stream, raw.as<double>(),
+                       g.nx, g.ny, g.nz, (int64_t)g.nxp, h->Tmm.as<double>());
+    OQ_HIP(hipGetLastError());
+    OQ_HIP(hipStreamSynchronize(h->stream));
+    return OQ_OK;
+}
+
 // Coupled model with the DFT K_ff (OQ_K_FFT, ne > 0): keeps st, sets up the
 // DFT, and allocates (zeroed) the fault rows' K_fm (nf x ldm), the mantle
 // rows [K_mf K_mm] (6ne x ld, in h->K), the mantle source copy and the K_ff
@@ -938,7 +1339,10 @@ static int fft_coupled_alloc(oq_rhs *h, DevBuf &st)
     int rc = fft_setup(h, h->stk.as<double>());
     const int64_t m6 = 6 * h->ne;
     h->ldm = (m6 + kPad - 1) / kPad * kPad;
-    const size_t bf = (size_t)(h->nf * h->ldm) * 8, bm = (size_t)(m6 * h->ld) * 8;
+    h->ldf = (h->nf + kPad - 1) / kPad * kPad;
+    if (h->mmgrid) mm_grid_chunks(h->grid, h->ldm, h->ldf);
+    // grid mantle: the mantle rows hold only K_mf (6ne x ldf)
+    const size_t bf = (size_t)(h->nf * h->ldm) * 8, bm = (size_t)(m6 * (h->mmgrid ? h->ldf : h->ld)) * 8;
     if (!rc) rc = h->Kf.alloc(bf);
     if (!rc) rc = h->K.alloc(bm);
     if (!rc) rc = h->sm.alloc((size_t)h->ldm * 8);
@@ -1131,8 +1535,22 @@ int oq_rhs_create_from_mesh(int device, const oq_fault_mesh *mf, const oq_hex8_m
     const int64_t ne = ma ? ma->ne : 0;
     if (ma && (rc = check_hex8(ma))) return rc;
     if (ma && (rc = check_quad(quad))) return rc;
+    // OQ_K_FFT_GRID = the coupled OQ_K_FFT model with K_mm on the mantle grid
+    const bool grid = kdtype == OQ_K_FFT_GRID;
+    MmGrid g;
+    double hx = 0.0, hy = 0.0;
+    if (grid) {
+        OQ_REQUIRE(ne > 0, "OQ_K_FFT_GRID is for the coupled model (a hex8 mantle)");
+        if ((rc = mm_grid_detect(ma, g, hx, hy))) return rc;
+        kdtype = OQ_K_FFT;
+    }
     oq_rhs *h = new oq_rhs();
     rc = alloc_common(h, device, mf->nx, mf->nxi, ne, kdtype);
+    if (!rc && grid) {
+        h->mmgrid = true;
+        h->grid = g;
+        rc = mm_grid_shape(h->grid);
+    }
     if (!rc) rc = upload_props(h, fault, visc);
     FaultUpload fu;
     Hex8Upload hu;
@@ -1151,8 +1569,9 @@ int oq_rhs_create_from_mesh(int device, const oq_fault_mesh *mf, const oq_hex8_m
         if (!rc) rc = launch_gf_fm(hu.dev, fu.dev, lambda, mu, ftype, h->Kf.as<double>(), h->ldm, 1, h->stream);
         if (!rc)
             rc = launch_gf_mf(fu.dev, hu.dev, qu.c.as<double>(), qu.w.as<double>(), qu.nq, lambda, mu, ftype,
-                              (int)nrept, h->K.as<double>(), ld, 1, h->stream);
-        if (!rc)
+                              (int)nrept, h->K.as<double>(), h->mmgrid ? h->ldf : ld, 1, h->stream);
+        if (!rc && h->mmgrid) rc = mm_grid_build(ma, hx, hy, qu, lambda, mu, h);
+        else if (!rc)
             rc = launch_gf_mm(hu.dev, qu.c.as<double>(), qu.w.as<double>(), qu.nq, lambda, mu,
                               h->K.as<double>() + nf, ld, 1, h->stream);
         if (!rc && hipStreamSynchronize(h->stream) != hipSuccess) {
@@ -1312,7 +1731,7 @@ int oq_rhs_info(oq_rhs *h, int64_t *nstate, int64_t *n, int64_t *kbytes)
     OQ_REQUIRE(h != nullptr, "handle is NULL");
     if (nstate) *nstate = h->lay.n;
     if (n) *n = h->N;
-    if (kbytes) *kbytes = (int64_t)(h->K.bytes + h->Kf.bytes + h->stk.bytes);
+    if (kbytes) *kbytes = (int64_t)(h->K.bytes + h->Kf.bytes + h->stk.bytes + h->Tmm.bytes);
     return OQ_OK;
 }
 
@@ -1333,8 +1752,18 @@ int oq_rhs_get_k(oq_rhs *h, double *k)
             return rc;
         OQ_HIP(hipMemcpy2DAsync(tmp.as<double>() + nf, (size_t)h->ld * 8, h->Kf.p, (size_t)h->ldm * 8,
                                 (size_t)m6 * 8, (size_t)nf, hipMemcpyDeviceToDevice, h->stream));
-        OQ_HIP(hipMemcpy2DAsync(tmp.as<double>() + nf * h->ld, (size_t)h->ld * 8, h->K.p, (size_t)h->ld * 8,
-                                (size_t)h->N * 8, (size_t)m6, hipMemcpyDeviceToDevice, h->stream));
+        if (h->mmgrid) {
+            OQ_HIP(hipMemcpy2DAsync(tmp.as<double>() + nf * h->ld, (size_t)h->ld * 8, h->K.p, (size_t)h->ldf * 8,
+                                    (size_t)nf * 8, (size_t)m6, hipMemcpyDeviceToDevice, h->stream));
+            const MmGrid &g = h->grid;
+            hipLaunchKernelGGL(k_expand_mm_grid, dim3(grid_capped(m6 * m6, 256)), dim3(256), 0, h->stream,
+                               h->Tmm.as<double>(), g.nx, g.ny, g.nz, (int64_t)g.nxp,
+                               tmp.as<double>() + nf * h->ld + nf, h->ld);
+            OQ_HIP(hipGetLastError());
+        } else {
+            OQ_HIP(hipMemcpy2DAsync(tmp.as<double>() + nf * h->ld, (size_t)h->ld * 8, h->K.p, (size_t)h->ld * 8,
+                                    (size_t)h->N * 8, (size_t)m6, hipMemcpyDeviceToDevice, h->stream));
+        }
         src = tmp.as<double>();
     } else if (h->kdtype == OQ_K_TOEPLITZ || h->kdtype == OQ_K_FFT) {
         int rc = tmp.alloc((size_t)n * 8);

@@ -169,6 +169,22 @@ __device__ __forceinline__ void fault_update(const RhsDev &d, int64_t i, double 
     du[d.lay.del + i] = v;
 }
 
+// K_mm of a mantle whose cells form a uniform horizontal grid (cell
+// c = ix + nx (iy + ny iz)): entry [(k, jz, jy, jx), (p, iz, iy, ix)] depends
+// on (k, jz, p, iz) and the offsets (jy - iy, jx - ix) only, so it is kept as
+// 36 nz^2 tiles of (2ny - 1) x (2nx - 1) and applied as 2D convolutions.
+struct MmGrid {
+    int64_t nx = 0, ny = 0, nz = 0;
+    int r = 0;        // outputs per thread along x (register window)
+    int nxp = 0;      // nx rounded up to r (tile rows hold 2 nxp entries)
+    int tp = 0;       // source planes (p, iz) per workgroup
+    int nchunk = 0;   // workgroups per output plane (k, jz) = partial sums per entry
+    int threads = 0;  // workgroup size (256)
+    size_t lds = 0;   // dynamic LDS bytes
+    int fs = 0;       // column chunks of the fault rows' K_fm pass
+    int64_t fcw = 0;  // columns per chunk
+};
+
 }  // namespace oq
 
 struct oq_rhs {
@@ -191,6 +207,15 @@ struct oq_rhs {
     // Toeplitz kernel; tauff the K_ff traction rates
     oq::DevBuf Kf, sm, stk, tauff;
     int64_t ldm = 0;
+    // OQ_K_FFT_GRID (the coupled DFT model with a grid mantle): K holds only
+    // the mantle rows' K_mf (6ne x ldf); K_mm is the grid kernel table Tmm,
+    // applied by the convolution blocks of k_grid_fused into nchunk partial
+    // sums per mantle row (mpart, nchunk x 6ne) that k_grid_finish adds
+    oq::DevBuf Tmm, mpart, fpart, mvout;  // fpart: K_ff rates + K_fm chunk partials ((fs + 1) x nf);
+                                          // mvout: K_mf s_f (6ne)
+    bool mmgrid = false;
+    int64_t ldf = 0;
+    oq::MmGrid grid{};
     oq::RhsDev dev{};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };

@@ -24,7 +24,7 @@ _vp = ctypes.c_void_p
 OQ_OK = 0
 OQ_ERR_ARG = 1
 OQ_ERR_STATE = 3
-OQ_K_FP64, OQ_K_FP32, OQ_K_BF16, OQ_K_TOEPLITZ, OQ_K_FP16X2, OQ_K_FFT = 0, 1, 2, 3, 4, 5
+OQ_K_FP64, OQ_K_FP32, OQ_K_BF16, OQ_K_TOEPLITZ, OQ_K_FP16X2, OQ_K_FFT, OQ_K_FFT_GRID = 0, 1, 2, 3, 4, 5, 6
 RETCODES = {0: "Running", 1: "Success", 2: "MaxIters", 3: "DtLessThanMin", 4: "NonFinite"}
 
 

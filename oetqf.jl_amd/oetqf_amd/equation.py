@@ -149,6 +149,9 @@ class ODEProblem:
 
 _KDTYPES = {"fp64": _lib.OQ_K_FP64, "fp32": _lib.OQ_K_FP32, "bf16": _lib.OQ_K_BF16,
             "toeplitz": _lib.OQ_K_TOEPLITZ, "fft": _lib.OQ_K_FFT}
+# assemble_from_mesh only: "fft_grid" = "fft" with K_mm applied as 2D
+# convolutions over a mantle whose cells form a uniform horizontal grid
+_KDTYPES_MESH = dict(_KDTYPES, fft_grid=_lib.OQ_K_FFT_GRID)
 
 
 def assemble(*args, se=DieterichStateLaw(), kdtype="fp64", device=0, **kwargs):
@@ -205,9 +208,11 @@ def assemble_from_mesh(mf, ma, lam, mu, pf, pa, u0, tspan, ftype=StrikeSlip(), n
                        buffer_ratio=0.0, qtype="Gauss1", kdtype="fp64", device=0, dila=None):
     """The four stress_greens_function calls followed by assemble, with the
     Green's functions built straight into the device operator (no host copy
-    of K).  ma = None gives the fault-only problem."""
-    if kdtype not in _KDTYPES:
-        raise ValueError("kdtype must be one of %s" % sorted(_KDTYPES))
+    of K).  ma = None gives the fault-only problem.  kdtype "fft_grid"
+    (coupled, mantle cells on a uniform horizontal grid): K_ff by DFT and
+    K_mm as 2D convolutions of its per-offset kernel; K_fm, K_mf dense."""
+    if kdtype not in _KDTYPES_MESH:
+        raise ValueError("kdtype must be one of %s" % sorted(_KDTYPES_MESH))
     keep = _Keep()
     L = _lib.load()
     h = ctypes.c_void_p()
@@ -220,12 +225,12 @@ def assemble_from_mesh(mf, ma, lam, mu, pf, pa, u0, tspan, ftype=StrikeSlip(), n
         _lib.check(L.oq_rhs_create_from_mesh(device, ctypes.byref(a), ctypes.byref(b), lam, mu,
                                              _ftype(ftype), nrept, buffer_ratio, ctypes.byref(q),
                                              ctypes.byref(rs), ctypes.byref(vs),
-                                             _KDTYPES[kdtype], ctypes.byref(h)))
+                                             _KDTYPES_MESH[kdtype], ctypes.byref(h)))
         ne = ma.ne
     else:
         _lib.check(L.oq_rhs_create_from_mesh(device, ctypes.byref(a), None, lam, mu,
                                              _ftype(ftype), nrept, buffer_ratio, None,
-                                             ctypes.byref(rs), None, _KDTYPES[kdtype],
+                                             ctypes.byref(rs), None, _KDTYPES_MESH[kdtype],
                                              ctypes.byref(h)))
         ne = 0
     f = RHS(h, mf.nx, mf.nxi, ne, keep, dila=dila)

@@ -129,3 +129,38 @@ def test_bench_refuses_multi_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_fft_grid_needs_a_grid_mantle(oq, rng):
+    """OQ_K_FFT_GRID (K_mm as 2D convolutions over a mantle on a uniform
+    horizontal grid) is refused before any device work: by oq_rhs_create,
+    which has no mesh, and for a mantle refined along x ("Bump").  A box with
+    graded layers passes the grid test (and then fails here only for want of
+    a device)."""
+    from helpers import rs_props, small_fault, visc_props
+    from oetqf_amd import _lib
+    lib = oq.load_library()
+    a = np.ones(4)
+    rs = _lib.RSProps(_lib.ptr(a), _lib.ptr(a), _lib.ptr(a), _lib.ptr(a), 1.0, 1e-9, 0.6, 1e-6)
+    h = ctypes.c_void_p()
+    st = np.zeros(8)
+    rc = lib.oq_rhs_create(0, 2, 2, 0, ctypes.byref(rs), None, _lib.ptr(st), 0, None, None, None,
+                           _lib.OQ_K_FFT_GRID, ctypes.byref(h))
+    assert rc == _lib.OQ_ERR_ARG and not h.value
+    assert "oq_rhs_create_from_mesh" in lib.oq_last_error().decode()
+    mf = small_fault(90.0)
+    pf = rs_props(mf, rng)
+
+    def build(ma):
+        pa = visc_props(ma, rng, 0.0)
+        u = oq.ArrayPartition(np.ones((mf.nx, mf.nxi)), np.ones((mf.nx, mf.nxi)), np.zeros((ma.ne, 6)),
+                              np.zeros((ma.ne, 6)), np.zeros((mf.nx, mf.nxi)))
+        return oq.assemble_from_mesh(mf, ma, 3e10, 3e10, pf, pa, u, (0.0, 1.0), kdtype="fft_grid")
+
+    with pytest.raises(ValueError, match="uniform horizontal grid"):
+        build(oq.hex8_box(-15e3, -5e3, -12e3, 30e3, 10e3, -20e3, 6, 2, 2, rfx=0.5))
+    with pytest.raises(Exception) as ei:
+        build(oq.hex8_box(-15e3, -5e3, -12e3, 30e3, 10e3, -20e3, 6, 2, 3, rfzh=[1.0, 1.5, 2.25]))
+    assert "grid" not in str(ei.value)
+    with pytest.raises(ValueError, match="kdtype"):
+        oq.assemble(np.zeros((2, 2, 2)), pf, None, (0.0, 1.0), kdtype="fft_grid")
