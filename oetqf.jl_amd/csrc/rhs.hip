@@ -663,11 +663,12 @@ __global__ void __launch_bounds__(256) k_fft_inv(RhsDev d, const double *__restr
 // planes PI = p nz + iz (stress component k, eigenstrain component p), row
 // yy = jy - iy + ny - 1 in [0, 2ny - 1), and applied as 2D convolutions:
 //   out[PO][jy][jx] = sum_PI sum_{iy, ix} T[PO][PI][jy - iy][jx - ix] s[PI][iy][ix].
-// A tile row holds the offsets m = jx - ix at m + nxp for m in [-nxp, nxp)
-// (Wp = 2 nxp entries, nxp = nx rounded up to R), zero where |m| >= nx, so
-// that a block stages it with straight copies and the sliding window below
-// runs over nxp source columns (s zero-padded the same way) without bounds
-// tests.
+// A tile row holds the offsets m = jx - ix at m + nxp for m in [-nxp, nxp]
+// (Wp = 2 nxp + 1 entries, nxp = nx rounded up to R), zero where |m| >= nx,
+// so that a block stages it with straight copies and the sliding window
+// below runs over nxp source columns (s zero-padded the same way) without
+// bounds tests; the odd pitch puts the rows of the 16 lanes of one LDS
+// access phase (4 rows x 4 windows) on distinct banks.
 __host__ __device__ inline int grid_nxp(int64_t nx, int r) { return (int)((nx + r - 1) / r * r); }
 
 // Block (PO, chunk c) stages the tiles of its tp source planes and those
@@ -683,14 +684,13 @@ __device__ __forceinline__ void mm_grid_conv_block(const double *__restrict__ T,
 {
     const int npl = 6 * nz, nxy = nx * ny;
     const int pi0 = c * tp, np = (pi0 + tp < npl ? pi0 + tp : npl) - pi0;
-    const int nxp = grid_nxp(nx, R), Wp = 2 * nxp, Wy = 2 * ny - 1, tile = Wy * Wp;
+    const int nxp = grid_nxp(nx, R), Wp = 2 * nxp + 1, Wy = 2 * ny - 1, tile = Wy * Wp;
     double *Ts = gsh;                 // tp x Wy x Wp
     double *Ss = Ts + tp * tile;      // tp x ny x nxp
     double *Os = Ss + tp * ny * nxp;  // tp x ny x nxp
     {
-        const double2 *src = reinterpret_cast<const double2 *>(T + (size_t)(PO * npl + pi0) * tile);
-        double2 *dst = reinterpret_cast<double2 *>(Ts);
-        for (int x = threadIdx.x; x < np * tile / 2; x += blockDim.x) dst[x] = src[x];
+        const double *src = T + (size_t)(PO * npl + pi0) * tile;
+        for (int x = threadIdx.x; x < np * tile; x += blockDim.x) Ts[x] = src[x];
     }
     for (int x = threadIdx.x; x < np * ny * nxp; x += blockDim.x) {
         const int row = x / nxp, ix = x - row * nxp;  // row = q ny + iy
@@ -740,7 +740,7 @@ __global__ void __launch_bounds__(256) k_mm_grid_table(const double *__restrict_
                                                        int64_t ny, int64_t nz, int64_t nxp,
                                                        double *__restrict__ T)
 {
-    const int64_t Wx = 2 * nx - 1, Wy = 2 * ny - 1, Wp = 2 * nxp, npl = 6 * nz, nrec = Wx * Wy * nz;
+    const int64_t Wx = 2 * nx - 1, Wy = 2 * ny - 1, Wp = 2 * nxp + 1, npl = 6 * nz, nrec = Wx * Wy * nz;
     const int64_t n = npl * npl * Wy * Wp;
     const int64_t step = (int64_t)gridDim.x * blockDim.x;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += step) {
@@ -756,7 +756,7 @@ __global__ void __launch_bounds__(256) k_expand_mm_grid(const double *__restrict
                                                         int64_t ny, int64_t nz, int64_t nxp,
                                                         double *__restrict__ k, int64_t ld)
 {
-    const int64_t nxy = nx * ny, m6 = 6 * nz * nxy, npl = 6 * nz, Wp = 2 * nxp, Wy = 2 * ny - 1;
+    const int64_t nxy = nx * ny, m6 = 6 * nz * nxy, npl = 6 * nz, Wp = 2 * nxp + 1, Wy = 2 * ny - 1;
     const int64_t step = (int64_t)gridDim.x * blockDim.x;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < m6 * m6; t += step) {
         const int64_t row = t / m6, col = t % m6;
@@ -1236,9 +1236,10 @@ static int mm_grid_detect(const oq_hex8_mesh *m, MmGrid &g, double &hx, double &
 // / R), at most 64 KB of LDS unless one plane needs more.
 static int mm_grid_shape(MmGrid &g)
 {
-    g.r = g.nx >= 24 ? 8 : 4;
+    // 8 outputs per thread only where that still gives 256 of them per plane
+    g.r = g.ny * ((g.nx + 7) / 8) >= 256 ? 8 : 4;
     g.nxp = grid_nxp(g.nx, g.r);
-    const int64_t Wy = 2 * g.ny - 1, Wp = 2 * g.nxp, nxg = g.nxp / g.r;
+    const int64_t Wy = 2 * g.ny - 1, Wp = 2 * g.nxp + 1, nxg = g.nxp / g.r;
     const int64_t U = g.ny * nxg, per = Wy * Wp + 2 * g.ny * g.nxp;
     OQ_REQUIRE(per * 8 <= 160 * 1024,
                "OQ_K_FFT_GRID: one source plane of the grid kernel exceeds the LDS of a workgroup "
@@ -1307,7 +1308,7 @@ static int mm_grid_build(oq_hex8_mesh const *m, double hx, double hy, const Quad
                              c[5].data(), c[6].data(), c[7].data(), c[8].data(), g.nz};
     Hex8Upload ru, su;
     DevBuf raw;
-    const int64_t npl = 6 * g.nz, nt = npl * npl * Wy * 2 * g.nxp;
+    const int64_t npl = 6 * g.nz, nt = npl * npl * Wy * (2 * g.nxp + 1);
     int rc = ru.load(&rm);
     if (!rc) rc = su.load(&sm);
     if (!rc) rc = raw.alloc((size_t)(6 * nrec * npl) * 8);

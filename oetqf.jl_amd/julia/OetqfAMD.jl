@@ -148,10 +148,12 @@ function cvisc(p::CompositePowerLawViscosityProperty, ne)
     return CViscProps(pointer(γ), pointer(n), length(p.piter), Tuple(Float64.(p.dϵ₀))), (γ, n)
 end
 
-# Kstorage: Float64, Float32, :bf16, :toeplitz (fault only: K_ff kept as st)
-# or :fft (fault only: DFT convolution along strike, equation.jl:44-61)
+# Kstorage: Float64, Float32, :bf16, :toeplitz (fault only: K_ff kept as st),
+# :fft (K_ff by DFT convolution along strike, equation.jl:44-61, :200) or,
+# with assemble_from_mesh only, :fft_grid (:fft, and K_mm as 2D convolutions
+# over a mantle whose cells form a uniform horizontal grid)
 kcode(T) = T === Float64 ? Cint(0) : T === Float32 ? Cint(1) : T === :toeplitz ? Cint(3) :
-           T === :fft ? Cint(5) : Cint(2)
+           T === :fft ? Cint(5) : T === :fft_grid ? Cint(6) : Cint(2)
 
 struct CDilaProps
     tp::Ptr{Float64}; eps::Ptr{Float64}; beta::Ptr{Float64}; p0::Ptr{Float64}
@@ -205,6 +207,31 @@ function assemble(gf₁₁::AbstractArray, gf₁₂::AbstractMatrix, gf₂₁::A
              Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cint, Ref{Ptr{Cvoid}}),
             device, nx, nξ, ne, Ref(crs(pf)), Ref(vs), pointer(g), Cint(isdft),
             pointer(gf₁₂), pointer(gf₂₁), pointer(gf₂₂), kcode(Kstorage), h))
+    end
+    f = GPURHS(h[], ne, (pf, pa, keepv))
+    return Oetqf.ODEProblem{true}((du, u, pp, t) -> rhs!(f, du, u, t), u0, tspan, (pf, pa, f))
+end
+
+"""assemble_from_mesh(mf, ma, λ, μ, pf, pa, u0, tspan; ftype, qtype, nrept,
+buffer_ratio, Kstorage=Float64): the four stress_greens_function calls
+(GF.jl:31-71,123-174,194-227,250-290) and assemble (equation.jl:141-154) in
+one step, the blocks built on the device with no host copy of K."""
+function assemble_from_mesh(mf::Oetqf.RectOkadaMesh, ma::Oetqf.BEMHex8Mesh, λ::Float64, μ::Float64,
+        pf::RateStateQuasiDynamicProperty, pa, u0::ArrayPartition, tspan::NTuple{2};
+        ftype=StrikeSlip(), qtype="Gauss1", nrept::Integer=2, buffer_ratio::Real=0, Kstorage=Float64,
+        device::Integer=0)
+    @assert buffer_ratio ≥ 0 "Argument `buffer_ratio` must be ≥ 0."
+    ne = length(ma.cx)
+    c, w = quadrature(qtype)
+    vs, keepv = cvisc(pa, ne)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve mf ma c w pf keepv begin
+        q = Ref(CQuadrature(pointer(c), pointer(w), length(w)))
+        check(ccall((:oq_rhs_create_from_mesh, libpath), Cint,
+            (Cint, Ref{CFaultMesh}, Ref{CHex8Mesh}, Float64, Float64, Cint, Int64, Float64, Ref{CQuadrature},
+             Ref{CRSProps}, Ref{CViscProps}, Cint, Ref{Ptr{Cvoid}}),
+            device, Ref(cmesh(mf)), Ref(cmesh(ma)), λ, μ, ftcode(ftype), nrept, buffer_ratio, q,
+            Ref(crs(pf)), Ref(vs), kcode(Kstorage), h))
     end
     f = GPURHS(h[], ne, (pf, pa, keepv))
     return Oetqf.ODEProblem{true}((du, u, pp, t) -> rhs!(f, du, u, t), u0, tspan, (pf, pa, f))
