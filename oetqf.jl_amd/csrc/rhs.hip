@@ -25,6 +25,8 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
+#include <utility>
 #include <vector>
 
 #include "rhs_impl.h"
@@ -827,38 +829,202 @@ __device__ __forceinline__ void grid_matvec_wave(const RhsDev &d, const GridMv &
     }
 }
 
-// One launch for the grid model's O(N^2) work: nconv convolution blocks
-// (PO = b mod 6nz, chunk b / 6nz) and the matvec blocks of four wavefronts,
-// interleaved one convolution block to two matvec blocks while both last,
-// so the LDS/FP64 work of the convolution runs beside the HBM stream.
-template <int R>
-__global__ void __launch_bounds__(256) k_grid_fused(RhsDev d, GridMv a, const double *__restrict__ T,
+// Along-strike tables of the grid model (see MmGrid).  F blocks (fault row
+// b, lines [c fqs, (c + 1) fqs)) stage fqb lines of F (rows padded to Fp
+// entries with zeros) and of s_m at a time; thread (line slot li, residue
+// a0 = a mod m, segment sg) accumulates the kRW outputs a = a0 + m (sg kRW +
+// r) with a register window over F[.][a - m jx + m (nxc - 1)] that slides
+// by one slot (m entries) per source column jx, one LDS load and one
+// broadcast load per kRW multiply-adds; the line slots are added in a
+// fixed order into fpart[(1 + c) nf + a + nx b].  G blocks (mantle line q',
+// fault rows [c gbs, (c + 1) gbs)) stage gbb rows of G (m leading zeros,
+// rows padded to Gp) and of s_f (rows padded to m nr) at a time; thread
+// (segment sg, row slot bi, residue a0) accumulates the kRW outputs jx = sg
+// kRW + r over a = a0 + m k with the window over G[.][m jx - a + nx - 1]
+// sliding by one slot per k; wavefront sums, then the four wavefronts'
+// sums in a fixed order, go to gpart[c 6ne + q' nxc + jx].
+static constexpr int kRW = 8;
+static constexpr size_t kGridOtherLds = 40 * 1024;  // LDS of an F or G block (4 per CU)
+
+struct GridSt {
+    const double *F, *G, *sf, *sm;
+    int nx, nxi, nxc, Q, m, D;
+    int fqs, fqb, gbs, gbb;
+    int nr, fseg, gseg, Fp, Gp;  // nr = ceil(nx / m); segments of kRW outputs; padded row widths
+    int flpp;                    // F: lines in flight (<= 256 / (m fseg))
+    int64_t nf, m6;
+    double *fpart, *gpart;
+};
+
+__device__ __forceinline__ void grid_f_block(const GridSt &g, int b, int c, double *sh)
+{
+    const int nx = g.nx, D = g.D, nxc = g.nxc, m = g.m, Q = g.Q, Fp = g.Fp;
+    const int q0 = c * g.fqs, q1 = (q0 + g.fqs < Q ? q0 + g.fqs : Q);
+    const int tpl = m * g.fseg, lpp = g.flpp;  // line slots
+    const int t = threadIdx.x, li = t / tpl, rr = t - li * tpl, sg = rr / m, a0 = rr - sg * m;
+    const bool active = li < lpp;
+    double acc[kRW];
+#pragma unroll
+    for (int r = 0; r < kRW; ++r) acc[r] = 0.0;
+    double *Fs = sh, *Ss = sh + g.fqb * Fp;
+    for (int qb = q0; qb < q1; qb += g.fqb) {
+        const int nq = (g.fqb < q1 - qb ? g.fqb : q1 - qb);
+        __syncthreads();
+        for (int x = t; x < nq * Fp; x += 256) {
+            const int q = x / Fp, dd = x - q * Fp;
+            Fs[x] = dd < D ? g.F[((size_t)b * Q + qb + q) * D + dd] : 0.0;
+        }
+        for (int x = t; x < nq * nxc; x += 256) Ss[x] = g.sm[(size_t)qb * nxc + x];
+        __syncthreads();
+        if (active)
+            for (int q = li; q < nq; q += lpp) {
+                // Fq[m x] = F[q][a0 + m sg kRW + m (nxc - 1) + m x]; output r, column jx: x = r - jx
+                const double *Fq = Fs + q * Fp + a0 + m * (sg * kRW) + m * (nxc - 1);
+                const double *Sq = Ss + q * nxc;
+                double w[kRW];
+#pragma unroll
+                for (int r = 0; r < kRW; ++r) w[r] = Fq[m * r];
+                for (int j0 = 0; j0 < nxc; j0 += kRW) {
+#pragma unroll
+                    for (int kk = 0; kk < kRW; ++kk) {
+                        // column jx = j0 + kk: slot (r - kk) mod kRW holds Fq[m (r - jx)]
+                        const int jx = j0 + kk;
+                        if (jx < nxc) {
+                            const double sv = Sq[jx];
+#pragma unroll
+                            for (int r = 0; r < kRW; ++r) acc[r] += w[(r - kk + kRW) % kRW] * sv;
+                            if (jx + 1 < nxc) w[(kRW - 1 - kk) % kRW] = Fq[-m * (jx + 1)];
+                        }
+                    }
+                }
+            }
+    }
+    __syncthreads();
+    double *Rs = sh;  // lpp x nx partial sums
+    if (active)
+#pragma unroll
+        for (int r = 0; r < kRW; ++r) {
+            const int a = a0 + m * (sg * kRW + r);
+            if (a < nx) Rs[li * nx + a] = acc[r];
+        }
+    __syncthreads();
+    for (int a = t; a < nx; a += 256) {
+        double y = 0.0;
+        for (int l = 0; l < lpp; ++l) y += Rs[l * nx + a];
+        g.fpart[(size_t)(1 + c) * g.nf + a + (size_t)nx * b] = y;
+    }
+}
+
+__device__ __forceinline__ void grid_g_block(const GridSt &g, int qq, int c, double *sh)
+{
+    const int nx = g.nx, D = g.D, nxc = g.nxc, m = g.m, nxi = g.nxi, Gp = g.Gp, nr = g.nr, sw = m * nr;
+    const int b0 = c * g.gbs, b1 = (b0 + g.gbs < nxi ? b0 + g.gbs : nxi);
+    // a segment takes whole wavefronts (wps of them); in it, thread (row slot bi, residue a0)
+    const int wps = 4 / g.gseg, tps = 64 * wps, bp = tps / m;
+    const int t = threadIdx.x, sg = t / tps, x0 = t - sg * tps, bi = x0 / m, a0 = x0 - bi * m;
+    const int w = t >> 6, lane = t & 63;
+    const bool active = sg < g.gseg && bi < bp;
+    double acc[kRW];
+#pragma unroll
+    for (int r = 0; r < kRW; ++r) acc[r] = 0.0;
+    double *Gs = sh, *Sf = sh + g.gbb * Gp;
+    for (int bb = b0; bb < b1; bb += g.gbb) {
+        const int nb = (g.gbb < b1 - bb ? g.gbb : b1 - bb);
+        __syncthreads();
+        for (int x = t; x < nb * Gp; x += 256) {
+            const int b = x / Gp, e = x - b * Gp - m;
+            Gs[x] = (e >= 0 && e < D) ? g.G[((size_t)qq * nxi + bb + b) * D + e] : 0.0;
+        }
+        for (int x = t; x < nb * sw; x += 256) {
+            const int b = x / sw, a = x - b * sw;
+            Sf[x] = a < nx ? g.sf[(size_t)(bb + b) * nx + a] : 0.0;
+        }
+        __syncthreads();
+        for (int b = bi; active && b < nb; b += bp) {
+            // Gq[m x] = G[b][m (sg kRW + x) - a0 + nx - 1] (m leading zeros); output r, step k: x = r - k
+            const double *Gq = Gs + b * Gp + m + (nx - 1 - a0) + m * (sg * kRW);
+            const double *Sq = Sf + b * sw + a0;
+            double wv[kRW];
+#pragma unroll
+            for (int r = 0; r < kRW; ++r) wv[r] = Gq[m * r];
+            for (int k0 = 0; k0 < nr; k0 += kRW) {
+#pragma unroll
+                for (int kk = 0; kk < kRW; ++kk) {
+                    const int k = k0 + kk;
+                    if (k < nr) {
+                        const double sv = Sq[m * k];
+#pragma unroll
+                        for (int r = 0; r < kRW; ++r) acc[r] += wv[(r - kk + kRW) % kRW] * sv;
+                        if (k + 1 < nr) wv[(kRW - 1 - kk) % kRW] = Gq[-m * (k + 1)];
+                    }
+                }
+            }
+        }
+    }
+    // wavefront sums (a wavefront holds one segment), then the wavefronts of a segment in order
+    __syncthreads();
+    double *Rs = sh;  // 4 x kRW
+#pragma unroll
+    for (int r = 0; r < kRW; ++r) {
+        const double y = wave_sum(acc[r]);  // every thread takes part; idle ones hold zeros
+        if (lane == 0) Rs[w * kRW + r] = y;
+    }
+    __syncthreads();
+    for (int o = t; o < g.gseg * kRW; o += 256) {
+        const int s2 = o / kRW, r = o - s2 * kRW, jx = s2 * kRW + r;
+        if (jx < nxc) {
+            double y = 0.0;
+            for (int v = 0; v < wps; ++v) y += Rs[(s2 * wps + v) * kRW + r];
+            g.gpart[(size_t)c * g.m6 + (size_t)qq * nxc + jx] = y;
+        }
+    }
+}
+
+// One launch for the grid model's O(N^2) work.  roles[block] = role << 28 |
+// index: 0 a K_mm convolution block (PO = index mod 6nz, chunk index / 6nz),
+// 1 four wavefronts of the dense K_fm / K_mf stream, 2 an F block (fault row
+// index mod nxi, chunk index / nxi), 3 a G block (line index mod Q, chunk
+// index / Q).  The host interleaves the roles evenly, so the LDS and FP64
+// work of the convolutions runs beside the HBM streams.
+// MASK selects the roles compiled into an instance (bit r for role r), so
+// that a launch of one kind of block gets that role's register budget.
+template <int R, int MASK>
+__global__ void __launch_bounds__(256) k_grid_fused(RhsDev d, GridMv a, GridSt g2, const double *__restrict__ T,
                                                     int nx, int ny, int nz, int tp, double *__restrict__ mpart,
-                                                    int nconv, int nmv)
+                                                    const unsigned *__restrict__ roles)
 {
     extern __shared__ double gsh[];
-    const int b = blockIdx.x, nmix = nconv < nmv / 2 ? nconv : nmv / 2;
-    int conv = -1, mv = -1;
-    if (b < 3 * nmix) {
-        if (b % 3 == 0) conv = b / 3;
-        else mv = 2 * (b / 3) + b % 3 - 1;
-    } else if (b - 3 * nmix < nconv - nmix) {
-        conv = nmix + (b - 3 * nmix);
-    } else {
-        mv = 2 * nmix + (b - 3 * nmix - (nconv - nmix));
+    const unsigned rv = roles[blockIdx.x];
+    const int role = (int)(rv >> 28), idx = (int)(rv & 0x0fffffffu);
+    if constexpr ((MASK & 1) != 0) {
+        if (role == 0) {
+            const int npl = 6 * nz;
+            mm_grid_conv_block<R>(T, a.sm, nx, ny, nz, tp, mpart, idx % npl, idx / npl, gsh);
+            return;
+        }
     }
-    if (conv >= 0) {
-        const int npl = 6 * nz;
-        mm_grid_conv_block<R>(T, a.sm, nx, ny, nz, tp, mpart, conv % npl, conv / npl, gsh);
-        return;
+    if constexpr ((MASK & 2) != 0) {
+        if (role == 1) {
+            grid_matvec_wave(d, a, (int64_t)idx * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
+            return;
+        }
     }
-    grid_matvec_wave(d, a, (int64_t)mv * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
+    if constexpr ((MASK & 4) != 0) {
+        if (role == 2) {
+            grid_f_block(g2, idx % g2.nxi, idx / g2.nxi, gsh);
+            return;
+        }
+    }
+    if constexpr ((MASK & 8) != 0) {
+        if (role == 3) grid_g_block(g2, idx % g2.Q, idx / g2.Q, gsh);
+    }
 }
+
 
 // Fault rows: the K_ff rates and K_fm chunk partials, then the friction
 // update; mantle rows: du.sigma = K_mf s_f + the convolution partials.
 __global__ void __launch_bounds__(256) k_grid_finish(RhsDev d, const double *__restrict__ fpart, int nfp,
-                                                     const double *__restrict__ mvout,
+                                                     const double *__restrict__ mvout, int nmv,
                                                      const double *__restrict__ mpart, int nchunk,
                                                      const double *__restrict__ u, double *__restrict__ du)
 {
@@ -870,9 +1036,56 @@ __global__ void __launch_bounds__(256) k_grid_finish(RhsDev d, const double *__r
         fault_update(d, q, tau, u, du);
     } else if (q < nf + m6) {
         const int64_t r = q - nf;
-        double y = mvout[r];
+        double y = 0.0;
+        for (int c = 0; c < nmv; ++c) y += mvout[c * m6 + r];
         for (int c = 0; c < nchunk; ++c) y += mpart[c * m6 + r];
         du[d.lay.sig + r] = y;
+    }
+}
+
+// F[b][q][d] from the fm pair kernel's block over the shifted virtual
+// patches (receiver d + D b, row pitch Q, column q).
+__global__ void __launch_bounds__(256) k_grid_ftable(const double *__restrict__ raw, int64_t nxi, int64_t Q,
+                                                     int64_t D, double *__restrict__ F)
+{
+    const int64_t n = nxi * Q * D, step = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += step) {
+        const int64_t dd = t % D, r = t / D, q = r % Q, b = r / Q;
+        F[t] = raw[(dd + D * b) * Q + q];
+    }
+}
+
+// G[q'][b][e] from the mf pair kernel's block over the shifted virtual cells
+// (receiver e + D (jy + ny jz), row (k ne' + receiver), row pitch nxi,
+// column b).
+__global__ void __launch_bounds__(256) k_grid_gtable(const double *__restrict__ raw, int64_t nxi, int64_t ny,
+                                                     int64_t nz, int64_t D, double *__restrict__ G)
+{
+    const int64_t Q = 6 * ny * nz, nev = D * ny * nz, n = Q * nxi * D;
+    const int64_t step = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += step) {
+        const int64_t e = t % D, r = t / D, b = r % nxi, qq = r / nxi;
+        const int64_t k = qq / (ny * nz), rem = qq % (ny * nz), jz = rem / ny, jy = rem % ny;
+        G[t] = raw[(k * nev + e + D * (jy + ny * jz)) * nxi + b];
+    }
+}
+
+// Dense K_fm (fault rows, mantle columns from nf) and K_mf (mantle rows,
+// fault columns) of a receiver-major operator (pitch ld) from the tables.
+__global__ void __launch_bounds__(256) k_expand_strike(const double *__restrict__ F, const double *__restrict__ G,
+                                                       int64_t nx, int64_t nxi, int64_t nxc, int64_t Q, int64_t m,
+                                                       int64_t D, double *__restrict__ k, int64_t ld)
+{
+    const int64_t nf = nx * nxi, m6 = Q * nxc, step = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < 2 * nf * m6; t += step) {
+        if (t < nf * m6) {  // K_fm: row f = a + nx b, column nf + q nxc + jx
+            const int64_t f = t / m6, col = t % m6, a = f % nx, b = f / nx, q = col / nxc, jx = col % nxc;
+            k[f * ld + nf + col] = F[(b * Q + q) * D + a - m * jx + m * (nxc - 1)];
+        } else {            // K_mf: row nf + q' nxc + jx, column a + nx b
+            const int64_t u = t - nf * m6, row = u / nf, f = u % nf, a = f % nx, b = f / nx;
+            const int64_t qq = row / nxc, jx = row % nxc;
+            k[(nf + row) * ld + f] = G[(qq * nxi + b) * D + m * jx - a + nx - 1];
+        }
     }
 }
 
@@ -898,6 +1111,40 @@ static void toep_shape(const oq_rhs *h, int *lchunk, int *nchunk, int *threads)
     *threads = (int)(t64 < 64 ? 64 : (t64 > 256 ? 256 : t64));
 }
 
+template <int R, int MASK>
+static void grid_launch1(const oq_rhs *h, int nblocks, size_t lds, const unsigned *roles, const GridMv &a,
+                         const GridSt &g2)
+{
+    const MmGrid &g = h->grid;
+    hipLaunchKernelGGL((k_grid_fused<R, MASK>), dim3((unsigned)nblocks), dim3(256), lds, h->stream, h->dev, a, g2,
+                       h->Tmm.as<double>(), (int)g.nx, (int)g.ny, (int)g.nz, g.tp, h->mpart.as<double>(), roles);
+}
+
+template <int R>
+static void grid_launch_r(const oq_rhs *h, const GridMv &a, const GridSt &g2)
+{
+    const MmGrid &g = h->grid;
+    const unsigned *r1 = h->roles.as<unsigned>(), *r2 = h->roles2.as<unsigned>();
+    if (!g.split) {
+        if (g.strike) grid_launch1<R, 1 | 4 | 8>(h, g.nblocks, g.lds_fused, r1, a, g2);
+        else grid_launch1<R, 1 | 2>(h, g.nblocks, g.lds_fused, r1, a, g2);
+        return;
+    }
+    grid_launch1<R, 1>(h, g.nconv, g.lds, r2, a, g2);
+    if (g.strike) grid_launch1<R, 4 | 8>(h, g.nblocks - g.nconv, g.lds_other, r2 + g.nconv, a, g2);
+    else grid_launch1<R, 2>(h, g.nblocks - g.nconv, g.lds_other, r2 + g.nconv, a, g2);
+}
+
+template <int R>
+static int grid_lds_attrs(size_t lds)
+{
+    const void *f[5] = {(const void *)k_grid_fused<R, 1 | 4 | 8>, (const void *)k_grid_fused<R, 1 | 2>,
+                        (const void *)k_grid_fused<R, 1>, (const void *)k_grid_fused<R, 4 | 8>,
+                        (const void *)k_grid_fused<R, 2>};
+    for (const void *p : f) OQ_HIP(hipFuncSetAttribute(p, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    return OQ_OK;
+}
+
 int rhs_enqueue_matvec(oq_rhs *h, const double *u, double *du)
 {
     if (h->kdtype == OQ_K_FFT && h->ne > 0) {
@@ -912,28 +1159,24 @@ int rhs_enqueue_matvec(oq_rhs *h, const double *u, double *du)
         hipLaunchKernelGGL(k_fft_mid, dim3((unsigned)(L / 2 + 1)), dim3(512), 0, h->stream, h->fg.as<double>(),
                            h->fsh.as<double>(), L / 2 + 1, h->nxi, h->fth.as<double>());
         if (h->mmgrid) {
-            // K_ff rates into fpart[0, nf); the grid convolution of K_mm, K_fm in column chunks
-            // and K_mf in one launch; then the friction update and du.sigma
+            // K_ff rates into fpart[0, nf); the grid convolution of K_mm with K_fm and K_mf
+            // (column-chunked dense streams, or along strike the F and G tables) in one launch;
+            // then the friction update and du.sigma
             const MmGrid &g = h->grid;
             hipLaunchKernelGGL(k_fft_inv, dim3((unsigned)h->nxi), dim3(256), fft_lds(h->nx), h->stream, h->dev,
                                h->fth.as<double>(), h->nx, L, bits, h->ftw.as<double>(), u, h->fpart.as<double>());
-            constexpr int R = KTraits<OQ_K_FP64>::kRows, W = KTraits<OQ_K_FP64>::kWaves;
-            static_assert(W == 4, "k_grid_fused runs four matvec wavefronts per block");
-            const int64_t waves = (h->nf + R - 1) / R * g.fs + (6 * h->ne + R - 1) / R;
-            const int nconv = (int)(6 * g.nz) * g.nchunk, nmv = (int)nblk(waves, W);
+            static_assert(KTraits<OQ_K_FP64>::kWaves == 4, "k_grid_fused runs four matvec wavefronts per block");
             const GridMv a = {h->Kf.as<double>(), h->K.as<double>(), h->s64.as<double>(), h->sm.as<double>(),
                               h->ldm, h->ldf, g.fcw, g.fs, h->fpart.as<double>(), h->mvout.as<double>()};
-            if (g.r == 8)
-                hipLaunchKernelGGL(k_grid_fused<8>, dim3((unsigned)(nconv + nmv)), dim3(256), g.lds, h->stream,
-                                   h->dev, a, h->Tmm.as<double>(), (int)g.nx, (int)g.ny, (int)g.nz, g.tp,
-                                   h->mpart.as<double>(), nconv, nmv);
-            else
-                hipLaunchKernelGGL(k_grid_fused<4>, dim3((unsigned)(nconv + nmv)), dim3(256), g.lds, h->stream,
-                                   h->dev, a, h->Tmm.as<double>(), (int)g.nx, (int)g.ny, (int)g.nz, g.tp,
-                                   h->mpart.as<double>(), nconv, nmv);
+            const GridSt g2 = {h->Ftab.as<double>(), h->Gtab.as<double>(), h->s64.as<double>(), h->sm.as<double>(),
+                               (int)h->nx, (int)h->nxi, (int)g.nx, (int)(6 * g.ny * g.nz), (int)g.m, (int)g.D,
+                               g.fqs, g.fqb, g.gbs, g.gbb, g.nr, g.fseg, g.gseg, g.Fp, g.Gp, g.flpp, h->nf, 6 * h->ne,
+                               h->fpart.as<double>(), h->mvout.as<double>()};
+            if (g.r == 8) grid_launch_r<8>(h, a, g2);
+            else grid_launch_r<4>(h, a, g2);
             hipLaunchKernelGGL(k_grid_finish, dim3(nblk(h->nf + 6 * h->ne, 256)), dim3(256), 0, h->stream, h->dev,
-                               h->fpart.as<double>(), g.fs + 1, h->mvout.as<double>(), h->mpart.as<double>(),
-                               g.nchunk, u, du);
+                               h->fpart.as<double>(), 1 + (g.strike ? g.nfc : g.fs), h->mvout.as<double>(),
+                               g.strike ? g.ngc : 1, h->mpart.as<double>(), g.nchunk, u, du);
             OQ_HIP(hipGetLastError());
             return OQ_OK;
         }
@@ -1252,13 +1495,177 @@ static int mm_grid_shape(MmGrid &g)
     g.nchunk = (int)((6 * g.nz + tp - 1) / tp);
     g.threads = 256;
     g.lds = (size_t)(tp * per * 8);
-    if (g.r == 8)
-        OQ_HIP(hipFuncSetAttribute((const void *)k_grid_fused<8>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)g.lds));
-    else
-        OQ_HIP(hipFuncSetAttribute((const void *)k_grid_fused<4>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)g.lds));
+    const size_t lmax = g.lds > kGridOtherLds ? g.lds : kGridOtherLds;
+    return g.r == 8 ? grid_lds_attrs<8>(lmax) : grid_lds_attrs<4>(lmax);
+}
+
+// Along strike: the fault's centres x[a] and extents ax[a] on one spacing
+// dxf (a RectOkada mesh is), and the mantle's x spacing hx = m dxf for an
+// integer m >= 1 (to 1e-11 of dxf and 1e-9 of hx).  Otherwise K_fm and K_mf
+// stay dense (no error).
+static bool mm_strike_detect(const oq_fault_mesh *mf, MmGrid &g, double hx)
+{
+    const int64_t nx = mf->nx;
+    const double dxf = nx > 1 ? (mf->x[nx - 1] - mf->x[0]) / (double)(nx - 1) : mf->dx;
+    if (!(dxf > 0.0) || g.nx > 64) return false;
+    const double tol = 1e-11 * dxf;
+    for (int64_t a = 0; a < nx; ++a) {
+        const double sh = (double)a * dxf;
+        if (fabs(mf->x[a] - (mf->x[0] + sh)) > tol || fabs(mf->ax[2 * a] - (mf->ax[0] + sh)) > tol ||
+            fabs(mf->ax[2 * a + 1] - (mf->ax[1] + sh)) > tol)
+            return false;
+    }
+    int64_t m = 1;
+    if (g.nx > 1) {
+        const double r = hx / dxf;
+        m = (int64_t)llround(r);
+        if (m < 1 || fabs(r - (double)m) > 1e-9 * r) return false;
+    }
+    g.m = m;
+    g.D = nx + m * (g.nx - 1);
+    return true;
+}
+
+// LDS batches and blocks of the F and G roles (within the convolution
+// blocks' LDS); false if a role does not fit (then K_fm and K_mf stay dense).
+static bool mm_strike_shape(MmGrid &g, int64_t nx, int64_t nxi)
+{
+    const int64_t cap = (int64_t)(kGridOtherLds / 8), Q = 6 * g.ny * g.nz, m = g.m;
+    const int64_t nr = (nx + m - 1) / m, fseg = (nr + kRW - 1) / kRW, gseg = (g.nx + kRW - 1) / kRW;
+    const int64_t Fp = m * (fseg * kRW + g.nx - 1), Gp = nx + m * gseg * kRW;
+    if (m * fseg > 256 || gseg > 4 || m > 64 * (4 / gseg) || nx > 2048) return false;
+    int64_t lpp = 256 / (m * fseg);
+    if (lpp * nx > cap) lpp = cap / nx;  // the line slots' partial sums share the LDS
+    const int64_t fqb = cap / (Fp + g.nx), gbb = cap / (Gp + m * nr);
+    if (fqb < 1 || gbb < 1 || lpp < 1 || 4 * kRW > cap) return false;
+    g.flpp = (int)lpp;
+    g.nr = (int)nr;
+    g.fseg = (int)fseg;
+    g.gseg = (int)gseg;
+    g.Fp = (int)Fp;
+    g.Gp = (int)Gp;
+    g.fqb = (int)fqb;
+    g.gbb = (int)gbb;
+    const int64_t nfc = Q < 8 ? Q : 8, fqs = (Q + nfc - 1) / nfc;
+    g.fqs = (int)fqs;
+    g.nfc = (int)((Q + fqs - 1) / fqs);
+    const int64_t ngc = nxi < 2 ? nxi : 2, gbs = (nxi + ngc - 1) / ngc;
+    g.gbs = (int)gbs;
+    g.ngc = (int)((nxi + gbs - 1) / gbs);
+    return true;
+}
+
+// F and G from the fm and mf pair kernels (GF.jl:194-227, :123-174) on
+// virtual meshes: for F, D patches per fault row at x[0] + dxf (d - m (nx -
+// 1)) as receivers of the cells (0, jy, jz); for G, D cells per (jy, jz) at
+// X0 + dxf (e - (nx_f - 1)) as receivers of the patches (0, b), with the
+// real fault's image spacing.
+static int mm_strike_build(const oq_fault_mesh *mf, const oq_hex8_mesh *ma, double lrept, const QuadUpload &qu,
+                           double lambda, double mu, int ftype, int nrept, oq_rhs *h)
+{
+    const MmGrid &g = h->grid;
+    const int64_t nx = mf->nx, nxi = mf->nxi, D = g.D, nyz = g.ny * g.nz, Q = 6 * nyz, nxy = g.nx * g.ny;
+    const double dxf = nx > 1 ? (mf->x[nx - 1] - mf->x[0]) / (double)(nx - 1) : mf->dx;
+    // virtual receiver patches of F
+    std::vector<double> vx((size_t)D), vax((size_t)(2 * D));
+    for (int64_t d = 0; d < D; ++d) {
+        const double sh = (double)(d - g.m * (g.nx - 1)) * dxf;
+        vx[d] = mf->x[0] + sh;
+        vax[2 * d] = mf->ax[0] + sh;
+        vax[2 * d + 1] = mf->ax[1] + sh;
+    }
+    oq_fault_mesh vf = *mf;
+    vf.x = vx.data();
+    vf.ax = vax.data();
+    vf.nx = D;
+    // the source column of F: cells (0, jy, jz), index jy + ny jz
+    std::vector<double> col[9];
+    const double *src[9] = {ma->cx, ma->cy, ma->cz, ma->qx, ma->qy, ma->qz, ma->dx, ma->dy, ma->dz};
+    for (int a = 0; a < 9; ++a)
+        for (int64_t jz = 0; jz < g.nz; ++jz)
+            for (int64_t jy = 0; jy < g.ny; ++jy) col[a].push_back(src[a][jz * nxy + jy * g.nx]);
+    const oq_hex8_mesh cm = {col[0].data(), col[1].data(), col[2].data(), col[3].data(), col[4].data(),
+                             col[5].data(), col[6].data(), col[7].data(), col[8].data(), nyz};
+    // virtual receiver cells of G: index e + D (jy + ny jz)
+    std::vector<double> vc[9];
+    for (auto &v : vc) v.resize((size_t)(D * nyz));
+    for (int64_t j = 0; j < nyz; ++j)
+        for (int64_t e = 0; e < D; ++e) {
+            const size_t i = (size_t)(e + D * j);
+            const double sh = (double)(e - (nx - 1)) * dxf;
+            vc[0][i] = col[0][j] + sh;
+            vc[1][i] = col[1][j];
+            vc[2][i] = col[2][j];
+            vc[3][i] = vc[0][i] + (col[3][j] - col[0][j]);
+            vc[4][i] = col[4][j];
+            vc[5][i] = col[5][j];
+            vc[6][i] = col[6][j];
+            vc[7][i] = col[7][j];
+            vc[8][i] = col[8][j];
+        }
+    const oq_hex8_mesh vm = {vc[0].data(), vc[1].data(), vc[2].data(), vc[3].data(), vc[4].data(),
+                             vc[5].data(), vc[6].data(), vc[7].data(), vc[8].data(), D * nyz};
+    // the source column of G: patches (0, b)
+    oq_fault_mesh sf = *mf;
+    sf.nx = 1;
+    FaultUpload fvu, fsu;
+    Hex8Upload cu, vu;
+    DevBuf raw;
+    const size_t nt = (size_t)(nxi * Q * D);
+    int rc = fvu.load(&vf);
+    if (!rc) rc = fsu.load(&sf);
+    if (!rc) rc = cu.load(&cm);
+    if (!rc) rc = vu.load(&vm);
+    if (rc) return rc;
+    fsu.dev.lrept = lrept;
+    if ((rc = raw.alloc(nt * 8)) || (rc = h->Ftab.alloc(nt * 8)) || (rc = h->Gtab.alloc(nt * 8))) return rc;
+    if ((rc = launch_gf_fm(cu.dev, fvu.dev, lambda, mu, ftype, raw.as<double>(), Q, 1, h->stream))) return rc;
+    hipLaunchKernelGGL(k_grid_ftable, dim3(grid_capped((int64_t)nt, 256)), dim3(256), 0, h->stream,
+                       raw.as<double>(), nxi, Q, D, h->Ftab.as<double>());
+    OQ_HIP(hipGetLastError());
+    if ((rc = launch_gf_mf(fsu.dev, vu.dev, qu.c.as<double>(), qu.w.as<double>(), qu.nq, lambda, mu, ftype, nrept,
+                           raw.as<double>(), nxi, 1, h->stream)))
+        return rc;
+    hipLaunchKernelGGL(k_grid_gtable, dim3(grid_capped((int64_t)nt, 256)), dim3(256), 0, h->stream,
+                       raw.as<double>(), nxi, g.ny, g.nz, D, h->Gtab.as<double>());
+    OQ_HIP(hipGetLastError());
+    OQ_HIP(hipStreamSynchronize(h->stream));
     return OQ_OK;
+}
+
+// The role table of k_grid_fused: every role's blocks spread evenly over the
+// launch (block b of role r sorts at (b + 1/2) / count_r).
+static int mm_roles_build(oq_rhs *h)
+{
+    MmGrid &g = h->grid;
+    constexpr int R = KTraits<OQ_K_FP64>::kRows, W = KTraits<OQ_K_FP64>::kWaves;
+    int64_t cnt[4] = {(int64_t)(6 * g.nz) * g.nchunk, 0, 0, 0};
+    if (g.strike) {
+        cnt[2] = h->nxi * g.nfc;
+        cnt[3] = 6 * g.ny * g.nz * g.ngc;
+    } else {
+        const int64_t waves = (h->nf + R - 1) / R * g.fs + (6 * h->ne + R - 1) / R;
+        cnt[1] = (waves + W - 1) / W;
+    }
+    std::vector<std::pair<double, unsigned>> e;
+    for (int r = 0; r < 4; ++r) {
+        OQ_REQUIRE(cnt[r] < (1 << 28), "OQ_K_FFT_GRID: too many blocks for the role table");
+        for (int64_t b = 0; b < cnt[r]; ++b)
+            e.push_back({((double)b + 0.5) / (double)cnt[r], ((unsigned)r << 28) | (unsigned)b});
+    }
+    std::vector<unsigned> t2;  // split: role order (convolutions first)
+    for (auto &x : e) t2.push_back(x.second);
+    std::stable_sort(e.begin(), e.end(), [](const std::pair<double, unsigned> &x,
+                                            const std::pair<double, unsigned> &y) { return x.first < y.first; });
+    std::vector<unsigned> t;
+    for (auto &x : e) t.push_back(x.second);
+    g.nblocks = (int)t.size();
+    g.nconv = (int)cnt[0];
+    g.lds_other = g.strike ? kGridOtherLds : 0;
+    g.lds_fused = g.lds > g.lds_other ? g.lds : g.lds_other;
+    int rc = h->roles.upload(t.data(), t.size() * sizeof(unsigned), h->stream);
+    if (!rc) rc = h->roles2.upload(t2.data(), t2.size() * sizeof(unsigned), h->stream);
+    return rc;
 }
 
 // Column chunks of the fault rows' K_fm pass (pitch ldm) near the mantle
@@ -1314,8 +1721,9 @@ static int mm_grid_build(oq_hex8_mesh const *m, double hx, double hy, const Quad
     if (!rc) rc = raw.alloc((size_t)(6 * nrec * npl) * 8);
     if (!rc) rc = h->Tmm.alloc((size_t)nt * 8);
     if (!rc) rc = h->mpart.alloc((size_t)(h->grid.nchunk * 6 * h->ne) * 8);
-    if (!rc) rc = h->fpart.alloc((size_t)((h->grid.fs + 1) * h->nf) * 8);
-    if (!rc) rc = h->mvout.alloc((size_t)(6 * h->ne) * 8);
+    const int nfp = 1 + (h->grid.strike ? h->grid.nfc : h->grid.fs), nmv = h->grid.strike ? h->grid.ngc : 1;
+    if (!rc) rc = h->fpart.alloc((size_t)(nfp * h->nf) * 8);
+    if (!rc) rc = h->mvout.alloc((size_t)(nmv * 6 * h->ne) * 8);
     if (!rc)
         rc = launch_gf_mm_pairs(ru.dev, su.dev, qu.c.as<double>(), qu.w.as<double>(), qu.nq, lambda, mu,
                                 raw.as<double>(), npl, 1, h->stream);
@@ -1324,6 +1732,31 @@ static int mm_grid_build(oq_hex8_mesh const *m, double hx, double hy, const Quad
                        g.nx, g.ny, g.nz, (int64_t)g.nxp, h->Tmm.as<double>());
     OQ_HIP(hipGetLastError());
     OQ_HIP(hipStreamSynchronize(h->stream));
+    return OQ_OK;
+}
+
+// Fused or split launches for the grid model: 20 operator applications of
+// each on zeroed state after 3 of warm-up, keep the faster (ties: fused).
+static int mm_grid_pick(oq_rhs *h)
+{
+    OQ_HIP(hipMemsetAsync(h->u.p, 0, h->u.bytes, h->stream));
+    OQ_HIP(hipMemsetAsync(h->s64.p, 0, h->s64.bytes, h->stream));
+    OQ_HIP(hipMemsetAsync(h->sm.p, 0, h->sm.bytes, h->stream));
+    double *u = h->u.as<double>(), *du = h->du.as<double>();
+    float ms[2] = {0.f, 0.f};
+    for (int k = 0; k < 2; ++k) {
+        h->grid.split = k == 1;
+        int rc;
+        for (int i = 0; i < 3; ++i)
+            if ((rc = rhs_enqueue_matvec(h, u, du))) return rc;
+        OQ_HIP(hipEventRecord(h->ev0, h->stream));
+        for (int i = 0; i < 20; ++i)
+            if ((rc = rhs_enqueue_matvec(h, u, du))) return rc;
+        OQ_HIP(hipEventRecord(h->ev1, h->stream));
+        OQ_HIP(hipEventSynchronize(h->ev1));
+        OQ_HIP(hipEventElapsedTime(&ms[k], h->ev0, h->ev1));
+    }
+    h->grid.split = ms[1] < ms[0];
     return OQ_OK;
 }
 
@@ -1342,15 +1775,17 @@ static int fft_coupled_alloc(oq_rhs *h, DevBuf &st)
     h->ldm = (m6 + kPad - 1) / kPad * kPad;
     h->ldf = (h->nf + kPad - 1) / kPad * kPad;
     if (h->mmgrid) mm_grid_chunks(h->grid, h->ldm, h->ldf);
-    // grid mantle: the mantle rows hold only K_mf (6ne x ldf)
+    // grid mantle: the mantle rows hold only K_mf (6ne x ldf); along strike
+    // neither dense block exists
+    const bool dense = !(h->mmgrid && h->grid.strike);
     const size_t bf = (size_t)(h->nf * h->ldm) * 8, bm = (size_t)(m6 * (h->mmgrid ? h->ldf : h->ld)) * 8;
-    if (!rc) rc = h->Kf.alloc(bf);
-    if (!rc) rc = h->K.alloc(bm);
+    if (!rc && dense) rc = h->Kf.alloc(bf);
+    if (!rc && dense) rc = h->K.alloc(bm);
     if (!rc) rc = h->sm.alloc((size_t)h->ldm * 8);
     if (!rc) rc = h->tauff.alloc((size_t)h->nf * 8);
     if (rc) return rc;
-    OQ_HIP(hipMemsetAsync(h->Kf.p, 0, bf, h->stream));
-    OQ_HIP(hipMemsetAsync(h->K.p, 0, bm, h->stream));
+    if (dense) OQ_HIP(hipMemsetAsync(h->Kf.p, 0, bf, h->stream));
+    if (dense) OQ_HIP(hipMemsetAsync(h->K.p, 0, bm, h->stream));
     OQ_HIP(hipMemsetAsync(h->sm.p, 0, (size_t)h->ldm * 8, h->stream));
     return OQ_OK;
 }
@@ -1551,6 +1986,8 @@ int oq_rhs_create_from_mesh(int device, const oq_fault_mesh *mf, const oq_hex8_m
         h->mmgrid = true;
         h->grid = g;
         rc = mm_grid_shape(h->grid);
+        if (!rc)
+            h->grid.strike = mm_strike_detect(mf, h->grid, hx) && mm_strike_shape(h->grid, mf->nx, mf->nxi);
     }
     if (!rc) rc = upload_props(h, fault, visc);
     FaultUpload fu;
@@ -1567,11 +2004,17 @@ int oq_rhs_create_from_mesh(int device, const oq_fault_mesh *mf, const oq_hex8_m
         if (!rc) rc = launch_gf_ff(fu.dev, lambda, mu, ftype, (int)nrept, st.as<double>(), h->stream);
         if (!rc && hipStreamSynchronize(h->stream) != hipSuccess) rc = OQ_ERR_HIP;
         if (!rc) rc = fft_coupled_alloc(h, st);
-        if (!rc) rc = launch_gf_fm(hu.dev, fu.dev, lambda, mu, ftype, h->Kf.as<double>(), h->ldm, 1, h->stream);
-        if (!rc)
-            rc = launch_gf_mf(fu.dev, hu.dev, qu.c.as<double>(), qu.w.as<double>(), qu.nq, lambda, mu, ftype,
-                              (int)nrept, h->K.as<double>(), h->mmgrid ? h->ldf : ld, 1, h->stream);
+        if (h->mmgrid && h->grid.strike) {
+            if (!rc) rc = mm_strike_build(mf, ma, fu.dev.lrept, qu, lambda, mu, ftype, (int)nrept, h);
+        } else {
+            if (!rc) rc = launch_gf_fm(hu.dev, fu.dev, lambda, mu, ftype, h->Kf.as<double>(), h->ldm, 1, h->stream);
+            if (!rc)
+                rc = launch_gf_mf(fu.dev, hu.dev, qu.c.as<double>(), qu.w.as<double>(), qu.nq, lambda, mu, ftype,
+                                  (int)nrept, h->K.as<double>(), h->mmgrid ? h->ldf : ld, 1, h->stream);
+        }
         if (!rc && h->mmgrid) rc = mm_grid_build(ma, hx, hy, qu, lambda, mu, h);
+        if (!rc && h->mmgrid) rc = mm_roles_build(h);
+        if (!rc && h->mmgrid) rc = mm_grid_pick(h);
         else if (!rc)
             rc = launch_gf_mm(hu.dev, qu.c.as<double>(), qu.w.as<double>(), qu.nq, lambda, mu,
                               h->K.as<double>() + nf, ld, 1, h->stream);
@@ -1732,7 +2175,8 @@ int oq_rhs_info(oq_rhs *h, int64_t *nstate, int64_t *n, int64_t *kbytes)
     OQ_REQUIRE(h != nullptr, "handle is NULL");
     if (nstate) *nstate = h->lay.n;
     if (n) *n = h->N;
-    if (kbytes) *kbytes = (int64_t)(h->K.bytes + h->Kf.bytes + h->stk.bytes + h->Tmm.bytes);
+    if (kbytes)
+        *kbytes = (int64_t)(h->K.bytes + h->Kf.bytes + h->stk.bytes + h->Tmm.bytes + h->Ftab.bytes + h->Gtab.bytes);
     return OQ_OK;
 }
 
@@ -1751,11 +2195,21 @@ int oq_rhs_get_k(oq_rhs *h, double *k)
         if ((rc = launch_expand_toeplitz(h->stk.as<double>(), h->nx, h->nxi, tmp.as<double>(), h->ld,
                                          h->stream)))
             return rc;
-        OQ_HIP(hipMemcpy2DAsync(tmp.as<double>() + nf, (size_t)h->ld * 8, h->Kf.p, (size_t)h->ldm * 8,
-                                (size_t)m6 * 8, (size_t)nf, hipMemcpyDeviceToDevice, h->stream));
+        if (!(h->mmgrid && h->grid.strike))
+            OQ_HIP(hipMemcpy2DAsync(tmp.as<double>() + nf, (size_t)h->ld * 8, h->Kf.p, (size_t)h->ldm * 8,
+                                    (size_t)m6 * 8, (size_t)nf, hipMemcpyDeviceToDevice, h->stream));
+        if (h->mmgrid && h->grid.strike) {
+            const MmGrid &g = h->grid;
+            hipLaunchKernelGGL(k_expand_strike, dim3(grid_capped(2 * nf * m6, 256)), dim3(256), 0, h->stream,
+                               h->Ftab.as<double>(), h->Gtab.as<double>(), h->nx, h->nxi, g.nx, 6 * g.ny * g.nz, g.m,
+                               g.D, tmp.as<double>(), h->ld);
+            OQ_HIP(hipGetLastError());
+        }
         if (h->mmgrid) {
-            OQ_HIP(hipMemcpy2DAsync(tmp.as<double>() + nf * h->ld, (size_t)h->ld * 8, h->K.p, (size_t)h->ldf * 8,
-                                    (size_t)nf * 8, (size_t)m6, hipMemcpyDeviceToDevice, h->stream));
+            if (!h->grid.strike)
+                OQ_HIP(hipMemcpy2DAsync(tmp.as<double>() + nf * h->ld, (size_t)h->ld * 8, h->K.p,
+                                        (size_t)h->ldf * 8, (size_t)nf * 8, (size_t)m6, hipMemcpyDeviceToDevice,
+                                        h->stream));
             const MmGrid &g = h->grid;
             hipLaunchKernelGGL(k_expand_mm_grid, dim3(grid_capped(m6 * m6, 256)), dim3(256), 0, h->stream,
                                h->Tmm.as<double>(), g.nx, g.ny, g.nz, (int64_t)g.nxp,

@@ -183,6 +183,24 @@ struct MmGrid {
     size_t lds = 0;   // dynamic LDS bytes
     int fs = 0;       // column chunks of the fault rows' K_fm pass
     int64_t fcw = 0;  // columns per chunk
+    // Along strike: when the mantle's x spacing is m times the fault's strike
+    // spacing, K_fm and K_mf depend on a patch-cell pair's x offset only
+    // through a - m jx and are kept as tables of D = nx_f + m (nx - 1)
+    // offsets per line (strike = true): F[b][q][a - m jx + m (nx - 1)] over
+    // fault rows b and mantle source lines q = (p nz + jz) ny + jy, and
+    // G[q'][b][m jx - a + nx_f - 1] over mantle receiver lines q' (stress
+    // component k in place of p) and fault rows b.
+    bool strike = false;
+    int64_t m = 0, D = 0;
+    int fqs = 0, fqb = 0, nfc = 0;  // F blocks: lines per block, per LDS batch; blocks per fault row
+    int gbs = 0, gbb = 0, ngc = 0;  // G blocks: fault rows per block, per LDS batch; blocks per line
+    int nr = 0, fseg = 0, gseg = 0;  // ceil(nx_f / m); register-window segments of F and G outputs
+    int Fp = 0, Gp = 0;              // LDS row widths of the F and G tiles
+    int flpp = 0;                    // F lines in flight per block
+    int nblocks = 0;                // blocks of the fused launch (role table entries)
+    int nconv = 0;                  // convolution blocks (the first launch when split)
+    size_t lds_fused = 0, lds_other = 0;  // dynamic LDS of the fused launch, of the second split launch
+    bool split = false;             // convolutions and the other roles as two launches (measured faster)
 };
 
 }  // namespace oq
@@ -211,8 +229,10 @@ struct oq_rhs {
     // the mantle rows' K_mf (6ne x ldf); K_mm is the grid kernel table Tmm,
     // applied by the convolution blocks of k_grid_fused into nchunk partial
     // sums per mantle row (mpart, nchunk x 6ne) that k_grid_finish adds
-    oq::DevBuf Tmm, mpart, fpart, mvout;  // fpart: K_ff rates + K_fm chunk partials ((fs + 1) x nf);
-                                          // mvout: K_mf s_f (6ne)
+    oq::DevBuf Tmm, mpart, fpart, mvout;  // fpart: K_ff rates + K_fm chunk partials; mvout: K_mf s_f
+                                          // partials (1 or ngc x 6ne)
+    oq::DevBuf Ftab, Gtab, roles, roles2;  // along-strike tables; block roles (fused, interleaved; split:
+                                           // convolutions first)
     bool mmgrid = false;
     int64_t ldf = 0;
     oq::MmGrid grid{};

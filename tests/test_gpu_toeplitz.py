@@ -153,12 +153,14 @@ def _blocks(K, nf):
 def test_fft_grid_matches_dense_and_oracle(oracle, rng, dims, rfzh):
     """OQ_K_FFT_GRID: K_ff by DFT, K_mm as 2D convolutions of its per-offset
     kernel over a mantle on a uniform horizontal grid (graded layers
-    allowed).  The fault rows and K_mf expand to the dense device assembly
-    bit for bit, K_mm to it within 1e-12 (its kernel comes from shifted
-    receivers, equal to rounding), every RHS component within 1e-12 of the
-    dense model's and of the oracle's coupled RHS on the oracle's blocks
-    (within the two assemblies' 1e-10 agreement), and a Tsit5 trajectory
-    follows the dense one."""
+    allowed).  With 5 cells over 40 km the mantle spacing (8 km) is 6.4
+    fault spacings and K_fm, K_mf stay dense (bit for bit the dense
+    assembly's); with 4 cells (10 km = 8 spacings) they are kept as tables
+    along strike.  Table-built blocks equal the dense assembly within 1e-12
+    (shifted virtual receivers, equal to rounding); every RHS component is
+    within 1e-12 of the dense model's and of the oracle's coupled RHS on the
+    oracle's blocks (within the two assemblies' 1e-10 agreement), and a
+    Tsit5 trajectory follows the dense one."""
     from helpers import visc_props
     mf = small_fault(60.0, x=40e3, xi=15e3, d=1.25e3)
     ma = oq.hex8_box(-20e3, -6e3, -10e3, 40e3, 12e3, -16e3, *dims, rfzh=rfzh)
@@ -170,8 +172,13 @@ def test_fft_grid_matches_dense_and_oracle(oracle, rng, dims, rfzh):
     pd = oq.assemble_from_mesh(mf, ma, lam, mu, pf, pa, u, (0.0, 0.05 * YEAR), **kw)
     nf = mf.nx * mf.nxi
     gk, dk = _blocks(pg.f.dense_operator(), nf), _blocks(pd.f.dense_operator(), nf)
-    assert np.array_equal(gk[0], dk[0]) and np.array_equal(gk[1], dk[1])
-    assert np.abs(gk[2] - dk[2]).max() <= 1e-12 * np.abs(dk[2]).max()
+    assert np.array_equal(gk[0][:, :nf], dk[0][:, :nf])  # K_ff
+    close = lambda a, b: np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
+    if dims[0] == 5:
+        assert np.array_equal(gk[0], dk[0]) and np.array_equal(gk[1], dk[1])
+    else:
+        assert close(gk[0][:, nf:], dk[0][:, nf:]) and close(gk[1], dk[1])
+    assert close(gk[2], dk[2])
     _, _, kb = pg.f.info()
     assert kb < pd.f.info()[2]
     dg, dd = oq.similar(u), oq.similar(u)
