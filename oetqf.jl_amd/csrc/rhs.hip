@@ -852,6 +852,7 @@ struct GridSt {
     int fqs, fqb, gbs, gbb;
     int nr, fseg, gseg, Fp, Gp;  // nr = ceil(nx / m); segments of kRW outputs; padded row widths
     int flpp;                    // F: lines in flight (<= 256 / (m fseg))
+    int sw;                      // G: LDS pitch of the s_f rows (>= m nr)
     int64_t nf, m6;
     double *fpart, *gpart;
 };
@@ -917,7 +918,7 @@ __device__ __forceinline__ void grid_f_block(const GridSt &g, int b, int c, doub
 
 __device__ __forceinline__ void grid_g_block(const GridSt &g, int qq, int c, double *sh)
 {
-    const int nx = g.nx, D = g.D, nxc = g.nxc, m = g.m, nxi = g.nxi, Gp = g.Gp, nr = g.nr, sw = m * nr;
+    const int nx = g.nx, D = g.D, nxc = g.nxc, m = g.m, nxi = g.nxi, Gp = g.Gp, nr = g.nr, sw = g.sw;
     const int b0 = c * g.gbs, b1 = (b0 + g.gbs < nxi ? b0 + g.gbs : nxi);
     // a segment takes whole wavefronts (wps of them); in it, thread (row slot bi, residue a0)
     const int wps = 4 / g.gseg, tps = 64 * wps, bp = tps / m;
@@ -937,7 +938,7 @@ __device__ __forceinline__ void grid_g_block(const GridSt &g, int qq, int c, dou
         }
         for (int x = t; x < nb * sw; x += 256) {
             const int b = x / sw, a = x - b * sw;
-            Sf[x] = a < nx ? g.sf[(size_t)(bb + b) * nx + a] : 0.0;
+            Sf[x] = a < nx ? g.sf[(size_t)(bb + b) * nx + a] : 0.0;  // a < m nr <= sw read below
         }
         __syncthreads();
         for (int b = bi; active && b < nb; b += bp) {
@@ -1170,7 +1171,7 @@ int rhs_enqueue_matvec(oq_rhs *h, const double *u, double *du)
                               h->ldm, h->ldf, g.fcw, g.fs, h->fpart.as<double>(), h->mvout.as<double>()};
             const GridSt g2 = {h->Ftab.as<double>(), h->Gtab.as<double>(), h->s64.as<double>(), h->sm.as<double>(),
                                (int)h->nx, (int)h->nxi, (int)g.nx, (int)(6 * g.ny * g.nz), (int)g.m, (int)g.D,
-                               g.fqs, g.fqb, g.gbs, g.gbb, g.nr, g.fseg, g.gseg, g.Fp, g.Gp, g.flpp, h->nf, 6 * h->ne,
+                               g.fqs, g.fqb, g.gbs, g.gbb, g.nr, g.fseg, g.gseg, g.Fp, g.Gp, g.flpp, g.sw, h->nf, 6 * h->ne,
                                h->fpart.as<double>(), h->mvout.as<double>()};
             if (g.r == 8) grid_launch_r<8>(h, a, g2);
             else grid_launch_r<4>(h, a, g2);
@@ -1532,11 +1533,14 @@ static bool mm_strike_shape(MmGrid &g, int64_t nx, int64_t nxi)
 {
     const int64_t cap = (int64_t)(kGridOtherLds / 8), Q = 6 * g.ny * g.nz, m = g.m;
     const int64_t nr = (nx + m - 1) / m, fseg = (nr + kRW - 1) / kRW, gseg = (g.nx + kRW - 1) / kRW;
-    const int64_t Fp = m * (fseg * kRW + g.nx - 1), Gp = nx + m * gseg * kRW;
+    // G's row pitches are 8 mod 32 doubles, so that the eight row slots of a
+    // wavefront fall on different LDS banks
+    const auto pitch8 = [](int64_t w) { return w + ((8 - w % 32) + 32) % 32; };
+    const int64_t Fp = m * (fseg * kRW + g.nx - 1), Gp = pitch8(nx + m * gseg * kRW), sw = pitch8(m * nr);
     if (m * fseg > 256 || gseg > 4 || m > 64 * (4 / gseg) || nx > 2048) return false;
     int64_t lpp = 256 / (m * fseg);
     if (lpp * nx > cap) lpp = cap / nx;  // the line slots' partial sums share the LDS
-    const int64_t fqb = cap / (Fp + g.nx), gbb = cap / (Gp + m * nr);
+    const int64_t fqb = cap / (Fp + g.nx), gbb = cap / (Gp + sw);
     if (fqb < 1 || gbb < 1 || lpp < 1 || 4 * kRW > cap) return false;
     g.flpp = (int)lpp;
     g.nr = (int)nr;
@@ -1544,12 +1548,13 @@ static bool mm_strike_shape(MmGrid &g, int64_t nx, int64_t nxi)
     g.gseg = (int)gseg;
     g.Fp = (int)Fp;
     g.Gp = (int)Gp;
+    g.sw = (int)sw;
     g.fqb = (int)fqb;
     g.gbb = (int)gbb;
-    const int64_t nfc = Q < 8 ? Q : 8, fqs = (Q + nfc - 1) / nfc;
+    const int64_t nfc = Q < 16 ? Q : 16, fqs = (Q + nfc - 1) / nfc;
     g.fqs = (int)fqs;
     g.nfc = (int)((Q + fqs - 1) / fqs);
-    const int64_t ngc = nxi < 2 ? nxi : 2, gbs = (nxi + ngc - 1) / ngc;
+    const int64_t ngc = 1, gbs = nxi;
     g.gbs = (int)gbs;
     g.ngc = (int)((nxi + gbs - 1) / gbs);
     return true;

@@ -197,6 +197,7 @@ struct MmGrid {
     int nr = 0, fseg = 0, gseg = 0;  // ceil(nx_f / m); register-window segments of F and G outputs
     int Fp = 0, Gp = 0;              // LDS row widths of the F and G tiles
     int flpp = 0;                    // F lines in flight per block
+    int sw = 0;                      // G: LDS pitch of the s_f rows
     int nblocks = 0;                // blocks of the fused launch (role table entries)
     int nconv = 0;                  // convolution blocks (the first launch when split)
     size_t lds_fused = 0, lds_other = 0;  // dynamic LDS of the fused launch, of the second split launch
