@@ -289,8 +289,8 @@ def ensemble_leg(oq, args, w, u0, value):
 
 def scale_leg(oq, args):
     """The precision variant at scale: 256x64 fault + 32x32x8 mantle
-    (N = 65536, K 34.4 GB in FP64; fft_grid: FP64 with K_ff by DFT and K_mm
-    on the mantle grid, 12.9 GB).  Per storage type: device assembly time,
+    (N = 65536, K 34.4 GB in FP64; fft_grid: FP64 with K_ff by DFT, K_mm on
+    the mantle grid and K_fm, K_mf along strike, 0.88 GB).  Per storage type: device assembly time,
     RHS rate and HBM rate, RHS error against FP64 on a perturbed state, and
     a trajectory of `scale_years` from the interseismic start with its
     event catalogue (max V > 1e-3 m/s, crossings within 1 yr merged); when

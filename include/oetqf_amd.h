@@ -69,9 +69,14 @@ extern "C" {
  * cells then depends on their horizontal offset and their two layers only,
  * so K_mm is held as 36 nz^2 tiles of (2ny-1) x (2nx-1) FP64 entries
  * (17.7 MB for 16 x 16 x 8 cells instead of 1.21 GB), assembled by the same
- * pair kernel for the shifted receivers; K_fm and K_mf stay dense FP64.
- * No counterpart in the reference (which multiplies the dense K_mm,
- * equation.jl:203); entries equal the dense block's to rounding. */
+ * pair kernel for the shifted receivers.  When the mantle's x spacing is m
+ * times the fault's strike spacing (m a whole number, at most 32 cells
+ * along x), K_fm and K_mf depend on a patch-cell pair's x offset only
+ * through a - m jx and are held as tables of nx_f + m (nx - 1) offsets per
+ * fault row and mantle line (49 MB each at the 128x32 fault with 16x16x8
+ * cells instead of 403 MB); otherwise they stay dense FP64.  All FP64.
+ * No counterpart in the reference (which multiplies the dense blocks,
+ * equation.jl:201-203); entries equal the dense blocks' to rounding. */
 #define OQ_K_FFT_GRID 6
 
 /* RectOkadaMesh (src/BEM/mesh.jl:5-21).  Only x, ax, y, z, axi, nx, nxi, dx,
