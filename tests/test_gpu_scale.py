@@ -5,8 +5,10 @@ The oracle rebuilds 12 fault rows and 24 mantle rows of K from the meshes
 with its own Okada and hex8 code (Toeplitz K_ff, then K_fm, K_mf and K_mm
 rows) and evaluates those rows of the RHS.  The device FP64 RHS must agree
 to 1e-10, the Green's-function tolerance: the two operators are assembled
-independently.  FP32 and BF16 storage of the same operator are then
-measured against the device FP64 RHS over the whole state."""
+independently.  FP32 and BF16 storage of the same operator, and the FP64
+grid operator (K_ff by DFT, K_mm on the 32x32x8 mantle grid, K_fm and K_mf
+as tables along strike), are then measured against the device FP64 RHS
+over the whole state."""
 from types import SimpleNamespace
 
 import numpy as np
@@ -77,7 +79,7 @@ def test_scale_rows_match_oracle(oracle, scale):
     assert e_dv <= 1e-10 and e_ds <= 1e-10
 
 
-@pytest.mark.parametrize("kdtype,tol", [("fp32", 1e-5), ("bf16", 5e-2)])
+@pytest.mark.parametrize("kdtype,tol", [("fp32", 1e-5), ("bf16", 5e-2), ("fft_grid", 1e-12)])
 def test_scale_precision_variants(scale, kdtype, tol):
     w, u, du = scale
     prob = oq.assemble_from_mesh(w["mf"], w["ma"], w["lam"], w["mu"], w["pf"], w["pa"], w["u0"],
