@@ -42,6 +42,23 @@ __device__ __forceinline__ void hex8_image_kernel(double X, double Y, double zr,
 #include "hex8_image.inc"
 }
 
+// The pair sums of the image kernel that the shear traction on a fault of
+// type FT takes (P[p], P[6 + p] for eigenstrain pair p; see
+// h8_image_traction), simplified as a whole before their common
+// subexpressions are taken: about half the operations of the 24 kernel
+// entries they are sums of.
+template <int FT>
+__device__ __forceinline__ void hex8_image_pairs(double X, double Y, double zr, double ys, double nu,
+                                                 double *P)
+{
+    const double R = sqrt(X * X + Y * Y + (zr + ys) * (zr + ys));
+    if constexpr (FT == 0) {
+#include "hex8_image_trac0.inc"
+    } else {
+#include "hex8_image_trac1.inc"
+    }
+}
+
 __device__ __forceinline__ double h8_logrx(double R, double xc, double rho2)
 {
     if (xc >= 0.0) return log(R + xc);
@@ -267,7 +284,7 @@ __device__ void h8_image_traction(double x, double y, double z, const double lo0
     const double xi[3] = {x, y, -z};
     unsigned long long path = 0;
     int depth = 0;
-    double Dk[54];
+    double Pk[12];
     for (;;) {
         double lo[3], hi[3], h[3], ctr[3], dist2 = 0.0, hd2 = 0.0;
         h8_subbox(lo0, hi0, path, depth, lo, hi);
@@ -292,16 +309,9 @@ __device__ void h8_image_traction(double x, double y, double z, const double lo0
                     const double ys2 = ctr[1] + h[1] * kGLX[OQ_H8_NQ][j];
                     const double ys3 = ctr[2] + h[2] * kGLX[OQ_H8_NQ][k];
                     const double w = kGLW[OQ_H8_NQ][i] * kGLW[OQ_H8_NQ][j] * kGLW[OQ_H8_NQ][k] * vol;
-                    hex8_image_kernel(x - ys1, y - ys2, z, ys3, nu, Dk);
+                    hex8_image_pairs<FT>(x - ys1, y - ys2, z, ys3, nu, Pk);
 #pragma unroll
-                    for (int q = 0; q < 6; ++q) {
-                        const double t = FT == 0
-                                             ? ca * (Dk[0 * 18 + 1 * 6 + q] + Dk[1 * 18 + 0 * 6 + q]) +
-                                                   cb * (Dk[0 * 18 + 2 * 6 + q] + Dk[2 * 18 + 0 * 6 + q])
-                                             : ca * (Dk[2 * 18 + 2 * 6 + q] - Dk[1 * 18 + 1 * 6 + q]) +
-                                                   cb * (Dk[1 * 18 + 2 * 6 + q] + Dk[2 * 18 + 1 * 6 + q]);
-                        E[q] += w * t;
-                    }
+                    for (int q = 0; q < 6; ++q) E[q] += w * (ca * Pk[q] + cb * Pk[6 + q]);
                 }
         while (depth > 0 && (path & 1ull)) {
             path >>= 1;

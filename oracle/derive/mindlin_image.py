@@ -15,6 +15,7 @@ Mindlin's solution is checked here before use: Navier equilibrium,
 traction-free surface and reciprocity, symbolically evaluated at a point.
 
 Run:  python oracle/derive/mindlin_image.py OUT.inc  [--check] [--device]
+      python oracle/derive/mindlin_image.py --traction FT OUT.inc [--halves]
 The output is written verbatim to oracle/hex8_image.inc; with --device, to
 oetqf.jl_amd/csrc/hex8_image.inc, where every negative power of R and of
 zr + ys - R is raised from the one reciprocal of each that the common
@@ -161,9 +162,86 @@ def emit(path, device=False):
     print("wrote", path, len(reps), "temporaries")
 
 
+# The pairs of D entries that the shear traction on a fault takes (the
+# traction projection of GF.jl:89-96 applied to the image stress): strike
+# slip -sigma_xy sin(dip) + sigma_xz cos(dip) needs D[0][1] + D[1][0] and
+# D[0][2] + D[2][0]; dip slip (sigma_zz - sigma_yy)/2 sin(2 dip) + sigma_yz
+# cos(2 dip) needs D[2][2] - D[1][1] and D[1][2] + D[2][1] (per eigenstrain
+# pair p; the coefficients multiply them in the kernel).
+TRACTION_PAIRS = {0: (((0, 1), (1, 0), 1), ((0, 2), (2, 0), 1)),
+                  1: (((2, 2), (1, 1), -1), ((1, 2), (2, 1), 1))}
+
+
+def traction_pairs(ft):
+    R = sp.Symbol("R", positive=True)
+    r2 = X**2 + Y**2 + (-ys - zr) ** 2
+    D = [sp.powsimp(e.subs(r2, R**2), force=True) for e in image_kernel()]
+    at = lambda il, p: D[il[0] * 18 + il[1] * 6 + p]
+    out = []
+    for a, b, sgn in TRACTION_PAIRS[ft]:
+        for p in range(6):
+            out.append(at(a, p) + sgn * at(b, p))
+    return out, [sp.simplify(e) for e in out]
+
+
+def emit_traction(path, ft, halves=False):
+    """P[12] = the two pair sums of TRACTION_PAIRS[ft] for the six p,
+    simplified before the common subexpressions are taken (half the
+    operations of the 24 entries they come from); checked against the
+    unsimplified sums at random points."""
+    import random
+    import re
+    R = sp.Symbol("R", positive=True)
+    raw, exprs = traction_pairs(ft)
+    rnd = random.Random(7)
+    worst = 0.0
+    for _ in range(20):
+        v = {X: rnd.uniform(-3, 3), Y: rnd.uniform(-3, 3), zr: -rnd.uniform(0.1, 3),
+             ys: -rnd.uniform(0.1, 3), nu: rnd.uniform(0.1, 0.45)}
+        v[R] = float(sp.sqrt(v[X] ** 2 + v[Y] ** 2 + (v[zr] + v[ys]) ** 2))
+        for a, b in zip(raw, exprs):
+            fa, fb = float(a.subs(v)), float(b.subs(v))
+            worst = max(worst, abs(fa - fb) / max(abs(fa), 1e-300))
+    print("traction pairs ft=%d: simplified vs raw, max relative difference %.1e" % (ft, worst))
+    assert worst < 1e-12
+    lines = ["/* generated by oracle/derive/mindlin_image.py --traction %d -- do not edit." % ft,
+             " * Inputs: X, Y (receiver - source, horizontal), zr, ys (receiver and",
+             " * source z, both <= 0), nu, R = |(X, Y, zr + ys)|.  Output P[12]: the",
+             " * pair sums of the image kernel the shear traction takes, P[p] and",
+             " * P[6 + p] for eigenstrain pair p. */"]
+
+    def cc(e):
+        t = sp.ccode(e)
+        t = re.sub(r"pow\(([A-Za-z0-9_]+), (-?\d+)\)", r"OQ_IPOW(\1, \2)", t)
+        assert "pow(" not in t and "sqrt(" not in t, t
+        return t
+    # halves: the two sums of each pair in scopes of their own (fewer values
+    # live at once, some subexpressions computed twice)
+    groups = [(0, exprs[:6]), (6, exprs[6:])] if halves else [(0, exprs)]
+    ntemp = 0
+    for base, grp in groups:
+        reps, red = sp.cse(grp, optimizations="basic")
+        ntemp += len(reps)
+        if halves:
+            lines.append("{")
+        for sym, e in reps:
+            lines.append("const double %s = %s;" % (sym, cc(e)))
+        for n, e in enumerate(red):
+            lines.append("P[%d] = %s;" % (base + n, cc(e)))
+        if halves:
+            lines.append("}")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("wrote", path, ntemp, "temporaries")
+
+
 if __name__ == "__main__":
     if "--check" in sys.argv:
         check()
+    if "--traction" in sys.argv:
+        ft = int(sys.argv[sys.argv.index("--traction") + 1])
+        emit_traction(sys.argv[sys.argv.index("--traction") + 2], ft, halves="--halves" in sys.argv)
+        sys.exit(0)
     for a in sys.argv[1:]:
         if not a.startswith("--"):
             emit(a, device="--device" in sys.argv)
