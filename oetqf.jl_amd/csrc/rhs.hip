@@ -1554,9 +1554,8 @@ static bool mm_strike_shape(MmGrid &g, int64_t nx, int64_t nxi)
     const int64_t nfc = Q < 16 ? Q : 16, fqs = (Q + nfc - 1) / nfc;
     g.fqs = (int)fqs;
     g.nfc = (int)((Q + fqs - 1) / fqs);
-    const int64_t ngc = 1, gbs = nxi;
-    g.gbs = (int)gbs;
-    g.ngc = (int)((nxi + gbs - 1) / gbs);
+    g.gbs = (int)nxi;  // one G block per mantle line takes every fault row
+    g.ngc = 1;
     return true;
 }
 
@@ -1787,7 +1786,7 @@ static int fft_coupled_alloc(oq_rhs *h, DevBuf &st)
     if (!rc && dense) rc = h->Kf.alloc(bf);
     if (!rc && dense) rc = h->K.alloc(bm);
     if (!rc) rc = h->sm.alloc((size_t)h->ldm * 8);
-    if (!rc) rc = h->tauff.alloc((size_t)h->nf * 8);
+    if (!rc && !h->mmgrid) rc = h->tauff.alloc((size_t)h->nf * 8);  // the grid model uses fpart
     if (rc) return rc;
     if (dense) OQ_HIP(hipMemsetAsync(h->Kf.p, 0, bf, h->stream));
     if (dense) OQ_HIP(hipMemsetAsync(h->K.p, 0, bm, h->stream));
