@@ -15,8 +15,13 @@
 //     their longest axis until each is >= 5 half-diagonals from the
 //     receiver's image point.
 //
+// The image kernel D[i][l][p] = d2 G_image_ij / dx_l dy_k summed over the
+// moment pair p = (j, k) (without 1/(16 pi mu (1-nu))) enters only through
+// the combinations a stress (36) or a fault's shear traction (12) takes,
+// generated and simplified by oracle/derive/mindlin_image.py.
+//
 // GPU shape: everything that does not depend on the eigenstrain (the
-// Eshelby tensors and the 54 quadrature sums of the image kernel) is
+// Eshelby tensors and the quadrature sums of the image combinations) is
 // accumulated once per receiver-cell pair; the 6 unit eigenstrains the
 // builders need are then contractions.  The reference calls the closed form
 // once per unit eigenstrain (6x the geometric work).
@@ -32,14 +37,6 @@ __device__ __forceinline__ double oq_ipow_dev(double a, int n)
     const int m = n < 0 ? -n : n;
     for (int i = 0; i < m; ++i) r *= a;
     return n < 0 ? 1.0 / r : r;
-}
-
-// Regular image kernel: D[i*18 + l*6 + p] (without 1/(16 pi mu (1-nu))).
-__device__ __forceinline__ void hex8_image_kernel(double X, double Y, double zr, double ys,
-                                                  double nu, double *D)
-{
-    const double R = sqrt(X * X + Y * Y + (zr + ys) * (zr + ys));
-#include "hex8_image.inc"
 }
 
 // The pair sums of the image kernel that the shear traction on a fault of
@@ -192,15 +189,28 @@ __device__ __forceinline__ void h8_subbox(const double lo0[3], const double hi0[
     }
 }
 
-// Accumulates the 54 image-kernel integrals over [lo0, hi0] into D
-// (depth-first over the bisection tree, lower child first).
+// The combinations of the image kernel that a stress takes: P[6 q + c] for
+// moment pair q, c = D[0][0], D[1][1], D[2][2], D[0][1] + D[1][0],
+// D[0][2] + D[2][0], D[1][2] + D[2][1] (the image part of grad u enters the
+// stress only through its diagonal and symmetric sums), simplified as a
+// whole before their common subexpressions are taken: about 60 % of the
+// operations of the 54 kernel entries.
+__device__ __forceinline__ void hex8_image_stress_combos(double X, double Y, double zr, double ys, double nu,
+                                                         double *P)
+{
+    const double R = sqrt(X * X + Y * Y + (zr + ys) * (zr + ys));
+#include "hex8_image_stress.inc"
+}
+
+// Accumulates the 36 image stress combinations integrated over [lo0, hi0]
+// into acc (depth-first over the bisection tree, lower child first).
 __device__ void h8_image_integral(double x, double y, double z, const double lo0[3],
                                   const double hi0[3], double nu, double *acc)
 {
     const double xi[3] = {x, y, -z};
     unsigned long long path = 0;
     int depth = 0;
-    double Dk[54];
+    double Dk[36];
     for (;;) {
         double lo[3], hi[3], h[3], ctr[3], dist2 = 0.0, hd2 = 0.0;
         h8_subbox(lo0, hi0, path, depth, lo, hi);
@@ -225,9 +235,9 @@ __device__ void h8_image_integral(double x, double y, double z, const double lo0
                     const double ys2 = ctr[1] + h[1] * kGLX[OQ_H8_NQ][j];
                     const double ys3 = ctr[2] + h[2] * kGLX[OQ_H8_NQ][k];
                     const double w = kGLW[OQ_H8_NQ][i] * kGLW[OQ_H8_NQ][j] * kGLW[OQ_H8_NQ][k] * vol;
-                    hex8_image_kernel(x - ys1, y - ys2, z, ys3, nu, Dk);
+                    hex8_image_stress_combos(x - ys1, y - ys2, z, ys3, nu, Dk);
 #pragma unroll
-                    for (int m = 0; m < 54; ++m) acc[m] += w * Dk[m];
+                    for (int m = 0; m < 36; ++m) acc[m] += w * Dk[m];
                 }
         // next leaf in depth-first order
         while (depth > 0 && (path & 1ull)) {
@@ -325,7 +335,7 @@ __device__ void h8_image_traction(double x, double y, double z, const double lo0
 // Per receiver-cell pair: everything independent of the eigenstrain.
 struct Hex8Pair {
     Hex8Kelvin K;
-    double D[54];
+    double D[36];  // image stress combinations, integrated (h8_image_integral)
     bool inside;
 };
 
@@ -337,7 +347,7 @@ __device__ void hex8_pair(double x, double y, double z, double qx, double qy, do
     // The image integral first: its quadrature loop then runs without the
     // 24 Kelvin sums live, which takes most of its register spills away.
 #pragma unroll
-    for (int m = 0; m < 54; ++m) P.D[m] = 0.0;
+    for (int m = 0; m < 36; ++m) P.D[m] = 0.0;
     h8_image_integral(x, y, z, lo, hi, nu, P.D);
     P.inside = hex8_kelvin(x, y, z, lo, hi, P.K);
 }
@@ -393,21 +403,27 @@ __device__ __forceinline__ void hex8_stress_unit_body(const Hex8Kelvin &K, const
             for (int k = 0; k < 3; ++k)
                 if (e[i][k] != 0.0) acc -= 4.0 * (1.0 - nu) * e[i][k] * h8_phi(K, k, l);
             acc -= 2.0 * nu * tr * h8_phi(K, i, l);
-            double g = 0.0;
-            if (IMG) {
-#pragma unroll
-                for (int q = 0; q < 6; ++q) g += D[i * 18 + l * 6 + q] * mv[q];
-            }
-            gu[i][l] = acc * ck + g * ci;
+            gu[i][l] = acc * ck;
         }
-    const double div = gu[0][0] + gu[1][1] + gu[2][2];
+    // image part: the diagonal and the symmetric sums of its grad u
+    double gi[6] = {0, 0, 0, 0, 0, 0};
+    if (IMG) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            double g = 0.0;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) g += D[6 * q + c] * mv[q];
+            gi[c] = g * ci;
+        }
+    }
+    const double div = gu[0][0] + gu[1][1] + gu[2][2] + (gi[0] + gi[1] + gi[2]);
     const double in = inside ? 1.0 : 0.0;
-    s[0] = 2.0 * mu * gu[0][0] + lambda * div - in * m[0][0];
-    s[1] = mu * (gu[0][1] + gu[1][0]) - in * m[0][1];
-    s[2] = mu * (gu[0][2] + gu[2][0]) - in * m[0][2];
-    s[3] = 2.0 * mu * gu[1][1] + lambda * div - in * m[1][1];
-    s[4] = mu * (gu[1][2] + gu[2][1]) - in * m[1][2];
-    s[5] = 2.0 * mu * gu[2][2] + lambda * div - in * m[2][2];
+    s[0] = 2.0 * mu * (gu[0][0] + gi[0]) + lambda * div - in * m[0][0];
+    s[1] = mu * (gu[0][1] + gu[1][0] + gi[3]) - in * m[0][1];
+    s[2] = mu * (gu[0][2] + gu[2][0] + gi[4]) - in * m[0][2];
+    s[3] = 2.0 * mu * (gu[1][1] + gi[1]) + lambda * div - in * m[1][1];
+    s[4] = mu * (gu[1][2] + gu[2][1] + gi[5]) - in * m[1][2];
+    s[5] = 2.0 * mu * (gu[2][2] + gi[2]) + lambda * div - in * m[2][2];
 }
 
 // Stress s[6] for the unit eigenstrain component p (0..5 = xx xy xz yy yz zz).

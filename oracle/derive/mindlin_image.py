@@ -16,11 +16,14 @@ traction-free surface and reciprocity, symbolically evaluated at a point.
 
 Run:  python oracle/derive/mindlin_image.py OUT.inc  [--check] [--device]
       python oracle/derive/mindlin_image.py --traction FT OUT.inc [--halves]
-The output is written verbatim to oracle/hex8_image.inc; with --device, to
-oetqf.jl_amd/csrc/hex8_image.inc, where every negative power of R and of
-zr + ys - R is raised from the one reciprocal of each that the common
-subexpressions already hold (3 IEEE divisions per kernel evaluation instead
-of 15; the values differ from the oracle's by a few ulps).
+      python oracle/derive/mindlin_image.py --stress OUT.inc [--halves]
+The output is written verbatim to oracle/hex8_image.inc (--device raises
+every negative power of R and of zr + ys - R from the one reciprocal of
+each that the common subexpressions already hold).  --stress and --traction
+write the device's combinations of the kernel (the diagonal and symmetric
+sums a stress takes, as oetqf.jl_amd/csrc/hex8_image_stress.inc; the pair
+sums a fault's shear traction takes, hex8_image_trac0.inc with --halves and
+hex8_image_trac1.inc), simplified as a whole.
 """
 import sys
 
@@ -184,6 +187,24 @@ def traction_pairs(ft):
     return out, [sp.simplify(e) for e in out]
 
 
+# The combinations of D entries that a full stress takes (the image part of
+# grad u enters sigma = lambda tr(eps) I + 2 mu eps only through the
+# diagonal and the symmetric sums): per moment pair q, D[0][0], D[1][1],
+# D[2][2], D[0][1] + D[1][0], D[0][2] + D[2][0], D[1][2] + D[2][1].
+STRESS_COMBOS = (((0, 0),), ((1, 1),), ((2, 2),), ((0, 1), (1, 0)), ((0, 2), (2, 0)), ((1, 2), (2, 1)))
+
+
+def stress_combos():
+    R = sp.Symbol("R", positive=True)
+    r2 = X**2 + Y**2 + (-ys - zr) ** 2
+    D = [sp.powsimp(e.subs(r2, R**2), force=True) for e in image_kernel()]
+    out = []
+    for q in range(6):
+        for combo in STRESS_COMBOS:
+            out.append(sum(D[i * 18 + l * 6 + q] for i, l in combo))
+    return out, [sp.simplify(e) for e in out]
+
+
 def emit_traction(path, ft, halves=False):
     """P[12] = the two pair sums of TRACTION_PAIRS[ft] for the six p,
     simplified before the common subexpressions are taken (half the
@@ -192,7 +213,7 @@ def emit_traction(path, ft, halves=False):
     import random
     import re
     R = sp.Symbol("R", positive=True)
-    raw, exprs = traction_pairs(ft)
+    raw, exprs = traction_pairs(ft) if ft >= 0 else stress_combos()
     rnd = random.Random(7)
     worst = 0.0
     for _ in range(20):
@@ -202,13 +223,21 @@ def emit_traction(path, ft, halves=False):
         for a, b in zip(raw, exprs):
             fa, fb = float(a.subs(v)), float(b.subs(v))
             worst = max(worst, abs(fa - fb) / max(abs(fa), 1e-300))
-    print("traction pairs ft=%d: simplified vs raw, max relative difference %.1e" % (ft, worst))
+    print("%s: simplified vs raw, max relative difference %.1e" % (
+        "traction pairs ft=%d" % ft if ft >= 0 else "stress combinations", worst))
     assert worst < 1e-12
-    lines = ["/* generated by oracle/derive/mindlin_image.py --traction %d -- do not edit." % ft,
-             " * Inputs: X, Y (receiver - source, horizontal), zr, ys (receiver and",
-             " * source z, both <= 0), nu, R = |(X, Y, zr + ys)|.  Output P[12]: the",
-             " * pair sums of the image kernel the shear traction takes, P[p] and",
-             " * P[6 + p] for eigenstrain pair p. */"]
+    if ft >= 0:
+        lines = ["/* generated by oracle/derive/mindlin_image.py --traction %d -- do not edit." % ft,
+                 " * Inputs: X, Y (receiver - source, horizontal), zr, ys (receiver and",
+                 " * source z, both <= 0), nu, R = |(X, Y, zr + ys)|.  Output P[12]: the",
+                 " * pair sums of the image kernel the shear traction takes, P[p] and",
+                 " * P[6 + p] for eigenstrain pair p. */"]
+    else:
+        lines = ["/* generated by oracle/derive/mindlin_image.py --stress -- do not edit.",
+                 " * Inputs: X, Y (receiver - source, horizontal), zr, ys (receiver and",
+                 " * source z, both <= 0), nu, R = |(X, Y, zr + ys)|.  Output P[36]:",
+                 " * P[6 q + c], moment pair q, c = D[0][0], D[1][1], D[2][2], D[0][1] +",
+                 " * D[1][0], D[0][2] + D[2][0], D[1][2] + D[2][1] of the image kernel. */"]
 
     def cc(e):
         t = sp.ccode(e)
@@ -217,7 +246,8 @@ def emit_traction(path, ft, halves=False):
         return t
     # halves: the two sums of each pair in scopes of their own (fewer values
     # live at once, some subexpressions computed twice)
-    groups = [(0, exprs[:6]), (6, exprs[6:])] if halves else [(0, exprs)]
+    h = len(exprs) // 2
+    groups = [(0, exprs[:h]), (h, exprs[h:])] if halves else [(0, exprs)]
     ntemp = 0
     for base, grp in groups:
         reps, red = sp.cse(grp, optimizations="basic")
@@ -238,6 +268,9 @@ def emit_traction(path, ft, halves=False):
 if __name__ == "__main__":
     if "--check" in sys.argv:
         check()
+    if "--stress" in sys.argv:
+        emit_traction(sys.argv[sys.argv.index("--stress") + 1], -1, halves="--halves" in sys.argv)
+        sys.exit(0)
     if "--traction" in sys.argv:
         ft = int(sys.argv[sys.argv.index("--traction") + 1])
         emit_traction(sys.argv[sys.argv.index("--traction") + 2], ft, halves="--halves" in sys.argv)

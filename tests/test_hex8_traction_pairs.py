@@ -1,6 +1,9 @@
 """The generated traction pair sums of the hex8 image kernel
 (oetqf.jl_amd/csrc/hex8_image_trac0.inc for strike slip, _trac1.inc for dip
-slip) against the oracle's generated 54-term kernel (oracle/hex8_image.inc):
+slip), and its generated stress combinations (hex8_image_stress.inc: per
+moment pair q the diagonal D[k][k] and the symmetric sums D[k][l] + D[l][k]
+that a full stress takes, for the K_mm block), against the oracle's
+generated 54-term kernel (oracle/hex8_image.inc):
 the device quadrature of the fault-traction block K_fm accumulates only
 these 12 sums, P[p] and P[6 + p] for eigenstrain pair p, which the generator
 simplifies as a whole.  Both bodies are evaluated here statement by
@@ -18,6 +21,9 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_INC = os.path.join(ROOT, "oracle", "hex8_image.inc")
 PAIRS_INC = {ft: os.path.join(ROOT, "oetqf.jl_amd", "csrc", "hex8_image_trac%d.inc" % ft) for ft in (0, 1)}
+STRESS_INC = os.path.join(ROOT, "oetqf.jl_amd", "csrc", "hex8_image_stress.inc")
+# per moment pair q: (i, l) entries summed for each of the six combinations
+COMBOS = (((0, 0),), ((1, 1),), ((2, 2),), ((0, 1), (1, 0)), ((0, 2), (2, 0)), ((1, 2), (2, 1)))
 # (first entry, second entry, sign of the second) of the two sums, as (i, l)
 PAIRS = {0: (((0, 1), (1, 0), 1), ((0, 2), (2, 0), 1)),
          1: (((2, 2), (1, 1), -1), ((1, 2), (2, 1), 1))}
@@ -45,14 +51,18 @@ def _sums(D, ft):
     return [at(a, p) + s * at(b, p) for a, b, s in PAIRS[ft] for p in range(6)]
 
 
-def _agree(f_ref, f_pairs, ft, seed, npts=40):
+def _combos(D, combos=COMBOS):
+    return [sum(D[i * 18 + l * 6 + q] for i, l in c) for q in range(6) for c in combos]
+
+
+def _agree(f_ref, f_pairs, ft, seed, npts=40, sums=None):
     rng = np.random.default_rng(seed)
     for _ in range(npts):
         X, Y = rng.uniform(-3e4, 3e4, 2)
         zr, ys = -rng.uniform(1e2, 4e4, 2)
         nu = rng.uniform(0.15, 0.35)
         R = math.sqrt(X * X + Y * Y + (zr + ys) ** 2)
-        a = np.array(_sums(f_ref(X, Y, zr, ys, nu, R), ft))
+        a = np.array((sums or (lambda D: _sums(D, ft)))(f_ref(X, Y, zr, ys, nu, R)))
         b = np.array(f_pairs(X, Y, zr, ys, nu, R))
         scale = np.abs(a).max()
         assert scale > 0
@@ -84,3 +94,22 @@ def test_planted_pair_mistakes_are_caught(ft):
         assert not _agree(f_ref, f_pairs, ft, 0, npts=10)
     finally:
         PAIRS[ft] = good
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_stress_combinations_match_oracle_kernel(seed):
+    f_ref = _compile(ORACLE_INC)
+    f_st = _compile(STRESS_INC, n_out=36, name="P")
+    assert _agree(f_ref, f_st, None, seed, sums=_combos)
+
+
+def test_planted_stress_combination_mistake_is_caught():
+    """The oracle side planted wrong: a symmetric sum missing its second entry."""
+    f_ref = _compile(ORACLE_INC)
+    f_st = _compile(STRESS_INC, n_out=36, name="P")
+
+    def bad(D):
+        c = list(COMBOS)
+        c[4] = ((0, 2),)
+        return _combos(D, tuple(c))
+    assert not _agree(f_ref, f_st, None, 0, npts=10, sums=bad)
