@@ -314,6 +314,12 @@ __device__ void h8_image_traction(double x, double y, double z, const double lo0
         const double vol = h[0] * h[1] * h[2];
         for (int i = 0; i < OQ_H8_NQ; ++i)
             for (int j = 0; j < OQ_H8_NQ; ++j)
+                // One Gauss point per iteration: unrolled, the six points'
+                // temporaries spill (512 registers + 264 B/lane of scratch
+                // for strike slip) and one wavefront fits a SIMD; rolled, the
+                // body fits 244 VGPRs and two wavefronts hide each other's
+                // FP64 latency (k_gf_fm 61 -> 47 ms at 4096 x 2048 pairs).
+#pragma unroll 1
                 for (int k = 0; k < OQ_H8_NQ; ++k) {
                     const double ys1 = ctr[0] + h[0] * kGLX[OQ_H8_NQ][i];
                     const double ys2 = ctr[1] + h[1] * kGLX[OQ_H8_NQ][j];
