@@ -227,13 +227,15 @@ def emit_traction(path, ft, halves=False):
         "traction pairs ft=%d" % ft if ft >= 0 else "stress combinations", worst))
     assert worst < 1e-12
     if ft >= 0:
-        lines = ["/* generated by oracle/derive/mindlin_image.py --traction %d -- do not edit." % ft,
+        lines = ["/* generated by oracle/derive/mindlin_image.py --traction %d OUT%s -- do not edit."
+                 % (ft, " --halves" if halves else ""),
                  " * Inputs: X, Y (receiver - source, horizontal), zr, ys (receiver and",
                  " * source z, both <= 0), nu, R = |(X, Y, zr + ys)|.  Output P[12]: the",
                  " * pair sums of the image kernel the shear traction takes, P[p] and",
                  " * P[6 + p] for eigenstrain pair p. */"]
     else:
-        lines = ["/* generated by oracle/derive/mindlin_image.py --stress -- do not edit.",
+        lines = ["/* generated by oracle/derive/mindlin_image.py --stress OUT%s -- do not edit."
+                 % (" --halves" if halves else ""),
                  " * Inputs: X, Y (receiver - source, horizontal), zr, ys (receiver and",
                  " * source z, both <= 0), nu, R = |(X, Y, zr + ys)|.  Output P[36]:",
                  " * P[6 q + c], moment pair q, c = D[0][0], D[1][1], D[2][2], D[0][1] +",
