@@ -224,12 +224,13 @@ __global__ void __launch_bounds__(256) k_gf_ff(DevFault f, OkConst c, double lam
 }
 
 // K_mf = gf12 (GF.jl:141-170): thread (source patch j fastest, receiver
-// cell i); rows k*ne + i, column j.
+// cell i); rows k*ne + i, column j.  Held to two wavefronts per SIMD (256
+// registers, 84 B/lane of scratch for strike slip, against 282 registers at
+// one): 6.8 -> 6.1 ms for 12288 receivers x 4096 patches x 5 images.
 template <int SLIP>
-__global__ void __launch_bounds__(256) k_gf_mf(DevFault f, DevHex8 h, const double *qc,
-                                               const double *qw, int nq, OkConst c,
-                                               double lambda, double mu, int nrept,
-                                               double *out, int64_t rs, int64_t cs)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+k_gf_mf(DevFault f, DevHex8 h, const double *qc, const double *qw, int nq, OkConst c, double lambda, double mu,
+        int nrept, double *out, int64_t rs, int64_t cs)
 {
     const int64_t nf = f.nx * f.nxi;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
