@@ -27,6 +27,38 @@ def test_library_exports_every_declared_symbol(oq):
     assert set(_lib.PROTOTYPES) == set(syms)
 
 
+def _c_kind(p):
+    p = p.strip()
+    if "*" in p or "[" in p:
+        return "ptr"
+    for kind in ("double", "int64_t", "int"):
+        if re.match(r"(const\s+)?%s\b" % kind, p):
+            return kind
+    return "unknown: " + p
+
+
+def _ctypes_kind(t):
+    if t in (ctypes.c_void_p, ctypes.c_char_p) or (isinstance(t, type) and issubclass(t, ctypes._Pointer)):
+        return "ptr"
+    return {ctypes.c_double: "double", ctypes.c_int64: "int64_t", ctypes.c_int: "int"}.get(t, repr(t))
+
+
+def test_bindings_match_declared_signatures(oq):
+    """Every ctypes prototype has the header's return type and, parameter
+    by parameter, its kind (pointer, double, int64_t, int)."""
+    from oetqf_amd import _lib
+    src = open(os.path.join(ROOT, "include", "oetqf_amd.h")).read()
+    decls = re.findall(r"^(int|void|const char \*)\s*(oq_[a-z0-9_]+)\s*\(([^;]*?)\)\s*;", src, re.M | re.S)
+    assert len(decls) == len(declared_symbols())
+    rest = {"int": ctypes.c_int, "void": None, "const char *": ctypes.c_char_p}
+    for ret, name, params in decls:
+        params = " ".join(params.split())
+        want = [] if params in ("", "void") else [_c_kind(p) for p in params.split(",")]
+        res, args = _lib.PROTOTYPES[name]
+        assert res is rest[ret], name
+        assert [_ctypes_kind(t) for t in args] == want, name
+
+
 def test_version_and_error_string(oq):
     lib = oq.load_library()
     assert lib.oq_version() >= 1
