@@ -941,7 +941,7 @@ def main():
     ap.add_argument("--spans", default="",
                     help="instead of the bench line: whole simulated spans with event statistics, "
                          "comma-separated from composite, ensemble, members, group (JSON to --spans-out)")
-    ap.add_argument("--spans-out", default="gpurun_out/spans.json")
+    ap.add_argument("--spans-out", default="spans.json", help="where --spans writes its JSON")
     ap.add_argument("--spans-members", default="0,37,64,101,128,170,203,255",
                     help="ensemble members sampled by --spans ensemble and run alone in FP64 by --spans members")
     ap.add_argument("--spans-ref", default="",

@@ -1,8 +1,8 @@
-"""The documents point at real things: every repository path written in
-backquotes in DESIGN.md, INTEGRATION.md, README.md and profiles/*.md exists
-(globs match something), and every test named there (test_<name>, a file or
-a function) is defined under tests/.  oracle/_ref is named only to say why
-there is no reference build."""
+"""The repository's documents (the design notes, the integration guide, the
+read-me and the profile summaries) point at real things: every repository
+path they write in backquotes exists (globs match something), and every test
+they name (test_<name>, a file or a function) is defined under tests/.
+oracle/_ref is named only to say why there is no reference build."""
 import glob
 import os
 import re

@@ -1,4 +1,4 @@
-"""INTEGRATION.md's examples against the code they describe: its C blocks
+"""The integration guide's examples against the code they describe: its C blocks
 compile against include/oetqf_amd.h (gcc -fsyntax-only, with the example's
 variables declared around them), its Julia ccall matches the header's
 signature, and every oq.<name> and Ensemble / Group method it uses exists in
